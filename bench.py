@@ -1,0 +1,153 @@
+"""Benchmark: Mcell-updates/s of the 3D Yee hot path at 512^3 (gold dimer,
+12-cell split-field PML on all faces, 64-wavelength DFT on 3 planes, 8
+probes), FP32 fields beside the plain FP64 stepper, on one MI355X.
+
+One "step" is one full time step of the workload (E half step with the
+dispersive cells and the PML, H half step, DFT monitors and probes) over the
+whole 512^3 grid; inputs are resident in HBM before the timed region.
+
+Prints ONE JSON line.  ``roofline`` is measured live: HIP events around every
+launch of the dominant kernel (the fused E update) on the handle's stream,
+its algorithmic bytes per launch divided by its mean duration.
+``cpu_baseline`` times the FP64 CPU oracle (OpenMP) on a few steps of the
+same 512^3 workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "fortran-mpi-fdtd_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mcell-updates/s at 512³ gold+PML, FP32 vs FP64; %HBM roofline; p99 job latency"
+
+
+def run_gpu(spec, precision, steps, warmup):
+    import fdtd_amd
+    from fdtd_amd import abi
+
+    t0 = time.time()
+    sim, ids, pids = fdtd_amd.make_simulation(spec, precision=precision)
+    sim.commit()
+    setup_s = time.time() - t0
+    sim.step(warmup)
+    sim.sync()
+    sim.set_profiling(True)
+    t1 = time.perf_counter()
+    sim.step(steps)
+    sim.sync()
+    elapsed = time.perf_counter() - t1
+    nx, ny, nz = sim.shape
+    cells = nx * ny * nz
+    kern = {}
+    for k in (abi.K_E, abi.K_H, abi.K_ADE, abi.K_MON):
+        ms, n = sim.kernel_stats(k)
+        if n:
+            kern[abi.KERNEL_NAMES[k]] = {"mean_ms": ms / n, "launches": n,
+                                         "bytes_per_launch": sim.algorithmic_bytes(k) if k != abi.K_MON else None}
+    e = kern["e_update"]
+    ach = e["bytes_per_launch"] / (e["mean_ms"] * 1e-3) / 1e9
+    h = kern["h_update"]
+    ach_h = h["bytes_per_launch"] / (h["mean_ms"] * 1e-3) / 1e9
+    out = {
+        "value": cells * steps / elapsed / 1e6,
+        "ms_per_step": elapsed * 1e3 / steps,
+        "setup_s": setup_s,
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "k_fused_e",
+                     "bytes_per_launch": e["bytes_per_launch"], "mean_launch_ms": e["mean_ms"]},
+        "h_update": {"achieved_GBs": ach_h, "frac": ach_h / HBM_PEAK_GBS, "mean_launch_ms": h["mean_ms"]},
+        "kernels": kern,
+        "device_bytes": sim.memory_bytes(),
+        "cell_counts": dict(zip(("pml", "ade"), sim.cell_counts())),
+    }
+    flux = [float(v) for v in sim.flux(ids[2])[:4]]
+    out["flux_sample"] = flux
+    sim.close()
+    return out
+
+
+def cpu_baseline(spec, steps):
+    """FP64 CPU oracle (OpenMP) on `steps` steps of the same workload."""
+    import oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    sim, _, _ = oracle.make_simulation(spec)
+    sim.commit()
+    sim.step(1)
+    t0 = time.perf_counter()
+    sim.step(steps)
+    el = time.perf_counter() - t0
+    nx, ny, nz = sim.shape
+    sim.close()
+    return {"value": nx * ny * nz * steps / el / 1e6, "unit": "Mcell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"FP64 oracle (fdtd_oracle.c, OpenMP) on the same {nx}x{ny}x{nz} workload, "
+                      f"{steps} timed steps after 1 warm-up step ({el:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from fdtd_amd import workloads
+
+    spec = workloads.c2_gold_dimer(seed=args.seed, n=args.n)
+    print(f"[bench] workload {spec['name']} gap {spec['gap_nm']:.3f} nm, fp32 {args.steps} steps", file=sys.stderr,
+          flush=True)
+    r32 = run_gpu(spec, "fp32", args.steps, args.warmup)
+    print(f"[bench] fp32 {r32['value']:.0f} Mcell/s", file=sys.stderr, flush=True)
+    r64 = None
+    if not args.no_fp64:
+        r64 = run_gpu(spec, "fp64", args.steps, args.warmup)
+        print(f"[bench] fp64 {r64['value']:.0f} Mcell/s", file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(spec, args.cpu_steps)
+        print(f"[bench] cpu {cpu['value']:.1f} Mcell/s", file=sys.stderr, flush=True)
+    line = {
+        "metric": METRIC,
+        "value": r32["value"],
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": spec["name"], "grid": [args.n] * 3, "ds_nm": 1.0, "pml_cells": 12,
+                   "gap_nm": spec["gap_nm"], "seed": args.seed, "dft_planes": 3, "nfreq": 64, "probes": 8,
+                   "engine": "fused", "dft_accumulation": "f64"},
+        "roofline": r32["roofline"],
+        "fp64": None if r64 is None else {"value": r64["value"], "ms_per_step": r64["ms_per_step"],
+                                          "dtype": "f64", "roofline": r64["roofline"]},
+        "fp32_over_fp64": None if r64 is None else r32["value"] / r64["value"],
+        "cpu_baseline": cpu,
+        "p99_job_latency_ms": None,
+        "detail": {"fp32": {k: r32[k] for k in ("kernels", "h_update", "device_bytes", "cell_counts", "setup_s")},
+                   "fp64": None if r64 is None else {k: r64[k] for k in ("kernels", "h_update", "device_bytes")}},
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

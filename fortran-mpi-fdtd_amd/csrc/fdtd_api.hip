@@ -1,0 +1,1021 @@
+// fdtd_api.hip -- C ABI of the gfx950 FDTD library (include/fdtd.h): set-up,
+// geometry rasterisation, coefficient tables, device state, results.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <map>
+
+#include "fdtd_internal.h"
+
+namespace fdtd {
+
+const Consts& consts() {
+  static const Consts c = [] {
+    Consts k{};
+    k.pi = 4.0 * std::atan(1.0);      // misc/constants_mod.f90:4
+    k.ep0 = 8.85418781762039e-12;     // :6
+    k.mu0 = k.pi * 4e-7;              // :7
+    k.c0 = 1.0 / std::sqrt(k.ep0 * k.mu0);
+    k.z0 = std::sqrt(k.mu0 / k.ep0);  // mod3/BERpmly_mod.f90:625
+    const double h = 6.62607015e-34;
+    k.hbar = h / (2.0 * k.pi);
+    k.kb = 1.380649e-23;
+    k.echarge = 1.602176634e-19;
+    return k;
+  }();
+  return c;
+}
+
+static thread_local std::string g_err;
+
+int set_error(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_check(hipError_t e, const char* what) {
+  return set_error(FDTD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void* dev_alloc(Sim* s, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+  if (hipMemset(p, 0, bytes) != hipSuccess) {
+    (void)hipFree(p);
+    return nullptr;
+  }
+  s->allocs.push_back({p, bytes});
+  s->dev_bytes += bytes;
+  return p;
+}
+
+double* scratch(Sim* s, size_t bytes) {
+  if ((size_t)s->red_cap < bytes) {
+    s->d_red = (double*)dev_alloc(s, bytes);
+    s->red_cap = s->d_red ? (int)bytes : 0;
+  }
+  return s->d_red;
+}
+
+namespace {
+
+using cplx = std::complex<double>;
+
+// ---- regions: box clipping of mod3/material_mod.f90:60-119 + shapes -------
+bool region_bounds(const Sim* s, const fdtd_region& r, int lo[3], int hi[3]) {
+  const int n[3] = {s->nx, s->ny, s->nz};
+  if (r.kind == FDTD_REGION_BOX) {
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = r.lo[a];
+      hi[a] = r.hi[a];
+    }
+  } else if (r.kind == FDTD_REGION_SPHERE || r.kind == FDTD_REGION_CYLINDER) {
+    if (r.kind == FDTD_REGION_CYLINDER && (r.axis < 0 || r.axis > 2)) return false;
+    for (int a = 0; a < 3; ++a) {
+      double ext = r.radius;
+      if (r.kind == FDTD_REGION_CYLINDER && a == r.axis) ext = r.half_length;
+      lo[a] = (int)std::ceil(r.center[a] - ext);
+      hi[a] = (int)std::floor(r.center[a] + ext);
+      if (r.lo[a] <= r.hi[a] && (r.lo[a] != 0 || r.hi[a] != 0)) {
+        lo[a] = std::max(lo[a], r.lo[a]);
+        hi[a] = std::min(hi[a], r.hi[a]);
+      }
+    }
+  } else {
+    return false;
+  }
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = std::max(lo[a], 1);
+    hi[a] = std::min(hi[a], n[a]);
+  }
+  return true;
+}
+
+bool region_inside(const fdtd_region& r, int i, int j, int k) {
+  const double p[3] = {(double)i, (double)j, (double)k};
+  if (r.kind == FDTD_REGION_BOX) return true;
+  if (r.kind == FDTD_REGION_SPHERE) {
+    double d2 = 0;
+    for (int a = 0; a < 3; ++a) d2 += (p[a] - r.center[a]) * (p[a] - r.center[a]);
+    return d2 <= r.radius * r.radius;
+  }
+  double d2 = 0;
+  for (int a = 0; a < 3; ++a)
+    if (a != r.axis) d2 += (p[a] - r.center[a]) * (p[a] - r.center[a]);
+  const double dz = std::fabs(p[r.axis] - r.center[r.axis]);
+  return d2 <= r.radius * r.radius && dz <= r.half_length;
+}
+
+// ---- dispersive coefficients ---------------------------------------------
+void drude2cp_coef(const Sim* s, const fdtd_drude2cp& p, AdeCoef<double>& c, double* eps_inf) {
+  const double dt = s->dt;
+  const cplx I(0.0, 1.0);
+  std::memset(&c, 0, sizeof c);
+  // setexp, mod3/gold_mod.f90:187-195
+  const cplx expi1 = std::exp(-I * p.phi[0]);
+  const cplx expi2 = std::exp(-I * p.phi[1]);
+  const cplx exp1 = std::exp((-p.Gamma[0] + I * p.Omega[0]) * dt);
+  const cplx exp2 = std::exp((-p.Gamma[1] + I * p.Omega[1]) * dt);
+  // initialize_GoldVars, :175-181
+  c.ncp = 2;
+  c.raman = 0;
+  c.ha = std::exp(-p.gamma * dt);
+  c.hb = (p.omega_d * p.omega_d) * dt / p.gamma;
+  const cplx gc1 = dt * 2.0 * I * p.A[0] * p.Omega[0] * expi1;
+  const cplx gc2 = dt * 2.0 * I * p.A[1] * p.Omega[1] * expi2;
+  const cplx ga = p.eps_inf - gc1 - gc2;
+  c.gbr[0] = exp1.real(); c.gbi[0] = exp1.imag();
+  c.gbr[1] = exp2.real(); c.gbi[1] = exp2.imag();
+  c.gcr[0] = gc1.real(); c.gci[0] = gc1.imag();
+  c.gcr[1] = gc2.real(); c.gci[1] = gc2.imag();
+  c.gar = ga.real();
+  c.gai = ga.imag();
+  *eps_inf = p.eps_inf;
+}
+
+int graphene_coef(const Sim* s, const fdtd_graphene& p, AdeCoef<double>& c, double* eps_inf) {
+  const Consts& K = consts();
+  const double dt = s->dt;
+  const cplx I(0.0, 1.0);
+  std::memset(&c, 0, sizeof c);
+  c.ncp = 4;
+  if (p.model == FDTD_GRAPHENE_CP4) {
+    // initialize_GrapheneVars + scaleParams, mod3/graphene_mod.f90:202-269
+    const double scale = p.thickness / s->ds;
+    const double wD = std::sqrt(scale) * p.omega_d;
+    c.ha = std::exp(-p.gamma * dt);
+    c.hb = (wD * wD) * dt / p.gamma;
+    cplx ga = p.eps_inf;
+    for (int q = 0; q < 4; ++q) {
+      cplx expi = std::exp(-I * p.phi[q]);
+      cplx ex = std::exp((-p.Gamma[q] + I * p.Omega[q]) * dt);
+      if (p.ref_init_order) {  // locals read before setexp (:222-231)
+        expi = 0.0;
+        ex = 0.0;
+      }
+      const cplx gc = dt * 2.0 * I * (scale * p.A[q]) * p.Omega[q] * expi;
+      c.gbr[q] = ex.real(); c.gbi[q] = ex.imag();
+      c.gcr[q] = gc.real(); c.gci[q] = gc.imag();
+      ga = ga - gc;
+    }
+    c.gar = ga.real();
+    c.gai = ga.imag();
+  } else if (p.model == FDTD_GRAPHENE_KUBO) {
+    // Kubo intraband Drude weight spread over one cell (see oracle)
+    if (!(p.tau > 0) || !(p.temperature > 0)) return set_error(FDTD_ERR_ARG, "tau and T must be > 0");
+    const double kT = K.kb * p.temperature;
+    const double ef = p.fermi_ev * K.echarge;
+    const double Dw = 2.0 * K.echarge * K.echarge * kT / (K.hbar * K.hbar) * std::log(2.0 * std::cosh(ef / (2.0 * kT)));
+    const double wp2 = Dw / (K.pi * K.ep0 * s->ds);
+    const double gam = 1.0 / p.tau;
+    c.ha = std::exp(-gam * dt);
+    c.hb = wp2 * dt / gam;
+    c.gar = p.eps_inf;
+    c.gai = 0.0;
+  } else {
+    return set_error(FDTD_ERR_ARG, "bad graphene model");
+  }
+  if (p.wNL > 0) {
+    // exact-exponential Raman recursion, mod3/graphene_mod.f90:216-218,256-258
+    const double alfaR = 0.5 * p.dnuNL;
+    const double betaR = std::sqrt((p.wNL * p.wNL) - (alfaR * alfaR));
+    const double omegaR = (p.wNL * p.wNL) / betaR;
+    c.raman = 1;
+    c.r0 = p.xi3 * omegaR * dt * std::exp(-alfaR * dt) * std::sin(betaR * dt);
+    c.r1 = 2.0 * std::exp(-alfaR * dt) * std::cos(betaR * dt);
+    c.r2 = -std::exp(-2.0 * alfaR * dt);
+  }
+  *eps_inf = p.eps_inf;
+  return 0;
+}
+
+// ---- PML profiles (reference formulas, one axis at a time) ----------------
+struct Prof {
+  const Sim* s;
+  double sigmam;
+  int P;
+  void e_low(double ep, int idx, double& f1, double& f2) const {
+    const Consts& K = consts();
+    const double sig = ep * sigmam * std::pow(((double)(P - idx) + 0.5) / (double)P, s->pml.order);
+    const double alpha = sig * s->dt / ep;
+    f1 = std::exp(-alpha);
+    f2 = (1.0 - f1) / (K.z0 * sig * s->ds);
+  }
+  void h_low(int idx, double& g1, double& g2) const {
+    const Consts& K = consts();
+    const double sig = K.mu0 * sigmam * std::pow(((double)(P - idx)) / (double)P, s->pml.order);
+    const double alpha = sig * s->dt / K.mu0;
+    g1 = std::exp(-alpha);
+    g2 = K.z0 * (1.0 - g1) / (sig * s->ds);
+  }
+  void e_high(double ep, int q, double& f1, double& f2) const {
+    const Consts& K = consts();
+    const double sig = ep * sigmam * std::pow(((double)(P - q)) / (double)P, s->pml.order);
+    const double alpha = sig * s->dt / ep;
+    f1 = std::exp(-alpha);
+    f2 = (1.0 - f1) / (K.z0 * sig * s->ds);
+  }
+  void h_high(int q, double& g1, double& g2) const {
+    const Consts& K = consts();
+    const double sig = K.mu0 * sigmam * std::pow(((double)(P - q) + 0.5) / (double)P, s->pml.order);
+    const double alpha = sig * s->dt / K.mu0;
+    g1 = std::exp(-alpha);
+    g2 = K.z0 * (1.0 - g1) / (sig * s->ds);
+  }
+  // init_varx1 then init_varx2 shape on indices 1..n
+  void grade(double ep, int n, double* f1, double* f2, double* g1, double* g2) const {
+    for (int x = 1; x <= P; ++x) {
+      e_low(ep, x, f1[x], f2[x]);
+      if (x < P) h_low(x, g1[x], g2[x]);
+    }
+    int q = P;
+    for (int x = n - P + 1; x <= n; ++x) {
+      if (q < P) e_high(ep, q, f1[x], f2[x]);
+      h_high(q, g1[x], g2[x]);
+      --q;
+    }
+  }
+};
+
+double sigmam_for(const Sim* s, int P) {
+  const Consts& K = consts();
+  if (s->pml.sigmam > 0) return s->pml.sigmam;
+  return (s->pml.order + 1.0) * K.c0 * std::log(1.0 / s->pml.R0) / (2.0 * P * s->ds);  // sim3/program.f90:102
+}
+
+template <typename real>
+void* upload(Sim* s, const std::vector<double>& v) {
+  std::vector<real> h(v.begin(), v.end());
+  void* p = dev_alloc(s, h.size() * sizeof(real));
+  if (p && !h.empty()) (void)hipMemcpy(p, h.data(), h.size() * sizeof(real), hipMemcpyHostToDevice);
+  return p;
+}
+
+void* upload_bytes(Sim* s, const void* data, size_t bytes) {
+  void* p = dev_alloc(s, bytes);
+  if (p && bytes) (void)hipMemcpy(p, data, bytes, hipMemcpyHostToDevice);
+  return p;
+}
+
+int setup_all_faces(Sim* s) {
+  const int n[3] = {s->nx, s->ny, s->nz};
+  for (int a = 0; a < 3; ++a) {
+    s->P[a] = s->pml.npml[a];
+    if (s->P[a] < 0) return set_error(FDTD_ERR_ARG, "negative PML thickness");
+    if (s->P[a] > 0 && s->g.periodic[a]) return set_error(FDTD_ERR_ARG, "PML on a periodic axis");
+    if (2 * s->P[a] > n[a]) return set_error(FDTD_ERR_ARG, "PML thicker than the grid");
+  }
+  if (s->P[0] + s->P[1] + s->P[2] > 0 && !(s->pml.order > 0)) return set_error(FDTD_ERR_ARG, "bad PML order");
+  const int Px = s->P[0], Py = s->P[1], Pz = s->P[2];
+  s->npml_cells = (int64_t)2 * Pz * s->nx * s->ny + (int64_t)(s->nz - 2 * Pz) * 2 * Py * s->nx +
+                  (int64_t)(s->nz - 2 * Pz) * (s->ny - 2 * Py) * 2 * Px;
+  std::vector<double> prof;
+  for (int a = 0; a < 3; ++a) {
+    std::vector<double> f1(n[a] + 2, 1.0), f2(n[a] + 2, s->ku_pml), g1(n[a] + 2, 1.0), g2(n[a] + 2, s->ku_pml);
+    if (s->P[a] > 0) {
+      Prof pr{s, sigmam_for(s, s->P[a]), s->P[a]};
+      pr.grade(consts().ep0, n[a], f1.data(), f2.data(), g1.data(), g2.data());
+    }
+    prof.insert(prof.end(), f1.begin(), f1.end());
+    prof.insert(prof.end(), f2.begin(), f2.end());
+    prof.insert(prof.end(), g1.begin(), g1.end());
+    prof.insert(prof.end(), g2.begin(), g2.end());
+  }
+  s->d_prof = s->prec == FDTD_FP32 ? upload<float>(s, prof) : upload<double>(s, prof);
+  s->d_split = dev_alloc(s, (size_t)S_N * (size_t)std::max<int64_t>(s->npml_cells, 1) * s->wsize);
+  if (!s->d_prof || !s->d_split) return set_error(FDTD_ERR_NOMEM, "PML allocation");
+  return 0;
+}
+
+int setup_ref_y(Sim* s) {
+  const Consts& K = consts();
+  const int P = s->pml.npml[1];
+  if (P < 1 || 2 * P + 2 > s->ny) return set_error(FDTD_ERR_ARG, "bad y PML thickness");
+  if (!(s->pml.eps_low > 0) || !(s->pml.eps_high > 0)) return set_error(FDTD_ERR_ARG, "bad PML eps");
+  if (!(s->pml.order > 0)) return set_error(FDTD_ERR_ARG, "bad PML order");
+  if (s->g.periodic[1]) return set_error(FDTD_ERR_ARG, "REF_Y PML needs a non-periodic y axis");
+  const int npx = s->pml.npml[0], npz = s->pml.npml[2];
+  if (npx < 0 || npz < 0 || npx > s->nx || npz > s->nz) return set_error(FDTD_ERR_ARG, "bad x/z grading depth");
+  s->slab_cells = (int64_t)(s->nx + 2) * (P + 2) * (s->nz + 2);
+  s->d_slab = dev_alloc(s, (size_t)2 * S_N * (size_t)s->slab_cells * s->wsize);
+  const int ps = std::max(s->nx, std::max(s->ny, s->nz)) + 2;
+  s->ref_prof_stride = ps;
+  const double eplow = s->pml.eps_low * K.ep0, ephigh = s->pml.eps_high * K.ep0;
+  const double sigmam = sigmam_for(s, P);
+  std::vector<double> all((size_t)2 * 12 * ps, 0.0);
+  for (int sl = 0; sl < 2; ++sl) {
+    const bool low = sl == 0;
+    double* pr = all.data() + (size_t)sl * 12 * ps;
+    double *f1x = pr, *f2x = pr + ps, *f1y = pr + 2 * ps, *f2y = pr + 3 * ps, *f1z = pr + 4 * ps, *f2z = pr + 5 * ps;
+    double *g1x = pr + 6 * ps, *g2x = pr + 7 * ps, *g1y = pr + 8 * ps, *g2y = pr + 9 * ps, *g1z = pr + 10 * ps,
+           *g2z = pr + 11 * ps;
+    // allocate_vars defaults (mod3/BERpmly_mod.f90:567-601)
+    const double epr = low ? eplow / K.ep0 : ephigh / K.ep0;
+    for (int x = 0; x < ps; ++x) {
+      f1x[x] = f1y[x] = f1z[x] = 1.0;
+      f2x[x] = s->ku_pml / epr;
+      f2y[x] = s->ku_pml / epr;
+      f2z[x] = s->ku_pml / epr;
+      g1x[x] = g1y[x] = g1z[x] = 1.0;
+      g2x[x] = g2y[x] = g2z[x] = s->ku_pml;
+    }
+    const double ep = low ? eplow : ephigh;
+    if (npx > 0) Prof{s, sigmam, npx}.grade(ep, s->nx, f1x, f2x, g1x, g2x);  // :630-656
+    Prof py{s, sigmam, P};                                                   // :674-714
+    if (low) {
+      for (int j = 1; j <= P; ++j) {
+        py.e_low(eplow, j, f1y[j], f2y[j]);
+        if (j < P) py.h_low(j, g1y[j], g2y[j]);
+      }
+    } else {
+      int jj = P;
+      for (int j = s->ny - P + 1; j <= s->ny; ++j) {
+        if (jj < P) py.e_high(ephigh, jj, f1y[j], f2y[j]);
+        py.h_high(jj, g1y[j], g2y[j]);
+        --jj;
+      }
+    }
+    if (npz > 0) Prof{s, sigmam, npz}.grade(ep, s->nz, f1z, f2z, g1z, g2z);  // :719-745
+  }
+  s->d_ref_prof = s->prec == FDTD_FP32 ? upload<float>(s, all) : upload<double>(s, all);
+  if (!s->d_slab || !s->d_ref_prof) return set_error(FDTD_ERR_NOMEM, "PML allocation");
+  return 0;
+}
+
+inline bool cell_in_pml(const Sim* s, int i, int j, int k) {
+  if (s->pml_mode != FDTD_PML_ALL_FACES) return false;
+  const int* P = s->P;
+  return i <= P[0] || i > s->nx - P[0] || j <= P[1] || j > s->ny - P[1] || k <= P[2] || k > s->nz - P[2];
+}
+
+template <typename real>
+void* upload_table(Sim* s) {
+  std::vector<MatEntry<real>> t(s->tab.size());
+  for (size_t q = 0; q < t.size(); ++q) {
+    const MatEntry<double>& m = s->tab[q];
+    for (int c = 0; c < 3; ++c) {
+      t[q].coef[c] = (real)m.coef[c];
+      t[q].gax[c] = (real)m.gax[c];
+      t[q].act[c] = m.act[c];
+    }
+    t[q].pgax = (real)m.pgax;
+    t[q].eps = (real)m.eps;
+    t[q].pad = 0;
+  }
+  return upload_bytes(s, t.data(), t.size() * sizeof(MatEntry<real>));
+}
+
+int commit_impl(Sim* s) {
+  const Consts& K = consts();
+  (void)K;
+  // engine
+  int eng = s->g.engine;
+  if (eng == FDTD_ENGINE_AUTO) eng = s->pml_mode == FDTD_PML_REF_Y ? FDTD_ENGINE_STAGED : FDTD_ENGINE_FUSED;
+  if (eng == FDTD_ENGINE_FUSED && s->pml_mode == FDTD_PML_REF_Y)
+    return set_error(FDTD_ERR_UNSUPPORTED, "the REF_Y PML runs on the STAGED engine");
+  if (eng == FDTD_ENGINE_STAGED && s->pml_mode == FDTD_PML_ALL_FACES)
+    return set_error(FDTD_ERR_UNSUPPORTED, "the ALL_FACES PML runs on the FUSED engine");
+  if (eng != FDTD_ENGINE_FUSED && eng != FDTD_ENGINE_STAGED) return set_error(FDTD_ERR_ARG, "bad engine");
+  s->engine = eng;
+  if ((int)s->srcs.size() > kMaxSrc) return set_error(FDTD_ERR_ARG, "too many sources");
+
+  // layout
+  s->px = ((int64_t)s->nx + 2 + 15) / 16 * 16;
+  s->pxy = s->px * (s->ny + 2);
+  s->ncell_alloc = s->pxy * (s->nz + 2);
+  if (s->ncell_alloc >= ((int64_t)1 << 31)) return set_error(FDTD_ERR_ARG, "grid too large for one handle");
+  const size_t fbytes = (size_t)s->ncell_alloc * s->wsize;
+  for (int q = 0; q < 3; ++q) {
+    s->E[q] = dev_alloc(s, fbytes);
+    s->H[q] = dev_alloc(s, fbytes);
+    if (!s->E[q] || !s->H[q]) return set_error(FDTD_ERR_NOMEM, "field allocation");
+    if (s->engine == FDTD_ENGINE_STAGED) {
+      s->D[q] = dev_alloc(s, fbytes);
+      if (!s->D[q]) return set_error(FDTD_ERR_NOMEM, "field allocation");
+    }
+  }
+
+  // PML
+  if (s->pml_mode == FDTD_PML_ALL_FACES) {
+    int rc = setup_all_faces(s);
+    if (rc) return rc;
+  } else if (s->pml_mode == FDTD_PML_REF_Y) {
+    int rc = setup_ref_y(s);
+    if (rc) return rc;
+  }
+
+  // dispersive groups (identical coefficients share one group)
+  s->groups.clear();
+  for (auto& m : s->mats) {
+    if (m.type == 0) continue;
+    AdeCoef<double> c;
+    double einf = 1.0;
+    if (m.type == 1) {
+      drude2cp_coef(s, m.d2, c, &einf);
+    } else {
+      int rc = graphene_coef(s, m.gr, c, &einf);
+      if (rc) return rc;
+    }
+    int gi = -1;
+    for (size_t q = 0; q < s->groups.size(); ++q)
+      if (std::memcmp(&s->groups[q].coef, &c, sizeof c) == 0) gi = (int)q;
+    if (gi < 0) {
+      AdeGroup g;
+      g.coef = c;
+      g.eps_inf = einf;
+      g.nstate = (s->engine == FDTD_ENGINE_FUSED ? 1 : 0) + 2 + 2 * c.ncp + (c.raman ? 2 : 0);
+      s->groups.push_back(g);
+      gi = (int)s->groups.size() - 1;
+    }
+    m.group = gi;
+  }
+
+  // rasterise: per cell, the combination of last writers per component
+  // (materials applied in call order, sim3/program.f90:411-414)
+  using Key = std::array<int16_t, 5>;  // writer x,y,z, last dielectric, last any
+  std::vector<Key> combos;
+  std::map<Key, int> index;
+  combos.push_back({-1, -1, -1, -1, -1});
+  index[combos[0]] = 0;
+  std::vector<uint8_t> cid((size_t)s->ncell_alloc, 0);
+  for (size_t mi = 0; mi < s->mats.size(); ++mi) {
+    const HostMaterial& m = s->mats[mi];
+    int lo[3], hi[3];
+    if (!region_bounds(s, m.reg, lo, hi)) return set_error(FDTD_ERR_ARG, "bad region");
+    std::vector<int> trans(256, -1);
+    for (int k = lo[2]; k <= hi[2]; ++k)
+      for (int j = lo[1]; j <= hi[1]; ++j)
+        for (int i = lo[0]; i <= hi[0]; ++i) {
+          if (!region_inside(m.reg, i, j, k)) continue;
+          const size_t c = (size_t)(i + s->px * j + s->pxy * k);
+          const int old = cid[c];
+          if (trans[old] < 0) {
+            Key key = combos[old];
+            for (int q = 0; q < 3; ++q)
+              if (m.writes[q]) key[q] = (int16_t)mi;
+            if (m.type == 0) key[3] = (int16_t)mi;
+            key[4] = (int16_t)mi;
+            auto it = index.find(key);
+            if (it == index.end()) {
+              if (combos.size() >= 256) return set_error(FDTD_ERR_UNSUPPORTED, "more than 256 material combinations");
+              combos.push_back(key);
+              it = index.emplace(key, (int)combos.size() - 1).first;
+            }
+            trans[old] = it->second;
+          }
+          cid[c] = (uint8_t)trans[old];
+        }
+  }
+  // material table
+  s->tab.assign(combos.size(), MatEntry<double>{});
+  for (size_t q = 0; q < combos.size(); ++q) {
+    const Key& key = combos[q];
+    MatEntry<double>& t = s->tab[q];
+    for (int c = 0; c < 3; ++c) {
+      const int w = key[c];
+      t.act[c] = ACT_KEEP;
+      t.gax[c] = 1.0;
+      t.coef[c] = s->ku;
+      if (w < 0) continue;
+      const HostMaterial& m = s->mats[w];
+      if (m.type == 0) {
+        const double gax = (m.eps_r == 0.0) ? 0.0 : 1.0 / m.eps_r;  // simpledielectric_mod.f90:91-97
+        t.act[c] = ACT_DIEL;
+        t.gax[c] = gax;
+        t.coef[c] = s->ku * gax;
+      } else {
+        t.act[c] = ACT_ADE;
+      }
+    }
+    t.pgax = key[3] >= 0 ? (s->mats[key[3]].eps_r == 0.0 ? 0.0 : 1.0 / s->mats[key[3]].eps_r) : 1.0;
+    if (key[4] >= 0) {
+      const HostMaterial& m = s->mats[key[4]];
+      t.eps = m.type == 0 ? m.eps_r : s->groups[m.group].eps_inf;
+    } else {
+      t.eps = 1.0;
+    }
+    t.pad = 0;
+  }
+  s->ntab = (int)s->tab.size();
+  s->d_mattab = s->prec == FDTD_FP32 ? upload_table<float>(s) : upload_table<double>(s);
+  s->d_matid = (uint8_t*)upload_bytes(s, cid.data(), cid.size());
+  if (!s->d_mattab || !s->d_matid) return set_error(FDTD_ERR_NOMEM, "material table");
+
+  // dispersive cell lists
+  std::vector<std::vector<int32_t>> gcells(s->groups.size());
+  std::vector<std::vector<uint8_t>> gmask(s->groups.size());
+  s->n_pml = 0;
+  s->n_ade = 0;
+  for (int k = 1; k <= s->nz; ++k)
+    for (int j = 1; j <= s->ny; ++j)
+      for (int i = 1; i <= s->nx; ++i) {
+        const bool pm = cell_in_pml(s, i, j, k);
+        if (pm) ++s->n_pml;
+        const int64_t c = i + s->px * j + s->pxy * k;
+        const Key& key = combos[cid[(size_t)c]];
+        if (pm) continue;
+        int gm[3] = {-1, -1, -1};
+        for (int q = 0; q < 3; ++q)
+          if (key[q] >= 0 && s->mats[key[q]].type != 0) gm[q] = s->mats[key[q]].group;
+        bool any = false;
+        for (size_t g = 0; g < s->groups.size(); ++g) {
+          uint8_t mk = 0;
+          for (int q = 0; q < 3; ++q)
+            if (gm[q] == (int)g) mk |= (uint8_t)(1 << q);
+          if (mk) {
+            gcells[g].push_back((int32_t)c);
+            gmask[g].push_back(mk);
+            any = true;
+          }
+        }
+        if (any) ++s->n_ade;
+      }
+  s->n_main = (int64_t)s->nx * s->ny * s->nz - s->n_pml;
+  for (size_t g = 0; g < s->groups.size(); ++g) {
+    AdeGroup& G = s->groups[g];
+    G.ncells = (int64_t)gcells[g].size();
+    if (!G.ncells) continue;
+    G.d_cells = (int32_t*)upload_bytes(s, gcells[g].data(), gcells[g].size() * sizeof(int32_t));
+    G.d_mask = (uint8_t*)upload_bytes(s, gmask[g].data(), gmask[g].size());
+    G.d_state = dev_alloc(s, (size_t)G.nstate * 3 * (size_t)G.ncells * s->wsize);
+    if (!G.d_cells || !G.d_mask || !G.d_state) return set_error(FDTD_ERR_NOMEM, "dispersive state");
+  }
+
+  // monitors
+  for (auto& d : s->dfts) {
+    d.nslots = 0;
+    for (int c = 0; c < 6; ++c) d.slot[c] = (d.mask & (1 << c)) ? d.nslots++ : -1;
+    d.d_omega = (double*)upload_bytes(s, d.omega.data(), d.omega.size() * sizeof(double));
+    d.d_acc = (double*)dev_alloc(s, sizeof(double) * 2 * (size_t)d.nslots * d.nfreq * (size_t)d.n1 * d.n2);
+    if (!d.d_omega || !d.d_acc) return set_error(FDTD_ERR_NOMEM, "DFT accumulators");
+  }
+  if (!s->probes.empty()) {
+    std::vector<int32_t> pos;
+    for (auto& p : s->probes) {
+      pos.push_back(p.i);
+      pos.push_back(p.j);
+      pos.push_back(p.k);
+    }
+    s->d_probe_pos = (int32_t*)upload_bytes(s, pos.data(), pos.size() * sizeof(int32_t));
+    if (!s->d_probe_pos) return set_error(FDTD_ERR_NOMEM, "probe table");
+  }
+  FDTD_HIP(hipDeviceSynchronize());
+  s->committed = true;
+  return 0;
+}
+
+int ensure_committed(Sim* s) {
+  if (s->committed) return 0;
+  FDTD_HIP(hipSetDevice(s->device));
+  return commit_impl(s);
+}
+
+int flush_profile(Sim* s) {
+  if (s->pending.empty()) return 0;
+  FDTD_HIP(hipStreamSynchronize(s->stream));
+  for (auto& pe : s->pending) {
+    float ms = 0.f;
+    FDTD_HIP(hipEventElapsedTime(&ms, pe.second.first, pe.second.second));
+    s->k_ms[pe.first] += ms;
+    s->k_launches[pe.first] += 1;
+    s->event_pool.push_back(pe.second.first);
+    s->event_pool.push_back(pe.second.second);
+  }
+  s->pending.clear();
+  return 0;
+}
+
+template <typename real>
+int read_field(Sim* s, void* src, double* dst) {
+  std::vector<real> h((size_t)s->ncell_alloc);
+  FDTD_HIP(hipMemcpyAsync(h.data(), src, h.size() * sizeof(real), hipMemcpyDeviceToHost, s->stream));
+  FDTD_HIP(hipStreamSynchronize(s->stream));
+  size_t o = 0;
+  for (int k = 1; k <= s->nz; ++k)
+    for (int j = 1; j <= s->ny; ++j) {
+      const real* row = h.data() + (size_t)(s->px * j + s->pxy * k);
+      for (int i = 1; i <= s->nx; ++i) dst[o++] = (double)row[i];
+    }
+  return 0;
+}
+
+template <typename real>
+int write_field(Sim* s, void* dstp, const double* src) {
+  std::vector<real> h((size_t)s->ncell_alloc);
+  FDTD_HIP(hipMemcpyAsync(h.data(), dstp, h.size() * sizeof(real), hipMemcpyDeviceToHost, s->stream));
+  FDTD_HIP(hipStreamSynchronize(s->stream));
+  size_t o = 0;
+  for (int k = 1; k <= s->nz; ++k)
+    for (int j = 1; j <= s->ny; ++j) {
+      real* row = h.data() + (size_t)(s->px * j + s->pxy * k);
+      for (int i = 1; i <= s->nx; ++i) row[i] = (real)src[o++];
+    }
+  FDTD_HIP(hipMemcpyAsync(dstp, h.data(), h.size() * sizeof(real), hipMemcpyHostToDevice, s->stream));
+  FDTD_HIP(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+void* field_ptr(Sim* s, int comp) {
+  if (comp < 3) return s->E[comp];
+  if (comp < 6) return s->H[comp - 3];
+  return s->D[comp - 6];
+}
+
+}  // namespace
+}  // namespace fdtd
+
+using namespace fdtd;
+
+#define GUARD(s)                                                  \
+  do {                                                            \
+    if (!(s)) return set_error(FDTD_ERR_ARG, "null handle");      \
+  } while (0)
+#define SETUP_ONLY(s)                                                             \
+  do {                                                                            \
+    if ((s)->committed) return set_error(FDTD_ERR_STATE, "set-up after commit"); \
+  } while (0)
+
+extern "C" {
+
+struct fdtd_sim : public Sim {};
+
+const char* fdtd_last_error(void) { return g_err.c_str(); }
+
+void fdtd_drude2cp_gold(fdtd_drude2cp* p) {
+  std::memset(p, 0, sizeof *p);
+  p->eps_inf = 1.1431;  // mod3/gold_mod.f90:28-34
+  p->gamma = 1.0805e14;
+  p->omega_d = 1.3202e16;
+  p->A[0] = 0.26698;
+  p->A[1] = 3.0834;
+  p->phi[0] = -1.2371;
+  p->phi[1] = -1.0968;
+  p->Omega[0] = 3.8711e15;
+  p->Omega[1] = 4.1684e15;
+  p->Gamma[0] = 4.4642e14;
+  p->Gamma[1] = 2.4555e15;
+}
+
+void fdtd_graphene_ref(fdtd_graphene* p) {
+  static const double ap[4] = {13.1068, 1.8417, 0.4191, 0.7737};  // mod3/graphene_mod.f90:33-41
+  static const double phip[4] = {-1.8391, 0.3505, 3.0565, -0.1199};
+  static const double omp[4] = {6.1264e14, 6.6843e15, 5.6289e15, 1.118e16};
+  static const double rp[4] = {1e14, 7.905e14, 6.7526e14, 1.0005e14};
+  std::memset(p, 0, sizeof *p);
+  p->model = FDTD_GRAPHENE_CP4;
+  p->normal_axis = 1;
+  p->ref_init_order = 1;
+  p->thickness = 3.4e-10;
+  p->eps_inf = 1.0;
+  p->gamma = 9e15;
+  p->omega_d = 5.2796e15;
+  for (int q = 0; q < 4; ++q) {
+    p->A[q] = ap[q];
+    p->phi[q] = phip[q];
+    p->Omega[q] = omp[q];
+    p->Gamma[q] = rp[q];
+  }
+}
+
+int fdtd_create(fdtd_sim** out, const fdtd_grid* g) {
+  if (!out || !g) return set_error(FDTD_ERR_ARG, "null argument");
+  if (g->nx < 1 || g->ny < 1 || g->nz < 1 || !(g->ds > 0) || !(g->courant > 0))
+    return set_error(FDTD_ERR_ARG, "bad grid");
+  if (g->precision != FDTD_FP64 && g->precision != FDTD_FP32) return set_error(FDTD_ERR_ARG, "bad precision");
+  int dev = 0;
+  FDTD_HIP(hipGetDevice(&dev));
+  fdtd_sim* s = new fdtd_sim();
+  s->g = *g;
+  s->nx = g->nx;
+  s->ny = g->ny;
+  s->nz = g->nz;
+  s->ds = g->ds;
+  s->ku = g->courant;
+  s->dt = s->ku * s->ds / consts().c0;      // sim3/program.f90:50
+  s->ku_pml = s->dt * consts().c0 / s->ds;  // mod3/BERpmly_mod.f90:506
+  s->prec = g->precision;
+  s->wsize = g->precision == FDTD_FP32 ? 4 : 8;
+  s->device = dev;
+  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete s;
+    return hip_check(e, "hipStreamCreate");
+  }
+  *out = s;
+  return 0;
+}
+
+int fdtd_destroy(fdtd_sim* s) {
+  if (!s) return 0;
+  (void)hipSetDevice(s->device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (auto& pe : s->pending) {
+    (void)hipEventDestroy(pe.second.first);
+    (void)hipEventDestroy(pe.second.second);
+  }
+  for (auto ev : s->event_pool) (void)hipEventDestroy(ev);
+  for (auto& b : s->allocs) (void)hipFree(b.p);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return 0;
+}
+
+int fdtd_set_pml(fdtd_sim* s, const fdtd_pml* p) {
+  GUARD(s);
+  SETUP_ONLY(s);
+  if (!p || p->mode < FDTD_PML_NONE || p->mode > FDTD_PML_ALL_FACES) return set_error(FDTD_ERR_ARG, "bad pml");
+  s->pml = *p;
+  s->pml_mode = p->mode;
+  return 0;
+}
+
+static int add_material(fdtd_sim* s, const fdtd_region* r, HostMaterial m) {
+  int lo[3], hi[3];
+  if (!r || !region_bounds(s, *r, lo, hi)) return set_error(FDTD_ERR_ARG, "bad region");
+  m.reg = *r;
+  s->mats.push_back(m);
+  return 0;
+}
+
+int fdtd_add_dielectric(fdtd_sim* s, const fdtd_region* r, double eps_r) {
+  GUARD(s);
+  SETUP_ONLY(s);
+  HostMaterial m{};
+  m.type = 0;
+  m.eps_r = eps_r;
+  m.writes[0] = m.writes[1] = m.writes[2] = 1;
+  return add_material(s, r, m);
+}
+
+int fdtd_add_drude2cp(fdtd_sim* s, const fdtd_region* r, const fdtd_drude2cp* p) {
+  GUARD(s);
+  SETUP_ONLY(s);
+  if (!p || !(p->gamma > 0)) return set_error(FDTD_ERR_ARG, "bad Drude+2CP parameters");
+  HostMaterial m{};
+  m.type = 1;
+  m.d2 = *p;
+  m.writes[0] = m.writes[1] = m.writes[2] = 1;
+  return add_material(s, r, m);
+}
+
+int fdtd_add_graphene(fdtd_sim* s, const fdtd_region* r, const fdtd_graphene* p) {
+  GUARD(s);
+  SETUP_ONLY(s);
+  if (!p || p->normal_axis < 0 || p->normal_axis > 2) return set_error(FDTD_ERR_ARG, "bad graphene parameters");
+  if (p->model == FDTD_GRAPHENE_CP4 && !(p->gamma > 0)) return set_error(FDTD_ERR_ARG, "bad graphene gamma");
+  if (p->model != FDTD_GRAPHENE_CP4 && p->model != FDTD_GRAPHENE_KUBO)
+    return set_error(FDTD_ERR_ARG, "bad graphene model");
+  if (p->model == FDTD_GRAPHENE_KUBO && (!(p->tau > 0) || !(p->temperature > 0)))
+    return set_error(FDTD_ERR_ARG, "tau and T must be > 0");
+  HostMaterial m{};
+  m.type = 2;
+  m.gr = *p;
+  m.writes[0] = m.writes[1] = m.writes[2] = 1;
+  m.writes[p->normal_axis] = 0;  // mod3/graphene_mod.f90:59-105 leaves Ey alone
+  return add_material(s, r, m);
+}
+
+int fdtd_add_plane_source(fdtd_sim* s, int axis, int index, int comp, const fdtd_pulse* p) {
+  GUARD(s);
+  SETUP_ONLY(s);
+  const int n[3] = {s->nx, s->ny, s->nz};
+  if (!p || axis < 0 || axis > 2 || comp < 0 || comp > 2 || index < 1 || index > n[axis])
+    return set_error(FDTD_ERR_ARG, "bad source plane");
+  if ((int)s->srcs.size() >= kMaxSrc) return set_error(FDTD_ERR_ARG, "too many sources");
+  if (!(p->st != 0.0)) return set_error(FDTD_ERR_ARG, "pulse width must be non-zero");
+  s->srcs.push_back({axis, index, comp, *p});
+  return 0;
+}
+
+int fdtd_add_dft_plane(fdtd_sim* s, int axis, int index, int nfreq, const double* omega, int comp_mask, int n_begin,
+                       int n_end, int* id) {
+  GUARD(s);
+  SETUP_ONLY(s);
+  const int n[3] = {s->nx, s->ny, s->nz};
+  if (!omega || !id || axis < 0 || axis > 2 || index < 1 || index > n[axis] || nfreq < 1 || nfreq > 4096 ||
+      (comp_mask & ~63) || !comp_mask)
+    return set_error(FDTD_ERR_ARG, "bad dft plane");
+  DftPlane d;
+  d.axis = axis;
+  d.index = index;
+  d.nfreq = nfreq;
+  d.mask = comp_mask;
+  d.n_begin = n_begin;
+  d.n_end = n_end;
+  d.n1 = n[axis == 0 ? 1 : 0];
+  d.n2 = n[axis == 2 ? 1 : 2];
+  d.omega.assign(omega, omega + nfreq);
+  s->dfts.push_back(d);
+  *id = (int)s->dfts.size() - 1;
+  return 0;
+}
+
+int fdtd_add_probe(fdtd_sim* s, int i, int j, int k, int* id) {
+  GUARD(s);
+  SETUP_ONLY(s);
+  if (!id || i < 1 || i > s->nx || j < 1 || j > s->ny || k < 1 || k > s->nz)
+    return set_error(FDTD_ERR_ARG, "bad probe");
+  if (s->probes.size() >= 64) return set_error(FDTD_ERR_ARG, "too many probes");
+  Probe p;
+  p.i = i;
+  p.j = j;
+  p.k = k;
+  s->probes.push_back(p);
+  *id = (int)s->probes.size() - 1;
+  return 0;
+}
+
+int fdtd_commit(fdtd_sim* s) {
+  GUARD(s);
+  return ensure_committed(s);
+}
+
+int fdtd_step(fdtd_sim* s, int nsteps) {
+  GUARD(s);
+  if (nsteps < 0) return set_error(FDTD_ERR_ARG, "negative step count");
+  int rc = ensure_committed(s);
+  if (rc) return rc;
+  FDTD_HIP(hipSetDevice(s->device));
+  const int np = (int)s->probes.size();
+  if (np && nsteps > s->probe_cap) {
+    s->d_probe = (double*)dev_alloc(s, sizeof(double) * 7 * (size_t)np * nsteps);
+    if (!s->d_probe) return set_error(FDTD_ERR_NOMEM, "probe buffer");
+    s->probe_cap = nsteps;
+  }
+  s->call_base = s->n_done;
+  for (int it = 0; it < nsteps; ++it) {
+    rc = launch_step(s);
+    if (rc) return rc;
+    s->n_done += 1;
+  }
+  if (np && nsteps) {
+    std::vector<double> h(7 * (size_t)np * nsteps);
+    FDTD_HIP(hipMemcpyAsync(h.data(), s->d_probe, h.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    FDTD_HIP(hipStreamSynchronize(s->stream));
+    for (int it = 0; it < nsteps; ++it)
+      for (int p = 0; p < np; ++p) {
+        const double* r = h.data() + 7 * ((size_t)it * np + p);
+        s->probes[p].rec.insert(s->probes[p].rec.end(), r, r + 7);
+      }
+  }
+  return flush_profile(s);
+}
+
+int fdtd_sync(fdtd_sim* s) {
+  GUARD(s);
+  if (!s->stream) return 0;
+  FDTD_HIP(hipStreamSynchronize(s->stream));
+  return flush_profile(s);
+}
+
+int fdtd_get_step(fdtd_sim* s, int* n) {
+  GUARD(s);
+  if (!n) return set_error(FDTD_ERR_ARG, "null argument");
+  *n = s->n_done;
+  return 0;
+}
+
+int fdtd_get_dt(fdtd_sim* s, double* dt) {
+  GUARD(s);
+  if (!dt) return set_error(FDTD_ERR_ARG, "null argument");
+  *dt = s->dt;
+  return 0;
+}
+
+int fdtd_get_field(fdtd_sim* s, int comp, double* dst) {
+  GUARD(s);
+  if (!dst || comp < 0 || comp > 8) return set_error(FDTD_ERR_ARG, "bad component");
+  int rc = ensure_committed(s);
+  if (rc) return rc;
+  void* f = field_ptr(s, comp);
+  if (!f) return set_error(FDTD_ERR_UNSUPPORTED, "D is kept only by the STAGED engine");
+  return s->prec == FDTD_FP32 ? read_field<float>(s, f, dst) : read_field<double>(s, f, dst);
+}
+
+int fdtd_set_field(fdtd_sim* s, int comp, const double* src) {
+  GUARD(s);
+  if (!src || comp < 0 || comp > 8) return set_error(FDTD_ERR_ARG, "bad component");
+  int rc = ensure_committed(s);
+  if (rc) return rc;
+  void* f = field_ptr(s, comp);
+  if (!f) return set_error(FDTD_ERR_UNSUPPORTED, "D is kept only by the STAGED engine");
+  rc = s->prec == FDTD_FP32 ? write_field<float>(s, f, src) : write_field<double>(s, f, src);
+  if (rc) return rc;
+  rc = launch_set_ghosts(s, comp);
+  if (rc) return rc;
+  FDTD_HIP(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int fdtd_get_dft(fdtd_sim* s, int id, int comp, double* re, double* im) {
+  GUARD(s);
+  if (id < 0 || id >= (int)s->dfts.size() || comp < 0 || comp > 5) return set_error(FDTD_ERR_ARG, "bad argument");
+  int rc = ensure_committed(s);
+  if (rc) return rc;
+  DftPlane& d = s->dfts[id];
+  if (d.slot[comp] < 0) return set_error(FDTD_ERR_ARG, "component not monitored");
+  const size_t np = (size_t)d.n1 * d.n2 * d.nfreq;
+  const double* base = d.d_acc + (size_t)d.slot[comp] * 2 * np;
+  if (re) FDTD_HIP(hipMemcpyAsync(re, base, np * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  if (im) FDTD_HIP(hipMemcpyAsync(im, base + np, np * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  FDTD_HIP(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int fdtd_get_flux(fdtd_sim* s, int id, double* S) {
+  GUARD(s);
+  if (!S || id < 0 || id >= (int)s->dfts.size()) return set_error(FDTD_ERR_ARG, "bad argument");
+  int rc = ensure_committed(s);
+  if (rc) return rc;
+  return launch_flux(s, id, S);
+}
+
+int fdtd_get_probe_count(fdtd_sim* s, int id, int* nrec) {
+  GUARD(s);
+  if (!nrec || id < 0 || id >= (int)s->probes.size()) return set_error(FDTD_ERR_ARG, "bad argument");
+  *nrec = (int)(s->probes[id].rec.size() / 7);
+  return 0;
+}
+
+int fdtd_get_probe(fdtd_sim* s, int id, double* out) {
+  GUARD(s);
+  if (!out || id < 0 || id >= (int)s->probes.size()) return set_error(FDTD_ERR_ARG, "bad argument");
+  std::memcpy(out, s->probes[id].rec.data(), s->probes[id].rec.size() * sizeof(double));
+  return 0;
+}
+
+int fdtd_energy(fdtd_sim* s, double* we, double* wh) {
+  GUARD(s);
+  if (!we || !wh) return set_error(FDTD_ERR_ARG, "null argument");
+  int rc = ensure_committed(s);
+  if (rc) return rc;
+  return launch_energy(s, we, wh);
+}
+
+int fdtd_memory_bytes(fdtd_sim* s, size_t* bytes) {
+  GUARD(s);
+  if (!bytes) return set_error(FDTD_ERR_ARG, "null argument");
+  *bytes = s->dev_bytes;
+  return 0;
+}
+
+int fdtd_set_profiling(fdtd_sim* s, int on) {
+  GUARD(s);
+  int rc = flush_profile(s);
+  if (rc) return rc;
+  s->profiling = on != 0;
+  return 0;
+}
+
+int fdtd_kernel_stats(fdtd_sim* s, int kernel, double* total_ms, int64_t* launches) {
+  GUARD(s);
+  if (kernel < 0 || kernel >= FDTD_K_COUNT) return set_error(FDTD_ERR_ARG, "bad kernel id");
+  int rc = flush_profile(s);
+  if (rc) return rc;
+  if (total_ms) *total_ms = s->k_ms[kernel];
+  if (launches) *launches = s->k_launches[kernel];
+  return 0;
+}
+
+int fdtd_algorithmic_bytes(fdtd_sim* s, int kernel, double* bytes) {
+  GUARD(s);
+  if (!bytes || kernel < 0 || kernel >= FDTD_K_COUNT) return set_error(FDTD_ERR_ARG, "bad argument");
+  int rc = ensure_committed(s);
+  if (rc) return rc;
+  const double w = (double)s->wsize;
+  const double N = (double)s->nx * s->ny * s->nz;
+  double b = 0;
+  if (s->engine == FDTD_ENGINE_FUSED) {
+    // per non-PML cell: read H(3) + read/write E(6); per PML cell: read the
+    // other field (3), read/write 6 splits (12), write the main field (3).
+    if (kernel == FDTD_K_E || kernel == FDTD_K_H) b = w * (9.0 * (N - s->n_pml) + 18.0 * s->n_pml);
+    if (kernel == FDTD_K_ADE)
+      for (auto& g : s->groups) b += w * (6.0 + 2.0 * 3.0 * g.nstate) * (double)g.ncells;
+  } else {
+    const int P = s->pml_mode == FDTD_PML_REF_Y ? s->pml.npml[1] : 0;
+    const double Nm = (double)s->nx * (s->ny - 2 * P) * s->nz;
+    const double Ns = (double)s->nx * 2 * P * s->nz;
+    if (kernel == FDTD_K_E || kernel == FDTD_K_H) b = w * 9.0 * Nm;
+    if (kernel == FDTD_K_MAT) b = w * 6.0 * N;
+    if (kernel == FDTD_K_PML_E || kernel == FDTD_K_PML_H) b = w * 24.0 * Ns;
+    if (kernel == FDTD_K_ADE)
+      for (auto& g : s->groups) b += w * (6.0 + 2.0 * 3.0 * g.nstate) * (double)g.ncells;
+  }
+  *bytes = b;
+  return 0;
+}
+
+int fdtd_cell_counts(fdtd_sim* s, int64_t* n_pml, int64_t* n_ade) {
+  GUARD(s);
+  int rc = ensure_committed(s);
+  if (rc) return rc;
+  if (n_pml) *n_pml = s->n_pml;
+  if (n_ade) *n_ade = s->n_ade;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,181 @@
+// fdtd_internal.h -- shared state of the gfx950 FDTD library (host + device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/fdtd.h"
+
+namespace fdtd {
+
+// misc/constants_mod.f90:3-24, evaluated the way the reference evaluates them
+struct Consts {
+  double pi, ep0, mu0, c0, z0, hbar, kb, echarge;
+};
+const Consts& consts();
+
+// per-component action of a material-table entry
+enum : int32_t { ACT_KEEP = 0, ACT_DIEL = 1, ACT_ADE = 2 };
+
+// Material-table entry, one per distinct per-cell combination of writers.
+// coef = Ku*gax (E-form update), gax = 1/eps (source scaling and D-form),
+// pgax = medium of an ALL_FACES PML cell, eps = energy weight.
+template <typename real>
+struct alignas(16) MatEntry {
+  real coef[3];
+  real pgax;
+  real gax[3];
+  real eps;
+  int32_t act[3];
+  int32_t pad;
+};
+
+// Coefficients of one group of dispersive cells (Drude + up to 4 CP + Raman).
+template <typename real>
+struct AdeCoef {
+  int32_t ncp, raman;
+  real ku, ha, hb, gar, gai;
+  real gbr[4], gbi[4], gcr[4], gci[4];
+  real r0, r1, r2;
+};
+
+// Up to this many plane sources per handle.
+constexpr int kMaxSrc = 8;
+
+template <typename real>
+struct SrcArgs {
+  int32_t n;
+  int32_t axis[kMaxSrc], index[kMaxSrc], comp[kMaxSrc];
+  real val[kMaxSrc];
+};
+
+// split-field components
+enum { S_EXY = 0, S_EXZ, S_EYZ, S_EYX, S_EZX, S_EZY, S_HXY, S_HXZ, S_HYZ, S_HYX, S_HZX, S_HZY, S_N };
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct HostMaterial {
+  int type;  // 0 dielectric, 1 drude2cp, 2 graphene
+  fdtd_region reg;
+  double eps_r;
+  fdtd_drude2cp d2;
+  fdtd_graphene gr;
+  int writes[3];
+  int group;  // ADE group index (dispersive only)
+};
+
+struct HostSource {
+  int axis, index, comp;
+  fdtd_pulse p;
+};
+
+struct DftPlane {
+  int axis, index, nfreq, mask, n_begin, n_end, n1, n2;
+  std::vector<double> omega;
+  double* d_omega = nullptr;
+  double* d_acc = nullptr;  // [comp slot][re/im][m][v][u]
+  int slot[6];
+  int nslots = 0;
+};
+
+struct Probe {
+  int i, j, k;
+  std::vector<double> rec;
+};
+
+struct AdeGroup {
+  AdeCoef<double> coef;  // FP64 master copy
+  double eps_inf;
+  int nstate;                 // state words per component
+  int64_t ncells = 0;
+  int32_t* d_cells = nullptr;  // linear cell offsets
+  uint8_t* d_mask = nullptr;   // component mask per cell
+  void* d_state = nullptr;     // SoA [word][cell]
+};
+
+struct Sim {
+  fdtd_grid g{};
+  int nx = 0, ny = 0, nz = 0;
+  int64_t px = 0, pxy = 0, ncell_alloc = 0;  // ghosted layout strides
+  double ds = 0, ku = 0, dt = 0, ku_pml = 0;
+  int prec = FDTD_FP64;
+  int engine = FDTD_ENGINE_AUTO;
+  size_t wsize = 8;
+  fdtd_pml pml{};
+  int pml_mode = FDTD_PML_NONE;
+  std::vector<HostMaterial> mats;
+  std::vector<HostSource> srcs;
+  std::vector<DftPlane> dfts;
+  std::vector<Probe> probes;
+  bool committed = false;
+  int n_done = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+
+  // device state
+  void* E[3] = {nullptr, nullptr, nullptr};
+  void* H[3] = {nullptr, nullptr, nullptr};
+  void* D[3] = {nullptr, nullptr, nullptr};
+  uint8_t* d_matid = nullptr;  // per ghosted cell
+  void* d_mattab = nullptr;    // MatEntry<real>[ntab]
+  int ntab = 0;
+  std::vector<MatEntry<double>> tab;  // FP64 master copy
+  std::vector<AdeGroup> groups;
+  // ALL_FACES PML
+  int P[3] = {0, 0, 0};
+  int64_t npml_cells = 0;
+  void* d_split = nullptr;  // [S_N][npml_cells]
+  void* d_prof = nullptr;   // per axis: f1E, f2E, g1H, g2H each n_a+2
+  // REF_Y PML
+  int64_t slab_cells = 0;   // (nx+2)*(P+2)*(nz+2)
+  void* d_slab = nullptr;   // [2][S_N][slab_cells]
+  void* d_ref_prof = nullptr;  // [2 slabs][12 profiles][max_n+2]
+  int ref_prof_stride = 0;
+  // probes
+  double* d_probe = nullptr;
+  int32_t* d_probe_pos = nullptr;
+  int probe_cap = 0;
+  int call_base = 0;  // steps done before the current fdtd_step call
+  // reductions
+  double* d_red = nullptr;
+  int red_cap = 0;
+
+  std::vector<DevBuf> allocs;
+  size_t dev_bytes = 0;
+
+  // profiling
+  bool profiling = false;
+  double k_ms[FDTD_K_COUNT] = {0};
+  int64_t k_launches[FDTD_K_COUNT] = {0};
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::vector<hipEvent_t> event_pool;
+
+  // cell counts
+  int64_t n_pml = 0, n_ade = 0, n_main = 0;
+};
+
+int set_error(int code, const std::string& msg);
+int hip_check(hipError_t e, const char* what);
+#define FDTD_HIP(call)                                 \
+  do {                                                 \
+    hipError_t e_ = (call);                            \
+    if (e_ != hipSuccess) return hip_check(e_, #call); \
+  } while (0)
+
+void* dev_alloc(Sim* s, size_t bytes);  // zero-filled; nullptr on failure
+double* scratch(Sim* s, size_t bytes); // reusable reduction buffer
+
+// kernels (fdtd_kernels.hip)
+int launch_step(Sim* s);           // one full time step at s->n_done + 1
+int launch_energy(Sim* s, double* we, double* wh);
+int launch_flux(Sim* s, int id, double* S);
+int launch_set_ghosts(Sim* s, int comp);
+
+}  // namespace fdtd

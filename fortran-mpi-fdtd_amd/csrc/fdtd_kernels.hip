@@ -1,0 +1,893 @@
+// fdtd_kernels.hip -- gfx950 kernels of the 3D Yee time step.
+//
+// Field layout: one array per component, x fastest, with a one-cell ghost
+// ring on every axis (index 0..n+1, the reference's (0:N+1) bounds,
+// sim3/program.f90:77).  Offset of (i,j,k) = i + px*j + pxy*k, px padded.
+// Non-periodic ghosts stay zero (PEC); periodic axes keep the ghost planes
+// that the reference's halo exchange fills (sim3/include/mympi_func.f90:
+// 241-690) up to date from inside the kernels: the E writer of plane 1
+// also writes ghost n+1, the H writer of plane n also writes ghost 0.
+//
+// Two engines:
+//  FUSED  (production): E = E + Ku*gax*curl H per cell (D is kept only in
+//         dispersive cells), split-field PML cells in the same kernel,
+//         source folded in; one H kernel; one kernel per group of
+//         dispersive cells.
+//  STAGED (reference order): updateD, source, material E = D*gax,
+//         dispersive ADE, y-slab PML with interface copies, updateH,
+//         y-slab PML H -- one kernel per stage, D kept everywhere.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "fdtd_internal.h"
+
+namespace fdtd {
+
+template <typename real>
+struct FieldArgs {
+  real* e[3];
+  real* h[3];
+  real* d[3];
+  const uint8_t* matid;
+  const MatEntry<real>* tab;
+  int32_t nx, ny, nz;
+  int32_t per[3];
+  int64_t px, pxy;
+  real ku;
+};
+
+template <typename real>
+struct PmlArgs {
+  int32_t P[3];
+  real* s[S_N];
+  const real* f1[3];
+  const real* f2[3];
+  const real* g1[3];
+  const real* g2[3];
+};
+
+template <typename real>
+struct RefPmlArgs {
+  int32_t P, jb;
+  int64_t sx, sxy;  // slab strides: i + sx*jl + sxy*k
+  real* s[2][S_N];
+  const real* prof[2];  // 12 profiles per slab, stride ps
+  int32_t ps;
+};
+
+// ---------------------------------------------------------------------------
+// ghost maintenance for periodic axes
+template <typename real>
+__device__ __forceinline__ void put_e_ghost(real* f, real v, int i, int j, int k, int64_t c,
+                                            const FieldArgs<real>& a) {
+  if (a.per[0] && i == 1) f[c + a.nx] = v;
+  if (a.per[1] && j == 1) f[c + a.px * a.ny] = v;
+  if (a.per[2] && k == 1) f[c + a.pxy * a.nz] = v;
+}
+template <typename real>
+__device__ __forceinline__ void put_h_ghost(real* f, real v, int i, int j, int k, int64_t c,
+                                            const FieldArgs<real>& a) {
+  if (a.per[0] && i == a.nx) f[c - a.nx] = v;
+  if (a.per[1] && j == a.ny) f[c - a.px * a.ny] = v;
+  if (a.per[2] && k == a.nz) f[c - a.pxy * a.nz] = v;
+}
+
+__device__ __forceinline__ bool in_pml(const int32_t* P, int i, int j, int k, int nx, int ny, int nz) {
+  return i <= P[0] || i > nx - P[0] || j <= P[1] || j > ny - P[1] || k <= P[2] || k > nz - P[2];
+}
+
+// Compact index of an ALL_FACES PML cell: z slabs (full xy planes), then y
+// slabs of the middle planes (full x rows), then x slabs of the rest.
+__device__ __forceinline__ int64_t pml_index(const int32_t* P, int i, int j, int k, int nx, int ny, int nz) {
+  const int Px = P[0], Py = P[1], Pz = P[2];
+  if (k <= Pz || k > nz - Pz) {
+    int kk = k <= Pz ? k - 1 : Pz + (k - (nz - Pz + 1));
+    return (int64_t)kk * nx * ny + (int64_t)(j - 1) * nx + (i - 1);
+  }
+  const int64_t baseY = (int64_t)2 * Pz * nx * ny;
+  const int km = k - Pz - 1;
+  if (j <= Py || j > ny - Py) {
+    int jj = j <= Py ? j - 1 : Py + (j - (ny - Py + 1));
+    return baseY + ((int64_t)km * 2 * Py + jj) * nx + (i - 1);
+  }
+  const int64_t baseX = baseY + (int64_t)(nz - 2 * Pz) * 2 * Py * nx;
+  const int jm = j - Py - 1;
+  int ii = i <= Px ? i - 1 : Px + (i - (nx - Px + 1));
+  return baseX + ((int64_t)km * (ny - 2 * Py) + jm) * 2 * Px + ii;
+}
+
+// ---------------------------------------------------------------------------
+// FUSED engine: E half step (myfdtd_func.f90:19-41 in E-form, material
+// update_E folded in, ALL_FACES split-field PML, plane sources).
+template <typename real>
+__global__ __launch_bounds__(256) void k_fused_e(FieldArgs<real> a, PmlArgs<real> p, SrcArgs<real> src,
+                                                 int nsrc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  if (i > a.nx || j > a.ny) return;
+  const int64_t c = i + a.px * j + a.pxy * k;
+  const real hx = a.h[0][c], hy = a.h[1][c], hz = a.h[2][c];
+  const real hz_jm = a.h[2][c - a.px], hx_jm = a.h[0][c - a.px];
+  const real hy_km = a.h[1][c - a.pxy], hx_km = a.h[0][c - a.pxy];
+  const real hz_im = a.h[2][c - 1], hy_im = a.h[1][c - 1];
+  const MatEntry<real> m = a.tab[a.matid[c]];
+  real sv[3] = {0, 0, 0};
+  int sa[3] = {-1, -1, -1};
+  for (int q = 0; q < nsrc; ++q) {
+    const int pos = src.axis[q] == 0 ? i : (src.axis[q] == 1 ? j : k);
+    if (pos == src.index[q]) {
+      sv[src.comp[q]] += src.val[q];
+      sa[src.comp[q]] = src.axis[q];
+    }
+  }
+  if (in_pml(p.P, i, j, k, a.nx, a.ny, a.nz)) {
+    // mod3/BERpmly_mod.f90:150-155 with the splits driven by total H
+    const int64_t s = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
+    const real r = m.pgax;
+    real eyx = p.f1[0][i] * p.s[S_EYX][s] + (p.f2[0][i] * r) * (-hz + hz_im);
+    real ezx = p.f1[0][i] * p.s[S_EZX][s] + (p.f2[0][i] * r) * (hy + (-hy_im));
+    real exy = p.f1[1][j] * p.s[S_EXY][s] + (p.f2[1][j] * r) * (hz + (-hz_jm));
+    real ezy = p.f1[1][j] * p.s[S_EZY][s] + (p.f2[1][j] * r) * (-hx + hx_jm);
+    real exz = p.f1[2][k] * p.s[S_EXZ][s] + (p.f2[2][k] * r) * (-hy + hy_km);
+    real eyz = p.f1[2][k] * p.s[S_EYZ][s] + (p.f2[2][k] * r) * (hx + (-hx_km));
+    // a source plane through the PML feeds the split along its normal
+    if (sa[0] == 1) exy = exy + sv[0] * r;
+    if (sa[0] == 2) exz = exz + sv[0] * r;
+    if (sa[1] == 0) eyx = eyx + sv[1] * r;
+    if (sa[1] == 2) eyz = eyz + sv[1] * r;
+    if (sa[2] == 0) ezx = ezx + sv[2] * r;
+    if (sa[2] == 1) ezy = ezy + sv[2] * r;
+    p.s[S_EYX][s] = eyx; p.s[S_EZX][s] = ezx; p.s[S_EXY][s] = exy;
+    p.s[S_EZY][s] = ezy; p.s[S_EXZ][s] = exz; p.s[S_EYZ][s] = eyz;
+    a.e[0][c] = exy + exz;
+    a.e[1][c] = eyz + eyx;
+    a.e[2][c] = ezx + ezy;
+    return;  // periodic axes carry no PML: no ghosts to refresh here
+  }
+  const real cx = (hz - hz_jm) + (-hy + hy_km);  // curlhzy + curlhyz
+  const real cy = (hx - hx_km) + (-hz + hz_im);  // curlhxz + curlhzx
+  const real cz = (hy - hy_im) + (-hx + hx_jm);  // curlhyx + curlhxy
+  const real cur[3] = {cx, cy, cz};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    if (m.act[q] != ACT_DIEL) continue;
+    real ev = a.e[q][c] + m.coef[q] * cur[q];
+    if (sa[q] >= 0) ev = ev + m.gax[q] * sv[q];
+    a.e[q][c] = ev;
+    put_e_ghost(a.e[q], ev, i, j, k, c, a);
+  }
+}
+
+// FUSED engine: H half step (myfdtd_func.f90:45-64 + split-field H).
+template <typename real>
+__global__ __launch_bounds__(256) void k_fused_h(FieldArgs<real> a, PmlArgs<real> p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  if (i > a.nx || j > a.ny) return;
+  const int64_t c = i + a.px * j + a.pxy * k;
+  const real ex = a.e[0][c], ey = a.e[1][c], ez = a.e[2][c];
+  const real ey_kp = a.e[1][c + a.pxy], ex_kp = a.e[0][c + a.pxy];
+  const real ez_jp = a.e[2][c + a.px], ex_jp = a.e[0][c + a.px];
+  const real ez_ip = a.e[2][c + 1], ey_ip = a.e[1][c + 1];
+  if (in_pml(p.P, i, j, k, a.nx, a.ny, a.nz)) {
+    // mod3/BERpmly_mod.f90:186-191 with the splits driven by total E
+    const int64_t s = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
+    real hyx = p.g1[0][i] * p.s[S_HYX][s] + p.g2[0][i] * (-ez + ez_ip);
+    real hzx = p.g1[0][i] * p.s[S_HZX][s] + p.g2[0][i] * (ey + (-ey_ip));
+    real hxy = p.g1[1][j] * p.s[S_HXY][s] + p.g2[1][j] * (ez + (-ez_jp));
+    real hzy = p.g1[1][j] * p.s[S_HZY][s] + p.g2[1][j] * (-ex + ex_jp);
+    real hxz = p.g1[2][k] * p.s[S_HXZ][s] + p.g2[2][k] * (-ey + ey_kp);
+    real hyz = p.g1[2][k] * p.s[S_HYZ][s] + p.g2[2][k] * (ex + (-ex_kp));
+    p.s[S_HYX][s] = hyx; p.s[S_HZX][s] = hzx; p.s[S_HXY][s] = hxy;
+    p.s[S_HZY][s] = hzy; p.s[S_HXZ][s] = hxz; p.s[S_HYZ][s] = hyz;
+    a.h[0][c] = hxy + hxz;
+    a.h[1][c] = hyz + hyx;
+    a.h[2][c] = hzx + hzy;
+    return;
+  }
+  const real curleyz = ey_kp - ey, curlezy = -ez_jp + ez;
+  const real curlezx = ez_ip - ez, curlexz = -ex_kp + ex;
+  const real curlexy = ex_jp - ex, curleyx = -ey_ip + ey;
+  const real hx = a.h[0][c] + a.ku * (curleyz + curlezy);
+  const real hy = a.h[1][c] + a.ku * (curlezx + curlexz);
+  const real hz = a.h[2][c] + a.ku * (curlexy + curleyx);
+  a.h[0][c] = hx;
+  a.h[1][c] = hy;
+  a.h[2][c] = hz;
+  put_h_ghost(a.h[0], hx, i, j, k, c, a);
+  put_h_ghost(a.h[1], hy, i, j, k, c, a);
+  put_h_ghost(a.h[2], hz, i, j, k, c, a);
+}
+
+// ---------------------------------------------------------------------------
+// Dispersive cells: Drude + up to 4 critical points + Raman
+// (mod3/gold_mod.f90:61-83, mod3/graphene_mod.f90:70-100).  State SoA:
+// word w of component q for cell e at state[(w*3+q)*n + e];
+// words: [D (FUSED only)], I, S, g_p re, g_p im (p < ncp), sn, snm2 (Raman).
+template <typename real, bool FUSED>
+__global__ __launch_bounds__(256) void k_ade(FieldArgs<real> a, AdeCoef<real> co, const int32_t* cells,
+                                             const uint8_t* masks, real* state, int64_t n, SrcArgs<real> src,
+                                             int nsrc) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int64_t c = cells[e];
+  const int mask = masks[e];
+  const int k = (int)(c / a.pxy);
+  const int64_t rem = c - (int64_t)k * a.pxy;
+  const int j = (int)(rem / a.px);
+  const int i = (int)(rem - (int64_t)j * a.px);
+  real cur[3] = {0, 0, 0};
+  real sv[3] = {0, 0, 0};
+  if (FUSED) {
+    const real hx = a.h[0][c], hy = a.h[1][c], hz = a.h[2][c];
+    cur[0] = (hz - a.h[2][c - a.px]) + (-hy + a.h[1][c - a.pxy]);
+    cur[1] = (hx - a.h[0][c - a.pxy]) + (-hz + a.h[2][c - 1]);
+    cur[2] = (hy - a.h[1][c - 1]) + (-hx + a.h[0][c - a.px]);
+    for (int q = 0; q < nsrc; ++q) {
+      const int pos = src.axis[q] == 0 ? i : (src.axis[q] == 1 ? j : k);
+      if (pos == src.index[q]) sv[src.comp[q]] += src.val[q];
+    }
+  }
+  const int w0 = FUSED ? 1 : 0;  // word offset of I
+  const int ncp = co.ncp;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    if (!(mask & (1 << q))) continue;
+    real dv;
+    if (FUSED) {
+      real* dp = state + (int64_t)(0 * 3 + q) * n + e;
+      dv = *dp + co.ku * cur[q];
+      dv = dv + sv[q];
+      *dp = dv;
+    } else {
+      dv = a.d[q][c];
+    }
+    real* Ip = state + (int64_t)((w0 + 0) * 3 + q) * n + e;
+    real* Sp = state + (int64_t)((w0 + 1) * 3 + q) * n + e;
+    const real Iv = *Ip, Sv = *Sp;
+    real xr = (dv - Iv) + co.ha * Sv;
+    real xi = 0;
+    real gr[4], gi[4];
+#pragma unroll
+    for (int pp = 0; pp < 4; ++pp) {
+      if (pp >= ncp) break;
+      gr[pp] = state[(int64_t)((w0 + 2 + pp) * 3 + q) * n + e];
+      gi[pp] = state[(int64_t)((w0 + 2 + ncp + pp) * 3 + q) * n + e];
+      xr = xr + (co.gbr[pp] * gr[pp] - co.gbi[pp] * gi[pp]);
+      xi = xi + (co.gbr[pp] * gi[pp] + co.gbi[pp] * gr[pp]);
+    }
+    real snv = 0, snm2 = 0;
+    real* snp = nullptr;
+    real* snm2p = nullptr;
+    if (co.raman) {
+      snp = state + (int64_t)((w0 + 2 + 2 * ncp) * 3 + q) * n + e;
+      snm2p = state + (int64_t)((w0 + 3 + 2 * ncp) * 3 + q) * n + e;
+      snv = *snp;
+      snm2 = *snm2p;
+    }
+    // E = Re[X / (ga + sn)]
+    const real dr = co.gar + snv, di = co.gai;
+    const real ev = (xr * dr + xi * di) / (dr * dr + di * di);
+    a.e[q][c] = ev;
+    put_e_ghost(a.e[q], ev, i, j, k, c, a);
+    if (co.raman) {
+      const real nsn = co.r1 * snv + co.r2 * snm2 + co.r0 * (ev * ev);
+      *snm2p = snv;
+      *snp = nsn;
+    }
+    *Ip = Iv + co.hb * ev;
+    *Sp = co.ha * Sv + co.hb * ev;
+#pragma unroll
+    for (int pp = 0; pp < 4; ++pp) {
+      if (pp >= ncp) break;
+      state[(int64_t)((w0 + 2 + pp) * 3 + q) * n + e] = (co.gbr[pp] * gr[pp] - co.gbi[pp] * gi[pp]) + co.gcr[pp] * ev;
+      state[(int64_t)((w0 + 2 + ncp + pp) * 3 + q) * n + e] =
+          (co.gbr[pp] * gi[pp] + co.gbi[pp] * gr[pp]) + co.gci[pp] * ev;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// STAGED engine
+// updateD, sim3/include/myfdtd_func.f90:19-41, over rows [j0, j1]
+template <typename real>
+__global__ __launch_bounds__(256) void k_staged_d(FieldArgs<real> a, int j0, int j1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y + j0;
+  const int k = blockIdx.z + 1;
+  if (i > a.nx || j > j1) return;
+  const int64_t c = i + a.px * j + a.pxy * k;
+  const real hx = a.h[0][c], hy = a.h[1][c], hz = a.h[2][c];
+  const real curlhzy = hz - a.h[2][c - a.px];
+  const real curlhyz = -hy + a.h[1][c - a.pxy];
+  const real curlhxz = hx - a.h[0][c - a.pxy];
+  const real curlhzx = -hz + a.h[2][c - 1];
+  const real curlhyx = hy - a.h[1][c - 1];
+  const real curlhxy = -hx + a.h[0][c - a.px];
+  a.d[0][c] = a.d[0][c] + a.ku * (curlhzy + curlhyz);
+  a.d[1][c] = a.d[1][c] + a.ku * (curlhxz + curlhzx);
+  a.d[2][c] = a.d[2][c] + a.ku * (curlhyx + curlhxy);
+}
+
+// plane source, sim3/program.f90:396-406
+template <typename real>
+__global__ void k_staged_src(FieldArgs<real> a, int axis, int index, int comp, real val) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int v = blockIdx.y + 1;
+  int p[3];
+  const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+  const int n[3] = {a.nx, a.ny, a.nz};
+  if (u > n[a1]) return;
+  p[axis] = index;
+  p[a1] = u;
+  p[a2] = v;
+  const int64_t c = p[0] + a.px * p[1] + a.pxy * p[2];
+  a.d[comp][c] = a.d[comp][c] + val;
+}
+
+// ConstantDielectric%update_E for every cell/component whose last writer is
+// a dielectric (mod3/simpledielectric_mod.f90:181-183).
+template <typename real>
+__global__ __launch_bounds__(256) void k_staged_mat(FieldArgs<real> a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  if (i > a.nx || j > a.ny) return;
+  const int64_t c = i + a.px * j + a.pxy * k;
+  const MatEntry<real> m = a.tab[a.matid[c]];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    if (m.act[q] != ACT_DIEL) continue;
+    const real ev = a.d[q][c] * m.gax[q];
+    a.e[q][c] = ev;
+    put_e_ghost(a.e[q], ev, i, j, k, c, a);
+  }
+}
+
+// updateH, sim3/include/myfdtd_func.f90:45-64, over rows [j0, j1]
+template <typename real>
+__global__ __launch_bounds__(256) void k_staged_h(FieldArgs<real> a, int j0, int j1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y + j0;
+  const int k = blockIdx.z + 1;
+  if (i > a.nx || j > j1) return;
+  const int64_t c = i + a.px * j + a.pxy * k;
+  const real ex = a.e[0][c], ey = a.e[1][c], ez = a.e[2][c];
+  const real curleyz = a.e[1][c + a.pxy] - ey;
+  const real curlezy = -a.e[2][c + a.px] + ez;
+  const real curlezx = a.e[2][c + 1] - ez;
+  const real curlexz = -a.e[0][c + a.pxy] + ex;
+  const real curlexy = a.e[0][c + a.px] - ex;
+  const real curleyx = -a.e[1][c + 1] + ey;
+  const real hx = a.h[0][c] + a.ku * (curleyz + curlezy);
+  const real hy = a.h[1][c] + a.ku * (curlezx + curlexz);
+  const real hz = a.h[2][c] + a.ku * (curlexy + curleyx);
+  a.h[0][c] = hx;
+  a.h[1][c] = hy;
+  a.h[2][c] = hz;
+  put_h_ghost(a.h[0], hx, i, j, k, c, a);
+  put_h_ghost(a.h[1], hy, i, j, k, c, a);
+  put_h_ghost(a.h[2], hz, i, j, k, c, a);
+}
+
+// Reference y-slab PML, E: updateEs (mod3/BERpmly_mod.f90:117-159) driven by
+// split H only, then setboundayEya/Eyb (:200-225,255-278).  Slab 0 holds
+// rows 0..P+1, slab 1 rows jb-1..ny+1; split ghosts stay zero.
+template <typename real>
+__global__ __launch_bounds__(256) void k_ref_pml_e(FieldArgs<real> a, RefPmlArgs<real> p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int jr = blockIdx.y * blockDim.y + threadIdx.y;  // 0..2P-1
+  const int k = blockIdx.z + 1;
+  if (i > a.nx || jr >= 2 * p.P) return;
+  const int sl = jr < p.P ? 0 : 1;
+  const int j = sl == 0 ? jr + 1 : p.jb + (jr - p.P);
+  const int jl = sl == 0 ? j : j - (p.jb - 1);
+  real* const* S = p.s[sl];
+  const real* pr = p.prof[sl];
+  const int ps = p.ps;
+  const int64_t o = i + p.sx * jl + p.sxy * k;
+  const int64_t ojm = o - p.sx, okm = o - p.sxy, oim = o - 1;
+  const real curlHzy_p = S[S_HZX][o] + S[S_HZY][o];
+  const real curlHzy_m = -S[S_HZX][ojm] - S[S_HZY][ojm];
+  const real curlHyz_p = -S[S_HYZ][o] - S[S_HYX][o];
+  const real curlHyz_m = S[S_HYZ][okm] + S[S_HYX][okm];
+  const real curlHxz_p = S[S_HXY][o] + S[S_HXZ][o];
+  const real curlHxz_m = -S[S_HXY][okm] - S[S_HXZ][okm];
+  const real curlHzx_p = -S[S_HZX][o] - S[S_HZY][o];
+  const real curlHzx_m = S[S_HZX][oim] + S[S_HZY][oim];
+  const real curlHyx_p = S[S_HYZ][o] + S[S_HYX][o];
+  const real curlHyx_m = -S[S_HYZ][oim] - S[S_HYX][oim];
+  const real curlHxy_p = -S[S_HXY][o] - S[S_HXZ][o];
+  const real curlHxy_m = S[S_HXY][ojm] + S[S_HXZ][ojm];
+  const real f1x = pr[0 * ps + i], f2x = pr[1 * ps + i];
+  const real f1y = pr[2 * ps + j], f2y = pr[3 * ps + j];
+  const real f1z = pr[4 * ps + k], f2z = pr[5 * ps + k];
+  S[S_EYX][o] = f1x * S[S_EYX][o] + f2x * (curlHzx_p + curlHzx_m);
+  S[S_EZX][o] = f1x * S[S_EZX][o] + f2x * (curlHyx_p + curlHyx_m);
+  const real exy = f1y * S[S_EXY][o] + f2y * (curlHzy_p + curlHzy_m);
+  const real ezy = f1y * S[S_EZY][o] + f2y * (curlHxy_p + curlHxy_m);
+  S[S_EXY][o] = exy;
+  S[S_EZY][o] = ezy;
+  S[S_EXZ][o] = f1z * S[S_EXZ][o] + f2z * (curlHyz_p + curlHyz_m);
+  S[S_EYZ][o] = f1z * S[S_EYZ][o] + f2z * (curlHxz_p + curlHxz_m);
+  const int64_t c = i + a.px * j + a.pxy * k;
+  if (sl == 0 && j == p.P) {
+    S[S_EXY][o + p.sx] = a.e[0][c + a.px];
+    S[S_EZY][o + p.sx] = a.e[2][c + a.px];
+    a.e[0][c] = exy;
+    a.e[2][c] = ezy;
+  } else if (sl == 1 && j == p.jb) {
+    S[S_EXY][o - p.sx] = a.e[0][c - a.px];
+    S[S_EZY][o - p.sx] = a.e[2][c - a.px];
+    a.e[0][c] = exy;
+    a.e[2][c] = ezy;
+  }
+}
+
+// Reference y-slab PML, H: updateHs (:161-195) and setboundayHya/Hyb.
+template <typename real>
+__global__ __launch_bounds__(256) void k_ref_pml_h(FieldArgs<real> a, RefPmlArgs<real> p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int jr = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z + 1;
+  if (i > a.nx || jr >= 2 * p.P) return;
+  const int sl = jr < p.P ? 0 : 1;
+  const int j = sl == 0 ? jr + 1 : p.jb + (jr - p.P);
+  const int jl = sl == 0 ? j : j - (p.jb - 1);
+  real* const* S = p.s[sl];
+  const real* pr = p.prof[sl];
+  const int ps = p.ps;
+  const int64_t o = i + p.sx * jl + p.sxy * k;
+  const int64_t ojp = o + p.sx, okp = o + p.sxy, oip = o + 1;
+  const real curlEzy_p = -S[S_EZX][ojp] - S[S_EZY][ojp];
+  const real curlEzy_m = S[S_EZX][o] + S[S_EZY][o];
+  const real curlEyz_p = S[S_EYZ][okp] + S[S_EYX][okp];
+  const real curlEyz_m = -S[S_EYZ][o] - S[S_EYX][o];
+  const real curlExz_p = -S[S_EXY][okp] - S[S_EXZ][okp];
+  const real curlExz_m = S[S_EXY][o] + S[S_EXZ][o];
+  const real curlEzx_p = S[S_EZX][oip] + S[S_EZY][oip];
+  const real curlEzx_m = -S[S_EZX][o] - S[S_EZY][o];
+  const real curlEyx_p = -S[S_EYZ][oip] - S[S_EYX][oip];
+  const real curlEyx_m = S[S_EYZ][o] + S[S_EYX][o];
+  const real curlExy_p = S[S_EXY][ojp] + S[S_EXZ][ojp];
+  const real curlExy_m = -S[S_EXY][o] - S[S_EXZ][o];
+  const real g1x = pr[6 * ps + i], g2x = pr[7 * ps + i];
+  const real g1y = pr[8 * ps + j], g2y = pr[9 * ps + j];
+  const real g1z = pr[10 * ps + k], g2z = pr[11 * ps + k];
+  S[S_HYX][o] = g1x * S[S_HYX][o] + g2x * (curlEzx_m + curlEzx_p);
+  S[S_HZX][o] = g1x * S[S_HZX][o] + g2x * (curlEyx_m + curlEyx_p);
+  const real hxy = g1y * S[S_HXY][o] + g2y * (curlEzy_m + curlEzy_p);
+  const real hzy = g1y * S[S_HZY][o] + g2y * (curlExy_m + curlExy_p);
+  S[S_HXY][o] = hxy;
+  S[S_HZY][o] = hzy;
+  S[S_HXZ][o] = g1z * S[S_HXZ][o] + g2z * (curlEyz_m + curlEyz_p);
+  S[S_HYZ][o] = g1z * S[S_HYZ][o] + g2z * (curlExz_m + curlExz_p);
+  const int64_t c = i + a.px * j + a.pxy * k;
+  if (sl == 0 && j == p.P) {
+    S[S_HXY][o + p.sx] = a.h[0][c + a.px];
+    S[S_HZY][o + p.sx] = a.h[2][c + a.px];
+    a.h[0][c] = hxy;
+    a.h[2][c] = hzy;
+  } else if (sl == 1 && j == p.jb) {
+    S[S_HXY][o - p.sx] = a.h[0][c - a.px];
+    S[S_HZY][o - p.sx] = a.h[2][c - a.px];
+    a.h[0][c] = hxy;
+    a.h[2][c] = hzy;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Monitors: running DFT on a plane, sim3/program.f90:456-472.
+// acc layout: [slot][re|im][m][v][u] (FP64).
+template <typename real>
+__global__ __launch_bounds__(256) void k_dft(FieldArgs<real> a, int axis, int index, int n1, int n2, int nfreq,
+                                             const double* omega, double* acc, int comp0, int comp1, int comp2,
+                                             int comp3, int comp4, int comp5, double t, double dt) {
+  extern __shared__ double tabcs[];
+  for (int m = threadIdx.x; m < nfreq; m += blockDim.x) {
+    double sw, cw;
+    sincos(omega[m] * t, &sw, &cw);
+    tabcs[m] = cw;
+    tabcs[nfreq + m] = sw;
+  }
+  __syncthreads();
+  const int64_t np = (int64_t)n1 * n2;
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= np) return;
+  const int slot = blockIdx.y;
+  const int comps[6] = {comp0, comp1, comp2, comp3, comp4, comp5};
+  const int comp = comps[slot];
+  const int u = (int)(o % n1) + 1, v = (int)(o / n1) + 1;
+  int p[3];
+  const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+  p[axis] = index;
+  p[a1] = u;
+  p[a2] = v;
+  const int64_t c = p[0] + a.px * p[1] + a.pxy * p[2];
+  const real* F = comp < 3 ? a.e[comp] : a.h[comp - 3];
+  const double f = (double)F[c];
+  double* re = acc + (int64_t)slot * 2 * nfreq * np;
+  double* im = re + (int64_t)nfreq * np;
+  for (int m = 0; m < nfreq; ++m) {
+    const int64_t q = (int64_t)m * np + o;
+    re[q] = re[q] + tabcs[m] * f * dt;
+    im[q] = im[q] + tabcs[nfreq + m] * f * dt;
+  }
+}
+
+// point probes, sim3/program.f90:445-451
+template <typename real>
+__global__ void k_probe(FieldArgs<real> a, const int32_t* pos, int np, double* out, double tn) {
+  const int p = threadIdx.x;
+  if (p >= np) return;
+  const int64_t c = pos[3 * p] + a.px * pos[3 * p + 1] + a.pxy * pos[3 * p + 2];
+  double* r = out + 7 * p;
+  r[0] = tn;
+  for (int q = 0; q < 3; ++q) {
+    r[1 + q] = (double)a.e[q][c];
+    r[4 + q] = (double)a.h[q][c];
+  }
+}
+
+// Flux of a DFT plane, one block per frequency, fixed-order tree reduction.
+__global__ __launch_bounds__(256) void k_flux(const double* acc, int64_t np, int nfreq, int shA, int seA, int shB,
+                                              int seB, double ds, double* S) {
+  __shared__ double red[256];
+  const int m = blockIdx.x;
+  auto re = [&](int slot) { return acc + (int64_t)slot * 2 * nfreq * np + (int64_t)m * np; };
+  auto im = [&](int slot) { return acc + (int64_t)slot * 2 * nfreq * np + (int64_t)nfreq * np + (int64_t)m * np; };
+  const double *rhA = re(shA), *ihA = im(shA), *reA = re(seA), *ieA = im(seA);
+  const double *rhB = re(shB), *ihB = im(shB), *reB = re(seB), *ieB = im(seB);
+  double acc_l = 0.0;
+  for (int64_t o = threadIdx.x; o < np; o += blockDim.x)
+    acc_l += ((rhA[o] * reA[o]) + (ihA[o] * ieA[o]) - (rhB[o] * reB[o]) - (ihB[o] * ieB[o])) * ds;
+  red[threadIdx.x] = acc_l;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) S[m] = red[0];
+}
+
+// Energy sums over interior cells; one partial pair per block.
+template <typename real>
+__global__ __launch_bounds__(256) void k_energy(FieldArgs<real> a, double* part) {
+  __shared__ double r1[256], r2[256];
+  const int64_t n = (int64_t)a.nx * a.ny * a.nz;
+  double s1 = 0, s2 = 0;
+  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < n; o += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(o % a.nx) + 1;
+    const int64_t r = o / a.nx;
+    const int j = (int)(r % a.ny) + 1, k = (int)(r / a.ny) + 1;
+    const int64_t c = i + a.px * j + a.pxy * k;
+    const double ex = a.e[0][c], ey = a.e[1][c], ez = a.e[2][c];
+    const double hx = a.h[0][c], hy = a.h[1][c], hz = a.h[2][c];
+    s1 += (double)a.tab[a.matid[c]].eps * (ex * ex + ey * ey + ez * ez);
+    s2 += hx * hx + hy * hy + hz * hz;
+  }
+  r1[threadIdx.x] = s1;
+  r2[threadIdx.x] = s2;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      r1[threadIdx.x] += r1[threadIdx.x + w];
+      r2[threadIdx.x] += r2[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = r1[0];
+    part[2 * blockIdx.x + 1] = r2[0];
+  }
+}
+
+// periodic ghost refresh of one component after a host write
+template <typename real>
+__global__ void k_ghosts(FieldArgs<real> a, real* f, int axis) {
+  const int n[3] = {a.nx, a.ny, a.nz};
+  const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = blockIdx.y;
+  if (u > n[a1] + 1) return;
+  int p[3];
+  p[a1] = u;
+  p[a2] = v;
+  const int64_t st[3] = {1, a.px, a.pxy};
+  p[axis] = 0;
+  const int64_t c0 = p[0] + a.px * p[1] + a.pxy * p[2];
+  f[c0 + st[axis] * (n[axis] + 1)] = f[c0 + st[axis] * 1];
+  f[c0] = f[c0 + st[axis] * n[axis]];
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers
+namespace {
+
+template <typename real>
+FieldArgs<real> field_args(Sim* s) {
+  FieldArgs<real> a{};
+  for (int q = 0; q < 3; ++q) {
+    a.e[q] = (real*)s->E[q];
+    a.h[q] = (real*)s->H[q];
+    a.d[q] = (real*)s->D[q];
+  }
+  a.matid = s->d_matid;
+  a.tab = (const MatEntry<real>*)s->d_mattab;
+  a.nx = s->nx;
+  a.ny = s->ny;
+  a.nz = s->nz;
+  for (int q = 0; q < 3; ++q) a.per[q] = s->g.periodic[q] ? 1 : 0;
+  a.px = s->px;
+  a.pxy = s->pxy;
+  a.ku = (real)s->ku;
+  return a;
+}
+
+template <typename real>
+PmlArgs<real> pml_args(Sim* s) {
+  PmlArgs<real> p{};
+  for (int q = 0; q < 3; ++q) p.P[q] = s->P[q];
+  for (int q = 0; q < S_N; ++q) p.s[q] = s->d_split ? (real*)s->d_split + (int64_t)q * s->npml_cells : nullptr;
+  if (s->d_prof) {
+    const real* base = (const real*)s->d_prof;
+    int64_t off = 0;
+    const int n[3] = {s->nx, s->ny, s->nz};
+    for (int ax = 0; ax < 3; ++ax) {
+      p.f1[ax] = base + off; off += n[ax] + 2;
+      p.f2[ax] = base + off; off += n[ax] + 2;
+      p.g1[ax] = base + off; off += n[ax] + 2;
+      p.g2[ax] = base + off; off += n[ax] + 2;
+    }
+  }
+  return p;
+}
+
+template <typename real>
+RefPmlArgs<real> ref_pml_args(Sim* s) {
+  RefPmlArgs<real> p{};
+  p.P = s->pml.npml[1];
+  p.jb = s->ny - p.P + 1;
+  p.sx = s->nx + 2;
+  p.sxy = (int64_t)(s->nx + 2) * (p.P + 2);
+  for (int sl = 0; sl < 2; ++sl) {
+    for (int q = 0; q < S_N; ++q) p.s[sl][q] = (real*)s->d_slab + ((int64_t)sl * S_N + q) * s->slab_cells;
+    p.prof[sl] = (const real*)s->d_ref_prof + (int64_t)sl * 12 * s->ref_prof_stride;
+  }
+  p.ps = s->ref_prof_stride;
+  return p;
+}
+
+struct Timer {
+  Sim* s;
+  int kid;
+  hipEvent_t a = nullptr, b = nullptr;
+  Timer(Sim* s_, int kid_) : s(s_), kid(kid_) {
+    if (!s->profiling) return;
+    a = get();
+    if (a && hipEventRecord(a, s->stream) != hipSuccess) a = nullptr;
+  }
+  hipEvent_t get() {
+    if (!s->event_pool.empty()) {
+      hipEvent_t e = s->event_pool.back();
+      s->event_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+  ~Timer() {
+    if (!s->profiling || !a) return;
+    b = get();
+    if (!b || hipEventRecord(b, s->stream) != hipSuccess) return;
+    s->pending.push_back({kid, {a, b}});
+  }
+};
+
+inline dim3 cell_grid(int nx, int nrows, int nz, dim3 blk) {
+  return dim3((unsigned)((nx + blk.x - 1) / blk.x), (unsigned)((nrows + blk.y - 1) / blk.y), (unsigned)nz);
+}
+
+template <typename real>
+SrcArgs<real> source_args(Sim* s, int n, int* nsrc) {
+  SrcArgs<real> sa{};
+  sa.n = n;
+  const double t = (double)(n - 1) * s->dt;
+  int q = 0;
+  for (const auto& src : s->srcs) {
+    const fdtd_pulse& p = src.p;
+    // sim3/program.f90:400
+    const double g = p.amplitude * (std::exp(-0.5 * ((t - p.t0) / p.st) * ((t - p.t0) / p.st)) * std::sin(p.w0 * t));
+    sa.axis[q] = src.axis;
+    sa.index[q] = src.index;
+    sa.comp[q] = src.comp;
+    sa.val[q] = (real)g;
+    ++q;
+  }
+  *nsrc = q;
+  return sa;
+}
+
+template <typename real>
+int step_impl(Sim* s) {
+  const int n = s->n_done + 1;
+  const double t = (double)(n - 1) * s->dt;
+  FieldArgs<real> a = field_args<real>(s);
+  int nsrc = 0;
+  SrcArgs<real> src = source_args<real>(s, n, &nsrc);
+  const dim3 blk(64, 4, 1);
+  hipStream_t st = s->stream;
+  auto ade_coef = [&](const AdeGroup& g) {
+    AdeCoef<real> c{};
+    c.ncp = g.coef.ncp;
+    c.raman = g.coef.raman;
+    c.ku = (real)s->ku;
+    c.ha = (real)g.coef.ha;
+    c.hb = (real)g.coef.hb;
+    c.gar = (real)g.coef.gar;
+    c.gai = (real)g.coef.gai;
+    for (int p = 0; p < 4; ++p) {
+      c.gbr[p] = (real)g.coef.gbr[p];
+      c.gbi[p] = (real)g.coef.gbi[p];
+      c.gcr[p] = (real)g.coef.gcr[p];
+      c.gci[p] = (real)g.coef.gci[p];
+    }
+    c.r0 = (real)g.coef.r0;
+    c.r1 = (real)g.coef.r1;
+    c.r2 = (real)g.coef.r2;
+    return c;
+  };
+  if (s->engine == FDTD_ENGINE_FUSED) {
+    PmlArgs<real> p = pml_args<real>(s);
+    {
+      Timer tm(s, FDTD_K_E);
+      hipLaunchKernelGGL(k_fused_e<real>, cell_grid(s->nx, s->ny, s->nz, blk), blk, 0, st, a, p, src, nsrc);
+    }
+    for (auto& g : s->groups) {
+      if (!g.ncells) continue;
+      Timer tm(s, FDTD_K_ADE);
+      hipLaunchKernelGGL((k_ade<real, true>), dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, a,
+                         ade_coef(g), g.d_cells, g.d_mask, (real*)g.d_state, g.ncells, src, nsrc);
+    }
+    {
+      Timer tm(s, FDTD_K_H);
+      hipLaunchKernelGGL(k_fused_h<real>, cell_grid(s->nx, s->ny, s->nz, blk), blk, 0, st, a, p);
+    }
+  } else {
+    const bool refy = s->pml_mode == FDTD_PML_REF_Y;
+    const int P = refy ? s->pml.npml[1] : 0;
+    const int j0 = 1 + P, j1 = s->ny - P;
+    {
+      Timer tm(s, FDTD_K_E);
+      hipLaunchKernelGGL(k_staged_d<real>, cell_grid(s->nx, j1 - j0 + 1, s->nz, blk), blk, 0, st, a, j0, j1);
+    }
+    for (int q = 0; q < nsrc; ++q) {
+      const int ax = src.axis[q];
+      const int a1 = ax == 0 ? 1 : 0, a2 = ax == 2 ? 1 : 2;
+      const int nn[3] = {s->nx, s->ny, s->nz};
+      Timer tm(s, FDTD_K_MAT);
+      hipLaunchKernelGGL(k_staged_src<real>, dim3((unsigned)((nn[a1] + 63) / 64), (unsigned)nn[a2]), dim3(64), 0, st,
+                         a, ax, src.index[q], src.comp[q], src.val[q]);
+    }
+    {
+      Timer tm(s, FDTD_K_MAT);
+      hipLaunchKernelGGL(k_staged_mat<real>, cell_grid(s->nx, s->ny, s->nz, blk), blk, 0, st, a);
+    }
+    for (auto& g : s->groups) {
+      if (!g.ncells) continue;
+      Timer tm(s, FDTD_K_ADE);
+      hipLaunchKernelGGL((k_ade<real, false>), dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, a,
+                         ade_coef(g), g.d_cells, g.d_mask, (real*)g.d_state, g.ncells, src, 0);
+    }
+    RefPmlArgs<real> rp{};
+    if (refy) {
+      rp = ref_pml_args<real>(s);
+      Timer tm(s, FDTD_K_PML_E);
+      hipLaunchKernelGGL(k_ref_pml_e<real>, cell_grid(s->nx, 2 * P, s->nz, blk), blk, 0, st, a, rp);
+    }
+    {
+      Timer tm(s, FDTD_K_H);
+      hipLaunchKernelGGL(k_staged_h<real>, cell_grid(s->nx, j1 - j0 + 1, s->nz, blk), blk, 0, st, a, j0, j1);
+    }
+    if (refy) {
+      Timer tm(s, FDTD_K_PML_H);
+      hipLaunchKernelGGL(k_ref_pml_h<real>, cell_grid(s->nx, 2 * P, s->nz, blk), blk, 0, st, a, rp);
+    }
+  }
+  // monitors
+  for (auto& d : s->dfts) {
+    if (n < d.n_begin) continue;
+    if (d.n_end > 0 && n > d.n_end) continue;
+    int comps[6] = {0, 0, 0, 0, 0, 0};
+    int ns = 0;
+    for (int c = 0; c < 6; ++c)
+      if (d.mask & (1 << c)) comps[ns++] = c;
+    const int64_t np = (int64_t)d.n1 * d.n2;
+    Timer tm(s, FDTD_K_MON);
+    hipLaunchKernelGGL(k_dft<real>, dim3((unsigned)((np + 255) / 256), (unsigned)ns), dim3(256),
+                       (size_t)(2 * d.nfreq) * sizeof(double), st, a, d.axis, d.index, d.n1, d.n2, d.nfreq, d.d_omega,
+                       d.d_acc, comps[0], comps[1], comps[2], comps[3], comps[4], comps[5], t, s->dt);
+  }
+  if (!s->probes.empty()) {
+    Timer tm(s, FDTD_K_MON);
+    const int np = (int)s->probes.size();
+    double* out = s->d_probe + (int64_t)7 * np * (n - 1 - s->call_base);
+    hipLaunchKernelGGL(k_probe<real>, dim3(1), dim3(64), 0, st, a, (const int32_t*)s->d_probe_pos, np, out,
+                       (double)n * s->dt);
+  }
+  return 0;
+}
+
+}  // namespace
+
+int launch_step(Sim* s) {
+  int rc = s->prec == FDTD_FP32 ? step_impl<float>(s) : step_impl<double>(s);
+  if (rc) return rc;
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_check(e, "kernel launch");
+  return 0;
+}
+
+int launch_flux(Sim* s, int id, double* S) {
+  DftPlane& d = s->dfts[id];
+  int eA, eB, hA, hB;
+  if (d.axis == 1) { hA = FDTD_HX; eA = FDTD_EZ; hB = FDTD_HZ; eB = FDTD_EX; }
+  else if (d.axis == 2) { hA = FDTD_HY; eA = FDTD_EX; hB = FDTD_HX; eB = FDTD_EY; }
+  else { hA = FDTD_HZ; eA = FDTD_EY; hB = FDTD_HY; eB = FDTD_EZ; }
+  int need[4] = {hA, eA, hB, eB};
+  for (int q = 0; q < 4; ++q)
+    if (!(d.mask & (1 << need[q]))) return set_error(FDTD_ERR_ARG, "flux needs the four tangential components");
+  double* dS = scratch(s, sizeof(double) * (size_t)d.nfreq);
+  if (!dS) return set_error(FDTD_ERR_NOMEM, "flux buffer");
+  const int64_t np = (int64_t)d.n1 * d.n2;
+  hipLaunchKernelGGL(k_flux, dim3((unsigned)d.nfreq), dim3(256), 0, s->stream, d.d_acc, np, d.nfreq, d.slot[hA],
+                     d.slot[eA], d.slot[hB], d.slot[eB], s->ds, dS);
+  FDTD_HIP(hipGetLastError());
+  FDTD_HIP(hipMemcpyAsync(S, dS, sizeof(double) * (size_t)d.nfreq, hipMemcpyDeviceToHost, s->stream));
+  FDTD_HIP(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int launch_energy(Sim* s, double* we, double* wh) {
+  const int nb = 1024;
+  double* part = scratch(s, sizeof(double) * 2 * nb);
+  if (!part) return set_error(FDTD_ERR_NOMEM, "energy buffer");
+  if (s->prec == FDTD_FP32)
+    hipLaunchKernelGGL(k_energy<float>, dim3(nb), dim3(256), 0, s->stream, field_args<float>(s), part);
+  else
+    hipLaunchKernelGGL(k_energy<double>, dim3(nb), dim3(256), 0, s->stream, field_args<double>(s), part);
+  FDTD_HIP(hipGetLastError());
+  std::vector<double> h(2 * nb);
+  FDTD_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, s->stream));
+  FDTD_HIP(hipStreamSynchronize(s->stream));
+  double a = 0, b = 0;
+  for (int q = 0; q < nb; ++q) {
+    a += h[2 * q];
+    b += h[2 * q + 1];
+  }
+  *we = a;
+  *wh = b;
+  return 0;
+}
+
+int launch_set_ghosts(Sim* s, int comp) {
+  const int n[3] = {s->nx, s->ny, s->nz};
+  for (int ax = 0; ax < 3; ++ax) {
+    if (!s->g.periodic[ax]) continue;
+    const int a1 = ax == 0 ? 1 : 0, a2 = ax == 2 ? 1 : 2;
+    dim3 grid((unsigned)((n[a1] + 2 + 63) / 64), (unsigned)(n[a2] + 2));
+    void* f = comp < 3 ? s->E[comp] : (comp < 6 ? s->H[comp - 3] : s->D[comp - 6]);
+    if (s->prec == FDTD_FP32)
+      hipLaunchKernelGGL(k_ghosts<float>, grid, dim3(64), 0, s->stream, field_args<float>(s), (float*)f, ax);
+    else
+      hipLaunchKernelGGL(k_ghosts<double>, grid, dim3(64), 0, s->stream, field_args<double>(s), (double*)f, ax);
+  }
+  FDTD_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace fdtd

@@ -1,0 +1,168 @@
+"""ctypes mirror of include/fdtd.h (structs, enums, prototypes).
+
+The same prototypes are bound for any shared library that exports the ABI
+under a symbol prefix (``fdtd_`` for the HIP library).
+"""
+import ctypes as C
+
+# status codes
+FDTD_OK = 0
+FDTD_ERR_ARG = -1
+FDTD_ERR_STATE = -2
+FDTD_ERR_NOMEM = -3
+FDTD_ERR_HIP = -4
+FDTD_ERR_UNSUPPORTED = -5
+
+FP64, FP32 = 0, 1
+PML_NONE, PML_REF_Y, PML_ALL_FACES = 0, 1, 2
+EX, EY, EZ, HX, HY, HZ, DX, DY, DZ = range(9)
+COMP_NAMES = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz", "Dx", "Dy", "Dz")
+REGION_BOX, REGION_SPHERE, REGION_CYLINDER = 0, 1, 2
+GRAPHENE_CP4, GRAPHENE_KUBO = 0, 1
+ENGINE_AUTO, ENGINE_STAGED, ENGINE_FUSED = 0, 1, 2
+K_E, K_H, K_ADE, K_PML_E, K_PML_H, K_MAT, K_MON, K_COUNT = range(8)
+KERNEL_NAMES = ("e_update", "h_update", "ade", "pml_e", "pml_h", "material", "monitor")
+
+
+class Grid(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("precision", C.c_int32),
+        ("periodic", C.c_int32 * 3),
+        ("engine", C.c_int32),
+        ("ds", C.c_double),
+        ("courant", C.c_double),
+    ]
+
+
+class Pml(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int32),
+        ("npml", C.c_int32 * 3),
+        ("order", C.c_double),
+        ("R0", C.c_double),
+        ("sigmam", C.c_double),
+        ("eps_low", C.c_double),
+        ("eps_high", C.c_double),
+    ]
+
+
+class Region(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("axis", C.c_int32),
+        ("lo", C.c_int32 * 3),
+        ("hi", C.c_int32 * 3),
+        ("center", C.c_double * 3),
+        ("radius", C.c_double),
+        ("half_length", C.c_double),
+    ]
+
+
+class Drude2CP(C.Structure):
+    _fields_ = [
+        ("eps_inf", C.c_double),
+        ("gamma", C.c_double),
+        ("omega_d", C.c_double),
+        ("A", C.c_double * 2),
+        ("phi", C.c_double * 2),
+        ("Omega", C.c_double * 2),
+        ("Gamma", C.c_double * 2),
+    ]
+
+
+class Graphene(C.Structure):
+    _fields_ = [
+        ("model", C.c_int32),
+        ("normal_axis", C.c_int32),
+        ("ref_init_order", C.c_int32),
+        ("reserved", C.c_int32),
+        ("thickness", C.c_double),
+        ("eps_inf", C.c_double),
+        ("gamma", C.c_double),
+        ("omega_d", C.c_double),
+        ("A", C.c_double * 4),
+        ("phi", C.c_double * 4),
+        ("Omega", C.c_double * 4),
+        ("Gamma", C.c_double * 4),
+        ("fermi_ev", C.c_double),
+        ("tau", C.c_double),
+        ("temperature", C.c_double),
+        ("wNL", C.c_double),
+        ("dnuNL", C.c_double),
+        ("xi3", C.c_double),
+    ]
+
+
+class Pulse(C.Structure):
+    _fields_ = [("t0", C.c_double), ("st", C.c_double), ("w0", C.c_double),
+                ("amplitude", C.c_double)]
+
+
+_P = C.c_void_p
+_I = C.c_int
+_D = C.c_double
+_PD = C.POINTER(C.c_double)
+
+PROTOTYPES = {
+    "drude2cp_gold": (None, [C.POINTER(Drude2CP)]),
+    "graphene_ref": (None, [C.POINTER(Graphene)]),
+    "create": (_I, [C.POINTER(_P), C.POINTER(Grid)]),
+    "destroy": (_I, [_P]),
+    "last_error": (C.c_char_p, []),
+    "set_pml": (_I, [_P, C.POINTER(Pml)]),
+    "add_dielectric": (_I, [_P, C.POINTER(Region), _D]),
+    "add_drude2cp": (_I, [_P, C.POINTER(Region), C.POINTER(Drude2CP)]),
+    "add_graphene": (_I, [_P, C.POINTER(Region), C.POINTER(Graphene)]),
+    "add_plane_source": (_I, [_P, _I, _I, _I, C.POINTER(Pulse)]),
+    "add_dft_plane": (_I, [_P, _I, _I, _I, _PD, _I, _I, _I, C.POINTER(_I)]),
+    "add_probe": (_I, [_P, _I, _I, _I, C.POINTER(_I)]),
+    "commit": (_I, [_P]),
+    "step": (_I, [_P, _I]),
+    "sync": (_I, [_P]),
+    "get_step": (_I, [_P, C.POINTER(_I)]),
+    "get_dt": (_I, [_P, _PD]),
+    "get_field": (_I, [_P, _I, _PD]),
+    "set_field": (_I, [_P, _I, _PD]),
+    "get_dft": (_I, [_P, _I, _I, _PD, _PD]),
+    "get_flux": (_I, [_P, _I, _PD]),
+    "get_probe_count": (_I, [_P, _I, C.POINTER(_I)]),
+    "get_probe": (_I, [_P, _I, _PD]),
+    "energy": (_I, [_P, _PD, _PD]),
+}
+
+# entry points only the HIP library has
+GPU_PROTOTYPES = {
+    "memory_bytes": (_I, [_P, C.POINTER(C.c_size_t)]),
+    "set_profiling": (_I, [_P, _I]),
+    "kernel_stats": (_I, [_P, _I, _PD, C.POINTER(C.c_int64)]),
+    "algorithmic_bytes": (_I, [_P, _I, _PD]),
+    "cell_counts": (_I, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+}
+
+
+class FdtdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fdtd error {code}: {msg}")
+        self.code = code
+
+
+class Binding:
+    """Prototypes of one shared library exporting the ABI under `prefix`."""
+
+    def __init__(self, dll, prefix, extra=True):
+        self.dll = dll
+        self.prefix = prefix
+        table = dict(PROTOTYPES)
+        if extra:
+            table.update(GPU_PROTOTYPES)
+        for name, (res, args) in table.items():
+            fn = getattr(dll, prefix + name)
+            fn.restype = res
+            fn.argtypes = args
+            setattr(self, name, fn)
+
+    def check(self, rc):
+        if rc != 0:
+            raise FdtdError(rc, (self.last_error() or b"").decode(errors="replace"))
+        return rc

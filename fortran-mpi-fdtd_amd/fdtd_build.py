@@ -1,0 +1,61 @@
+"""Build steps for the gfx950 library, the Fortran host and the test oracles.
+
+``build_library`` compiles csrc/*.hip with hipcc for gfx950 into
+libfdtd_amd.so next to this file (in-tree, so it travels with the tree).
+``build_fortran_host`` compiles the iso_c_binding interface module and the
+Fortran90 host program against it.  ``build_oracles`` runs oracle/Makefile.
+"""
+import os
+import shutil
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "libfdtd_amd.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("fdtd_api.hip", "fdtd_kernels.hip")]
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # packed FP32 instructions stay off on this target
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+
+
+def _run(cmd, cwd=None):
+    res = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    return res
+
+
+def build_library():
+    tmp = LIB + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-o", tmp, *SOURCES])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_fortran_host():
+    """Fortran90 host (fortran/host3d.f90) over the C ABI via iso_c_binding."""
+    fdir = os.path.join(HERE, "fortran")
+    out = os.path.join(fdir, "host3d")
+    mod = os.path.join(fdir, "_mod")
+    os.makedirs(mod, exist_ok=True)
+    _run([FLANG, "-O2", "-c", os.path.join(fdir, "fdtd_c.f90"), "-module-dir", mod, "-o",
+          os.path.join(mod, "fdtd_c.o")])
+    _run([FLANG, "-O2", "-I", mod, os.path.join(fdir, "host3d.f90"), os.path.join(mod, "fdtd_c.o"),
+          "-L", HERE, "-lfdtd_amd", f"-Wl,-rpath,{HERE}", "-o", out])
+    return out
+
+
+def build_oracles(with_reference=True):
+    odir = os.path.join(REPO, "oracle")
+    _run(["make", "-C", odir, "all"])
+    if with_reference and os.path.isdir("/root/reference") and shutil.which(FLANG):
+        _run(["make", "-C", odir, "ref"])
+
+
+if __name__ == "__main__":
+    print(build_library())

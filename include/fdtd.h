@@ -1,0 +1,274 @@
+/*
+ * fdtd.h -- C ABI of the MI355X (gfx950) 3D Yee FDTD time-stepper.
+ *
+ * The library replaces the 3D hot path of lsmayoral/Fortran-MPI-FDTD:
+ * the Yee leapfrog, the split-field PML, the Drude+2CP gold ADE, the graphene
+ * CP/Raman ADE, soft plane sources, running-DFT monitors and flux integrals.
+ * Every entry point below names the reference routine(s) it stands in for
+ * (paths relative to the reference checkout):
+ *
+ *   sim3/  = FDTDsimulations/3D/air-AuAntennasSiO2-Si/
+ *   mod3/  = modules/src/3D/
+ *   misc/  = modules/src/misc/
+ *
+ * Conventions shared with the reference
+ *   - Grid indices are 1-based and inclusive, global (the reference's
+ *     xinit..zend, mod3/material_mod.f90:8).  Interior cells are 1..n per axis.
+ *   - Field arrays crossing the ABI are Fortran order (x fastest), interior
+ *     cells only, FP64, caller-owned host buffers.
+ *   - Units are the reference's normalised ones: D and E share one scale,
+ *     H is scaled by the vacuum impedance, Ku = c0*dt/ds multiplies every curl
+ *     (sim3/include/myfdtd_func.f90:32-34,58-60).
+ *   - Materials are applied in call order; where regions overlap, the last
+ *     one added writes E (sim3/program.f90:411-414).
+ *
+ * All functions return 0 on success or a negative FDTD_ERR_* code; the text
+ * of the most recent error on the calling thread is fdtd_last_error().
+ * A handle owns all its device memory and one HIP stream; it is not
+ * thread-safe, but separate handles may be driven from separate threads.
+ */
+#ifndef FDTD_AMD_FDTD_H
+#define FDTD_AMD_FDTD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+enum {
+  FDTD_OK = 0,
+  FDTD_ERR_ARG = -1,         /* invalid argument                           */
+  FDTD_ERR_STATE = -2,       /* call not valid in the handle's state       */
+  FDTD_ERR_NOMEM = -3,       /* device or host allocation failed           */
+  FDTD_ERR_HIP = -4,         /* HIP runtime error                          */
+  FDTD_ERR_UNSUPPORTED = -5  /* combination not supported by this path     */
+};
+
+/* ---- enumerations ---------------------------------------------------- */
+enum { FDTD_FP64 = 0, FDTD_FP32 = 1 };
+
+/* PML modes.
+ * REF_Y     : the reference's Berenger split-field PML on the two y faces,
+ *             with x/z grading inside the y slabs and interface coupling by
+ *             copy (mod3/BERpmly_mod.f90), multi-rank semantics (the y-low
+ *             slab's default f2 uses eps_low, the y-high slab's eps_high).
+ * ALL_FACES : the same split-field formulation and profiles on every face
+ *             whose npml > 0; main fields hold the split sums, splits are
+ *             driven by the main (total) fields of the other kind.          */
+enum { FDTD_PML_NONE = 0, FDTD_PML_REF_Y = 1, FDTD_PML_ALL_FACES = 2 };
+
+/* Field components for get/set. */
+enum {
+  FDTD_EX = 0, FDTD_EY = 1, FDTD_EZ = 2,
+  FDTD_HX = 3, FDTD_HY = 4, FDTD_HZ = 5,
+  FDTD_DX = 6, FDTD_DY = 7, FDTD_DZ = 8
+};
+
+/* Region shapes used to place materials. */
+enum { FDTD_REGION_BOX = 0, FDTD_REGION_SPHERE = 1, FDTD_REGION_CYLINDER = 2 };
+
+/* Graphene models. */
+enum {
+  FDTD_GRAPHENE_CP4 = 0,  /* reference: Drude + 4 critical points, thickness-scaled */
+  FDTD_GRAPHENE_KUBO = 1  /* Drude weight from E_F, tau, T (Kubo intraband)        */
+};
+
+/* Stepping engines (fdtd_grid.engine). */
+enum {
+  FDTD_ENGINE_AUTO = 0,   /* STAGED for REF_Y, FUSED otherwise                  */
+  FDTD_ENGINE_STAGED = 1, /* one kernel per reference stage, D kept everywhere  */
+  FDTD_ENGINE_FUSED = 2   /* fused E / H kernels, D kept only in ADE cells      */
+};
+
+/* Kernel ids for fdtd_kernel_stats. */
+enum {
+  FDTD_K_E = 0,       /* E half step (fused) or updateD (staged)   */
+  FDTD_K_H = 1,       /* H half step (fused) or updateH (staged)   */
+  FDTD_K_ADE = 2,     /* dispersive cells (gold, graphene)         */
+  FDTD_K_PML_E = 3,   /* staged split-field E                       */
+  FDTD_K_PML_H = 4,   /* staged split-field H                       */
+  FDTD_K_MAT = 5,     /* staged dielectric E = D*gax + source       */
+  FDTD_K_MON = 6,     /* DFT monitors and probes                    */
+  FDTD_K_COUNT = 7
+};
+
+typedef struct fdtd_sim fdtd_sim;
+
+/* Grid: sim3/program.f90:44-72 (ds, Ku, nx/ny/nz, PBC_x/y/z). */
+typedef struct {
+  int32_t nx, ny, nz;     /* interior cells                                   */
+  int32_t precision;      /* FDTD_FP64 | FDTD_FP32                            */
+  int32_t periodic[3];    /* PBC_x, PBC_y, PBC_z (sim3/program.f90:12)        */
+  int32_t engine;         /* FDTD_ENGINE_*                                    */
+  double ds;              /* cell size [m]                                    */
+  double courant;         /* Ku; dt = Ku*ds/c0 (sim3/program.f90:48-50)      */
+} fdtd_grid;
+
+/* PML: sim3/program.f90:97-102 and pmly%initialize (mod3/BERpmly_mod.f90:485). */
+typedef struct {
+  int32_t mode;           /* FDTD_PML_*                                        */
+  int32_t npml[3];        /* cells per axis; REF_Y: x/z grading depth in slabs */
+  double order;           /* polynomial grading order m                        */
+  double R0;              /* design reflection; used when sigmam <= 0          */
+  double sigmam;          /* > 0: peak conductivity factor used as given       */
+  double eps_low;         /* REF_Y: relative eps at the y-low slab             */
+  double eps_high;        /* REF_Y: relative eps at the y-high slab            */
+} fdtd_pml;
+
+/* Region: the box semantics of material_mod (1-based inclusive global
+ * indices, clipped to the grid) plus staircased spheres and cylinders in
+ * cell-index coordinates (cell (i,j,k) has its centre at (i,j,k)).          */
+typedef struct {
+  int32_t kind;           /* FDTD_REGION_*                                     */
+  int32_t axis;           /* CYLINDER axis 0..2                                */
+  int32_t lo[3];          /* BOX lower corner (also clips SPHERE/CYLINDER)     */
+  int32_t hi[3];          /* BOX upper corner                                  */
+  double center[3];       /* SPHERE/CYLINDER centre                            */
+  double radius;          /* SPHERE/CYLINDER radius [cells]                    */
+  double half_length;     /* CYLINDER half length along axis [cells]           */
+} fdtd_region;
+
+/* Drude + 2 critical points (mod3/gold_mod.f90:28-34). */
+typedef struct {
+  double eps_inf;         /* epinf                                             */
+  double gamma;           /* dnur  [1/s]                                       */
+  double omega_d;         /* wD    [rad/s]                                     */
+  double A[2];            /* ap                                                */
+  double phi[2];          /* phip  [rad]                                       */
+  double Omega[2];        /* omp   [rad/s]                                     */
+  double Gamma[2];        /* rp    [1/s]                                       */
+} fdtd_drude2cp;
+
+/* Graphene sheet (mod3/graphene_mod.f90:9-55). */
+typedef struct {
+  int32_t model;          /* FDTD_GRAPHENE_*                                   */
+  int32_t normal_axis;    /* sheet normal; that E component is not written     */
+  int32_t ref_init_order; /* CP4: 1 keeps the reference's init order, in which
+                             the CP coefficients are read before they are set
+                             (mod3/graphene_mod.f90:222-231): gb = gc = 0      */
+  int32_t reserved;
+  double thickness;       /* CP4: physical thickness scaled into ds (3.4e-10)  */
+  double eps_inf;         /* graphene_epinf                                    */
+  double gamma;           /* graphene_dnur [1/s] (CP4)                         */
+  double omega_d;         /* graphene_wD [rad/s] (CP4)                         */
+  double A[4], phi[4], Omega[4], Gamma[4];   /* CP4 critical points          */
+  double fermi_ev;        /* KUBO: Fermi level [eV]                            */
+  double tau;             /* KUBO: relaxation time [s]                         */
+  double temperature;     /* KUBO: [K]                                         */
+  double wNL;             /* Raman resonance [rad/s]                           */
+  double dnuNL;           /* Raman linewidth [1/s]                             */
+  double xi3;             /* Raman strength                                    */
+} fdtd_graphene;
+
+/* Gaussian-modulated soft source, sim3/program.f90:107-111,400:
+ * g(t) = amplitude * exp(-0.5*((t-t0)/st)^2) * sin(w0*t), t = (n-1)*dt.     */
+typedef struct {
+  double t0, st, w0, amplitude;
+} fdtd_pulse;
+
+/* ---- parameter presets ------------------------------------------------ */
+/* Gold of mod3/gold_mod.f90:28-34 (Vial & Laroche, Appl Phys A 103:849). */
+void fdtd_drude2cp_gold(fdtd_drude2cp* p);
+/* Graphene CP4 + Raman defaults of mod3/graphene_mod.f90:33-41 (the Raman
+ * triple wNL, dnuNL, xi3 is left zero: the reference driver sets it).      */
+void fdtd_graphene_ref(fdtd_graphene* p);
+
+/* ---- lifetime ---------------------------------------------------------- */
+/* Replaces the static field arrays and grid parameters of
+ * sim3/program.f90:44-77,209-214 and the rank layout of
+ * sim3/include/mympi_func.f90:137-214 (one domain, no ranks).              */
+int fdtd_create(fdtd_sim** out, const fdtd_grid* grid);
+int fdtd_destroy(fdtd_sim* sim);
+const char* fdtd_last_error(void);
+
+/* mypmly%initialize, mod3/BERpmly_mod.f90:485-508 (coefficients of
+ * :612-874, sigmam of sim3/program.f90:102).                               */
+int fdtd_set_pml(fdtd_sim* sim, const fdtd_pml* pml);
+
+/* ConstantDielectric%initialize_material + update_E,
+ * mod3/simpledielectric_mod.f90:22-103 (eps_r = 0 gives gax = 0, :91-97). */
+int fdtd_add_dielectric(fdtd_sim* sim, const fdtd_region* region, double eps_r);
+
+/* Gold%initialize_material + update_E, mod3/gold_mod.f90:50-195. */
+int fdtd_add_drude2cp(fdtd_sim* sim, const fdtd_region* region,
+                      const fdtd_drude2cp* params);
+
+/* Graphene%initialize_material + update_E, mod3/graphene_mod.f90:59-283. */
+int fdtd_add_graphene(fdtd_sim* sim, const fdtd_region* region,
+                      const fdtd_graphene* params);
+
+/* Plane source, sim3/program.f90:396-406: D_comp(plane) += g(t) after the
+ * D update and before E = f(D).  axis = plane normal, index 1-based.       */
+int fdtd_add_plane_source(fdtd_sim* sim, int axis, int index, int comp,
+                          const fdtd_pulse* pulse);
+
+/* Running DFT on a plane, sim3/program.f90:441-540: at step n with
+ * n_begin <= n <= n_end (n_end <= 0: no end), for every selected component
+ * F: re += cos(omega*t)*F*dt, im += sin(omega*t)*F*dt, t = (n-1)*dt,
+ * sampled after the H update.  comp_mask bit c selects component c
+ * (FDTD_EX..FDTD_HZ).  Accumulators are FP64.                              */
+int fdtd_add_dft_plane(fdtd_sim* sim, int axis, int index, int nfreq,
+                       const double* omega, int comp_mask, int n_begin,
+                       int n_end, int* id);
+
+/* Point probe, sim3/program.f90:445-451: one record per step after the H
+ * update: (n*dt, Ex, Ey, Ez, Hx, Hy, Hz).                                  */
+int fdtd_add_probe(fdtd_sim* sim, int i, int j, int k, int* id);
+
+/* Optional: freeze the set-up and allocate device state now (otherwise done
+ * by the first call that needs it).                                        */
+int fdtd_commit(fdtd_sim* sim);
+
+/* ---- time loop ----------------------------------------------------------- */
+/* nsteps iterations of the body of sim3/program.f90:382-540:
+ * updateD, source, material update_E, pmly%updateE, updateH, pmly%updateH,
+ * monitors; the halo exchanges become periodic index wrap on one domain.   */
+int fdtd_step(fdtd_sim* sim, int nsteps);
+int fdtd_sync(fdtd_sim* sim);
+int fdtd_get_step(fdtd_sim* sim, int* n_done);
+int fdtd_get_dt(fdtd_sim* sim, double* dt);
+
+/* ---- results ------------------------------------------------------------- */
+/* comp FDTD_EX..FDTD_DZ; dst holds nx*ny*nz doubles, x fastest.  D is
+ * available with the STAGED engine only.                                   */
+int fdtd_get_field(fdtd_sim* sim, int comp, double* dst);
+int fdtd_set_field(fdtd_sim* sim, int comp, const double* src);
+
+/* DFT accumulators of plane `id`, component comp: n1*n2*nfreq doubles each,
+ * in-plane index order (x-normal: j,k; y-normal: i,k; z-normal: i,j),
+ * frequency slowest (the reference's real_exin(i,k,m), :127-141).          */
+int fdtd_get_dft(fdtd_sim* sim, int id, int comp, double* re, double* im);
+
+/* Flux through DFT plane `id`, sim3/program.f90:613-685, generalised to
+ * the plane normal: S(m) = ds * sum (Re(Et)Re(Ht') ... ) with
+ * y-normal: Hx*Ez - Hz*Ex; z-normal: Hy*Ex - Hx*Ey; x-normal: Hz*Ey - Hy*Ez
+ * (real*real + imag*imag).  Needs the four tangential components.        */
+int fdtd_get_flux(fdtd_sim* sim, int id, double* S);
+
+int fdtd_get_probe_count(fdtd_sim* sim, int id, int* nrec);
+int fdtd_get_probe(fdtd_sim* sim, int id, double* out /* nrec*7 */);
+
+/* Field energy sums over interior cells: we = sum eps_r*(Ex^2+Ey^2+Ez^2)
+ * (eps_inf for dispersive cells, 1 where no material), wh = sum |H|^2.
+ * New: the reference computes flux only.                                   */
+int fdtd_energy(fdtd_sim* sim, double* we, double* wh);
+
+/* ---- introspection -------------------------------------------------------- */
+int fdtd_memory_bytes(fdtd_sim* sim, size_t* bytes);
+/* HIP-event timing of every kernel launch while enabled (adds events to the
+ * handle's stream; off by default).                                        */
+int fdtd_set_profiling(fdtd_sim* sim, int on);
+int fdtd_kernel_stats(fdtd_sim* sim, int kernel, double* total_ms,
+                      int64_t* launches);
+/* Bytes moved per step as counted by the algorithm (fields, ADE state, PML
+ * splits; monitors excluded) and the cell mix it was computed from.        */
+int fdtd_algorithmic_bytes(fdtd_sim* sim, int kernel, double* bytes_per_launch);
+int fdtd_cell_counts(fdtd_sim* sim, int64_t* n_pml, int64_t* n_ade);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FDTD_AMD_FDTD_H */
