@@ -100,7 +100,19 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run a plain device copy of this size (scales rocprof FETCH/WRITE counters)")
     args = ap.parse_args()
+    if args.copy_calibration_mib:
+        import torch
+
+        n = args.copy_calibration_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.mul(src, 2.0, out=dst)  # elementwise kernel: reads and writes exactly n*4 bytes
+        torch.cuda.synchronize()
+        del src, dst
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 
@@ -354,19 +355,26 @@ inline bool cell_in_pml(const Sim* s, int i, int j, int k) {
 
 template <typename real>
 void* upload_table(Sim* s) {
-  std::vector<MatEntry<real>> t(s->tab.size());
-  for (size_t q = 0; q < t.size(); ++q) {
+  const size_t n = s->tab.size();
+  std::vector<Vec4<real>> coef(n), gaxe(n);
+  std::vector<uint32_t> flags(n);
+  for (size_t q = 0; q < n; ++q) {
     const MatEntry<double>& m = s->tab[q];
+    uint32_t fl = 0;
     for (int c = 0; c < 3; ++c) {
-      t[q].coef[c] = (real)m.coef[c];
-      t[q].gax[c] = (real)m.gax[c];
-      t[q].act[c] = m.act[c];
+      coef[q].v[c] = (real)m.coef[c];
+      gaxe[q].v[c] = (real)m.gax[c];
+      if (m.act[c] == ACT_DIEL) fl |= 1u << c;
     }
-    t[q].pgax = (real)m.pgax;
-    t[q].eps = (real)m.eps;
-    t[q].pad = 0;
+    coef[q].v[3] = (real)m.pgax;
+    gaxe[q].v[3] = (real)m.eps;
+    flags[q] = fl;
   }
-  return upload_bytes(s, t.data(), t.size() * sizeof(MatEntry<real>));
+  std::vector<uint8_t> blob(n * (2 * sizeof(Vec4<real>) + sizeof(uint32_t)));
+  std::memcpy(blob.data(), coef.data(), n * sizeof(Vec4<real>));
+  std::memcpy(blob.data() + n * sizeof(Vec4<real>), gaxe.data(), n * sizeof(Vec4<real>));
+  std::memcpy(blob.data() + 2 * n * sizeof(Vec4<real>), flags.data(), n * sizeof(uint32_t));
+  return upload_bytes(s, blob.data(), blob.size());
 }
 
 int commit_impl(Sim* s) {
@@ -383,6 +391,26 @@ int commit_impl(Sim* s) {
   s->engine = eng;
   if ((int)s->srcs.size() > kMaxSrc) return set_error(FDTD_ERR_ARG, "too many sources");
 
+  // launch shape: whole 64-lane wavefronts along x, 256 threads per block
+  {
+    const int r = (s->nx + 63) / 64 * 64;
+    s->bx = std::min(256, r);
+    s->by = 256 / s->bx;
+    // tuning overrides: FDTD_BLOCK_X in {64,128,256,512}, FDTD_BLOCK_Y in
+    // {1,2,4,8} (at most 512 threads), FDTD_XCD_MAP in {0,1}
+    if (const char* e = std::getenv("FDTD_BLOCK_X")) {
+      const int v = std::atoi(e);
+      if (v == 64 || v == 128 || v == 256 || v == 512) {
+        s->bx = v;
+        s->by = std::max(1, 256 / v);
+      }
+    }
+    if (const char* e = std::getenv("FDTD_BLOCK_Y")) {
+      const int v = std::atoi(e);
+      if ((v == 1 || v == 2 || v == 4 || v == 8) && v * s->bx <= 512) s->by = v;
+    }
+    if (const char* e = std::getenv("FDTD_XCD_MAP")) s->xcdmap = std::atoi(e) != 0;
+  }
   // layout
   s->px = ((int64_t)s->nx + 2 + 15) / 16 * 16;
   s->pxy = s->px * (s->ny + 2);
@@ -503,6 +531,38 @@ int commit_impl(Sim* s) {
   s->ntab = (int)s->tab.size();
   s->d_mattab = s->prec == FDTD_FP32 ? upload_table<float>(s) : upload_table<double>(s);
   s->d_matid = (uint8_t*)upload_bytes(s, cid.data(), cid.size());
+
+  // segment form: x segments of at most 64 cells (x-PML cells apart) and the
+  // uniform material id of every (j,k) row (-1 where the row mixes ids)
+  {
+    std::vector<int32_t> seg;
+    std::vector<uint8_t> segpml;
+    const int Px = s->pml_mode == FDTD_PML_ALL_FACES ? s->P[0] : 0;
+    auto cut = [&](int lo, int hi, uint8_t pml) {
+      for (int x = lo; x <= hi; x += 64) {
+        seg.push_back(x);
+        seg.push_back(std::min(64, hi - x + 1));
+        segpml.push_back(pml);
+      }
+    };
+    if (Px > 0) cut(1, Px, 1);
+    cut(Px + 1, s->nx - Px, 0);
+    if (Px > 0) cut(s->nx - Px + 1, s->nx, 1);
+    s->nseg = (int)segpml.size();
+    std::vector<int16_t> rowmat((size_t)(s->ny + 2) * (s->nz + 2), -1);
+    for (int k = 1; k <= s->nz; ++k)
+      for (int j = 1; j <= s->ny; ++j) {
+        const uint8_t* row = cid.data() + (size_t)(s->px * j + s->pxy * k);
+        bool uni = true;
+        for (int i = 2; i <= s->nx && uni; ++i) uni = row[i] == row[1];
+        if (uni) rowmat[(size_t)j + (size_t)(s->ny + 2) * k] = row[1];
+      }
+    s->d_seg = upload_bytes(s, seg.data(), seg.size() * sizeof(int32_t));
+    s->d_segpml = (uint8_t*)upload_bytes(s, segpml.data(), segpml.size());
+    s->d_rowmat = (int16_t*)upload_bytes(s, rowmat.data(), rowmat.size() * sizeof(int16_t));
+    if (!s->d_seg || !s->d_segpml || !s->d_rowmat) return set_error(FDTD_ERR_NOMEM, "segment tables");
+    if (const char* e = std::getenv("FDTD_SEG")) s->segmode = std::atoi(e) != 0;
+  }
   if (!s->d_mattab || !s->d_matid) return set_error(FDTD_ERR_NOMEM, "material table");
 
   // dispersive cell lists
@@ -551,7 +611,10 @@ int commit_impl(Sim* s) {
     for (int c = 0; c < 6; ++c) d.slot[c] = (d.mask & (1 << c)) ? d.nslots++ : -1;
     d.d_omega = (double*)upload_bytes(s, d.omega.data(), d.omega.size() * sizeof(double));
     d.d_acc = (double*)dev_alloc(s, sizeof(double) * 2 * (size_t)d.nslots * d.nfreq * (size_t)d.n1 * d.n2);
-    if (!d.d_omega || !d.d_acc) return set_error(FDTD_ERR_NOMEM, "DFT accumulators");
+    d.d_samp = dev_alloc(s, s->wsize * (size_t)kDftBlock * d.nslots * (size_t)d.n1 * d.n2);
+    d.d_tab = (double*)dev_alloc(s, sizeof(double) * 2 * (size_t)kDftBlock * d.nfreq);
+    if (!d.d_omega || !d.d_acc || !d.d_samp || !d.d_tab) return set_error(FDTD_ERR_NOMEM, "DFT accumulators");
+    d.nbuf = 0;
   }
   if (!s->probes.empty()) {
     std::vector<int32_t> pos;
@@ -852,6 +915,9 @@ int fdtd_step(fdtd_sim* s, int nsteps) {
     if (rc) return rc;
     s->n_done += 1;
   }
+  // accumulators are complete whenever fdtd_step returns
+  rc = launch_dft_flush(s);
+  if (rc) return rc;
   if (np && nsteps) {
     std::vector<double> h(7 * (size_t)np * nsteps);
     FDTD_HIP(hipMemcpyAsync(h.data(), s->d_probe, h.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));

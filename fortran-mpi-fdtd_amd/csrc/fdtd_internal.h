@@ -34,6 +34,13 @@ struct alignas(16) MatEntry {
   int32_t pad;
 };
 
+// Device form of the material table: coef = (Ku*gax x,y,z, PML medium gax),
+// gaxe = (gax x,y,z, energy eps), flags bit q = a dielectric writes E_q.
+template <typename real>
+struct alignas(4 * sizeof(real)) Vec4 {
+  real v[4];
+};
+
 // Coefficients of one group of dispersive cells (Drude + up to 4 CP + Raman).
 template <typename real>
 struct AdeCoef {
@@ -76,11 +83,19 @@ struct HostSource {
   fdtd_pulse p;
 };
 
+// Samples of a DFT plane are buffered for up to kDftBlock gated steps and
+// folded into the FP64 accumulators in one pass (time-blocked running DFT).
+constexpr int kDftBlock = 32;
+
 struct DftPlane {
   int axis, index, nfreq, mask, n_begin, n_end, n1, n2;
   std::vector<double> omega;
   double* d_omega = nullptr;
-  double* d_acc = nullptr;  // [comp slot][re/im][m][v][u]
+  double* d_acc = nullptr;   // [comp slot][re/im][m][v][u]
+  void* d_samp = nullptr;    // [b][slot][v][u], field precision
+  double* d_tab = nullptr;   // [b][m][cos, sin]
+  int nbuf = 0;              // buffered samples
+  int n_first = 0;           // step of the first buffered sample
   int slot[6];
   int nslots = 0;
 };
@@ -124,7 +139,7 @@ struct Sim {
   void* H[3] = {nullptr, nullptr, nullptr};
   void* D[3] = {nullptr, nullptr, nullptr};
   uint8_t* d_matid = nullptr;  // per ghosted cell
-  void* d_mattab = nullptr;    // MatEntry<real>[ntab]
+  void* d_mattab = nullptr;    // Vec4 coef[ntab], Vec4 gaxe[ntab], uint32 flags[ntab]
   int ntab = 0;
   std::vector<MatEntry<double>> tab;  // FP64 master copy
   std::vector<AdeGroup> groups;
@@ -150,6 +165,18 @@ struct Sim {
   std::vector<DevBuf> allocs;
   size_t dev_bytes = 0;
 
+  // launch shape of the cell-parallel kernels (x threads, y threads) and
+  // whether fused kernels deal tiles to XCDs in contiguous runs
+  int bx = 64, by = 4;
+  int xcdmap = 1;
+  // segment form of the fused kernels (one 64-cell x segment per wavefront);
+  // measured slower than whole-row tiles on 512^3, kept selectable
+  int segmode = 0;
+  int nseg = 0;
+  void* d_seg = nullptr;        // int2 (first i, length) per segment
+  uint8_t* d_segpml = nullptr;  // segment inside an x PML slab
+  int16_t* d_rowmat = nullptr;  // uniform material id of a (j,k) row, or -1
+
   // profiling
   bool profiling = false;
   double k_ms[FDTD_K_COUNT] = {0};
@@ -174,6 +201,7 @@ double* scratch(Sim* s, size_t bytes); // reusable reduction buffer
 
 // kernels (fdtd_kernels.hip)
 int launch_step(Sim* s);           // one full time step at s->n_done + 1
+int launch_dft_flush(Sim* s);      // fold every buffered DFT sample
 int launch_energy(Sim* s, double* we, double* wh);
 int launch_flux(Sim* s, int id, double* S);
 int launch_set_ghosts(Sim* s, int comp);

@@ -30,12 +30,32 @@ struct FieldArgs {
   real* h[3];
   real* d[3];
   const uint8_t* matid;
-  const MatEntry<real>* tab;
+  const Vec4<real>* coef;   // Ku*gax x,y,z, PML medium gax
+  const Vec4<real>* gaxe;   // gax x,y,z, energy eps
+  const uint32_t* flags;    // bit q: a dielectric writes E_q
   int32_t nx, ny, nz;
   int32_t per[3];
-  int64_t px, pxy;
+  int32_t px, pxy;          // offsets fit 32 bits (checked at commit)
   real ku;
+  // tile grid of the cell-parallel kernels: ntx*nty*ntz tiles of
+  // blockDim.x*blockDim.y cells, dealt out so each of the 8 XCDs (one L2
+  // each) walks a contiguous run of `tper` tiles: neighbouring rows and
+  // planes are then read from the same L2.
+  int32_t ntx, nty, ntz, tper, xcdmap;
 };
+
+// Tile coordinates of this block (false: padding block, nothing to do).
+template <typename real>
+__device__ __forceinline__ bool tile_of(const FieldArgs<real>& a, int& tx, int& ty, int& tz) {
+  const int b = blockIdx.x;
+  const int t = a.xcdmap ? (b & 7) * a.tper + (b >> 3) : b;
+  if (t >= a.ntx * a.nty * a.ntz) return false;
+  tx = t % a.ntx;
+  const int r = t / a.ntx;
+  ty = r % a.nty;
+  tz = r / a.nty;
+  return true;
+}
 
 template <typename real>
 struct PmlArgs {
@@ -79,53 +99,60 @@ __device__ __forceinline__ bool in_pml(const int32_t* P, int i, int j, int k, in
 
 // Compact index of an ALL_FACES PML cell: z slabs (full xy planes), then y
 // slabs of the middle planes (full x rows), then x slabs of the rest.
-__device__ __forceinline__ int64_t pml_index(const int32_t* P, int i, int j, int k, int nx, int ny, int nz) {
+// The count of PML cells is below the ghosted cell count, which fits 31 bits.
+__device__ __forceinline__ int pml_index(const int32_t* P, int i, int j, int k, int nx, int ny, int nz) {
   const int Px = P[0], Py = P[1], Pz = P[2];
   if (k <= Pz || k > nz - Pz) {
-    int kk = k <= Pz ? k - 1 : Pz + (k - (nz - Pz + 1));
-    return (int64_t)kk * nx * ny + (int64_t)(j - 1) * nx + (i - 1);
+    const int kk = k <= Pz ? k - 1 : Pz + (k - (nz - Pz + 1));
+    return kk * nx * ny + (j - 1) * nx + (i - 1);
   }
-  const int64_t baseY = (int64_t)2 * Pz * nx * ny;
+  const int baseY = 2 * Pz * nx * ny;
   const int km = k - Pz - 1;
   if (j <= Py || j > ny - Py) {
-    int jj = j <= Py ? j - 1 : Py + (j - (ny - Py + 1));
-    return baseY + ((int64_t)km * 2 * Py + jj) * nx + (i - 1);
+    const int jj = j <= Py ? j - 1 : Py + (j - (ny - Py + 1));
+    return baseY + (km * 2 * Py + jj) * nx + (i - 1);
   }
-  const int64_t baseX = baseY + (int64_t)(nz - 2 * Pz) * 2 * Py * nx;
+  const int baseX = baseY + (nz - 2 * Pz) * 2 * Py * nx;
   const int jm = j - Py - 1;
-  int ii = i <= Px ? i - 1 : Px + (i - (nx - Px + 1));
-  return baseX + ((int64_t)km * (ny - 2 * Py) + jm) * 2 * Px + ii;
+  const int ii = i <= Px ? i - 1 : Px + (i - (nx - Px + 1));
+  return baseX + (km * (ny - 2 * Py) + jm) * 2 * Px + ii;
 }
 
 // ---------------------------------------------------------------------------
 // FUSED engine: E half step (myfdtd_func.f90:19-41 in E-form, material
 // update_E folded in, ALL_FACES split-field PML, plane sources).
 template <typename real>
-__global__ __launch_bounds__(256) void k_fused_e(FieldArgs<real> a, PmlArgs<real> p, SrcArgs<real> src,
+__global__ __launch_bounds__(512) void k_fused_e(FieldArgs<real> a, PmlArgs<real> p, SrcArgs<real> src,
                                                  int nsrc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  const int k = blockIdx.z + 1;
+  int tx, ty, tz;
+  if (!tile_of(a, tx, ty, tz)) return;
+  const int i = tx * blockDim.x + threadIdx.x + 1;
+  const int j = ty * blockDim.y + threadIdx.y + 1;
+  const int k = tz + 1;
   if (i > a.nx || j > a.ny) return;
-  const int64_t c = i + a.px * j + a.pxy * k;
+  const int c = i + a.px * j + a.pxy * k;
   const real hx = a.h[0][c], hy = a.h[1][c], hz = a.h[2][c];
   const real hz_jm = a.h[2][c - a.px], hx_jm = a.h[0][c - a.px];
   const real hy_km = a.h[1][c - a.pxy], hx_km = a.h[0][c - a.pxy];
   const real hz_im = a.h[2][c - 1], hy_im = a.h[1][c - 1];
-  const MatEntry<real> m = a.tab[a.matid[c]];
+  const unsigned id = a.matid[c];
+  // plane sources: value and normal axis per E component (-1: none here)
   real sv[3] = {0, 0, 0};
   int sa[3] = {-1, -1, -1};
   for (int q = 0; q < nsrc; ++q) {
-    const int pos = src.axis[q] == 0 ? i : (src.axis[q] == 1 ? j : k);
+    const int ax = src.axis[q];
+    const int pos = ax == 0 ? i : (ax == 1 ? j : k);
     if (pos == src.index[q]) {
-      sv[src.comp[q]] += src.val[q];
-      sa[src.comp[q]] = src.axis[q];
+      const int cq = src.comp[q];
+      if (cq == 0) { sv[0] += src.val[q]; sa[0] = ax; }
+      else if (cq == 1) { sv[1] += src.val[q]; sa[1] = ax; }
+      else { sv[2] += src.val[q]; sa[2] = ax; }
     }
   }
   if (in_pml(p.P, i, j, k, a.nx, a.ny, a.nz)) {
     // mod3/BERpmly_mod.f90:150-155 with the splits driven by total H
-    const int64_t s = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
-    const real r = m.pgax;
+    const int s = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
+    const real r = a.coef[id].v[3];
     real eyx = p.f1[0][i] * p.s[S_EYX][s] + (p.f2[0][i] * r) * (-hz + hz_im);
     real ezx = p.f1[0][i] * p.s[S_EZX][s] + (p.f2[0][i] * r) * (hy + (-hy_im));
     real exy = p.f1[1][j] * p.s[S_EXY][s] + (p.f2[1][j] * r) * (hz + (-hz_jm));
@@ -150,11 +177,16 @@ __global__ __launch_bounds__(256) void k_fused_e(FieldArgs<real> a, PmlArgs<real
   const real cy = (hx - hx_km) + (-hz + hz_im);  // curlhxz + curlhzx
   const real cz = (hy - hy_im) + (-hx + hx_jm);  // curlhyx + curlhxy
   const real cur[3] = {cx, cy, cz};
+  const uint32_t fl = a.flags[id];
+  const Vec4<real> cf = a.coef[id];
+  const bool any_src = sa[0] >= 0 || sa[1] >= 0 || sa[2] >= 0;
+  Vec4<real> gx;
+  if (any_src) gx = a.gaxe[id];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    if (m.act[q] != ACT_DIEL) continue;
-    real ev = a.e[q][c] + m.coef[q] * cur[q];
-    if (sa[q] >= 0) ev = ev + m.gax[q] * sv[q];
+    if (!(fl & (1u << q))) continue;
+    real ev = a.e[q][c] + cf.v[q] * cur[q];
+    if (sa[q] >= 0) ev = ev + gx.v[q] * sv[q];
     a.e[q][c] = ev;
     put_e_ghost(a.e[q], ev, i, j, k, c, a);
   }
@@ -162,19 +194,172 @@ __global__ __launch_bounds__(256) void k_fused_e(FieldArgs<real> a, PmlArgs<real
 
 // FUSED engine: H half step (myfdtd_func.f90:45-64 + split-field H).
 template <typename real>
-__global__ __launch_bounds__(256) void k_fused_h(FieldArgs<real> a, PmlArgs<real> p) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  const int k = blockIdx.z + 1;
+__global__ __launch_bounds__(512) void k_fused_h(FieldArgs<real> a, PmlArgs<real> p) {
+  int tx, ty, tz;
+  if (!tile_of(a, tx, ty, tz)) return;
+  const int i = tx * blockDim.x + threadIdx.x + 1;
+  const int j = ty * blockDim.y + threadIdx.y + 1;
+  const int k = tz + 1;
   if (i > a.nx || j > a.ny) return;
-  const int64_t c = i + a.px * j + a.pxy * k;
+  const int c = i + a.px * j + a.pxy * k;
   const real ex = a.e[0][c], ey = a.e[1][c], ez = a.e[2][c];
   const real ey_kp = a.e[1][c + a.pxy], ex_kp = a.e[0][c + a.pxy];
   const real ez_jp = a.e[2][c + a.px], ex_jp = a.e[0][c + a.px];
   const real ez_ip = a.e[2][c + 1], ey_ip = a.e[1][c + 1];
   if (in_pml(p.P, i, j, k, a.nx, a.ny, a.nz)) {
     // mod3/BERpmly_mod.f90:186-191 with the splits driven by total E
-    const int64_t s = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
+    const int s = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
+    real hyx = p.g1[0][i] * p.s[S_HYX][s] + p.g2[0][i] * (-ez + ez_ip);
+    real hzx = p.g1[0][i] * p.s[S_HZX][s] + p.g2[0][i] * (ey + (-ey_ip));
+    real hxy = p.g1[1][j] * p.s[S_HXY][s] + p.g2[1][j] * (ez + (-ez_jp));
+    real hzy = p.g1[1][j] * p.s[S_HZY][s] + p.g2[1][j] * (-ex + ex_jp);
+    real hxz = p.g1[2][k] * p.s[S_HXZ][s] + p.g2[2][k] * (-ey + ey_kp);
+    real hyz = p.g1[2][k] * p.s[S_HYZ][s] + p.g2[2][k] * (ex + (-ex_kp));
+    p.s[S_HYX][s] = hyx; p.s[S_HZX][s] = hzx; p.s[S_HXY][s] = hxy;
+    p.s[S_HZY][s] = hzy; p.s[S_HXZ][s] = hxz; p.s[S_HYZ][s] = hyz;
+    a.h[0][c] = hxy + hxz;
+    a.h[1][c] = hyz + hyx;
+    a.h[2][c] = hzx + hzy;
+    return;
+  }
+  const real curleyz = ey_kp - ey, curlezy = -ez_jp + ez;
+  const real curlezx = ez_ip - ez, curlexz = -ex_kp + ex;
+  const real curlexy = ex_jp - ex, curleyx = -ey_ip + ey;
+  const real hx = a.h[0][c] + a.ku * (curleyz + curlezy);
+  const real hy = a.h[1][c] + a.ku * (curlezx + curlexz);
+  const real hz = a.h[2][c] + a.ku * (curlexy + curleyx);
+  a.h[0][c] = hx;
+  a.h[1][c] = hy;
+  a.h[2][c] = hz;
+  put_h_ghost(a.h[0], hx, i, j, k, c, a);
+  put_h_ghost(a.h[1], hy, i, j, k, c, a);
+  put_h_ghost(a.h[2], hz, i, j, k, c, a);
+}
+
+// ---------------------------------------------------------------------------
+// FUSED engine, segment form.  A row of cells is cut into x segments of at
+// most 64 cells: the low x-PML cells, the interior in 64-cell pieces, the
+// high x-PML cells.  Each wavefront owns one segment of one (j,k) row, so
+// the PML / interior choice is uniform over a wavefront (no divergence at
+// the x slabs).  Rows whose interior cells all share one material id carry
+// it in `rowmat`; their table lookups are wave-uniform (scalar) loads.
+struct SegArgs {
+  const int2* seg;  // (first i, length) per segment
+  const uint8_t* segpml;  // 1: segment lies in an x PML slab
+  int32_t nseg;
+  int32_t nwork;    // nseg * ny * nz wavefronts of work
+  int32_t per;      // blocks per XCD run (XCD-contiguous dealing)
+  const int16_t* rowmat;  // per ghosted (j,k) row: uniform material id or -1
+  int32_t ry;       // row stride: rowmat[j + ry*k]
+};
+
+__device__ __forceinline__ bool seg_work(const SegArgs& sg, int ny, int& seg, int& j, int& k) {
+  const int b = blockIdx.x;
+  const int blk = (b & 7) * sg.per + (b >> 3);
+  const int w = blk * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w >= sg.nwork) return false;
+  seg = w % sg.nseg;
+  const int r = w / sg.nseg;
+  j = r % ny + 1;
+  k = r / ny + 1;
+  return true;
+}
+
+template <typename real, bool UNI>
+__device__ __forceinline__ void e_cell_update(const FieldArgs<real>& a, unsigned id, int i, int j, int k, int c,
+                                              real cx, real cy, real cz, const real* sv, const int* sa) {
+  const uint32_t fl = a.flags[id];
+  const Vec4<real> cf = a.coef[id];
+  const real cur[3] = {cx, cy, cz};
+  const bool any_src = sa[0] >= 0 || sa[1] >= 0 || sa[2] >= 0;
+  Vec4<real> gx;
+  if (any_src) gx = a.gaxe[id];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    if (!(fl & (1u << q))) continue;
+    real ev = a.e[q][c] + cf.v[q] * cur[q];
+    if (sa[q] >= 0) ev = ev + gx.v[q] * sv[q];
+    a.e[q][c] = ev;
+    put_e_ghost(a.e[q], ev, i, j, k, c, a);
+  }
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_seg_e(FieldArgs<real> a, PmlArgs<real> p, SrcArgs<real> src, int nsrc,
+                                               SegArgs sg) {
+  int seg, j, k;
+  if (!seg_work(sg, a.ny, seg, j, k)) return;
+  const int2 sd = sg.seg[seg];
+  const int lane = threadIdx.x & 63;
+  if (lane >= sd.y) return;
+  const int i = sd.x + lane;
+  const int c = i + a.px * j + a.pxy * k;
+  const real hx = a.h[0][c], hy = a.h[1][c], hz = a.h[2][c];
+  const real hz_jm = a.h[2][c - a.px], hx_jm = a.h[0][c - a.px];
+  const real hy_km = a.h[1][c - a.pxy], hx_km = a.h[0][c - a.pxy];
+  const real hz_im = a.h[2][c - 1], hy_im = a.h[1][c - 1];
+  real sv[3] = {0, 0, 0};
+  int sa[3] = {-1, -1, -1};
+  for (int q = 0; q < nsrc; ++q) {
+    const int ax = src.axis[q];
+    const int pos = ax == 0 ? i : (ax == 1 ? j : k);
+    if (pos == src.index[q]) {
+      const int cq = src.comp[q];
+      if (cq == 0) { sv[0] += src.val[q]; sa[0] = ax; }
+      else if (cq == 1) { sv[1] += src.val[q]; sa[1] = ax; }
+      else { sv[2] += src.val[q]; sa[2] = ax; }
+    }
+  }
+  const int rm = __builtin_amdgcn_readfirstlane((int)sg.rowmat[j + sg.ry * k]);
+  const bool pml = sg.segpml[seg] || j <= p.P[1] || j > a.ny - p.P[1] || k <= p.P[2] || k > a.nz - p.P[2];
+  if (pml) {
+    const unsigned id = rm >= 0 ? (unsigned)rm : (unsigned)a.matid[c];
+    const int s = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
+    const real r = a.coef[id].v[3];
+    real eyx = p.f1[0][i] * p.s[S_EYX][s] + (p.f2[0][i] * r) * (-hz + hz_im);
+    real ezx = p.f1[0][i] * p.s[S_EZX][s] + (p.f2[0][i] * r) * (hy + (-hy_im));
+    real exy = p.f1[1][j] * p.s[S_EXY][s] + (p.f2[1][j] * r) * (hz + (-hz_jm));
+    real ezy = p.f1[1][j] * p.s[S_EZY][s] + (p.f2[1][j] * r) * (-hx + hx_jm);
+    real exz = p.f1[2][k] * p.s[S_EXZ][s] + (p.f2[2][k] * r) * (-hy + hy_km);
+    real eyz = p.f1[2][k] * p.s[S_EYZ][s] + (p.f2[2][k] * r) * (hx + (-hx_km));
+    if (sa[0] == 1) exy = exy + sv[0] * r;
+    if (sa[0] == 2) exz = exz + sv[0] * r;
+    if (sa[1] == 0) eyx = eyx + sv[1] * r;
+    if (sa[1] == 2) eyz = eyz + sv[1] * r;
+    if (sa[2] == 0) ezx = ezx + sv[2] * r;
+    if (sa[2] == 1) ezy = ezy + sv[2] * r;
+    p.s[S_EYX][s] = eyx; p.s[S_EZX][s] = ezx; p.s[S_EXY][s] = exy;
+    p.s[S_EZY][s] = ezy; p.s[S_EXZ][s] = exz; p.s[S_EYZ][s] = eyz;
+    a.e[0][c] = exy + exz;
+    a.e[1][c] = eyz + eyx;
+    a.e[2][c] = ezx + ezy;
+    return;
+  }
+  const real cx = (hz - hz_jm) + (-hy + hy_km);  // curlhzy + curlhyz
+  const real cy = (hx - hx_km) + (-hz + hz_im);  // curlhxz + curlhzx
+  const real cz = (hy - hy_im) + (-hx + hx_jm);  // curlhyx + curlhxy
+  if (rm >= 0)
+    e_cell_update<real, true>(a, (unsigned)rm, i, j, k, c, cx, cy, cz, sv, sa);
+  else
+    e_cell_update<real, false>(a, (unsigned)a.matid[c], i, j, k, c, cx, cy, cz, sv, sa);
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_seg_h(FieldArgs<real> a, PmlArgs<real> p, SegArgs sg) {
+  int seg, j, k;
+  if (!seg_work(sg, a.ny, seg, j, k)) return;
+  const int2 sd = sg.seg[seg];
+  const int lane = threadIdx.x & 63;
+  if (lane >= sd.y) return;
+  const int i = sd.x + lane;
+  const int c = i + a.px * j + a.pxy * k;
+  const real ex = a.e[0][c], ey = a.e[1][c], ez = a.e[2][c];
+  const real ey_kp = a.e[1][c + a.pxy], ex_kp = a.e[0][c + a.pxy];
+  const real ez_jp = a.e[2][c + a.px], ex_jp = a.e[0][c + a.px];
+  const real ez_ip = a.e[2][c + 1], ey_ip = a.e[1][c + 1];
+  const bool pml = sg.segpml[seg] || j <= p.P[1] || j > a.ny - p.P[1] || k <= p.P[2] || k > a.nz - p.P[2];
+  if (pml) {
+    const int s = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
     real hyx = p.g1[0][i] * p.s[S_HYX][s] + p.g2[0][i] * (-ez + ez_ip);
     real hzx = p.g1[0][i] * p.s[S_HZX][s] + p.g2[0][i] * (ey + (-ey_ip));
     real hxy = p.g1[1][j] * p.s[S_HXY][s] + p.g2[1][j] * (ez + (-ez_jp));
@@ -336,12 +521,14 @@ __global__ __launch_bounds__(256) void k_staged_mat(FieldArgs<real> a) {
   const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
   const int k = blockIdx.z + 1;
   if (i > a.nx || j > a.ny) return;
-  const int64_t c = i + a.px * j + a.pxy * k;
-  const MatEntry<real> m = a.tab[a.matid[c]];
+  const int c = i + a.px * j + a.pxy * k;
+  const unsigned id = a.matid[c];
+  const uint32_t fl = a.flags[id];
+  const Vec4<real> gx = a.gaxe[id];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    if (m.act[q] != ACT_DIEL) continue;
-    const real ev = a.d[q][c] * m.gax[q];
+    if (!(fl & (1u << q))) continue;
+    const real ev = a.d[q][c] * gx.v[q];
     a.e[q][c] = ev;
     put_e_ghost(a.e[q], ev, i, j, k, c, a);
   }
@@ -480,26 +667,22 @@ __global__ __launch_bounds__(256) void k_ref_pml_h(FieldArgs<real> a, RefPmlArgs
 }
 
 // ---------------------------------------------------------------------------
-// Monitors: running DFT on a plane, sim3/program.f90:456-472.
-// acc layout: [slot][re|im][m][v][u] (FP64).
+// Monitors: running DFT on a plane (sim3/program.f90:456-472), time-blocked:
+// every gated step copies the plane's selected components into a ring of
+// kDftBlock samples; a flush adds dt * sum_b cos/sin(w t_b) F_b, t_b =
+// (n_b - 1) dt, to the FP64 accumulators [slot][re|im][m][v][u].
+struct CompList {
+  int32_t c[6];
+};
+
 template <typename real>
-__global__ __launch_bounds__(256) void k_dft(FieldArgs<real> a, int axis, int index, int n1, int n2, int nfreq,
-                                             const double* omega, double* acc, int comp0, int comp1, int comp2,
-                                             int comp3, int comp4, int comp5, double t, double dt) {
-  extern __shared__ double tabcs[];
-  for (int m = threadIdx.x; m < nfreq; m += blockDim.x) {
-    double sw, cw;
-    sincos(omega[m] * t, &sw, &cw);
-    tabcs[m] = cw;
-    tabcs[nfreq + m] = sw;
-  }
-  __syncthreads();
+__global__ __launch_bounds__(256) void k_dft_sample(FieldArgs<real> a, int axis, int index, int n1, int n2,
+                                                    int nslots, CompList comps, real* samp) {
   const int64_t np = (int64_t)n1 * n2;
   const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (o >= np) return;
   const int slot = blockIdx.y;
-  const int comps[6] = {comp0, comp1, comp2, comp3, comp4, comp5};
-  const int comp = comps[slot];
+  const int comp = comps.c[slot];
   const int u = (int)(o % n1) + 1, v = (int)(o / n1) + 1;
   int p[3];
   const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
@@ -508,13 +691,59 @@ __global__ __launch_bounds__(256) void k_dft(FieldArgs<real> a, int axis, int in
   p[a2] = v;
   const int64_t c = p[0] + a.px * p[1] + a.pxy * p[2];
   const real* F = comp < 3 ? a.e[comp] : a.h[comp - 3];
-  const double f = (double)F[c];
+  samp[(int64_t)slot * np + o] = F[c];
+  (void)nslots;
+}
+
+// cos/sin table of one block of samples: tab[(b*nfreq + m)*2 + {0,1}]
+__global__ void k_dft_table(const double* omega, int nfreq, int n_first, int count, double dt, double* tab) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nfreq * count) return;
+  const int b = q / nfreq, m = q - b * nfreq;
+  const double t = (double)(n_first + b - 1) * dt;
+  double sw, cw;
+  sincos(omega[m] * t, &sw, &cw);
+  tab[2 * q] = cw;
+  tab[2 * q + 1] = sw;
+}
+
+constexpr int kDftMChunk = 32;
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_dft_accum(const real* samp, int64_t np, int nslots, int nfreq, int count,
+                                                   const double* tab, double dt, double* acc) {
+  __shared__ double lt[kDftBlock * kDftMChunk * 2];
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int slot = blockIdx.y;
+  const bool act = o < np;
+  double f[kDftBlock];
+#pragma unroll
+  for (int b = 0; b < kDftBlock; ++b)
+    f[b] = (act && b < count) ? (double)samp[((int64_t)b * nslots + slot) * np + o] : 0.0;
   double* re = acc + (int64_t)slot * 2 * nfreq * np;
   double* im = re + (int64_t)nfreq * np;
-  for (int m = 0; m < nfreq; ++m) {
-    const int64_t q = (int64_t)m * np + o;
-    re[q] = re[q] + tabcs[m] * f * dt;
-    im[q] = im[q] + tabcs[nfreq + m] * f * dt;
+  for (int m0 = 0; m0 < nfreq; m0 += kDftMChunk) {
+    const int mc = min(kDftMChunk, nfreq - m0);
+    __syncthreads();
+    for (int q = threadIdx.x; q < kDftBlock * mc; q += blockDim.x) {
+      const int b = q / mc, mm = q - b * mc;
+      const bool live = b < count;
+      lt[2 * (b * kDftMChunk + mm)] = live ? tab[2 * (b * nfreq + m0 + mm)] : 0.0;
+      lt[2 * (b * kDftMChunk + mm) + 1] = live ? tab[2 * (b * nfreq + m0 + mm) + 1] : 0.0;
+    }
+    __syncthreads();
+    if (!act) continue;
+    for (int mm = 0; mm < mc; ++mm) {
+      double sr = 0.0, si = 0.0;
+#pragma unroll
+      for (int b = 0; b < kDftBlock; ++b) {
+        sr += lt[2 * (b * kDftMChunk + mm)] * f[b];
+        si += lt[2 * (b * kDftMChunk + mm) + 1] * f[b];
+      }
+      const int64_t qq = (int64_t)(m0 + mm) * np + o;
+      re[qq] += sr * dt;
+      im[qq] += si * dt;
+    }
   }
 }
 
@@ -566,7 +795,7 @@ __global__ __launch_bounds__(256) void k_energy(FieldArgs<real> a, double* part)
     const int64_t c = i + a.px * j + a.pxy * k;
     const double ex = a.e[0][c], ey = a.e[1][c], ez = a.e[2][c];
     const double hx = a.h[0][c], hy = a.h[1][c], hz = a.h[2][c];
-    s1 += (double)a.tab[a.matid[c]].eps * (ex * ex + ey * ey + ez * ez);
+    s1 += (double)a.gaxe[a.matid[c]].v[3] * (ex * ex + ey * ey + ez * ez);
     s2 += hx * hx + hy * hy + hz * hz;
   }
   r1[threadIdx.x] = s1;
@@ -616,14 +845,23 @@ FieldArgs<real> field_args(Sim* s) {
     a.d[q] = (real*)s->D[q];
   }
   a.matid = s->d_matid;
-  a.tab = (const MatEntry<real>*)s->d_mattab;
+  const size_t nt = (size_t)s->ntab;
+  a.coef = (const Vec4<real>*)s->d_mattab;
+  a.gaxe = a.coef + nt;
+  a.flags = (const uint32_t*)(a.gaxe + nt);
   a.nx = s->nx;
   a.ny = s->ny;
   a.nz = s->nz;
   for (int q = 0; q < 3; ++q) a.per[q] = s->g.periodic[q] ? 1 : 0;
-  a.px = s->px;
-  a.pxy = s->pxy;
+  a.px = (int32_t)s->px;
+  a.pxy = (int32_t)s->pxy;
   a.ku = (real)s->ku;
+  a.ntx = (s->nx + s->bx - 1) / s->bx;
+  a.nty = (s->ny + s->by - 1) / s->by;
+  a.ntz = s->nz;
+  const int tiles = a.ntx * a.nty * a.ntz;
+  a.tper = (tiles + 7) / 8;
+  a.xcdmap = s->xcdmap;
   return a;
 }
 
@@ -688,6 +926,15 @@ struct Timer {
   }
 };
 
+template <typename real>
+inline dim3 tile_grid(const FieldArgs<real>& a) {
+  const int tiles = a.ntx * a.nty * a.ntz;
+  return dim3((unsigned)(a.xcdmap ? 8 * a.tper : tiles));
+}
+
+// 8 XCD runs of `per` blocks of 4 wavefronts (seg_work undoes the dealing)
+inline dim3 seg_grid(const SegArgs& sg) { return dim3((unsigned)(8 * sg.per)); }
+
 inline dim3 cell_grid(int nx, int nrows, int nz, dim3 blk) {
   return dim3((unsigned)((nx + blk.x - 1) / blk.x), (unsigned)((nrows + blk.y - 1) / blk.y), (unsigned)nz);
 }
@@ -713,13 +960,26 @@ SrcArgs<real> source_args(Sim* s, int n, int* nsrc) {
 }
 
 template <typename real>
+void dft_flush_plane(Sim* s, DftPlane& d) {
+  if (d.nbuf == 0) return;
+  const int64_t np = (int64_t)d.n1 * d.n2;
+  Timer tm(s, FDTD_K_MON);
+  const int nt = d.nfreq * d.nbuf;
+  hipLaunchKernelGGL(k_dft_table, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s->stream, d.d_omega, d.nfreq,
+                     d.n_first, d.nbuf, s->dt, d.d_tab);
+  hipLaunchKernelGGL(k_dft_accum<real>, dim3((unsigned)((np + 255) / 256), (unsigned)d.nslots), dim3(256), 0,
+                     s->stream, (const real*)d.d_samp, np, d.nslots, d.nfreq, d.nbuf, d.d_tab, s->dt, d.d_acc);
+  d.nbuf = 0;
+}
+
+template <typename real>
 int step_impl(Sim* s) {
   const int n = s->n_done + 1;
   const double t = (double)(n - 1) * s->dt;
   FieldArgs<real> a = field_args<real>(s);
   int nsrc = 0;
   SrcArgs<real> src = source_args<real>(s, n, &nsrc);
-  const dim3 blk(64, 4, 1);
+  const dim3 blk((unsigned)s->bx, (unsigned)s->by, 1);
   hipStream_t st = s->stream;
   auto ade_coef = [&](const AdeGroup& g) {
     AdeCoef<real> c{};
@@ -743,9 +1003,20 @@ int step_impl(Sim* s) {
   };
   if (s->engine == FDTD_ENGINE_FUSED) {
     PmlArgs<real> p = pml_args<real>(s);
+    SegArgs sg{};
+    sg.seg = (const int2*)s->d_seg;
+    sg.segpml = s->d_segpml;
+    sg.nseg = s->nseg;
+    sg.nwork = s->nseg * s->ny * s->nz;
+    sg.per = ((sg.nwork + 3) / 4 + 7) / 8;
+    sg.rowmat = s->d_rowmat;
+    sg.ry = s->ny + 2;
     {
       Timer tm(s, FDTD_K_E);
-      hipLaunchKernelGGL(k_fused_e<real>, cell_grid(s->nx, s->ny, s->nz, blk), blk, 0, st, a, p, src, nsrc);
+      if (s->segmode)
+        hipLaunchKernelGGL(k_seg_e<real>, seg_grid(sg), dim3(256), 0, st, a, p, src, nsrc, sg);
+      else
+        hipLaunchKernelGGL(k_fused_e<real>, tile_grid(a), blk, 0, st, a, p, src, nsrc);
     }
     for (auto& g : s->groups) {
       if (!g.ncells) continue;
@@ -755,7 +1026,10 @@ int step_impl(Sim* s) {
     }
     {
       Timer tm(s, FDTD_K_H);
-      hipLaunchKernelGGL(k_fused_h<real>, cell_grid(s->nx, s->ny, s->nz, blk), blk, 0, st, a, p);
+      if (s->segmode)
+        hipLaunchKernelGGL(k_seg_h<real>, seg_grid(sg), dim3(256), 0, st, a, p, sg);
+      else
+        hipLaunchKernelGGL(k_fused_h<real>, tile_grid(a), blk, 0, st, a, p);
     }
   } else {
     const bool refy = s->pml_mode == FDTD_PML_REF_Y;
@@ -798,19 +1072,25 @@ int step_impl(Sim* s) {
       hipLaunchKernelGGL(k_ref_pml_h<real>, cell_grid(s->nx, 2 * P, s->nz, blk), blk, 0, st, a, rp);
     }
   }
-  // monitors
+  // monitors: buffer this step's plane samples, fold a full block
+  (void)t;
   for (auto& d : s->dfts) {
     if (n < d.n_begin) continue;
     if (d.n_end > 0 && n > d.n_end) continue;
-    int comps[6] = {0, 0, 0, 0, 0, 0};
+    CompList cl{};
     int ns = 0;
     for (int c = 0; c < 6; ++c)
-      if (d.mask & (1 << c)) comps[ns++] = c;
+      if (d.mask & (1 << c)) cl.c[ns++] = c;
     const int64_t np = (int64_t)d.n1 * d.n2;
-    Timer tm(s, FDTD_K_MON);
-    hipLaunchKernelGGL(k_dft<real>, dim3((unsigned)((np + 255) / 256), (unsigned)ns), dim3(256),
-                       (size_t)(2 * d.nfreq) * sizeof(double), st, a, d.axis, d.index, d.n1, d.n2, d.nfreq, d.d_omega,
-                       d.d_acc, comps[0], comps[1], comps[2], comps[3], comps[4], comps[5], t, s->dt);
+    if (d.nbuf == 0) d.n_first = n;
+    real* dst = (real*)d.d_samp + (int64_t)d.nbuf * ns * np;
+    {
+      Timer tm(s, FDTD_K_MON);
+      hipLaunchKernelGGL(k_dft_sample<real>, dim3((unsigned)((np + 255) / 256), (unsigned)ns), dim3(256), 0, st, a,
+                         d.axis, d.index, d.n1, d.n2, ns, cl, dst);
+    }
+    d.nbuf += 1;
+    if (d.nbuf == kDftBlock) dft_flush_plane<real>(s, d);
   }
   if (!s->probes.empty()) {
     Timer tm(s, FDTD_K_MON);
@@ -829,6 +1109,18 @@ int launch_step(Sim* s) {
   if (rc) return rc;
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_check(e, "kernel launch");
+  return 0;
+}
+
+int launch_dft_flush(Sim* s) {
+  for (auto& d : s->dfts) {
+    if (s->prec == FDTD_FP32)
+      dft_flush_plane<float>(s, d);
+    else
+      dft_flush_plane<double>(s, d);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_check(e, "DFT flush");
   return 0;
 }
 

@@ -135,6 +135,54 @@ def ref_c1_twin(nsteps=2000):
     }
 
 
+def sim3_shipped(nsteps=None):
+    """The shipped 3D case of sim3/program.f90:44-156,230-287 on one domain:
+    125 x 1266 x 25 cells of 3.999 nm, Ku = 0.5, y-PML 10 (x/z grading 10),
+    air / Au antenna / SiO2 / Si, 633 nm source, 100 frequencies over
+    500-800 nm on the incident, reflected and transmitted planes.  The same
+    parameters as fortran/host3d.f90."""
+    ds = 3.999 * NANO
+    nx, ny, nz = math.floor(500e-9 / ds), math.floor(5064e-9 / ds), math.floor(25 * ds / ds)
+    ic, kc = nx // 2, nz // 2
+    P = 10
+    lam0 = 633e-9
+    f0 = C0 / lam0
+    st = 1.0 / f0
+    air = (P + 1, int(0.8 * ny))
+    au_x = (int(ic - 0.5 * math.floor(100e-9 / ds) + 1), int(ic + 0.5 * math.floor(100e-9 / ds)))
+    au_y = (air[1] - math.floor(20e-9 / ds) + 1, air[1])
+    sio2 = (air[1] + 1, air[1] + 1 + math.floor(284e-9 / ds) - 1)
+    si = (sio2[1] + 1, ny - P)
+    ptty = int(0.5 * (si[1] + si[0]))
+    sposy = P + 2
+    ptiny = sposy + 2
+    dt = 0.5 * ds / C0
+    tfin = math.floor(((ds * (air[1] - sposy)) + 1.0 * 4.5 * lam0 / 1.0) / (C0 / 1.0) / dt)
+    arg = list(reference_frequencies(500e-9, 800e-9, 100))
+    comps = [EX, EZ, HX, HZ]
+    return {
+        "name": "sim3_shipped",
+        "tfinStop": tfin,
+        "grid": {"nx": nx, "ny": ny, "nz": nz, "ds": ds, "courant": 0.5, "periodic": [1, 0, 1]},
+        "pml": {"mode": "ref_y", "npml": [P, P, P], "order": 3.0, "R0": math.exp(-16.0),
+                "eps_low": 1.0, "eps_high": 3.88 ** 2},
+        "materials": [
+            {"type": "dielectric", "region": box((1, air[0], 1), (nx, air[1], nz)), "eps": 1.0},
+            {"type": "drude2cp", "region": box((au_x[0], au_y[0], 1), (au_x[1], au_y[1], nz)),
+             "params": gold_params_dict()},
+            {"type": "dielectric", "region": box((1, sio2[0], 1), (nx, sio2[1], nz)), "eps": 1.46 ** 2},
+            {"type": "dielectric", "region": box((1, si[0], 1), (nx, si[1], nz)), "eps": 3.88 ** 2},
+        ],
+        "sources": [{"axis": 1, "index": sposy, "comp": EX, "t0": 4.0 * st, "st": st, "w0": 2.0 * PI * f0,
+                     "amplitude": 1.0}],
+        "dft": [{"axis": 1, "index": ptiny, "omega": arg, "comps": comps, "n_begin": 1, "n_end": tfin - 1},
+                {"axis": 1, "index": ptiny, "omega": arg, "comps": comps, "n_begin": tfin + 1, "n_end": 0},
+                {"axis": 1, "index": ptty, "omega": arg, "comps": comps, "n_begin": 1, "n_end": 0}],
+        "probes": [[ic, ptiny, kc], [ic, ptty, kc]],
+        "nsteps": 4 * tfin if nsteps is None else nsteps,
+    }
+
+
 # ---------------------------------------------------------------------------
 # all-faces PML workloads (FUSED engine)
 def c1_gold_sphere(nsteps=2000, n=96):
