@@ -9,8 +9,11 @@ whole 512^3 grid; inputs are resident in HBM before the timed region.
 Prints ONE JSON line.  ``roofline`` is measured live: HIP events around every
 launch of the dominant kernel (the fused E update) on the handle's stream,
 its algorithmic bytes per launch divided by its mean duration.
-``cpu_baseline`` times the FP64 CPU oracle (OpenMP) on a few steps of the
-same 512^3 workload on this host.
+``cpu_baseline`` times the FP64 CPU oracle (OpenMP) on 60 steps of the
+same 512^3 workload on this host.  ``served_stream`` replays a seeded
+Poisson stream of smaller gold jobs (64-320 cells a side) through the
+library's job pool and reports throughput and p50/p99 job latency beside the
+same jobs run one at a time.
 """
 import argparse
 import json
@@ -90,6 +93,28 @@ def cpu_baseline(spec, steps):
                       f"{steps} timed steps after 1 warm-up step ({el:.1f} s)"}
 
 
+def served(args):
+    """Served job stream: Poisson arrivals replayed in real time through the
+    pool, then the same jobs one at a time (FP32, and the plain FP64 stepper)."""
+    import numpy as np
+    from fdtd_amd import serve
+
+    jobs = serve.job_stream(seed=args.seed, njobs=args.jobs, rate_hz=args.rate)
+    mix = serve.served_run(jobs, precision="fp32", max_active=args.max_active, chunk_steps=args.chunk)
+    seq32 = serve.sequential_run(jobs, precision="fp32")
+    seq64 = serve.sequential_run(jobs, precision="fp64") if not args.no_fp64 else None
+    same = all(np.array_equal(a, b) for a, b in zip(mix["flux"], seq32["flux"]))
+    out = {"jobs": len(jobs), "rate_hz": args.rate, "max_active": args.max_active, "chunk_steps": args.chunk,
+           "seconds": mix["seconds"], "mcells_per_s": mix["mcells_per_s"], "p50_ms": mix["p50_ms"],
+           "p99_ms": mix["p99_ms"], "sequential_fp32_mcells_per_s": seq32["mcells_per_s"],
+           "speedup_vs_sequential_fp32": mix["mcells_per_s"] / seq32["mcells_per_s"],
+           "results_identical_alone_and_mixed": bool(same)}
+    if seq64:
+        out["sequential_fp64_mcells_per_s"] = seq64["mcells_per_s"]
+        out["speedup_vs_sequential_fp64"] = mix["mcells_per_s"] / seq64["mcells_per_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,9 +122,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=60, help="timed oracle steps (about 15 s on 16 host cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--jobs", type=int, default=24)
+    ap.add_argument("--rate", type=float, default=20.0, help="served-stream arrival rate [jobs/s]")
+    ap.add_argument("--max-active", type=int, default=16)
+    ap.add_argument("--chunk", type=int, default=64)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run a plain device copy of this size (scales rocprof FETCH/WRITE counters)")
     args = ap.parse_args()
@@ -133,6 +163,11 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(spec, args.cpu_steps)
         print(f"[bench] cpu {cpu['value']:.1f} Mcell/s", file=sys.stderr, flush=True)
+    srv = None
+    if not args.no_serve:
+        srv = served(args)
+        print(f"[bench] served {srv['mcells_per_s']:.0f} Mcell/s p50 {srv['p50_ms']:.0f} ms p99 {srv['p99_ms']:.0f} ms",
+              file=sys.stderr, flush=True)
     line = {
         "metric": METRIC,
         "value": r32["value"],
@@ -154,7 +189,8 @@ def main():
                                           "dtype": "f64", "roofline": r64["roofline"]},
         "fp32_over_fp64": None if r64 is None else r32["value"] / r64["value"],
         "cpu_baseline": cpu,
-        "p99_job_latency_ms": None,
+        "p99_job_latency_ms": None if srv is None else srv["p99_ms"],
+        "served_stream": srv,
         "detail": {"fp32": {k: r32[k] for k in ("kernels", "h_update", "device_bytes", "cell_counts", "setup_s")},
                    "fp64": None if r64 is None else {k: r64[k] for k in ("kernels", "h_update", "device_bytes")}},
     }

@@ -411,19 +411,26 @@ int commit_impl(Sim* s) {
     }
     if (const char* e = std::getenv("FDTD_XCD_MAP")) s->xcdmap = std::atoi(e) != 0;
   }
-  // layout
-  s->px = ((int64_t)s->nx + 2 + 15) / 16 * 16;
+  // layout: rows are whole 128-byte lines and the first interior cell of
+  // every row starts a line (the field pointer is offset by line-1 cells
+  // inside its allocation), so each wavefront's 64 cells span the fewest lines
+  const int64_t le = 128 / (int64_t)s->wsize;  // cells per line
+  s->px = ((int64_t)s->nx + 2 + le - 1) / le * le;
   s->pxy = s->px * (s->ny + 2);
   s->ncell_alloc = s->pxy * (s->nz + 2);
-  if (s->ncell_alloc >= ((int64_t)1 << 31)) return set_error(FDTD_ERR_ARG, "grid too large for one handle");
-  const size_t fbytes = (size_t)s->ncell_alloc * s->wsize;
+  if (s->ncell_alloc + le >= ((int64_t)1 << 31)) return set_error(FDTD_ERR_ARG, "grid too large for one handle");
+  const size_t fbytes = (size_t)(s->ncell_alloc + le) * s->wsize;
+  const size_t shift = (size_t)(le - 1) * s->wsize;
   for (int q = 0; q < 3; ++q) {
-    s->E[q] = dev_alloc(s, fbytes);
-    s->H[q] = dev_alloc(s, fbytes);
-    if (!s->E[q] || !s->H[q]) return set_error(FDTD_ERR_NOMEM, "field allocation");
+    char* e = (char*)dev_alloc(s, fbytes);
+    char* h = (char*)dev_alloc(s, fbytes);
+    if (!e || !h) return set_error(FDTD_ERR_NOMEM, "field allocation");
+    s->E[q] = e + shift;
+    s->H[q] = h + shift;
     if (s->engine == FDTD_ENGINE_STAGED) {
-      s->D[q] = dev_alloc(s, fbytes);
-      if (!s->D[q]) return set_error(FDTD_ERR_NOMEM, "field allocation");
+      char* d = (char*)dev_alloc(s, fbytes);
+      if (!d) return set_error(FDTD_ERR_NOMEM, "field allocation");
+      s->D[q] = d + shift;
     }
   }
 
@@ -689,6 +696,45 @@ void* field_ptr(Sim* s, int comp) {
 }
 
 }  // namespace
+
+// Upper bound of the device bytes commit_impl allocates (admission control).
+size_t estimate_bytes(const Sim* s) {
+  const size_t w = s->prec == FDTD_FP32 ? 4 : 8;
+  const int64_t le = 128 / (int64_t)w;
+  const int64_t px = ((int64_t)s->nx + 2 + le - 1) / le * le;
+  const int64_t ncell = px * (s->ny + 2) * (s->nz + 2);
+  int eng = s->g.engine;
+  if (eng == FDTD_ENGINE_AUTO) eng = s->pml_mode == FDTD_PML_REF_Y ? FDTD_ENGINE_STAGED : FDTD_ENGINE_FUSED;
+  size_t b = (size_t)(eng == FDTD_ENGINE_STAGED ? 9 : 6) * (size_t)(ncell + le) * w;
+  b += (size_t)ncell + (size_t)(s->ny + 2) * (s->nz + 2) * 2;
+  const int n[3] = {s->nx, s->ny, s->nz};
+  if (s->pml_mode == FDTD_PML_ALL_FACES) {
+    const int Px = s->pml.npml[0], Py = s->pml.npml[1], Pz = s->pml.npml[2];
+    const int64_t np = (int64_t)2 * Pz * s->nx * s->ny + (int64_t)(s->nz - 2 * Pz) * 2 * Py * s->nx +
+                       (int64_t)(s->nz - 2 * Pz) * (s->ny - 2 * Py) * 2 * Px;
+    b += (size_t)S_N * (size_t)std::max<int64_t>(np, 1) * w + (size_t)4 * (n[0] + n[1] + n[2] + 6) * w;
+  } else if (s->pml_mode == FDTD_PML_REF_Y) {
+    b += (size_t)2 * S_N * (size_t)(s->nx + 2) * (s->pml.npml[1] + 2) * (s->nz + 2) * w;
+  }
+  for (const auto& m : s->mats) {
+    if (m.type == 0) continue;
+    int lo[3], hi[3];
+    if (!region_bounds(s, m.reg, lo, hi)) continue;
+    int64_t vol = 1;
+    for (int a = 0; a < 3; ++a) vol *= std::max(0, hi[a] - lo[a] + 1);
+    const int words = 1 + 2 + 2 * (m.type == 1 ? 2 : 4) + 2;
+    b += (size_t)vol * (3 * words * w + 5);
+  }
+  for (const auto& d : s->dfts) {
+    int ns = 0;
+    for (int c = 0; c < 6; ++c) ns += (d.mask >> c) & 1;
+    const size_t np = (size_t)d.n1 * d.n2;
+    b += 16 * (size_t)ns * d.nfreq * np + (size_t)kDftBlock * ns * np * w + 16 * (size_t)kDftBlock * d.nfreq;
+  }
+  b += (size_t)7 * 8 * 1024 * s->probes.size();
+  return b + ((size_t)1 << 20);
+}
+
 }  // namespace fdtd
 
 using namespace fdtd;
@@ -704,7 +750,6 @@ using namespace fdtd;
 
 extern "C" {
 
-struct fdtd_sim : public Sim {};
 
 const char* fdtd_last_error(void) { return g_err.c_str(); }
 
@@ -904,37 +949,55 @@ int fdtd_step(fdtd_sim* s, int nsteps) {
   if (rc) return rc;
   FDTD_HIP(hipSetDevice(s->device));
   const int np = (int)s->probes.size();
-  if (np && nsteps > s->probe_cap) {
-    s->d_probe = (double*)dev_alloc(s, sizeof(double) * 7 * (size_t)np * nsteps);
-    if (!s->d_probe) return set_error(FDTD_ERR_NOMEM, "probe buffer");
-    s->probe_cap = nsteps;
+  if (np && s->probe_pending + nsteps > s->probe_cap) {
+    // grow the device record buffer; records already written move along on
+    // the handle's stream (the old buffer is released with the handle)
+    const int cap = std::max(std::max(2 * s->probe_cap, s->probe_pending + nsteps), 1024);
+    double* nb = (double*)dev_alloc(s, sizeof(double) * 7 * (size_t)np * cap);
+    if (!nb) return set_error(FDTD_ERR_NOMEM, "probe buffer");
+    if (s->probe_pending)
+      FDTD_HIP(hipMemcpyAsync(nb, s->d_probe, sizeof(double) * 7 * (size_t)np * s->probe_pending,
+                              hipMemcpyDeviceToDevice, s->stream));
+    s->d_probe = nb;
+    s->probe_cap = cap;
   }
   s->call_base = s->n_done;
+  s->probe_row0 = s->probe_pending;
   for (int it = 0; it < nsteps; ++it) {
     rc = launch_step(s);
     if (rc) return rc;
     s->n_done += 1;
   }
-  // accumulators are complete whenever fdtd_step returns
-  rc = launch_dft_flush(s);
-  if (rc) return rc;
-  if (np && nsteps) {
-    std::vector<double> h(7 * (size_t)np * nsteps);
-    FDTD_HIP(hipMemcpyAsync(h.data(), s->d_probe, h.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-    FDTD_HIP(hipStreamSynchronize(s->stream));
-    for (int it = 0; it < nsteps; ++it)
-      for (int p = 0; p < np; ++p) {
-        const double* r = h.data() + 7 * ((size_t)it * np + p);
-        s->probes[p].rec.insert(s->probes[p].rec.end(), r, r + 7);
-      }
-  }
-  return flush_profile(s);
+  if (np) s->probe_pending += nsteps;
+  // DFT accumulators are complete once this call's work has run; the call
+  // itself only queues work on the handle's stream
+  return launch_dft_flush(s);
 }
+
+namespace fdtd {
+// Move probe records written since the last harvest to the host lists.
+static int harvest_probes(Sim* s) {
+  const int np = (int)s->probes.size();
+  if (!np || !s->probe_pending) return 0;
+  std::vector<double> h(7 * (size_t)np * s->probe_pending);
+  FDTD_HIP(hipMemcpyAsync(h.data(), s->d_probe, h.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  FDTD_HIP(hipStreamSynchronize(s->stream));
+  for (int it = 0; it < s->probe_pending; ++it)
+    for (int p = 0; p < np; ++p) {
+      const double* r = h.data() + 7 * ((size_t)it * np + p);
+      s->probes[p].rec.insert(s->probes[p].rec.end(), r, r + 7);
+    }
+  s->probe_pending = 0;
+  return 0;
+}
+}  // namespace fdtd
 
 int fdtd_sync(fdtd_sim* s) {
   GUARD(s);
   if (!s->stream) return 0;
   FDTD_HIP(hipStreamSynchronize(s->stream));
+  int rc = harvest_probes(s);
+  if (rc) return rc;
   return flush_profile(s);
 }
 
@@ -1003,6 +1066,8 @@ int fdtd_get_flux(fdtd_sim* s, int id, double* S) {
 int fdtd_get_probe_count(fdtd_sim* s, int id, int* nrec) {
   GUARD(s);
   if (!nrec || id < 0 || id >= (int)s->probes.size()) return set_error(FDTD_ERR_ARG, "bad argument");
+  int rc = harvest_probes(s);
+  if (rc) return rc;
   *nrec = (int)(s->probes[id].rec.size() / 7);
   return 0;
 }
@@ -1010,6 +1075,8 @@ int fdtd_get_probe_count(fdtd_sim* s, int id, int* nrec) {
 int fdtd_get_probe(fdtd_sim* s, int id, double* out) {
   GUARD(s);
   if (!out || id < 0 || id >= (int)s->probes.size()) return set_error(FDTD_ERR_ARG, "bad argument");
+  int rc = harvest_probes(s);
+  if (rc) return rc;
   std::memcpy(out, s->probes[id].rec.data(), s->probes[id].rec.size() * sizeof(double));
   return 0;
 }

@@ -154,9 +154,11 @@ struct Sim {
   void* d_ref_prof = nullptr;  // [2 slabs][12 profiles][max_n+2]
   int ref_prof_stride = 0;
   // probes
-  double* d_probe = nullptr;
+  double* d_probe = nullptr;     // [row][probe][7] records not yet harvested
   int32_t* d_probe_pos = nullptr;
-  int probe_cap = 0;
+  int probe_cap = 0;             // rows
+  int probe_pending = 0;         // rows written since the last harvest
+  int probe_row0 = 0;            // first row of the current fdtd_step call
   int call_base = 0;  // steps done before the current fdtd_step call
   // reductions
   double* d_red = nullptr;
@@ -206,4 +208,9 @@ int launch_energy(Sim* s, double* we, double* wh);
 int launch_flux(Sim* s, int id, double* S);
 int launch_set_ghosts(Sim* s, int comp);
 
+size_t estimate_bytes(const Sim* s);  // device bytes commit would allocate
+
 }  // namespace fdtd
+
+// The opaque handle of the C ABI is the simulation state itself.
+struct fdtd_sim : public fdtd::Sim {};

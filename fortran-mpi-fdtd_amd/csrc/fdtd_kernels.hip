@@ -33,6 +33,7 @@ struct FieldArgs {
   const Vec4<real>* coef;   // Ku*gax x,y,z, PML medium gax
   const Vec4<real>* gaxe;   // gax x,y,z, energy eps
   const uint32_t* flags;    // bit q: a dielectric writes E_q
+  const int16_t* rowmat;    // per ghosted (j,k) row: uniform id or -1
   int32_t nx, ny, nz;
   int32_t per[3];
   int32_t px, pxy;          // offsets fit 32 bits (checked at commit)
@@ -118,6 +119,27 @@ __device__ __forceinline__ int pml_index(const int32_t* P, int i, int j, int k, 
   return baseX + (km * (ny - 2 * Py) + jm) * 2 * Px + ii;
 }
 
+// Non-PML E update of one cell: E_q += Ku*gax_q*curl_q (+ gax_q * source)
+// for every component a dielectric writes.  UNI: `id` is wave-uniform.
+template <typename real, bool UNI>
+__device__ __forceinline__ void e_cell_update(const FieldArgs<real>& a, unsigned id, int i, int j, int k, int c,
+                                              real cx, real cy, real cz, const real* sv, const int* sa) {
+  const uint32_t fl = a.flags[id];
+  const Vec4<real> cf = a.coef[id];
+  const real cur[3] = {cx, cy, cz};
+  const bool any_src = sa[0] >= 0 || sa[1] >= 0 || sa[2] >= 0;
+  Vec4<real> gx;
+  if (any_src) gx = a.gaxe[id];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    if (!(fl & (1u << q))) continue;
+    real ev = a.e[q][c] + cf.v[q] * cur[q];
+    if (sa[q] >= 0) ev = ev + gx.v[q] * sv[q];
+    a.e[q][c] = ev;
+    put_e_ghost(a.e[q], ev, i, j, k, c, a);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // FUSED engine: E half step (myfdtd_func.f90:19-41 in E-form, material
 // update_E folded in, ALL_FACES split-field PML, plane sources).
@@ -135,7 +157,10 @@ __global__ __launch_bounds__(512) void k_fused_e(FieldArgs<real> a, PmlArgs<real
   const real hz_jm = a.h[2][c - a.px], hx_jm = a.h[0][c - a.px];
   const real hy_km = a.h[1][c - a.pxy], hx_km = a.h[0][c - a.pxy];
   const real hz_im = a.h[2][c - 1], hy_im = a.h[1][c - 1];
-  const unsigned id = a.matid[c];
+  // a row with one material id (most rows) gives a wave-uniform id: scalar
+  // table loads and no per-cell id gather
+  const int rm = __builtin_amdgcn_readfirstlane((int)a.rowmat[j + (a.ny + 2) * k]);
+  const unsigned id = rm >= 0 ? (unsigned)rm : (unsigned)a.matid[c];
   // plane sources: value and normal axis per E component (-1: none here)
   real sv[3] = {0, 0, 0};
   int sa[3] = {-1, -1, -1};
@@ -176,20 +201,10 @@ __global__ __launch_bounds__(512) void k_fused_e(FieldArgs<real> a, PmlArgs<real
   const real cx = (hz - hz_jm) + (-hy + hy_km);  // curlhzy + curlhyz
   const real cy = (hx - hx_km) + (-hz + hz_im);  // curlhxz + curlhzx
   const real cz = (hy - hy_im) + (-hx + hx_jm);  // curlhyx + curlhxy
-  const real cur[3] = {cx, cy, cz};
-  const uint32_t fl = a.flags[id];
-  const Vec4<real> cf = a.coef[id];
-  const bool any_src = sa[0] >= 0 || sa[1] >= 0 || sa[2] >= 0;
-  Vec4<real> gx;
-  if (any_src) gx = a.gaxe[id];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    if (!(fl & (1u << q))) continue;
-    real ev = a.e[q][c] + cf.v[q] * cur[q];
-    if (sa[q] >= 0) ev = ev + gx.v[q] * sv[q];
-    a.e[q][c] = ev;
-    put_e_ghost(a.e[q], ev, i, j, k, c, a);
-  }
+  if (rm >= 0)
+    e_cell_update<real, true>(a, (unsigned)rm, i, j, k, c, cx, cy, cz, sv, sa);
+  else
+    e_cell_update<real, false>(a, id, i, j, k, c, cx, cy, cz, sv, sa);
 }
 
 // FUSED engine: H half step (myfdtd_func.f90:45-64 + split-field H).
@@ -263,25 +278,6 @@ __device__ __forceinline__ bool seg_work(const SegArgs& sg, int ny, int& seg, in
   j = r % ny + 1;
   k = r / ny + 1;
   return true;
-}
-
-template <typename real, bool UNI>
-__device__ __forceinline__ void e_cell_update(const FieldArgs<real>& a, unsigned id, int i, int j, int k, int c,
-                                              real cx, real cy, real cz, const real* sv, const int* sa) {
-  const uint32_t fl = a.flags[id];
-  const Vec4<real> cf = a.coef[id];
-  const real cur[3] = {cx, cy, cz};
-  const bool any_src = sa[0] >= 0 || sa[1] >= 0 || sa[2] >= 0;
-  Vec4<real> gx;
-  if (any_src) gx = a.gaxe[id];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    if (!(fl & (1u << q))) continue;
-    real ev = a.e[q][c] + cf.v[q] * cur[q];
-    if (sa[q] >= 0) ev = ev + gx.v[q] * sv[q];
-    a.e[q][c] = ev;
-    put_e_ghost(a.e[q], ev, i, j, k, c, a);
-  }
 }
 
 template <typename real>
@@ -849,6 +845,7 @@ FieldArgs<real> field_args(Sim* s) {
   a.coef = (const Vec4<real>*)s->d_mattab;
   a.gaxe = a.coef + nt;
   a.flags = (const uint32_t*)(a.gaxe + nt);
+  a.rowmat = s->d_rowmat;
   a.nx = s->nx;
   a.ny = s->ny;
   a.nz = s->nz;
@@ -1095,7 +1092,7 @@ int step_impl(Sim* s) {
   if (!s->probes.empty()) {
     Timer tm(s, FDTD_K_MON);
     const int np = (int)s->probes.size();
-    double* out = s->d_probe + (int64_t)7 * np * (n - 1 - s->call_base);
+    double* out = s->d_probe + (int64_t)7 * np * (s->probe_row0 + (n - 1 - s->call_base));
     hipLaunchKernelGGL(k_probe<real>, dim3(1), dim3(64), 0, st, a, (const int32_t*)s->d_probe_pos, np, out,
                        (double)n * s->dt);
   }

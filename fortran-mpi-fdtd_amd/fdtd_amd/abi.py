@@ -138,7 +138,18 @@ GPU_PROTOTYPES = {
     "kernel_stats": (_I, [_P, _I, _PD, C.POINTER(C.c_int64)]),
     "algorithmic_bytes": (_I, [_P, _I, _PD]),
     "cell_counts": (_I, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "estimate_bytes": (_I, [_P, C.POINTER(C.c_size_t)]),
+    "pool_create": (_I, [C.POINTER(_P), C.c_size_t, _I, _I]),
+    "pool_destroy": (_I, [_P]),
+    "pool_submit": (_I, [_P, _P, _I, C.POINTER(_I)]),
+    "pool_progress": (_I, [_P, C.POINTER(_I), C.POINTER(_I)]),
+    "pool_status": (_I, [_P, _I, C.POINTER(_I), _PD, _PD]),
+    "pool_cancel": (_I, [_P, _I]),
+    "pool_release": (_I, [_P, _I]),
+    "pool_usage": (_I, [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
 }
+
+JOB_WAITING, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(5)
 
 
 class FdtdError(RuntimeError):

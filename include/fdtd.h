@@ -267,6 +267,34 @@ int fdtd_kernel_stats(fdtd_sim* sim, int kernel, double* total_ms,
  * splits; monitors excluded) and the cell mix it was computed from.        */
 int fdtd_algorithmic_bytes(fdtd_sim* sim, int kernel, double* bytes_per_launch);
 int fdtd_cell_counts(fdtd_sim* sim, int64_t* n_pml, int64_t* n_ade);
+/* Device bytes the handle holds (committed) or would allocate at commit. */
+int fdtd_estimate_bytes(fdtd_sim* sim, size_t* bytes);
+
+/* ---- serving: many independent simulations on one GPU ---------------------
+ * No counterpart in the reference (it runs one simulation per mpirun).
+ * A pool admits set-up handles in arrival order while their device bytes fit
+ * the HBM budget and fewer than max_active run; every admitted handle runs
+ * on its own stream and advances chunk_steps at a time, one chunk queued per
+ * handle per fdtd_pool_progress call, so co-resident jobs overlap on the
+ * device.  A finished job keeps its memory until fdtd_pool_release; the
+ * caller owns the handles (read results, then release, then destroy).      */
+typedef struct fdtd_pool fdtd_pool;
+enum {
+  FDTD_JOB_WAITING = 0, FDTD_JOB_RUNNING = 1, FDTD_JOB_DONE = 2,
+  FDTD_JOB_FAILED = 3, FDTD_JOB_CANCELLED = 4
+};
+int fdtd_pool_create(fdtd_pool** out, size_t hbm_budget, int max_active, int chunk_steps);
+int fdtd_pool_destroy(fdtd_pool* pool);
+/* FDTD_ERR_NOMEM when the job alone exceeds the budget. */
+int fdtd_pool_submit(fdtd_pool* pool, fdtd_sim* sim, int nsteps, int* ticket);
+/* One non-blocking scheduling pass: retire finished chunks, queue the next
+ * chunk of every running job, admit waiting jobs. */
+int fdtd_pool_progress(fdtd_pool* pool, int* n_running, int* n_waiting);
+int fdtd_pool_status(fdtd_pool* pool, int ticket, int* state, double* wait_ms, double* run_ms);
+/* Waiting: never runs.  Running: stops after its in-flight chunk. */
+int fdtd_pool_cancel(fdtd_pool* pool, int ticket);
+int fdtd_pool_release(fdtd_pool* pool, int ticket);
+int fdtd_pool_usage(fdtd_pool* pool, size_t* used, size_t* budget);
 
 #ifdef __cplusplus
 }

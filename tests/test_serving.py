@@ -1,0 +1,126 @@
+"""Serving layer: the served job stream, admission by HBM budget, overlap of
+co-resident jobs, cancellation, and identical results alone and in the mix."""
+import numpy as np
+import pytest
+
+
+def _small_jobs(n=4, seed=5):
+    """Served-stream jobs shrunk to 40-64 cells so the GPU tests stay short."""
+    from fdtd_amd import box, gold_params_dict, serve, sphere
+
+    jobs = []
+    for q in range(n):
+        spec = serve.job_spec(np.random.default_rng(seed + q))
+        side = 40 + 8 * q
+        c = (side + 1) / 2.0
+        spec["grid"].update(nx=side, ny=side, nz=side)
+        spec["materials"] = [{"type": "dielectric", "region": box((1, 1, 1), (side,) * 3), "eps": 1.0},
+                             {"type": "drude2cp", "region": sphere((c, c, c), 4.0 + q), "params": gold_params_dict()}]
+        spec["dft"][0]["index"] = side - 12
+        spec["probes"] = [[int(c), int(c), side - 13]]
+        spec["nsteps"] = 300 + 50 * q
+        jobs.append((0.0, spec))
+    return jobs
+
+
+def test_job_stream_is_reproducible_and_in_range():
+    from fdtd_amd import serve
+
+    a = serve.job_stream(seed=3, njobs=40)
+    b = serve.job_stream(seed=3, njobs=40)
+    assert [t for t, _ in a] == [t for t, _ in b]
+    assert all(x[1]["grid"]["nx"] == y[1]["grid"]["nx"] for x, y in zip(a, b))
+    for t, s in a:
+        n = s["grid"]["nx"]
+        assert 64 <= n <= 320
+        assert 1e-9 <= s["grid"]["ds"] <= 2e-9
+        assert 1500 <= s["nsteps"] <= 5000
+        assert 16 <= len(s["dft"][0]["omega"]) <= 64
+        assert s["pml"]["npml"] == [10, 10, 10]
+        assert {m["type"] for m in s["materials"]} == {"dielectric", "drude2cp"}
+    arrivals = np.array([t for t, _ in a])
+    assert (np.diff(arrivals) > 0).all()
+
+
+@pytest.mark.gpu
+def test_results_are_identical_alone_and_in_the_mix():
+    from fdtd_amd import serve
+
+    jobs = _small_jobs()
+    alone = serve.sequential_run(jobs)
+    mixed = serve.served_run(jobs, max_active=4, chunk_steps=32, realtime=False)
+    for a, b in zip(alone["flux"], mixed["flux"]):
+        assert np.array_equal(a, b)
+    assert all(v is not None for v in mixed["latency_ms"])
+
+
+@pytest.mark.gpu
+def test_admission_respects_the_budget_and_slots():
+    import fdtd_amd
+    from fdtd_amd import abi, serve
+
+    jobs = _small_jobs(n=4)
+    sims = [fdtd_amd.make_simulation(s)[0] for _, s in jobs]
+    est = [_estimate(s) for s in sims]
+    # every job fits alone; no three fit together (est[1] + est[2] > est[3])
+    budget = est[0] + est[3]
+    assert est[1] + est[2] > est[3]
+    pool = serve.Pool(budget, max_active=8, chunk_steps=16)
+    big = fdtd_amd.make_simulation(serve.job_spec(np.random.default_rng(0)))[0]
+    with pytest.raises(fdtd_amd.FdtdError):
+        serve.Pool(_estimate(big) - 1, max_active=1).submit(big, 10)
+    tickets = [pool.submit(s, spec["nsteps"]) for s, (_, spec) in zip(sims, jobs)]
+    seen_running = 0
+    import time
+
+    t_end = time.time() + 60
+    while time.time() < t_end:
+        r, w = pool.progress()
+        used, cap = pool.usage()
+        assert used <= cap
+        seen_running = max(seen_running, r)
+        states = [pool.status(t)[0] for t in tickets]
+        for t, st in zip(tickets, states):
+            if st == abi.JOB_DONE:
+                pool.release(t)
+        assert not any(st == abi.JOB_FAILED for st in states)
+        if all(st == abi.JOB_DONE for st in states):
+            break
+    assert all(pool.status(t)[0] == abi.JOB_DONE for t in tickets)
+    assert seen_running <= 2
+
+
+@pytest.mark.gpu
+def test_cancel_waiting_and_running_jobs():
+    import fdtd_amd
+    from fdtd_amd import abi, serve
+
+    jobs = _small_jobs(n=3)
+    sims = [fdtd_amd.make_simulation(s)[0] for _, s in jobs]
+    pool = serve.Pool(1 << 40, max_active=1, chunk_steps=8)
+    t = [pool.submit(s, 10000) for s in sims]
+    pool.progress()  # admits the first job only
+    assert pool.status(t[0])[0] == abi.JOB_RUNNING and pool.status(t[2])[0] == abi.JOB_WAITING
+    pool.cancel(t[2])
+    assert pool.status(t[2])[0] == abi.JOB_CANCELLED
+    pool.cancel(t[0])
+    for _ in range(10000):
+        pool.progress()
+        if pool.status(t[0])[0] == abi.JOB_CANCELLED:
+            break
+    assert pool.status(t[0])[0] == abi.JOB_CANCELLED
+    pool.release(t[0])
+    for _ in range(100000):
+        pool.progress()
+        if pool.status(t[1])[0] == abi.JOB_RUNNING:
+            break
+    assert pool.status(t[1])[0] == abi.JOB_RUNNING  # the freed slot went to the next job
+    pool.cancel(t[1])
+
+
+def _estimate(sim):
+    import ctypes as C
+
+    n = C.c_size_t()
+    sim.b.check(sim.b.estimate_bytes(sim.h, C.byref(n)))
+    return n.value
