@@ -74,6 +74,22 @@ def run_gpu(spec, precision, steps, warmup):
     return out
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_fused_e_gold_dimer_512.md")
+
+
+def profiled_traffic(kernel, prec):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
+    (FETCH_SIZE and WRITE_SIZE runs, each scaled by a 1 GiB copy's reading)."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            w = line.split()
+            if len(w) == 4 and w[0] == "traffic_bytes_per_launch" and w[1] == kernel and w[2] == prec:
+                return float(w[3])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(spec, steps):
     """FP64 CPU oracle (OpenMP) on `steps` steps of the same workload."""
     import oracle
@@ -168,6 +184,10 @@ def main():
         srv = served(args)
         print(f"[bench] served {srv['mcells_per_s']:.0f} Mcell/s p50 {srv['p50_ms']:.0f} ms p99 {srv['p99_ms']:.0f} ms",
               file=sys.stderr, flush=True)
+    if args.n == 512 and args.seed == 0:  # the workload the summary profiles
+        r32["roofline"]["traffic"] = profiled_traffic("k_fused_e", "fp32")
+        if r32["roofline"]["traffic"] is not None:
+            r32["roofline"]["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
     line = {
         "metric": METRIC,
         "value": r32["value"],

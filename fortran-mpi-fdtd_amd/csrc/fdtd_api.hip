@@ -539,24 +539,9 @@ int commit_impl(Sim* s) {
   s->d_mattab = s->prec == FDTD_FP32 ? upload_table<float>(s) : upload_table<double>(s);
   s->d_matid = (uint8_t*)upload_bytes(s, cid.data(), cid.size());
 
-  // segment form: x segments of at most 64 cells (x-PML cells apart) and the
-  // uniform material id of every (j,k) row (-1 where the row mixes ids)
+  // the uniform material id of every (j,k) row (-1 where the row mixes ids)
   {
-    std::vector<int32_t> seg;
-    std::vector<uint8_t> segpml;
-    const int Px = s->pml_mode == FDTD_PML_ALL_FACES ? s->P[0] : 0;
-    auto cut = [&](int lo, int hi, uint8_t pml) {
-      for (int x = lo; x <= hi; x += 64) {
-        seg.push_back(x);
-        seg.push_back(std::min(64, hi - x + 1));
-        segpml.push_back(pml);
-      }
-    };
-    if (Px > 0) cut(1, Px, 1);
-    cut(Px + 1, s->nx - Px, 0);
-    if (Px > 0) cut(s->nx - Px + 1, s->nx, 1);
-    s->nseg = (int)segpml.size();
-    std::vector<int16_t> rowmat((size_t)(s->ny + 2) * (s->nz + 2), -1);
+    std::vector<int32_t> rowmat((size_t)(s->ny + 2) * (s->nz + 2), -1);
     for (int k = 1; k <= s->nz; ++k)
       for (int j = 1; j <= s->ny; ++j) {
         const uint8_t* row = cid.data() + (size_t)(s->px * j + s->pxy * k);
@@ -564,11 +549,17 @@ int commit_impl(Sim* s) {
         for (int i = 2; i <= s->nx && uni; ++i) uni = row[i] == row[1];
         if (uni) rowmat[(size_t)j + (size_t)(s->ny + 2) * k] = row[1];
       }
-    s->d_seg = upload_bytes(s, seg.data(), seg.size() * sizeof(int32_t));
-    s->d_segpml = (uint8_t*)upload_bytes(s, segpml.data(), segpml.size());
-    s->d_rowmat = (int16_t*)upload_bytes(s, rowmat.data(), rowmat.size() * sizeof(int16_t));
-    if (!s->d_seg || !s->d_segpml || !s->d_rowmat) return set_error(FDTD_ERR_NOMEM, "segment tables");
-    if (const char* e = std::getenv("FDTD_SEG")) s->segmode = std::atoi(e) != 0;
+    s->d_rowmat = (int32_t*)upload_bytes(s, rowmat.data(), rowmat.size() * sizeof(int32_t));
+    if (!s->d_rowmat) return set_error(FDTD_ERR_NOMEM, "row material table");
+    std::vector<int32_t> mix;
+    for (int k = 1; k <= s->nz; ++k)
+      for (int j = 1; j <= s->ny; ++j)
+        if (rowmat[(size_t)j + (size_t)(s->ny + 2) * k] < 0) mix.push_back(j + (s->ny + 2) * k);
+    s->nmix = (int)mix.size();
+    if (s->nmix) {
+      s->d_mixrows = (int32_t*)upload_bytes(s, mix.data(), mix.size() * sizeof(int32_t));
+      if (!s->d_mixrows) return set_error(FDTD_ERR_NOMEM, "mixed row list");
+    }
   }
   if (!s->d_mattab || !s->d_matid) return set_error(FDTD_ERR_NOMEM, "material table");
 
@@ -706,7 +697,7 @@ size_t estimate_bytes(const Sim* s) {
   int eng = s->g.engine;
   if (eng == FDTD_ENGINE_AUTO) eng = s->pml_mode == FDTD_PML_REF_Y ? FDTD_ENGINE_STAGED : FDTD_ENGINE_FUSED;
   size_t b = (size_t)(eng == FDTD_ENGINE_STAGED ? 9 : 6) * (size_t)(ncell + le) * w;
-  b += (size_t)ncell + (size_t)(s->ny + 2) * (s->nz + 2) * 2;
+  b += (size_t)ncell + (size_t)(s->ny + 2) * (s->nz + 2) * 8;  // ids, row ids, mixed rows
   const int n[3] = {s->nx, s->ny, s->nz};
   if (s->pml_mode == FDTD_PML_ALL_FACES) {
     const int Px = s->pml.npml[0], Py = s->pml.npml[1], Pz = s->pml.npml[2];

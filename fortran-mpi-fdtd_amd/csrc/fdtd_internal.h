@@ -171,13 +171,9 @@ struct Sim {
   // whether fused kernels deal tiles to XCDs in contiguous runs
   int bx = 64, by = 4;
   int xcdmap = 1;
-  // segment form of the fused kernels (one 64-cell x segment per wavefront);
-  // measured slower than whole-row tiles on 512^3, kept selectable
-  int segmode = 0;
-  int nseg = 0;
-  void* d_seg = nullptr;        // int2 (first i, length) per segment
-  uint8_t* d_segpml = nullptr;  // segment inside an x PML slab
-  int16_t* d_rowmat = nullptr;  // uniform material id of a (j,k) row, or -1
+  int32_t* d_rowmat = nullptr;  // uniform material id of a (j,k) row, or -1
+  int32_t* d_mixrows = nullptr;  // rows j + (ny+2)*k whose material ids differ
+  int nmix = 0;
 
   // profiling
   bool profiling = false;

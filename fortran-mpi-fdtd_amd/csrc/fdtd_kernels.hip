@@ -33,7 +33,7 @@ struct FieldArgs {
   const Vec4<real>* coef;   // Ku*gax x,y,z, PML medium gax
   const Vec4<real>* gaxe;   // gax x,y,z, energy eps
   const uint32_t* flags;    // bit q: a dielectric writes E_q
-  const int16_t* rowmat;    // per ghosted (j,k) row: uniform id or -1
+  const int32_t* rowmat;    // per ghosted (j,k) row: uniform id or -1 (dword: scalar loads)
   int32_t nx, ny, nz;
   int32_t per[3];
   int32_t px, pxy;          // offsets fit 32 bits (checked at commit)
@@ -119,65 +119,31 @@ __device__ __forceinline__ int pml_index(const int32_t* P, int i, int j, int k, 
   return baseX + (km * (ny - 2 * Py) + jm) * 2 * Px + ii;
 }
 
-// Non-PML E update of one cell: E_q += Ku*gax_q*curl_q (+ gax_q * source)
-// for every component a dielectric writes.  UNI: `id` is wave-uniform.
-template <typename real, bool UNI>
-__device__ __forceinline__ void e_cell_update(const FieldArgs<real>& a, unsigned id, int i, int j, int k, int c,
-                                              real cx, real cy, real cz, const real* sv, const int* sa) {
-  const uint32_t fl = a.flags[id];
-  const Vec4<real> cf = a.coef[id];
-  const real cur[3] = {cx, cy, cz};
-  const bool any_src = sa[0] >= 0 || sa[1] >= 0 || sa[2] >= 0;
-  Vec4<real> gx;
-  if (any_src) gx = a.gaxe[id];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    if (!(fl & (1u << q))) continue;
-    real ev = a.e[q][c] + cf.v[q] * cur[q];
-    if (sa[q] >= 0) ev = ev + gx.v[q] * sv[q];
-    a.e[q][c] = ev;
-    put_e_ghost(a.e[q], ev, i, j, k, c, a);
-  }
-}
-
 // ---------------------------------------------------------------------------
 // FUSED engine: E half step (myfdtd_func.f90:19-41 in E-form, material
 // update_E folded in, ALL_FACES split-field PML, plane sources).
-template <typename real>
-__global__ __launch_bounds__(512) void k_fused_e(FieldArgs<real> a, PmlArgs<real> p, SrcArgs<real> src,
-                                                 int nsrc) {
-  int tx, ty, tz;
-  if (!tile_of(a, tx, ty, tz)) return;
-  const int i = tx * blockDim.x + threadIdx.x + 1;
-  const int j = ty * blockDim.y + threadIdx.y + 1;
-  const int k = tz + 1;
-  if (i > a.nx || j > a.ny) return;
+//
+// Every load is issued before the first store: the field pointers may alias
+// as far as the compiler knows, so a load placed after a store would wait
+// for it and cost one more memory round trip.  UNI: the row has one material
+// id `rm`, whose table entry is read with scalar loads; otherwise the entry
+// is gathered per cell.  The two forms are separate code paths so that no
+// merge of scalar and vector values makes the compiler wait for the loads.
+template <typename real, bool UNI>
+__device__ __forceinline__ void fused_e_cell(const FieldArgs<real>& a, const PmlArgs<real>& p, const real* sv,
+                                             const int* sa, int rm, int i, int j, int k) {
   const int c = i + a.px * j + a.pxy * k;
   const real hx = a.h[0][c], hy = a.h[1][c], hz = a.h[2][c];
   const real hz_jm = a.h[2][c - a.px], hx_jm = a.h[0][c - a.px];
   const real hy_km = a.h[1][c - a.pxy], hx_km = a.h[0][c - a.pxy];
   const real hz_im = a.h[2][c - 1], hy_im = a.h[1][c - 1];
-  // a row with one material id (most rows) gives a wave-uniform id: scalar
-  // table loads and no per-cell id gather
-  const int rm = __builtin_amdgcn_readfirstlane((int)a.rowmat[j + (a.ny + 2) * k]);
-  const unsigned id = rm >= 0 ? (unsigned)rm : (unsigned)a.matid[c];
-  // plane sources: value and normal axis per E component (-1: none here)
-  real sv[3] = {0, 0, 0};
-  int sa[3] = {-1, -1, -1};
-  for (int q = 0; q < nsrc; ++q) {
-    const int ax = src.axis[q];
-    const int pos = ax == 0 ? i : (ax == 1 ? j : k);
-    if (pos == src.index[q]) {
-      const int cq = src.comp[q];
-      if (cq == 0) { sv[0] += src.val[q]; sa[0] = ax; }
-      else if (cq == 1) { sv[1] += src.val[q]; sa[1] = ax; }
-      else { sv[2] += src.val[q]; sa[2] = ax; }
-    }
-  }
+  const unsigned id = UNI ? (unsigned)__builtin_amdgcn_readfirstlane(rm) : (unsigned)a.matid[c];
+  const uint32_t fl = a.flags[id];
+  const Vec4<real> cf = a.coef[id];
   if (in_pml(p.P, i, j, k, a.nx, a.ny, a.nz)) {
     // mod3/BERpmly_mod.f90:150-155 with the splits driven by total H
     const int s = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
-    const real r = a.coef[id].v[3];
+    const real r = cf.v[3];
     real eyx = p.f1[0][i] * p.s[S_EYX][s] + (p.f2[0][i] * r) * (-hz + hz_im);
     real ezx = p.f1[0][i] * p.s[S_EZX][s] + (p.f2[0][i] * r) * (hy + (-hy_im));
     real exy = p.f1[1][j] * p.s[S_EXY][s] + (p.f2[1][j] * r) * (hz + (-hz_jm));
@@ -198,104 +164,32 @@ __global__ __launch_bounds__(512) void k_fused_e(FieldArgs<real> a, PmlArgs<real
     a.e[2][c] = ezx + ezy;
     return;  // periodic axes carry no PML: no ghosts to refresh here
   }
-  const real cx = (hz - hz_jm) + (-hy + hy_km);  // curlhzy + curlhyz
-  const real cy = (hx - hx_km) + (-hz + hz_im);  // curlhxz + curlhzx
-  const real cz = (hy - hy_im) + (-hx + hx_jm);  // curlhyx + curlhxy
-  if (rm >= 0)
-    e_cell_update<real, true>(a, (unsigned)rm, i, j, k, c, cx, cy, cz, sv, sa);
-  else
-    e_cell_update<real, false>(a, id, i, j, k, c, cx, cy, cz, sv, sa);
-}
-
-// FUSED engine: H half step (myfdtd_func.f90:45-64 + split-field H).
-template <typename real>
-__global__ __launch_bounds__(512) void k_fused_h(FieldArgs<real> a, PmlArgs<real> p) {
-  int tx, ty, tz;
-  if (!tile_of(a, tx, ty, tz)) return;
-  const int i = tx * blockDim.x + threadIdx.x + 1;
-  const int j = ty * blockDim.y + threadIdx.y + 1;
-  const int k = tz + 1;
-  if (i > a.nx || j > a.ny) return;
-  const int c = i + a.px * j + a.pxy * k;
-  const real ex = a.e[0][c], ey = a.e[1][c], ez = a.e[2][c];
-  const real ey_kp = a.e[1][c + a.pxy], ex_kp = a.e[0][c + a.pxy];
-  const real ez_jp = a.e[2][c + a.px], ex_jp = a.e[0][c + a.px];
-  const real ez_ip = a.e[2][c + 1], ey_ip = a.e[1][c + 1];
-  if (in_pml(p.P, i, j, k, a.nx, a.ny, a.nz)) {
-    // mod3/BERpmly_mod.f90:186-191 with the splits driven by total E
-    const int s = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
-    real hyx = p.g1[0][i] * p.s[S_HYX][s] + p.g2[0][i] * (-ez + ez_ip);
-    real hzx = p.g1[0][i] * p.s[S_HZX][s] + p.g2[0][i] * (ey + (-ey_ip));
-    real hxy = p.g1[1][j] * p.s[S_HXY][s] + p.g2[1][j] * (ez + (-ez_jp));
-    real hzy = p.g1[1][j] * p.s[S_HZY][s] + p.g2[1][j] * (-ex + ex_jp);
-    real hxz = p.g1[2][k] * p.s[S_HXZ][s] + p.g2[2][k] * (-ey + ey_kp);
-    real hyz = p.g1[2][k] * p.s[S_HYZ][s] + p.g2[2][k] * (ex + (-ex_kp));
-    p.s[S_HYX][s] = hyx; p.s[S_HZX][s] = hzx; p.s[S_HXY][s] = hxy;
-    p.s[S_HZY][s] = hzy; p.s[S_HXZ][s] = hxz; p.s[S_HYZ][s] = hyz;
-    a.h[0][c] = hxy + hxz;
-    a.h[1][c] = hyz + hyx;
-    a.h[2][c] = hzx + hzy;
-    return;
+  // E_q += Ku*gax_q*curl_q (+ gax_q * source) for every component a
+  // dielectric writes
+  const real e0[3] = {a.e[0][c], a.e[1][c], a.e[2][c]};
+  const bool any_src = sa[0] >= 0 || sa[1] >= 0 || sa[2] >= 0;
+  Vec4<real> gx;
+  if (any_src) gx = a.gaxe[id];
+  const real cur[3] = {(hz - hz_jm) + (-hy + hy_km),   // curlhzy + curlhyz
+                       (hx - hx_km) + (-hz + hz_im),   // curlhxz + curlhzx
+                       (hy - hy_im) + (-hx + hx_jm)};  // curlhyx + curlhxy
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    if (!(fl & (1u << q))) continue;
+    real ev = e0[q] + cf.v[q] * cur[q];
+    if (sa[q] >= 0) ev = ev + gx.v[q] * sv[q];
+    a.e[q][c] = ev;
+    put_e_ghost(a.e[q], ev, i, j, k, c, a);
   }
-  const real curleyz = ey_kp - ey, curlezy = -ez_jp + ez;
-  const real curlezx = ez_ip - ez, curlexz = -ex_kp + ex;
-  const real curlexy = ex_jp - ex, curleyx = -ey_ip + ey;
-  const real hx = a.h[0][c] + a.ku * (curleyz + curlezy);
-  const real hy = a.h[1][c] + a.ku * (curlezx + curlexz);
-  const real hz = a.h[2][c] + a.ku * (curlexy + curleyx);
-  a.h[0][c] = hx;
-  a.h[1][c] = hy;
-  a.h[2][c] = hz;
-  put_h_ghost(a.h[0], hx, i, j, k, c, a);
-  put_h_ghost(a.h[1], hy, i, j, k, c, a);
-  put_h_ghost(a.h[2], hz, i, j, k, c, a);
 }
 
-// ---------------------------------------------------------------------------
-// FUSED engine, segment form.  A row of cells is cut into x segments of at
-// most 64 cells: the low x-PML cells, the interior in 64-cell pieces, the
-// high x-PML cells.  Each wavefront owns one segment of one (j,k) row, so
-// the PML / interior choice is uniform over a wavefront (no divergence at
-// the x slabs).  Rows whose interior cells all share one material id carry
-// it in `rowmat`; their table lookups are wave-uniform (scalar) loads.
-struct SegArgs {
-  const int2* seg;  // (first i, length) per segment
-  const uint8_t* segpml;  // 1: segment lies in an x PML slab
-  int32_t nseg;
-  int32_t nwork;    // nseg * ny * nz wavefronts of work
-  int32_t per;      // blocks per XCD run (XCD-contiguous dealing)
-  const int16_t* rowmat;  // per ghosted (j,k) row: uniform material id or -1
-  int32_t ry;       // row stride: rowmat[j + ry*k]
-};
-
-__device__ __forceinline__ bool seg_work(const SegArgs& sg, int ny, int& seg, int& j, int& k) {
-  const int b = blockIdx.x;
-  const int blk = (b & 7) * sg.per + (b >> 3);
-  const int w = blk * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (w >= sg.nwork) return false;
-  seg = w % sg.nseg;
-  const int r = w / sg.nseg;
-  j = r % ny + 1;
-  k = r / ny + 1;
-  return true;
-}
-
+// plane sources through cell (i,j,k): value and normal axis per E
+// component (-1: none here).  Called before any memory load is in flight (a
+// loop carrying per-lane values past outstanding loads makes the compiler
+// wait for them).
 template <typename real>
-__global__ __launch_bounds__(256) void k_seg_e(FieldArgs<real> a, PmlArgs<real> p, SrcArgs<real> src, int nsrc,
-                                               SegArgs sg) {
-  int seg, j, k;
-  if (!seg_work(sg, a.ny, seg, j, k)) return;
-  const int2 sd = sg.seg[seg];
-  const int lane = threadIdx.x & 63;
-  if (lane >= sd.y) return;
-  const int i = sd.x + lane;
-  const int c = i + a.px * j + a.pxy * k;
-  const real hx = a.h[0][c], hy = a.h[1][c], hz = a.h[2][c];
-  const real hz_jm = a.h[2][c - a.px], hx_jm = a.h[0][c - a.px];
-  const real hy_km = a.h[1][c - a.pxy], hx_km = a.h[0][c - a.pxy];
-  const real hz_im = a.h[2][c - 1], hy_im = a.h[1][c - 1];
-  real sv[3] = {0, 0, 0};
-  int sa[3] = {-1, -1, -1};
+__device__ __forceinline__ void plane_sources(const SrcArgs<real>& src, int nsrc, int i, int j, int k, real* sv,
+                                              int* sa) {
   for (int q = 0; q < nsrc; ++q) {
     const int ax = src.axis[q];
     const int pos = ax == 0 ? i : (ax == 1 ? j : k);
@@ -306,55 +200,60 @@ __global__ __launch_bounds__(256) void k_seg_e(FieldArgs<real> a, PmlArgs<real> 
       else { sv[2] += src.val[q]; sa[2] = ax; }
     }
   }
-  const int rm = __builtin_amdgcn_readfirstlane((int)sg.rowmat[j + sg.ry * k]);
-  const bool pml = sg.segpml[seg] || j <= p.P[1] || j > a.ny - p.P[1] || k <= p.P[2] || k > a.nz - p.P[2];
-  if (pml) {
-    const unsigned id = rm >= 0 ? (unsigned)rm : (unsigned)a.matid[c];
-    const int s = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
-    const real r = a.coef[id].v[3];
-    real eyx = p.f1[0][i] * p.s[S_EYX][s] + (p.f2[0][i] * r) * (-hz + hz_im);
-    real ezx = p.f1[0][i] * p.s[S_EZX][s] + (p.f2[0][i] * r) * (hy + (-hy_im));
-    real exy = p.f1[1][j] * p.s[S_EXY][s] + (p.f2[1][j] * r) * (hz + (-hz_jm));
-    real ezy = p.f1[1][j] * p.s[S_EZY][s] + (p.f2[1][j] * r) * (-hx + hx_jm);
-    real exz = p.f1[2][k] * p.s[S_EXZ][s] + (p.f2[2][k] * r) * (-hy + hy_km);
-    real eyz = p.f1[2][k] * p.s[S_EYZ][s] + (p.f2[2][k] * r) * (hx + (-hx_km));
-    if (sa[0] == 1) exy = exy + sv[0] * r;
-    if (sa[0] == 2) exz = exz + sv[0] * r;
-    if (sa[1] == 0) eyx = eyx + sv[1] * r;
-    if (sa[1] == 2) eyz = eyz + sv[1] * r;
-    if (sa[2] == 0) ezx = ezx + sv[2] * r;
-    if (sa[2] == 1) ezy = ezy + sv[2] * r;
-    p.s[S_EYX][s] = eyx; p.s[S_EZX][s] = ezx; p.s[S_EXY][s] = exy;
-    p.s[S_EZY][s] = ezy; p.s[S_EXZ][s] = exz; p.s[S_EYZ][s] = eyz;
-    a.e[0][c] = exy + exz;
-    a.e[1][c] = eyz + eyx;
-    a.e[2][c] = ezx + ezy;
-    return;
-  }
-  const real cx = (hz - hz_jm) + (-hy + hy_km);  // curlhzy + curlhyz
-  const real cy = (hx - hx_km) + (-hz + hz_im);  // curlhxz + curlhzx
-  const real cz = (hy - hy_im) + (-hx + hx_jm);  // curlhyx + curlhxy
-  if (rm >= 0)
-    e_cell_update<real, true>(a, (unsigned)rm, i, j, k, c, cx, cy, cz, sv, sa);
-  else
-    e_cell_update<real, false>(a, (unsigned)a.matid[c], i, j, k, c, cx, cy, cz, sv, sa);
 }
 
+// Rows with one material id (all but the rows that cross a particle
+// surface): the table entry is a wave-uniform scalar load.
 template <typename real>
-__global__ __launch_bounds__(256) void k_seg_h(FieldArgs<real> a, PmlArgs<real> p, SegArgs sg) {
-  int seg, j, k;
-  if (!seg_work(sg, a.ny, seg, j, k)) return;
-  const int2 sd = sg.seg[seg];
-  const int lane = threadIdx.x & 63;
-  if (lane >= sd.y) return;
-  const int i = sd.x + lane;
+__global__ __launch_bounds__(512) void k_fused_e(FieldArgs<real> a, PmlArgs<real> p, SrcArgs<real> src,
+                                                 int nsrc) {
+  int tx, ty, tz;
+  if (!tile_of(a, tx, ty, tz)) return;
+  const int i = tx * blockDim.x + threadIdx.x + 1;
+  // blockDim.x is a multiple of 64 (set at commit): a wavefront lies in one
+  // (j,k) row, so j and k are wave-uniform and row data are scalar loads
+  const int j = ty * blockDim.y + __builtin_amdgcn_readfirstlane((int)threadIdx.y) + 1;
+  const int k = tz + 1;
+  if (i > a.nx || j > a.ny) return;
+  const int rm = a.rowmat[__builtin_amdgcn_readfirstlane(j + (a.ny + 2) * k)];
+  if (rm < 0) return;  // a mixed row: k_fused_e_mixed updates it
+  real sv[3] = {0, 0, 0};
+  int sa[3] = {-1, -1, -1};
+  plane_sources(src, nsrc, i, j, k, sv, sa);
+  fused_e_cell<real, true>(a, p, sv, sa, rm, i, j, k);
+}
+
+// Rows that mix material ids: block b updates x tile b % ntx of row
+// rows[b / ntx] (row index j + (ny+2)*k), gathering table entries per cell.
+template <typename real>
+__global__ __launch_bounds__(256) void k_fused_e_mixed(FieldArgs<real> a, PmlArgs<real> p, SrcArgs<real> src,
+                                                       int nsrc, const int32_t* rows, int ntx) {
+  const int r = rows[blockIdx.x / ntx];
+  const int i = (blockIdx.x % ntx) * blockDim.x + threadIdx.x + 1;
+  const int j = r % (a.ny + 2), k = r / (a.ny + 2);
+  if (i > a.nx) return;
+  real sv[3] = {0, 0, 0};
+  int sa[3] = {-1, -1, -1};
+  plane_sources(src, nsrc, i, j, k, sv, sa);
+  fused_e_cell<real, false>(a, p, sv, sa, -1, i, j, k);
+}
+
+// FUSED engine: H half step (myfdtd_func.f90:45-64 + split-field H).
+template <typename real>
+__global__ __launch_bounds__(512) void k_fused_h(FieldArgs<real> a, PmlArgs<real> p) {
+  int tx, ty, tz;
+  if (!tile_of(a, tx, ty, tz)) return;
+  const int i = tx * blockDim.x + threadIdx.x + 1;
+  const int j = ty * blockDim.y + __builtin_amdgcn_readfirstlane((int)threadIdx.y) + 1;  // wave-uniform row
+  const int k = tz + 1;
+  if (i > a.nx || j > a.ny) return;
   const int c = i + a.px * j + a.pxy * k;
   const real ex = a.e[0][c], ey = a.e[1][c], ez = a.e[2][c];
   const real ey_kp = a.e[1][c + a.pxy], ex_kp = a.e[0][c + a.pxy];
   const real ez_jp = a.e[2][c + a.px], ex_jp = a.e[0][c + a.px];
   const real ez_ip = a.e[2][c + 1], ey_ip = a.e[1][c + 1];
-  const bool pml = sg.segpml[seg] || j <= p.P[1] || j > a.ny - p.P[1] || k <= p.P[2] || k > a.nz - p.P[2];
-  if (pml) {
+  if (in_pml(p.P, i, j, k, a.nx, a.ny, a.nz)) {
+    // mod3/BERpmly_mod.f90:186-191 with the splits driven by total E
     const int s = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
     real hyx = p.g1[0][i] * p.s[S_HYX][s] + p.g2[0][i] * (-ez + ez_ip);
     real hzx = p.g1[0][i] * p.s[S_HZX][s] + p.g2[0][i] * (ey + (-ey_ip));
@@ -930,7 +829,6 @@ inline dim3 tile_grid(const FieldArgs<real>& a) {
 }
 
 // 8 XCD runs of `per` blocks of 4 wavefronts (seg_work undoes the dealing)
-inline dim3 seg_grid(const SegArgs& sg) { return dim3((unsigned)(8 * sg.per)); }
 
 inline dim3 cell_grid(int nx, int nrows, int nz, dim3 blk) {
   return dim3((unsigned)((nx + blk.x - 1) / blk.x), (unsigned)((nrows + blk.y - 1) / blk.y), (unsigned)nz);
@@ -1000,20 +898,15 @@ int step_impl(Sim* s) {
   };
   if (s->engine == FDTD_ENGINE_FUSED) {
     PmlArgs<real> p = pml_args<real>(s);
-    SegArgs sg{};
-    sg.seg = (const int2*)s->d_seg;
-    sg.segpml = s->d_segpml;
-    sg.nseg = s->nseg;
-    sg.nwork = s->nseg * s->ny * s->nz;
-    sg.per = ((sg.nwork + 3) / 4 + 7) / 8;
-    sg.rowmat = s->d_rowmat;
-    sg.ry = s->ny + 2;
     {
       Timer tm(s, FDTD_K_E);
-      if (s->segmode)
-        hipLaunchKernelGGL(k_seg_e<real>, seg_grid(sg), dim3(256), 0, st, a, p, src, nsrc, sg);
-      else
-        hipLaunchKernelGGL(k_fused_e<real>, tile_grid(a), blk, 0, st, a, p, src, nsrc);
+      hipLaunchKernelGGL(k_fused_e<real>, tile_grid(a), blk, 0, st, a, p, src, nsrc);
+      if (s->nmix) {
+        const int bm = std::min(256, (s->nx + 63) / 64 * 64);
+        const int ntx = (s->nx + bm - 1) / bm;
+        hipLaunchKernelGGL(k_fused_e_mixed<real>, dim3((unsigned)(s->nmix * ntx)), dim3((unsigned)bm), 0, st, a, p,
+                           src, nsrc, (const int32_t*)s->d_mixrows, ntx);
+      }
     }
     for (auto& g : s->groups) {
       if (!g.ncells) continue;
@@ -1023,10 +916,7 @@ int step_impl(Sim* s) {
     }
     {
       Timer tm(s, FDTD_K_H);
-      if (s->segmode)
-        hipLaunchKernelGGL(k_seg_h<real>, seg_grid(sg), dim3(256), 0, st, a, p, sg);
-      else
-        hipLaunchKernelGGL(k_fused_h<real>, tile_grid(a), blk, 0, st, a, p);
+      hipLaunchKernelGGL(k_fused_h<real>, tile_grid(a), blk, 0, st, a, p);
     }
   } else {
     const bool refy = s->pml_mode == FDTD_PML_REF_Y;

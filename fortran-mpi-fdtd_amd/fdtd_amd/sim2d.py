@@ -1,0 +1,300 @@
+"""Host-side mirror of the reference's 2D TEz driver over include/fdtd2d.h.
+
+The reference's 2D driver (sim2/program.f90, sim2 =
+FDTDsimulations/2D/air-nanoAntennas_Si02-Sisubstrate/) sets up a y-slab
+split-field PML, materials applied in order, a soft line source on Dx, DFT
+lines of Ex and Hz, point traces, and steps a time loop.  ``Simulation2D``
+keeps that vocabulary; ``Batch2D`` steps many simulations together.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import C0, NANO, PI, _drude2cp, _graphene, gold_params_dict
+from . import abi
+from . import abi2d
+from .abi2d import DFT_EX, DFT_HZ, DX2, DY2, EX2, EY2, HZ2  # noqa: F401
+
+_binding2d = None
+
+
+def library2d():
+    """The HIP library's 2D entry points (fdtd2d_*).  Raises if not built."""
+    global _binding2d
+    if _binding2d is None:
+        from . import library
+        b = library()
+        _binding2d = abi2d.Binding2D(b.dll, "fdtd2d_", b.last_error)
+    return _binding2d
+
+
+def box2(lo, hi):
+    """1-based inclusive box (xinit, yinit) .. (xend, yend)."""
+    return {"kind": "box", "lo": [int(lo[0]), int(lo[1])], "hi": [int(hi[0]), int(hi[1])]}
+
+
+def disk(center, radius):
+    """Cells whose centres lie within `radius` (cells) of `center`."""
+    return {"kind": "disk", "center": [float(center[0]), float(center[1])], "radius": float(radius)}
+
+
+def _region2d(d):
+    r = abi2d.Region2D()
+    r.kind = abi2d.REGION_BOX if d["kind"] == "box" else abi2d.REGION_DISK
+    for a in range(2):
+        r.lo[a] = int(d.get("lo", [0, 0])[a])
+        r.hi[a] = int(d.get("hi", [0, 0])[a])
+        r.center[a] = float(d.get("center", [0.0, 0.0])[a])
+    r.radius = float(d.get("radius", 0.0))
+    return r
+
+
+class Simulation2D:
+    """One 2D TEz domain (HIP library by default, or the oracle's binding)."""
+
+    def __init__(self, nx, ny, ds, courant, precision="fp64", periodic_x=True, binding=None):
+        self.b = binding or library2d()
+        g = abi2d.Grid2D()
+        g.nx, g.ny = int(nx), int(ny)
+        g.precision = abi.FP32 if precision in ("fp32", abi.FP32) else abi.FP64
+        g.periodic_x = int(bool(periodic_x))
+        g.ds, g.courant = float(ds), float(courant)
+        self.grid = g
+        self.shape = (int(nx), int(ny))
+        self.h = C.c_void_p()
+        self.b.check(self.b.create(C.byref(self.h), C.byref(g)))
+        self._dfts = []
+
+    def close(self):
+        if self.h:
+            self.b.destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_pml(self, npml, order=3.0, R0=math.exp(-16.0), sigmam=0.0, eps_low=1.0, eps_high=1.0):
+        p = abi2d.Pml2D()
+        p.npml[0], p.npml[1] = int(npml[0]), int(npml[1])
+        p.order, p.R0, p.sigmam, p.eps_low, p.eps_high = order, R0, sigmam, eps_low, eps_high
+        self.b.check(self.b.set_pml(self.h, C.byref(p)))
+
+    def add_dielectric(self, region, eps):
+        self.b.check(self.b.add_dielectric(self.h, C.byref(_region2d(region)), float(eps)))
+
+    def add_drude2cp(self, region, params=None):
+        p = _drude2cp(params or gold_params_dict())
+        self.b.check(self.b.add_drude2cp(self.h, C.byref(_region2d(region)), C.byref(p)))
+
+    def add_graphene(self, region, params):
+        p = _graphene(params)
+        self.b.check(self.b.add_graphene(self.h, C.byref(_region2d(region)), C.byref(p)))
+
+    def add_line_source(self, j, t0, st, w0, amplitude=1.0):
+        p = abi.Pulse(t0, st, w0, amplitude)
+        self.b.check(self.b.add_line_source(self.h, int(j), C.byref(p)))
+
+    def add_dft_line(self, j, omega, mask=DFT_EX | DFT_HZ, n_begin=1, n_end=0):
+        om = np.ascontiguousarray(omega, dtype=np.float64)
+        i = C.c_int()
+        self.b.check(self.b.add_dft_line(self.h, int(j), len(om), om.ctypes.data_as(abi._PD), int(mask),
+                                         int(n_begin), int(n_end), C.byref(i)))
+        self._dfts.append(len(om))
+        return i.value
+
+    def add_probe(self, i, j):
+        r = C.c_int()
+        self.b.check(self.b.add_probe(self.h, int(i), int(j), C.byref(r)))
+        return r.value
+
+    def commit(self):
+        self.b.check(self.b.commit(self.h))
+
+    def step(self, n=1):
+        self.b.check(self.b.step(self.h, int(n)))
+
+    def sync(self):
+        self.b.check(self.b.sync(self.h))
+
+    @property
+    def steps_done(self):
+        n = C.c_int()
+        self.b.check(self.b.get_step(self.h, C.byref(n)))
+        return n.value
+
+    @property
+    def dt(self):
+        d = C.c_double()
+        self.b.check(self.b.get_dt(self.h, C.byref(d)))
+        return d.value
+
+    def field(self, comp):
+        """Interior field as an (nx, ny) Fortran-ordered float64 array."""
+        out = np.empty(self.shape, dtype=np.float64, order="F")
+        self.b.check(self.b.get_field(self.h, int(comp), out.ctypes.data_as(abi._PD)))
+        return out
+
+    def set_field(self, comp, values):
+        v = np.asfortranarray(values, dtype=np.float64)
+        assert v.shape == self.shape
+        self.b.check(self.b.set_field(self.h, int(comp), v.ctypes.data_as(abi._PD)))
+
+    def dft(self, line, comp):
+        nf = self._dfts[line]
+        re = np.empty((self.shape[0], nf), dtype=np.float64, order="F")
+        im = np.empty((self.shape[0], nf), dtype=np.float64, order="F")
+        self.b.check(self.b.get_dft(self.h, int(line), int(comp), re.ctypes.data_as(abi._PD),
+                                    im.ctypes.data_as(abi._PD)))
+        return re, im
+
+    def flux(self, line):
+        S = np.empty(self._dfts[line], dtype=np.float64)
+        self.b.check(self.b.get_flux(self.h, int(line), S.ctypes.data_as(abi._PD)))
+        return S
+
+    def probe(self, pid):
+        n = C.c_int()
+        self.b.check(self.b.get_probe_count(self.h, int(pid), C.byref(n)))
+        out = np.empty((n.value, 3), dtype=np.float64)
+        if n.value:
+            self.b.check(self.b.get_probe(self.h, int(pid), out.ctypes.data_as(abi._PD)))
+        return out
+
+
+class Batch2D:
+    """Many committed Simulation2D handles stepped together (HIP only)."""
+
+    def __init__(self, sims, binding=None):
+        self.b = binding or library2d()
+        self.sims = list(sims)
+        arr = (C.c_void_p * len(self.sims))(*[s.h.value for s in self.sims])
+        self.h = C.c_void_p()
+        self.b.check(self.b.batch_create(C.byref(self.h), arr, len(self.sims)))
+
+    def run(self, nsteps):
+        n = np.ascontiguousarray(nsteps, dtype=np.int32)
+        assert len(n) == len(self.sims)
+        self.b.check(self.b.batch_run(self.h, n.ctypes.data_as(C.POINTER(C.c_int32))))
+
+    def sync(self):
+        self.b.check(self.b.batch_sync(self.h))
+
+    def close(self):
+        if self.h:
+            self.b.batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def apply_spec2d(sim, spec):
+    p = spec.get("pml")
+    if p:
+        sim.set_pml(p["npml"], p.get("order", 3.0), p.get("R0", math.exp(-16.0)), p.get("sigmam", 0.0),
+                    p.get("eps_low", 1.0), p.get("eps_high", 1.0))
+    for m in spec.get("materials", []):
+        if m["type"] == "dielectric":
+            sim.add_dielectric(m["region"], m["eps"])
+        elif m["type"] == "drude2cp":
+            sim.add_drude2cp(m["region"], m.get("params") or gold_params_dict())
+        elif m["type"] == "graphene":
+            sim.add_graphene(m["region"], m["params"])
+        else:
+            raise ValueError(m["type"])
+    for s in spec.get("sources", []):
+        sim.add_line_source(s["j"], s["t0"], s["st"], s["w0"], s.get("amplitude", 1.0))
+    ids = [sim.add_dft_line(d["j"], np.array(d["omega"]), d.get("mask", DFT_EX | DFT_HZ), d.get("n_begin", 1),
+                            d.get("n_end", 0)) for d in spec.get("dft", [])]
+    pids = [sim.add_probe(*pr) for pr in spec.get("probes", [])]
+    return ids, pids
+
+
+def make_simulation2d(spec, precision="fp64", binding=None):
+    g = spec["grid"]
+    sim = Simulation2D(g["nx"], g["ny"], g["ds"], g["courant"], precision=precision,
+                       periodic_x=g.get("periodic_x", True), binding=binding)
+    ids, pids = apply_spec2d(sim, spec)
+    return sim, ids, pids
+
+
+# ---- workloads -------------------------------------------------------------------
+def ref2d_frequencies(lam_init, lam_end, nfreq):
+    """sim2/program.f90:121-126,294-298: arg(m) = 2*pi*freq with freq
+    advanced by repeated addition of df from c0/lam_end."""
+    finit = C0 / lam_end
+    fend = C0 / lam_init
+    df = (fend - finit) / nfreq
+    arg = []
+    freq = finit
+    for _ in range(nfreq):
+        arg.append(2.0 * PI * freq)
+        freq = freq + df
+    return np.array(arg)
+
+
+def ref2d_shipped(nsteps=None):
+    """The reference's shipped 2D case (sim2/program.f90:40-300): 125 x 1266
+    cells of 3.999 nm, Ku = 0.5, 10-cell y PML (air below, Si above), a gold
+    strip on air over SiO2 over Si, Dx line source at y = 12, DFT lines at
+    y = 14 (incident n < tfinStop, reflected n > tfinStop) and y = 1170
+    (transmitted), traces at x = 62 on both lines.  Parameters are computed
+    as the reference computes them (Fortran integer truncation included)."""
+    lambda0 = 633.0 * NANO
+    f0 = C0 / lambda0
+    ku = 0.5
+    ds = 3.999 * NANO
+    dt = ku * ds / C0
+    nx = math.floor(500.0 * NANO / ds)
+    ny = math.floor(5064.0 * NANO / ds)
+    ic = nx // 2
+    npml = 10
+    order = 3.0
+    R0 = math.exp(-16.0)
+    sigmam = (order + 1.0) * C0 * math.log(1.0 / R0) / (2.0 * npml * ds)
+    periodo = 1.0 / f0
+    w0 = 2.0 * PI * f0
+    st = 1.0 * periodo
+    t0 = 4.0 * st
+    sposy = npml + 2
+    air_y1 = int(0.8 * ny)
+    gl = math.floor(100.0 * NANO / ds)
+    au_x0 = int(ic - 0.5 * gl + 1)
+    au_x1 = int(ic + 0.5 * gl)
+    au_y0 = air_y1 - math.floor(20.0 * NANO / ds) + 1
+    sio2_y0 = air_y1 + 1
+    sio2_y1 = sio2_y0 + math.floor(284.0 * NANO / ds) - 1
+    si_y0 = sio2_y1 + 1
+    si_y1 = ny - npml
+    ptty = int(0.5 * (si_y0 + si_y1))
+    dist = ds * float(air_y1 - sposy) + 1.0 * 4.5 * lambda0 / 1.0
+    tfin = math.floor((dist / C0) / dt)
+    nt = 4 * tfin
+    arg = ref2d_frequencies(500e-9, 800e-9, 200)
+    return {
+        "name": "ref2d_shipped",
+        "grid": {"nx": nx, "ny": ny, "ds": ds, "courant": ku, "periodic_x": True},
+        "pml": {"npml": [npml, npml], "order": order, "sigmam": sigmam, "eps_low": 1.0, "eps_high": 3.88 ** 2},
+        "materials": [
+            {"type": "dielectric", "region": box2((1, npml), (nx, air_y1)), "eps": 1.0},
+            {"type": "drude2cp", "region": box2((au_x0, au_y0), (au_x1, air_y1))},
+            {"type": "dielectric", "region": box2((1, sio2_y0), (nx, sio2_y1)), "eps": 1.46 ** 2},
+            {"type": "dielectric", "region": box2((1, si_y0), (nx, si_y1)), "eps": 3.88 ** 2},
+        ],
+        "sources": [{"j": sposy, "t0": t0, "st": st, "w0": w0}],
+        "dft": [{"j": sposy + 2, "omega": arg, "n_begin": 1, "n_end": tfin - 1},
+                {"j": sposy + 2, "omega": arg, "n_begin": tfin + 1, "n_end": 0},
+                {"j": ptty, "omega": arg, "n_begin": 1, "n_end": 0}],
+        "probes": [[ic, sposy + 2], [ic, ptty]],
+        "nsteps": nt if nsteps is None else nsteps,
+        "tfinStop": tfin,
+        "frame_every": int(nt / 20.0),
+        "ic": ic,
+    }

@@ -1,0 +1,145 @@
+/*
+ * fdtd2d.h -- C ABI of the MI355X (gfx950) 2D TEz (Hz, Ex, Ey) Yee stepper:
+ * single simulations, and batches of many independent simulations stepped
+ * together (one launch per half step over all of them, the step loop
+ * replayed from a captured HIP graph).
+ *
+ * The library replaces the 2D driver of lsmayoral/Fortran-MPI-FDTD and the
+ * module API it calls (paths relative to the reference checkout):
+ *
+ *   sim2/  = FDTDsimulations/2D/air-nanoAntennas_Si02-Sisubstrate/
+ *   mod2/  = modules/src/2D/
+ *
+ * Conventions (shared with fdtd.h, whose status codes, precision enum and
+ * material/pulse structs this header reuses)
+ *   - Indices are 1-based, inclusive and global: interior cells are i=1..nx,
+ *     j=1..ny (sim2/program.f90:67, arrays (0:N_x+1,0:N_y+1)).
+ *   - Normalised units: D and E share one scale, Hz is scaled by the vacuum
+ *     impedance, Ku = c0*dt/ds multiplies every curl
+ *     (sim2/include/myfdtd_2Dfunc.f90:16-44).
+ *   - The main fields are updated on rows j = npml_y .. ny-npml_y; the y
+ *     slabs below and above hold the reference's split-field PML
+ *     (mod2/BERpmly_HzExEy_mod.f90) and hand Ex (rows 1..npml_y and
+ *     ny-npml_y+1..ny) and Hz (rows 1..npml_y-1 and ny-npml_y+1..ny) to the
+ *     main fields by copy (:144-191).  With periodic_x the main fields wrap
+ *     in x through their ghost columns (sim2/include/mympi_2Dfunc.f90,
+ *     interchange_E/H with PBC_x); the PML's split fields never wrap.
+ *   - Materials are applied in call order; where regions overlap the last
+ *     one added writes E (sim2/program.f90:345-348).
+ *
+ * All functions return 0 or a negative FDTD_ERR_* code; fdtd_last_error()
+ * gives the text of the last error on the calling thread.
+ */
+#ifndef FDTD_AMD_FDTD2D_H
+#define FDTD_AMD_FDTD2D_H
+
+#include "fdtd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* field components crossing the ABI */
+enum { FDTD2D_EX = 0, FDTD2D_EY = 1, FDTD2D_HZ = 2, FDTD2D_DX = 3, FDTD2D_DY = 4 };
+/* region kinds: box (the reference's xinit..yend) and disk (cell centres
+ * within radius of center, a cylinder seen end-on)                       */
+enum { FDTD2D_REGION_BOX = 0, FDTD2D_REGION_DISK = 1 };
+/* DFT line component mask bits */
+enum { FDTD2D_DFT_EX = 1, FDTD2D_DFT_HZ = 4 };
+
+typedef struct fdtd2d_sim fdtd2d_sim;
+typedef struct fdtd2d_batch fdtd2d_batch;
+
+typedef struct fdtd2d_grid {
+  int32_t nx, ny;        /* interior cells (sim2/program.f90:48-49)         */
+  int32_t precision;     /* FDTD_FP64 or FDTD_FP32 fields                  */
+  int32_t periodic_x;    /* PBC_x (sim2/program.f90:12)                    */
+  double ds;             /* cell size [m]                                   */
+  double courant;        /* Ku = c0*dt/ds (sim2/program.f90:43-45)         */
+} fdtd2d_grid;
+
+typedef struct fdtd2d_pml {
+  int32_t npml[2];       /* npmlx, npmly (sim2/program.f90:89-90)           */
+  double order;          /* grading exponent m                              */
+  double R0;             /* reflection target, used when sigmam <= 0        */
+  double sigmam;         /* > 0: taken as is (sim2/program.f90:93)          */
+  double eps_low;        /* relative permittivity beside the y-low slab     */
+  double eps_high;       /* ... beside the y-high slab (:237)               */
+} fdtd2d_pml;
+
+typedef struct fdtd2d_region {
+  int32_t kind;          /* FDTD2D_REGION_*                                  */
+  int32_t reserved;
+  int32_t lo[2], hi[2];  /* box: xinit,yinit .. xend,yend                   */
+  double center[2];      /* disk centre in cell units (cell i spans i-1/2..i+1/2) */
+  double radius;         /* disk radius in cells                            */
+} fdtd2d_region;
+
+/* ---- lifetime --------------------------------------------------------- */
+/* Replaces the static fields and grid parameters of sim2/program.f90:40-67
+ * and the rank layout of sim2/include/mympi_2Dfunc.f90 (one domain).      */
+int fdtd2d_create(fdtd2d_sim** out, const fdtd2d_grid* grid);
+int fdtd2d_destroy(fdtd2d_sim* sim);
+
+/* ---- set-up (before the first step) ------------------------------------ */
+/* pmly%initialize (mod2/BERpmly_HzExEy_mod.f90:229-247, profiles :249-426):
+ * y slabs of depth npml[1], x grading over npml[0] cells inside them.      */
+int fdtd2d_set_pml(fdtd2d_sim* sim, const fdtd2d_pml* pml);
+/* ConstantDielectric (mod2/simpledielectric_mod.f90:21-79): E = D/eps_r.   */
+int fdtd2d_add_dielectric(fdtd2d_sim* sim, const fdtd2d_region* region, double eps_r);
+/* Gold (mod2/gold_mod.f90:47-168): Drude + 2 critical points, Ex and Ey.   */
+int fdtd2d_add_drude2cp(fdtd2d_sim* sim, const fdtd2d_region* region, const fdtd_drude2cp* params);
+/* Graphene (mod2/graphene_mod.f90:56-236): Ey = Dy; Ex through the CP4 (or
+ * Kubo intraband Drude) ADE and the Raman recursion in the 2D module's
+ * central-difference form (:210-212).  model, thickness, CP table, Kubo
+ * and Raman fields as in fdtd.h; ref_init_order is ignored (the 2D module
+ * sets its exponentials before use).                                      */
+int fdtd2d_add_graphene(fdtd2d_sim* sim, const fdtd2d_region* region, const fdtd_graphene* params);
+/* Soft line source on Dx along row j, all x (sim2/program.f90:336-340):
+ * Dx(i,j) += amplitude*exp(-((t-t0)/st)^2/2)*sin(w0*t), t = (n-1)*dt.      */
+int fdtd2d_add_line_source(fdtd2d_sim* sim, int j, const fdtd_pulse* pulse);
+/* Running DFT of Ex and/or Hz along row j (sim2/program.f90:373-438): for
+ * steps n_begin <= n <= n_end, re += cos(omega*t)*F*dt, im += sin(omega*t)*F*dt,
+ * t = (n-1)*dt.  n_end <= 0: no upper gate.                                */
+int fdtd2d_add_dft_line(fdtd2d_sim* sim, int j, int nfreq, const double* omega, int comp_mask, int n_begin,
+                        int n_end, int* id);
+/* Time trace of (Ex, Hz) at (i,j) after each step (sim2/program.f90:376,397,417). */
+int fdtd2d_add_probe(fdtd2d_sim* sim, int i, int j, int* id);
+int fdtd2d_commit(fdtd2d_sim* sim);
+
+/* ---- time loop (sim2/program.f90:320-458) ------------------------------- */
+int fdtd2d_step(fdtd2d_sim* sim, int nsteps);
+int fdtd2d_sync(fdtd2d_sim* sim);
+int fdtd2d_get_step(fdtd2d_sim* sim, int* n_done);
+int fdtd2d_get_dt(fdtd2d_sim* sim, double* dt);
+
+/* ---- results -------------------------------------------------------------- */
+/* comp FDTD2D_*: nx*ny doubles, x fastest, interior cells.                 */
+int fdtd2d_get_field(fdtd2d_sim* sim, int comp, double* dst);
+int fdtd2d_set_field(fdtd2d_sim* sim, int comp, const double* src);
+/* DFT accumulators of line `id` for comp FDTD2D_EX or FDTD2D_HZ: nx*nfreq
+ * doubles each, x fastest, frequency slowest (real_exin(i,m)).             */
+int fdtd2d_get_dft(fdtd2d_sim* sim, int id, int comp, double* re, double* im);
+/* Sy(m) = -ds * sum_i (ReHz*ReEx + ImHz*ImEx) (sim2/program.f90:479-529).  */
+int fdtd2d_get_flux(fdtd2d_sim* sim, int id, double* S);
+int fdtd2d_get_probe_count(fdtd2d_sim* sim, int id, int* nsamples);
+/* 3 doubles per sample: n*dt, Ex, Hz.                                      */
+int fdtd2d_get_probe(fdtd2d_sim* sim, int id, double* trace);
+
+/* ---- batches (no reference counterpart: the reference runs one simulation
+ * per MPI job) ------------------------------------------------------------- */
+/* Pack `count` committed FP32 or FP64 simulations of one precision, each at
+ * step 0, into one batch.  The batch borrows the handles: they stay valid
+ * and readable through the calls above, but must not be stepped on their
+ * own while the batch lives.                                               */
+int fdtd2d_batch_create(fdtd2d_batch** out, fdtd2d_sim* const* sims, int count);
+/* Advance simulation q by nsteps[q] steps (each may differ; 0 leaves it). */
+int fdtd2d_batch_run(fdtd2d_batch* batch, const int32_t* nsteps);
+int fdtd2d_batch_sync(fdtd2d_batch* batch);
+int fdtd2d_batch_destroy(fdtd2d_batch* batch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -34,6 +34,27 @@ def binding():
     return _b
 
 
+_b2 = None
+
+
+def binding2d():
+    """The 2D restatement (fdtd2d_oracle.c) under the ofdtd2d_ prefix."""
+    global _b2
+    if _b2 is None:
+        from fdtd_amd import abi2d
+        dll = binding().dll
+        err = dll.ofdtd2d_last_error
+        err.restype = C.c_char_p
+        err.argtypes = []
+        _b2 = abi2d.Binding2D(dll, "ofdtd2d_", err, batches=False)
+    return _b2
+
+
+def make_simulation2d(spec, precision="fp64"):
+    from fdtd_amd.sim2d import make_simulation2d as ms
+    return ms(spec, precision=precision, binding=binding2d())
+
+
 def drude2cp_eps(params, w):
     """Closed-form Drude+2CP permittivity (misc/my_matfunctions_mod.f90:8-31)."""
     from fdtd_amd import _drude2cp
