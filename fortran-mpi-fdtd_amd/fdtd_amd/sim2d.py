@@ -226,6 +226,53 @@ def make_simulation2d(spec, precision="fp64", binding=None):
 
 
 # ---- workloads -------------------------------------------------------------------
+def sweep_instance(rng, side=None, nsteps=None, ds=1e-9, npml=16, nfreq=32):
+    """One simulation of the batched 2D sweep, drawn from `rng` in this order:
+    grid side (uniform 256-2048 cells), gold cylinder radius (uniform
+    10-80 nm), graphene Fermi level (uniform 0.1-0.6 eV), step count (uniform
+    3000-8000).  Layout along y: 16-cell PML, Dx line source, reflection
+    line, the gold disk at mid-height with a Kubo graphene line (tau 0.1 ps,
+    300 K) two cells above it, transmission line, PML; x periodic with the
+    reference's x grading inside the slabs.  `side`/`nsteps` override the
+    drawn values (the draws still happen, so the stream stays aligned)."""
+    from .workloads import pulse_for_band
+
+    s_draw = int(rng.integers(256, 2049))
+    r_nm = float(rng.uniform(10.0, 80.0))
+    ef = float(rng.uniform(0.1, 0.6))
+    n_draw = int(rng.integers(3000, 8001))
+    n = side or s_draw
+    steps = nsteps or n_draw
+    P = npml
+    c = (n + 1) / 2.0
+    r = min(r_nm * 1e-9 / ds, n / 2.0 - P - 8)
+    g_row = int(round(c + r)) + 2
+    gr = {"model": "kubo", "normal_axis": 1, "thickness": 3.4e-10, "eps_inf": 1.0, "fermi_ev": ef,
+          "tau": 1e-13, "temperature": 300.0, "wNL": 0.0, "dnuNL": 0.0, "xi3": 0.0}
+    pul = pulse_for_band(400e-9, 900e-9, 600e-9)
+    om = 2.0 * PI * C0 / np.linspace(400e-9, 900e-9, nfreq)
+    return {
+        "name": f"sweep2d_{n}",
+        "grid": {"nx": n, "ny": n, "ds": ds, "courant": 0.5, "periodic_x": True},
+        "pml": {"npml": [P, P], "order": 3.0, "R0": math.exp(-16.0), "eps_low": 1.0, "eps_high": 1.0},
+        "materials": [{"type": "dielectric", "region": box2((1, P), (n, n - P)), "eps": 1.0},
+                      {"type": "drude2cp", "region": disk((c, c), r)},
+                      {"type": "graphene", "region": box2((1, g_row), (n, g_row)), "params": gr}],
+        "sources": [dict(j=P + 2, **pul)],
+        "dft": [{"j": P + 4, "omega": om}, {"j": n - P - 4, "omega": om}],
+        "probes": [[int(c), n - P - 4]],
+        "nsteps": steps,
+        "radius_nm": r * ds / 1e-9,
+        "fermi_ev": ef,
+    }
+
+
+def sweep_stream(seed=0, count=256, side=None, nsteps=None):
+    """`count` sweep instances from one seed."""
+    rng = np.random.default_rng(seed)
+    return [sweep_instance(rng, side=side, nsteps=nsteps) for _ in range(count)]
+
+
 def ref2d_frequencies(lam_init, lam_end, nfreq):
     """sim2/program.f90:121-126,294-298: arg(m) = 2*pi*freq with freq
     advanced by repeated addition of df from c0/lam_end."""

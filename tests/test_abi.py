@@ -10,13 +10,27 @@ import pytest
 from conftest import PKG, ROOT
 
 HEADER = os.path.join(ROOT, "include", "fdtd.h")
+HEADER2D = os.path.join(ROOT, "include", "fdtd2d.h")
 
 
-def declared():
-    text = open(HEADER).read()
+def declared(header=HEADER, prefix="fdtd_"):
+    text = open(header).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    names = set(re.findall(r"\b(fdtd_[a-z0-9_]+)\s*\(", text))
+    names = set(re.findall(r"\b(" + prefix + r"[a-z0-9_]+)\s*\(", text))
     return sorted(names)
+
+
+def test_2d_header_library_oracle_and_prototypes_agree():
+    from fdtd_amd import abi2d
+
+    names = declared(HEADER2D, "fdtd2d_")
+    assert "fdtd2d_batch_run" in names and "fdtd2d_step" in names
+    dll = C.CDLL(os.path.join(PKG, "libfdtd_amd.so"))
+    assert not [n for n in names if not hasattr(dll, n)]
+    odll = C.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
+    assert not ["o" + n for n in names if "batch" not in n and not hasattr(odll, "o" + n)]
+    have = {"fdtd2d_" + n for n in list(abi2d.PROTOTYPES_2D) + list(abi2d.BATCH_PROTOTYPES_2D)}
+    assert set(names) == have
 
 
 def test_header_declares_the_abi():
@@ -66,14 +80,18 @@ def test_struct_layouts_match_the_header():
 
     from fdtd_amd import abi
 
-    code = ('#include "fdtd.h"\n#include <stdio.h>\nint main(void){printf("%zu %zu %zu %zu %zu %zu\\n",'
+    from fdtd_amd import abi2d
+
+    code = ('#include "fdtd2d.h"\n#include <stdio.h>\nint main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n",'
             "sizeof(fdtd_grid),sizeof(fdtd_pml),sizeof(fdtd_region),sizeof(fdtd_drude2cp),"
-            "sizeof(fdtd_graphene),sizeof(fdtd_pulse));return 0;}\n")
+            "sizeof(fdtd_graphene),sizeof(fdtd_pulse),sizeof(fdtd2d_grid),sizeof(fdtd2d_pml),"
+            "sizeof(fdtd2d_region));return 0;}\n")
     with tempfile.TemporaryDirectory() as t:
         src = os.path.join(t, "p.c")
         open(src, "w").write(code)
         exe = os.path.join(t, "p")
         subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), src, "-o", exe], check=True)
         sizes = [int(v) for v in subprocess.run([exe], capture_output=True, text=True).stdout.split()]
-    ours = [C.sizeof(t) for t in (abi.Grid, abi.Pml, abi.Region, abi.Drude2CP, abi.Graphene, abi.Pulse)]
+    ours = [C.sizeof(t) for t in (abi.Grid, abi.Pml, abi.Region, abi.Drude2CP, abi.Graphene, abi.Pulse,
+                                  abi2d.Grid2D, abi2d.Pml2D, abi2d.Region2D)]
     assert sizes == ours
