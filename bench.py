@@ -131,6 +131,74 @@ def served(args):
     return out
 
 
+def batched_2d(args):
+    """Batched 2D TEz sweep: `--b2-count` instances drawn from the seed
+    (sides 256-2048 cells, gold disk r 10-80 nm, Kubo graphene line E_F
+    0.1-0.6 eV, 16-cell PML, 32 wavelengths on 2 lines, FP32), all stepped
+    `--b2-steps` steps in one batch, then the same instances one at a time."""
+    import numpy as np
+    from fdtd_amd import sim2d
+
+    specs = sim2d.sweep_stream(seed=args.seed, count=args.b2_count, nsteps=args.b2_steps)
+    cells = sum(s["grid"]["nx"] * s["grid"]["ny"] for s in specs)
+    work = cells * args.b2_steps
+
+    def build():
+        sims = [sim2d.make_simulation2d(s, precision="fp32") for s in specs]
+        for s, _, _ in sims:
+            s.commit()
+        return sims
+
+    sims = build()
+    batch = sim2d.Batch2D([s for s, _, _ in sims])
+    batch.run([8] * len(sims))  # graph capture and first touch, untimed
+    batch.sync()
+    t0 = time.perf_counter()
+    batch.run([args.b2_steps] * len(sims))
+    batch.sync()
+    tb = time.perf_counter() - t0
+    fb = [s.flux(ids[1]) for s, ids, _ in sims]
+    batch.close()
+    for s, _, _ in sims:
+        s.close()
+    sims = build()
+    for s, _, _ in sims:
+        s.step(8)
+        s.sync()
+    t0 = time.perf_counter()
+    for s, _, _ in sims:
+        s.step(args.b2_steps)
+    for s, _, _ in sims:
+        s.sync()
+    ts = time.perf_counter() - t0
+    same = all(np.array_equal(a, s.flux(ids[1])) for a, (s, ids, _) in zip(fb, sims))
+    for s, _, _ in sims:
+        s.close()
+    # algorithmic words per main cell-update: E 7 (Hz, Dx, Dy rw, Ex, Ey), H 4
+    return {"instances": len(specs), "cells": cells, "steps": args.b2_steps, "dtype": "f32",
+            "batched_mcells_per_s": work / tb / 1e6, "one_at_a_time_mcells_per_s": work / ts / 1e6,
+            "speedup": ts / tb, "achieved_GBs_all_in": work * 11 * 4 / tb / 1e9,
+            "results_identical_batched_and_alone": bool(same)}
+
+
+def fp32_spectra_error():
+    """FP32 against FP64 spectra on the 96^3 gold sphere (all-faces PML,
+    2000 steps, 16 wavelengths): max over wavelengths of |S32 - S64| / max|S64|."""
+    import numpy as np
+    import fdtd_amd
+    from fdtd_amd import workloads
+
+    spec = workloads.c1_gold_sphere()
+    out = {}
+    for prec in ("fp64", "fp32"):
+        sim, ids, _ = fdtd_amd.make_simulation(spec, precision=prec)
+        sim.step(spec["nsteps"])
+        out[prec] = sim.flux(ids[0])
+        sim.close()
+    return {"workload": "gold_sphere_96 (2000 steps, 16 wavelengths)",
+            "max_rel_err": float(np.max(np.abs(out["fp32"] - out["fp64"])) / np.max(np.abs(out["fp64"])))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +210,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-2d", action="store_true")
+    ap.add_argument("--b2-count", type=int, default=64, help="batched 2D sweep: instances")
+    ap.add_argument("--b2-steps", type=int, default=1000, help="batched 2D sweep: timed steps")
     ap.add_argument("--jobs", type=int, default=24)
     ap.add_argument("--rate", type=float, default=20.0, help="served-stream arrival rate [jobs/s]")
     ap.add_argument("--max-active", type=int, default=16)
@@ -184,6 +255,12 @@ def main():
         srv = served(args)
         print(f"[bench] served {srv['mcells_per_s']:.0f} Mcell/s p50 {srv['p50_ms']:.0f} ms p99 {srv['p99_ms']:.0f} ms",
               file=sys.stderr, flush=True)
+    b2 = None
+    if not args.no_2d:
+        b2 = batched_2d(args)
+        print(f"[bench] batched 2d {b2['batched_mcells_per_s']:.0f} Mcell/s ({b2['speedup']:.2f}x one at a time)",
+              file=sys.stderr, flush=True)
+    spec_err = fp32_spectra_error()
     if args.n == 512 and args.seed == 0:  # the workload the summary profiles
         r32["roofline"]["traffic"] = profiled_traffic("k_fused_e", "fp32")
         if r32["roofline"]["traffic"] is not None:
@@ -208,9 +285,11 @@ def main():
         "fp64": None if r64 is None else {"value": r64["value"], "ms_per_step": r64["ms_per_step"],
                                           "dtype": "f64", "roofline": r64["roofline"]},
         "fp32_over_fp64": None if r64 is None else r32["value"] / r64["value"],
+        "fp32_spectra_error": spec_err,
         "cpu_baseline": cpu,
         "p99_job_latency_ms": None if srv is None else srv["p99_ms"],
         "served_stream": srv,
+        "batched_2d": b2,
         "detail": {"fp32": {k: r32[k] for k in ("kernels", "h_update", "device_bytes", "cell_counts", "setup_s")},
                    "fp64": None if r64 is None else {k: r64[k] for k in ("kernels", "h_update", "device_bytes")}},
     }

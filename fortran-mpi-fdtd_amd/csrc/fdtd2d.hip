@@ -1040,17 +1040,21 @@ int batch_run(Batch* b, const int32_t* nsteps) {
   int maxn = 0;
   for (int q = 0; q < count; ++q) {
     if (nsteps[q] < 0) return set_error(FDTD_ERR_ARG, "negative step count");
+    maxn = std::max(maxn, nsteps[q]);
+  }
+  if (maxn == 0) return 0;
+  // the previous run (of this or another batch of the same handles) must be
+  // done before a probe buffer is copied into a larger one
+  for (Sim* s : b->sims) {
+    int rc = sync_sim(s);
+    if (rc) return rc;
+  }
+  for (int q = 0; q < count; ++q) {
     Sim* s = b->sims[q];
     int rc = ensure_probe_cap(s, s->n_done + nsteps[q]);
     if (rc) return rc;
     b->ctl[2 * q] = s->n_done;
     b->ctl[2 * q + 1] = s->n_done + nsteps[q];
-    maxn = std::max(maxn, nsteps[q]);
-  }
-  if (maxn == 0) return 0;
-  for (Sim* s : b->sims) {  // another batch of the same handle may still run
-    int rc = sync_sim(s);
-    if (rc) return rc;
   }
   // descriptors again: a probe buffer may have moved (the graph reads them
   // from device memory, so it stays valid)

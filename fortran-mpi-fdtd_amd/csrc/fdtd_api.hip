@@ -41,12 +41,15 @@ int hip_check(hipError_t e, const char* what) {
   return set_error(FDTD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Stream-ordered on the handle's stream: allocating, clearing and (at
+// destroy) freeing never wait for other handles' work, so jobs that set up
+// or retire while others run do not drain the device.
 void* dev_alloc(Sim* s, size_t bytes) {
   if (bytes == 0) bytes = 16;
   void* p = nullptr;
-  if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
-  if (hipMemset(p, 0, bytes) != hipSuccess) {
-    (void)hipFree(p);
+  if (hipMallocAsync(&p, bytes, s->stream) != hipSuccess) return nullptr;
+  if (hipMemsetAsync(p, 0, bytes, s->stream) != hipSuccess) {
+    (void)hipFreeAsync(p, s->stream);
     return nullptr;
   }
   s->allocs.push_back({p, bytes});
@@ -248,17 +251,21 @@ double sigmam_for(const Sim* s, int P) {
   return (s->pml.order + 1.0) * K.c0 * std::log(1.0 / s->pml.R0) / (2.0 * P * s->ds);  // sim3/program.f90:102
 }
 
+void* upload_bytes(Sim* s, const void* data, size_t bytes);
+
 template <typename real>
 void* upload(Sim* s, const std::vector<double>& v) {
   std::vector<real> h(v.begin(), v.end());
-  void* p = dev_alloc(s, h.size() * sizeof(real));
-  if (p && !h.empty()) (void)hipMemcpy(p, h.data(), h.size() * sizeof(real), hipMemcpyHostToDevice);
-  return p;
+  return upload_bytes(s, h.data(), h.size() * sizeof(real));
 }
 
+// the copy is ordered after the allocation's clear on the handle's stream;
+// the host buffer is a temporary, so wait for the copy
 void* upload_bytes(Sim* s, const void* data, size_t bytes) {
   void* p = dev_alloc(s, bytes);
-  if (p && bytes) (void)hipMemcpy(p, data, bytes, hipMemcpyHostToDevice);
+  if (!p) return nullptr;
+  if (bytes && hipMemcpyAsync(p, data, bytes, hipMemcpyHostToDevice, s->stream) != hipSuccess) return nullptr;
+  if (hipStreamSynchronize(s->stream) != hipSuccess) return nullptr;
   return p;
 }
 
@@ -624,7 +631,7 @@ int commit_impl(Sim* s) {
     s->d_probe_pos = (int32_t*)upload_bytes(s, pos.data(), pos.size() * sizeof(int32_t));
     if (!s->d_probe_pos) return set_error(FDTD_ERR_NOMEM, "probe table");
   }
-  FDTD_HIP(hipDeviceSynchronize());
+  FDTD_HIP(hipStreamSynchronize(s->stream));
   s->committed = true;
   return 0;
 }
@@ -817,8 +824,11 @@ int fdtd_destroy(fdtd_sim* s) {
     (void)hipEventDestroy(pe.second.second);
   }
   for (auto ev : s->event_pool) (void)hipEventDestroy(ev);
-  for (auto& b : s->allocs) (void)hipFree(b.p);
-  if (s->stream) (void)hipStreamDestroy(s->stream);
+  for (auto& b : s->allocs) (void)hipFreeAsync(b.p, s->stream);
+  if (s->stream) {
+    (void)hipStreamSynchronize(s->stream);
+    (void)hipStreamDestroy(s->stream);
+  }
   delete s;
   return 0;
 }

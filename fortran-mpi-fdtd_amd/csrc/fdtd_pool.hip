@@ -2,8 +2,8 @@
 //
 // Jobs are set-up handles plus a step count.  A job waits until its device
 // footprint fits the HBM budget and a slot is free; admitted jobs each run on
-// their own HIP stream and advance in chunks, one chunk queued per job per
-// scheduling pass, so the chunks of co-resident jobs overlap on the device.
+// their own HIP stream and advance in chunks, up to two queued per job, so
+// the chunks of co-resident jobs overlap on the device.
 // A finished job keeps its memory (and its budget share) until the caller
 // releases it after reading its results.  No counterpart in the reference.
 #include <hip/hip_runtime.h>
@@ -31,11 +31,16 @@ struct Job {
   size_t bytes = 0;
   bool counted = false;
   double t_submit = 0, t_start = 0, t_done = 0;
-  hipEvent_t ev = nullptr;
-  bool ev_live = false;
+  // chunks queued and not yet seen complete: events ev[head], ev[head^1]
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int head = 0, inflight = 0;
   bool cancel = false;
   int err = 0;
 };
+
+// chunks kept queued per running job: the next one is on the device while
+// the host polls, so a busy host loop does not leave the GPU idle
+constexpr int kAhead = 2;
 
 }  // namespace
 }  // namespace fdtd
@@ -52,50 +57,52 @@ struct fdtd_pool {
 
 namespace {
 
-// Poll a running job; queue its next chunk once the previous one has run.
+// Poll a running job: retire completed chunks in order, then keep kAhead
+// chunks queued while steps remain.
 int advance(fdtd_pool* p, Job& j) {
   Sim* s = j.sim;
-  if (j.ev_live) {
-    const hipError_t q = hipEventQuery(j.ev);
-    if (q == hipErrorNotReady) return 0;
+  while (j.inflight > 0) {
+    const hipError_t q = hipEventQuery(j.ev[j.head]);
+    if (q == hipErrorNotReady) break;
     if (q != hipSuccess) {
       j.err = hip_check(q, "job stream");
       j.state = FDTD_JOB_FAILED;
       j.t_done = now_ms();
       return 0;
     }
-    j.ev_live = false;
+    j.head ^= 1;
+    --j.inflight;
   }
-  if (j.cancel) {
-    j.state = FDTD_JOB_CANCELLED;
-    j.t_done = now_ms();
+  if (j.cancel || j.left == 0) {
+    if (j.inflight == 0) {
+      j.state = j.cancel ? FDTD_JOB_CANCELLED : FDTD_JOB_DONE;
+      j.t_done = now_ms();
+    }
     return 0;
   }
-  if (j.left == 0) {
-    j.state = FDTD_JOB_DONE;
-    j.t_done = now_ms();
-    return 0;
+  while (j.inflight < kAhead && j.left > 0) {
+    const int k = std::min(j.left, p->chunk);
+    int rc = fdtd_step(j.sim, k);
+    if (rc) {
+      j.err = rc;
+      j.state = FDTD_JOB_FAILED;
+      j.t_done = now_ms();
+      return 0;
+    }
+    j.left -= k;
+    const int slot = (j.head + j.inflight) & 1;
+    if (!j.ev[slot] && hipEventCreateWithFlags(&j.ev[slot], hipEventDisableTiming) != hipSuccess) {
+      j.err = set_error(FDTD_ERR_HIP, "event create");
+      j.state = FDTD_JOB_FAILED;
+      return 0;
+    }
+    if (hipEventRecord(j.ev[slot], s->stream) != hipSuccess) {
+      j.err = set_error(FDTD_ERR_HIP, "event record");
+      j.state = FDTD_JOB_FAILED;
+      return 0;
+    }
+    ++j.inflight;
   }
-  const int k = std::min(j.left, p->chunk);
-  int rc = fdtd_step(j.sim, k);
-  if (rc) {
-    j.err = rc;
-    j.state = FDTD_JOB_FAILED;
-    j.t_done = now_ms();
-    return 0;
-  }
-  j.left -= k;
-  if (!j.ev && hipEventCreateWithFlags(&j.ev, hipEventDisableTiming) != hipSuccess) {
-    j.err = set_error(FDTD_ERR_HIP, "event create");
-    j.state = FDTD_JOB_FAILED;
-    return 0;
-  }
-  if (hipEventRecord(j.ev, s->stream) != hipSuccess) {
-    j.err = set_error(FDTD_ERR_HIP, "event record");
-    j.state = FDTD_JOB_FAILED;
-    return 0;
-  }
-  j.ev_live = true;
   return 0;
 }
 
@@ -122,10 +129,12 @@ int fdtd_pool_create(fdtd_pool** out, size_t hbm_budget, int max_active, int chu
 
 int fdtd_pool_destroy(fdtd_pool* p) {
   if (!p) return 0;
-  for (auto& j : p->jobs) {
-    if (j.ev_live) (void)hipEventSynchronize(j.ev);
-    if (j.ev) (void)hipEventDestroy(j.ev);
-  }
+  for (auto& j : p->jobs)
+    for (int e = 0; e < 2; ++e)
+      if (j.ev[e]) {
+        (void)hipEventSynchronize(j.ev[e]);
+        (void)hipEventDestroy(j.ev[e]);
+      }
   delete p;
   return 0;
 }

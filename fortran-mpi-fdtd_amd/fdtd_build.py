@@ -39,16 +39,23 @@ def build_library():
 
 
 def build_fortran_host():
-    """Fortran90 host (fortran/host3d.f90) over the C ABI via iso_c_binding."""
+    """Fortran90 hosts (fortran/host3d.f90, fortran/host2d.f90) over the C
+    ABI via iso_c_binding."""
     fdir = os.path.join(HERE, "fortran")
-    out = os.path.join(fdir, "host3d")
     mod = os.path.join(fdir, "_mod")
     os.makedirs(mod, exist_ok=True)
-    _run([FLANG, "-O2", "-c", os.path.join(fdir, "fdtd_c.f90"), "-module-dir", mod, "-o",
-          os.path.join(mod, "fdtd_c.o")])
-    _run([FLANG, "-O2", "-I", mod, os.path.join(fdir, "host3d.f90"), os.path.join(mod, "fdtd_c.o"),
-          "-L", HERE, "-lfdtd_amd", f"-Wl,-rpath,{HERE}", "-o", out])
-    return out
+    objs = []
+    for m in ("fdtd_c", "fdtd2d_c"):
+        obj = os.path.join(mod, m + ".o")
+        _run([FLANG, "-O2", "-c", os.path.join(fdir, m + ".f90"), "-module-dir", mod, "-I", mod, "-o", obj])
+        objs.append(obj)
+    outs = []
+    for host, deps in (("host3d", objs[:1]), ("host2d", objs)):
+        out = os.path.join(fdir, host)
+        _run([FLANG, "-O2", "-I", mod, os.path.join(fdir, host + ".f90"), *deps,
+              "-L", HERE, "-lfdtd_amd", f"-Wl,-rpath,{HERE}", "-o", out])
+        outs.append(out)
+    return outs[0]
 
 
 def build_oracles(with_reference=True):

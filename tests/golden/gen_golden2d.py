@@ -27,6 +27,7 @@ def main():
     out = {}
     for name in ("r0Syin", "r0Syr", "r2Syt"):
         out[name] = load(f"transforms/{name}.dat")            # lambda [nm], Sy
+    out["rta"] = load("transforms/RTA_2Dfdtd.data")            # lambda, R, T, A (calR_Sy.f90)
     out["infld"] = load("transforms/r0infld.dat")              # n*dt, Ex, Hz at (62,14), n < tfinStop
     out["rfld"] = load("transforms/r0rfld.dat")                # same point, n > tfinStop
     out["tfld"] = load("transforms/r2tfld.dat")                # (62,1170), every step

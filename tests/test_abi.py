@@ -31,6 +31,9 @@ def test_2d_header_library_oracle_and_prototypes_agree():
     assert not ["o" + n for n in names if "batch" not in n and not hasattr(odll, "o" + n)]
     have = {"fdtd2d_" + n for n in list(abi2d.PROTOTYPES_2D) + list(abi2d.BATCH_PROTOTYPES_2D)}
     assert set(names) == have
+    src = open(os.path.join(PKG, "fortran", "fdtd2d_c.f90")).read()
+    bound = set(re.findall(r'bind\(c,\s*name="(fdtd2d_[a-z0-9_]+)"\)', src))
+    assert bound and bound <= set(names)
 
 
 def test_header_declares_the_abi():

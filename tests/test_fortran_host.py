@@ -43,3 +43,39 @@ def test_fortran_host_matches_the_python_driven_library(tmp_path):
     inc = np.loadtxt(os.path.join(tmp_path, "r0infld.dat"))
     assert relerr(inc, sim.probe(pids[0])[: len(inc)]) < 1e-14
     assert np.abs(inc[:, 1]).max() > 0.1  # the pulse has passed the incident plane
+
+
+HOST2D = os.path.join(PKG, "fortran", "host2d")
+
+
+@pytest.mark.gpu
+def test_fortran_2d_host_writes_the_reference_files(tmp_path):
+    """fortran/host2d.f90 runs the reference's shipped 2D case through the
+    2D C ABI and writes the files the reference's 3-rank run wrote; each
+    equals the shipped file (golden fixture) to the parity bar."""
+    for d in ("transforms", "timesLiney", "timesGnuplotmap"):
+        os.makedirs(os.path.join(tmp_path, d))
+    res = subprocess.run([HOST2D, "13696", str(tmp_path), "1"], capture_output=True, text=True, timeout=600)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "3424" in res.stdout and "13696" in res.stdout
+    g = np.load(os.path.join(GOLDEN, "ref2d_shipped.npz"))
+    T = os.path.join(tmp_path, "transforms")
+    for name in ("r0Syin", "r0Syr", "r2Syt"):
+        f = np.loadtxt(os.path.join(T, name + ".dat"))
+        assert np.max(np.abs(f[:, 0] - g[name][:, 0])) <= 1e-9
+        assert relerr(f[:, 1], g[name][:, 1]) <= 1e-12, name
+    rta = np.loadtxt(os.path.join(T, "RTA_2Dfdtd.data"))
+    assert np.max(np.abs(rta[:, 1:] - g["rta"][:, 1:])) <= 1e-10
+    for name, key in (("r0infld", "infld"), ("r0rfld", "rfld"), ("r2tfld", "tfld")):
+        f = np.loadtxt(os.path.join(T, name + ".dat"))
+        assert f.shape == g[key].shape
+        for c in range(3):
+            assert relerr(f[:, c], g[key][:, c]) <= 1e-11, name
+    for fr in (5, 10, 20):
+        line = np.concatenate([np.loadtxt(os.path.join(tmp_path, "timesLiney", f"r{r}E_time{fr}.dat"))
+                               for r in range(3)])
+        for c in range(3):
+            assert relerr(line[:, c], g[f"line{fr}"][:, c]) <= 1e-11
+    m = np.loadtxt(os.path.join(tmp_path, "timesGnuplotmap", "r2E_time20.dat"))
+    assert np.array_equal(m[:, :2].astype(np.int32), g["map_ij"])
+    assert relerr(m[:, 2], g["map_hz"]) <= 1e-11
