@@ -163,6 +163,43 @@ typedef struct {
   double xi3;             /* Raman strength                                    */
 } fdtd_graphene;
 
+/* Lorentz poles in polarisation-current form (the Lorentz-Jp family:
+ * mod3/goldLorentzJp_mod.f90:37-91, siLorentzJp_mod.f90, titaniumJp_mod.f90,
+ * filmGoldLorentzJp_mod.f90, grapheneLorentzJp_mod.f90,
+ * graphene3LorentzJp1nm_mod.f90, NodispersiveGraphene.f90).  For every
+ * component q in jp_mask, with Ku' = dt*c0/ds and ga = 1/eps_inf:
+ *   E_q += Ku'*ga*(curlH_q - ds*JpT_q)          (JpT_q = sum_l Jp_q,l)
+ *   per pole l: P0 = xi_l*E_q/c0 (+ xi3*Q_q*E_q/c0 for l = 1 when nonlinear)
+ *               Jp_q,l = g1_l*Jp_q,l + g2_l*(P0 - P_q,l);  P_q,l += dt*Jp_q,l
+ *   g1 = (1 - dnu*dt)/(1 + dnu*dt),  g2 = w0^2*dt/(1 + dnu*dt)
+ *   nonlinear: G_q = g1NL*G_q + g2NL*((z0*E_q)^2 - Q_q);  Q_q += dt*G_q
+ * Components outside jp_mask are written as vacuum (E_q += Ku*curlH_q), as
+ * the graphene modules write Ey.  No reference driver calls update_EJp:
+ * overlapping regions follow the library's rule (last writer wins per
+ * component) instead of adding both updates.                                */
+typedef struct {
+  int32_t npols;          /* 1..4                                              */
+  int32_t jp_mask;        /* bit q: E_q carries the poles (7 = all)            */
+  int32_t nonlinear;      /* 1: graphene third-order term (wNL, dnuNL, xi3)    */
+  int32_t ez_uses_jx;     /* 1: E_z is driven by JpT_x, as
+                             mod3/grapheneLorentzJp_mod.f90:67-68 does         */
+  double eps_inf;         /* epinf                                             */
+  double xi[4];           /* xi1 (pole strengths)                              */
+  double w0[4];           /* w0l [rad/s]                                       */
+  double dnu[4];          /* dnu0l [rad/s]                                     */
+  double wNL, dnuNL, xi3; /* nonlinear oscillator (set by the caller)          */
+} fdtd_lorentz_jp;
+
+enum {
+  FDTD_LJP_GOLD = 0,          /* mod3/goldLorentzJp_mod.f90:22-26            */
+  FDTD_LJP_SI = 1,            /* mod3/siLorentzJp_mod.f90:22-26              */
+  FDTD_LJP_TITANIUM = 2,      /* mod3/titaniumJp_mod.f90:22-26               */
+  FDTD_LJP_FILM_GOLD = 3,     /* mod3/filmGoldLorentzJp_mod.f90:21-25        */
+  FDTD_LJP_GRAPHENE = 4,      /* mod3/grapheneLorentzJp_mod.f90:27-31        */
+  FDTD_LJP_GRAPHENE3_1NM = 5, /* mod3/graphene3LorentzJp1nm_mod.f90:27-31    */
+  FDTD_LJP_GRAPHENE_ND = 6    /* mod3/NodispersiveGraphene.f90               */
+};
+
 /* Gaussian-modulated soft source, sim3/program.f90:107-111,400:
  * g(t) = amplitude * exp(-0.5*((t-t0)/st)^2) * sin(w0*t), t = (n-1)*dt.     */
 typedef struct {
@@ -175,6 +212,12 @@ void fdtd_drude2cp_gold(fdtd_drude2cp* p);
 /* Graphene CP4 + Raman defaults of mod3/graphene_mod.f90:33-41 (the Raman
  * triple wNL, dnuNL, xi3 is left zero: the reference driver sets it).      */
 void fdtd_graphene_ref(fdtd_graphene* p);
+/* Pole tables of the Lorentz-Jp modules (FDTD_LJP_*; w0 and dnu converted
+ * with 2*pi*c0/micro as the modules do).  The graphene presets set
+ * nonlinear = 1 and leave wNL, dnuNL, xi3 zero (public fields the driver
+ * sets); FDTD_LJP_GRAPHENE sets ez_uses_jx = 1.  Returns 0, or
+ * FDTD_ERR_ARG for an unknown kind.                                         */
+int fdtd_lorentz_jp_preset(int32_t kind, fdtd_lorentz_jp* p);
 
 /* ---- lifetime ---------------------------------------------------------- */
 /* Replaces the static field arrays and grid parameters of
@@ -195,6 +238,12 @@ int fdtd_add_dielectric(fdtd_sim* sim, const fdtd_region* region, double eps_r);
 /* Gold%initialize_material + update_E, mod3/gold_mod.f90:50-195. */
 int fdtd_add_drude2cp(fdtd_sim* sim, const fdtd_region* region,
                       const fdtd_drude2cp* params);
+
+/* <material>_LJp%initialize_material + update_EJp of the Lorentz-Jp
+ * family (mod3/goldLorentzJp_mod.f90:37-163 and its siblings).  The update
+ * reads curl H directly (E form), so D is not used in these cells.          */
+int fdtd_add_lorentz_jp(fdtd_sim* sim, const fdtd_region* region,
+                        const fdtd_lorentz_jp* params);
 
 /* Graphene%initialize_material + update_E, mod3/graphene_mod.f90:59-283. */
 int fdtd_add_graphene(fdtd_sim* sim, const fdtd_region* region,
