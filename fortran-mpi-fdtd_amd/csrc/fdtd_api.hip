@@ -197,6 +197,90 @@ int graphene_coef(const Sim* s, const fdtd_graphene& p, AdeCoef<double>& c, doub
   return 0;
 }
 
+// Lorentz-Jp poles, mod3/goldLorentzJp_mod.f90:37-91,141-163 (g1, g2 of
+// initialize_GoldVars, evaluated inline by the film-gold and graphene
+// modules with the same expressions); Ku' = dt*c0/ds as the modules compute.
+// whether material m updates component q through a dispersive group
+bool is_ade_comp(const HostMaterial& m, int q) {
+  if (m.type == 0) return false;
+  if (m.type == 3) return (m.lj.jp_mask >> q) & 1;
+  return true;
+}
+
+void ljp_coef(const Sim* s, const fdtd_lorentz_jp& p, LjpCoef<double>& c) {
+  const Consts& K = consts();
+  const double dt = s->dt;
+  std::memset(&c, 0, sizeof c);
+  c.npols = p.npols;
+  c.jmask = p.jp_mask;
+  c.nl = p.nonlinear ? 1 : 0;
+  c.ezjx = p.ez_uses_jx ? 1 : 0;
+  c.ga = 1.0 / p.eps_inf;
+  c.kuga = (dt * K.c0 / s->ds) * c.ga;
+  c.ds = s->ds;
+  c.dt = dt;
+  c.c0 = K.c0;
+  c.z0 = std::sqrt(K.mu0 / K.ep0);
+  for (int l = 0; l < p.npols; ++l) {
+    c.xi[l] = p.xi[l];
+    c.g1[l] = (1.0 - p.dnu[l] * dt) / (1.0 + p.dnu[l] * dt);
+    c.g2[l] = (p.w0[l] * p.w0[l]) * dt / (1.0 + p.dnu[l] * dt);
+  }
+  if (c.nl) {
+    c.xi3 = p.xi3;
+    c.ng1 = (1.0 - p.dnuNL * dt) / (1.0 + p.dnuNL * dt);
+    c.ng2 = (p.wNL * p.wNL) * dt / (1.0 + p.dnuNL * dt);
+  }
+}
+
+// Profiles of a TFSF line's PML object (pmly_ExHz%initialize,
+// mod1/BERpmly_ExHz_mod.f90:156-255): the low slab of rank ycoord = 0 on
+// cells 1..npml and the high slab of rank Ymax-1 on n-npml+1..n, both in
+// the line's medium (eplow = ephigh = eps*ep0, sim1/program.f90:134).
+// out: f1, f2, g1, g2, each n+2 doubles.
+void tfsf_profiles(const Sim* s, const Tfsf& t, double* out) {
+  const Consts& K = consts();
+  const int N = t.n, np = t.p.npml;
+  const double dt = s->dt, ds = s->ds;
+  const double z0 = std::sqrt(K.mu0 / K.ep0);
+  const double Ku = dt * K.c0 / ds;
+  const double ep = t.p.eps * K.ep0, epr = ep / K.ep0;
+  const double order = t.p.order;
+  double* f1 = out;
+  double* f2 = out + (N + 2);
+  double* g1 = out + 2 * (N + 2);
+  double* g2 = out + 3 * (N + 2);
+  for (int j = 0; j <= N + 1; ++j) {
+    f1[j] = 1.0;
+    f2[j] = Ku / epr;
+    g1[j] = 1.0;
+    g2[j] = Ku;
+  }
+  if (np <= 0) return;
+  const double sigmam = (order + 1.0) * K.c0 * std::log(1.0 / t.p.R0) / (2.0 * np * ds);  // sim1/program.f90:132
+  for (int j = 1; j <= np; ++j) {
+    const double sE = ep * sigmam * std::pow(((double)(np - j) + 0.5) / (double)np, order);
+    f1[j] = std::exp(-(sE * dt / ep));
+    f2[j] = (1.0 - f1[j]) / (z0 * sE * ds);
+    if (j < np) {
+      const double sH = K.mu0 * sigmam * std::pow((double)(np - j) / (double)np, order);
+      g1[j] = std::exp(-(sH * dt / K.mu0));
+      g2[j] = z0 * (1.0 - g1[j]) / (sH * ds);
+    }
+  }
+  int jj = np;
+  for (int j = N - np + 1; j <= N; ++j, --jj) {
+    if (jj < np) {
+      const double sE = ep * sigmam * std::pow((double)(np - jj) / (double)np, order);
+      f1[j] = std::exp(-(sE * dt / ep));
+      f2[j] = (1.0 - f1[j]) / (z0 * sE * ds);
+    }
+    const double sH = K.mu0 * sigmam * std::pow(((double)(np - jj) + 0.5) / (double)np, order);
+    g1[j] = std::exp(-(sH * dt / K.mu0));
+    g2[j] = z0 * (1.0 - g1[j]) / (sH * ds);
+  }
+}
+
 // ---- PML profiles (reference formulas, one axis at a time) ----------------
 struct Prof {
   const Sim* s;
@@ -454,6 +538,25 @@ int commit_impl(Sim* s) {
   s->groups.clear();
   for (auto& m : s->mats) {
     if (m.type == 0) continue;
+    if (m.type == 3) {
+      LjpCoef<double> c;
+      ljp_coef(s, m.lj, c);
+      int gi = -1;
+      for (size_t q = 0; q < s->groups.size(); ++q)
+        if (s->groups[q].kind == 1 && std::memcmp(&s->groups[q].lj, &c, sizeof c) == 0) gi = (int)q;
+      if (gi < 0) {
+        AdeGroup g;
+        g.kind = 1;
+        std::memset(&g.coef, 0, sizeof g.coef);
+        g.lj = c;
+        g.eps_inf = m.lj.eps_inf;
+        g.nstate = 2 * c.npols + (c.nl ? 2 : 0);  // P_l, Jp_l, [Q, G]
+        s->groups.push_back(g);
+        gi = (int)s->groups.size() - 1;
+      }
+      m.group = gi;
+      continue;
+    }
     AdeCoef<double> c;
     double einf = 1.0;
     if (m.type == 1) {
@@ -464,7 +567,7 @@ int commit_impl(Sim* s) {
     }
     int gi = -1;
     for (size_t q = 0; q < s->groups.size(); ++q)
-      if (std::memcmp(&s->groups[q].coef, &c, sizeof c) == 0) gi = (int)q;
+      if (s->groups[q].kind == 0 && std::memcmp(&s->groups[q].coef, &c, sizeof c) == 0) gi = (int)q;
     if (gi < 0) {
       AdeGroup g;
       g.coef = c;
@@ -529,6 +632,12 @@ int commit_impl(Sim* s) {
         t.act[c] = ACT_DIEL;
         t.gax[c] = gax;
         t.coef[c] = s->ku * gax;
+      } else if (m.type == 3 && !(m.lj.jp_mask & (1 << c))) {
+        // a Lorentz-Jp material writes its other components as vacuum
+        // (mod3/grapheneLorentzJp_mod.f90:66: Ey += Ku*curl)
+        t.act[c] = ACT_DIEL;
+        t.gax[c] = 1.0;
+        t.coef[c] = s->ku;
       } else {
         t.act[c] = ACT_ADE;
       }
@@ -585,7 +694,7 @@ int commit_impl(Sim* s) {
         if (pm) continue;
         int gm[3] = {-1, -1, -1};
         for (int q = 0; q < 3; ++q)
-          if (key[q] >= 0 && s->mats[key[q]].type != 0) gm[q] = s->mats[key[q]].group;
+          if (key[q] >= 0 && is_ade_comp(s->mats[key[q]], q)) gm[q] = s->mats[key[q]].group;
         bool any = false;
         for (size_t g = 0; g < s->groups.size(); ++g) {
           uint8_t mk = 0;
@@ -606,8 +715,53 @@ int commit_impl(Sim* s) {
     if (!G.ncells) continue;
     G.d_cells = (int32_t*)upload_bytes(s, gcells[g].data(), gcells[g].size() * sizeof(int32_t));
     G.d_mask = (uint8_t*)upload_bytes(s, gmask[g].data(), gmask[g].size());
-    G.d_state = dev_alloc(s, (size_t)G.nstate * 3 * (size_t)G.ncells * s->wsize);
+    // Lorentz-Jp pole state is FP64 in both precisions (k_ljp)
+    G.d_state = dev_alloc(s, (size_t)G.nstate * 3 * (size_t)G.ncells * (G.kind == 1 ? 8 : s->wsize));
     if (!G.d_cells || !G.d_mask || !G.d_state) return set_error(FDTD_ERR_NOMEM, "dispersive state");
+  }
+
+  // TFSF incident lines
+  {
+    int used = (int)s->srcs.size();
+    for (auto& t : s->tfsfs) used += t.p.hi > 0 ? 2 : 1;
+    if (used > kMaxSrc) return set_error(FDTD_ERR_ARG, "TFSF corrections and sources exceed 8 source slots");
+  }
+  for (auto& t : s->tfsfs) {
+    const int a = t.p.axis;
+    const int n[3] = {s->nx, s->ny, s->nz};
+    int P = 0;
+    if (s->pml_mode == FDTD_PML_ALL_FACES) P = s->P[a];
+    if (s->pml_mode == FDTD_PML_REF_Y) {
+      if (a != 1) return set_error(FDTD_ERR_UNSUPPORTED, "with the reference y PML a TFSF wave runs along y");
+      P = s->pml.npml[1];
+    }
+    if (t.p.lo - 1 <= P || (t.p.hi > 0 && t.p.hi + 1 > n[a] - P) || (t.p.hi == 0 && t.p.lo > n[a] - P))
+      return set_error(FDTD_ERR_ARG, "TFSF planes inside the PML");
+    // Lorentz-Jp cells read curl H directly: keep them off the D correction planes
+    for (const auto& m : s->mats) {
+      if (m.type != 3) continue;
+      int lo[3], hi[3];
+      if (!region_bounds(s, m.reg, lo, hi)) continue;
+      const int pl[2] = {t.p.lo, t.p.hi > 0 ? t.p.hi + 1 : -1};
+      for (int q = 0; q < 2; ++q)
+        if (pl[q] >= lo[a] && pl[q] <= hi[a]) return set_error(FDTD_ERR_UNSUPPORTED, "TFSF plane through Lorentz-Jp cells");
+    }
+    std::vector<double> line((size_t)8 * (t.n + 2), 0.0);
+    tfsf_profiles(s, t, line.data() + (size_t)4 * (t.n + 2));
+    t.d_line = (double*)upload_bytes(s, line.data(), line.size() * sizeof(double));
+    if (!t.d_line) return set_error(FDTD_ERR_NOMEM, "TFSF line");
+    std::vector<int32_t> lp;
+    const int tid = (int)(&t - s->tfsfs.data());
+    for (size_t q = 0; q < s->probes.size(); ++q)
+      if (s->probes[q].line == tid) {
+        lp.push_back((int32_t)q);
+        lp.push_back(s->probes[q].j);
+      }
+    t.nlp = (int)lp.size() / 2;
+    if (t.nlp) {
+      t.d_lp = (int32_t*)upload_bytes(s, lp.data(), lp.size() * sizeof(int32_t));
+      if (!t.d_lp) return set_error(FDTD_ERR_NOMEM, "TFSF probes");
+    }
   }
 
   // monitors
@@ -624,7 +778,7 @@ int commit_impl(Sim* s) {
   if (!s->probes.empty()) {
     std::vector<int32_t> pos;
     for (auto& p : s->probes) {
-      pos.push_back(p.i);
+      pos.push_back(p.line >= 0 ? -1 : p.i);  // k_probe skips incident-line probes
       pos.push_back(p.j);
       pos.push_back(p.k);
     }
@@ -720,8 +874,8 @@ size_t estimate_bytes(const Sim* s) {
     if (!region_bounds(s, m.reg, lo, hi)) continue;
     int64_t vol = 1;
     for (int a = 0; a < 3; ++a) vol *= std::max(0, hi[a] - lo[a] + 1);
-    const int words = 1 + 2 + 2 * (m.type == 1 ? 2 : 4) + 2;
-    b += (size_t)vol * (3 * words * w + 5);
+    const int words = m.type == 3 ? 2 * m.lj.npols + 2 : 1 + 2 + 2 * (m.type == 1 ? 2 : 4) + 2;
+    b += (size_t)vol * (3 * words * (m.type == 3 ? 8 : w) + 5);
   }
   for (const auto& d : s->dfts) {
     int ns = 0;
@@ -730,6 +884,7 @@ size_t estimate_bytes(const Sim* s) {
     b += 16 * (size_t)ns * d.nfreq * np + (size_t)kDftBlock * ns * np * w + 16 * (size_t)kDftBlock * d.nfreq;
   }
   b += (size_t)7 * 8 * 1024 * s->probes.size();
+  for (const auto& t : s->tfsfs) b += (size_t)8 * 8 * (t.n + 2);
   return b + ((size_t)1 << 20);
 }
 
@@ -888,6 +1043,63 @@ int fdtd_add_graphene(fdtd_sim* s, const fdtd_region* r, const fdtd_graphene* p)
   return add_material(s, r, m);
 }
 
+int fdtd_add_lorentz_jp(fdtd_sim* s, const fdtd_region* r, const fdtd_lorentz_jp* p) {
+  GUARD(s);
+  SETUP_ONLY(s);
+  if (!p || p->npols < 1 || p->npols > 4 || p->jp_mask < 1 || p->jp_mask > 7 || !(p->eps_inf > 0))
+    return set_error(FDTD_ERR_ARG, "bad Lorentz-Jp parameters");
+  for (int l = 0; l < p->npols; ++l)
+    if (1.0 + p->dnu[l] * s->dt == 0.0) return set_error(FDTD_ERR_ARG, "pole with dnu*dt = -1");
+  if (p->ez_uses_jx && (!(p->jp_mask & 1) || !(p->jp_mask & 4)))
+    return set_error(FDTD_ERR_ARG, "ez_uses_jx needs Ex and Ez in jp_mask");
+  HostMaterial m{};
+  m.type = 3;
+  m.lj = *p;
+  m.writes[0] = m.writes[1] = m.writes[2] = 1;
+  return add_material(s, r, m);
+}
+
+int fdtd_lorentz_jp_preset(int32_t kind, fdtd_lorentz_jp* p) {
+  if (!p) return set_error(FDTD_ERR_ARG, "null preset");
+  struct Tab {
+    int npols, mask, nl, ezjx;
+    double einf, xi[4], w0[4], dnu[4];
+  };
+  // pole tables in units of 2*pi*c0/micro for w0 and dnu, as the modules
+  // declare them (file:line in include/fdtd.h, FDTD_LJP_*)
+  static const Tab tabs[7] = {
+      {3, 7, 0, 0, 1.053787, {2.862890, -0.0722398, 367.7668, 0},
+       {-2.445679, 1.480982, 0.3371279, 0}, {0.2170955, 0.1429645, 0.02285188, 0}},
+      {3, 7, 0, 0, 1.0, {8.0, 2.85, -0.107, 0}, {3.64, 2.76, 1.73, 0}, {0.0, 0.063, 2.5, 0}},
+      {4, 7, 0, 0, 1.1, {-12.74873, 0.9605442, 67.66233, 55.0525},
+       {4.36014, -1.261863, 0.4063879, -2.176176}, {-5.825585, 0.2087635, 1.246245, 8.059584}},
+      {2, 7, 0, 0, 3.497071, {2.301113, 400.0, 0, 0}, {-2.393546, 0.3441309, 0, 0},
+       {0.2460773, 0.02533959, 0, 0}},
+      {4, 5, 1, 1, 1.070682, {-0.163673, 0.843432, -0.170024, -0.121134},
+       {1.642784, 0.807691, 0.934364, 0.525964}, {2.267351, 1.018179, 0.621508, 0.033048}},
+      {3, 5, 1, 0, 1.023242, {4.957366, 4.455092, 1.905429, 0}, {0.573075, 1.713164, 4.598546, 0},
+       {0.368926, 2.42099, 1.35581, 0}},
+      {3, 5, 1, 0, 1.023242, {0.0, 0.0, 0.0, 0}, {0.573075, 1.713164, 4.598546, 0},
+       {0.368926, 2.42099, 1.35581, 0}},
+  };
+  if (kind < 0 || kind > 6) return set_error(FDTD_ERR_ARG, "unknown Lorentz-Jp preset");
+  const Consts& K = consts();
+  const double unit = 2.0 * K.pi * K.c0 / 1e-6;  // 2*pi*c0/micro
+  const Tab& t = tabs[kind];
+  std::memset(p, 0, sizeof *p);
+  p->npols = t.npols;
+  p->jp_mask = t.mask;
+  p->nonlinear = t.nl;
+  p->ez_uses_jx = t.ezjx;
+  p->eps_inf = t.einf;
+  for (int l = 0; l < t.npols; ++l) {
+    p->xi[l] = t.xi[l];
+    p->w0[l] = unit * t.w0[l];
+    p->dnu[l] = unit * t.dnu[l];
+  }
+  return 0;
+}
+
 int fdtd_add_plane_source(fdtd_sim* s, int axis, int index, int comp, const fdtd_pulse* p) {
   GUARD(s);
   SETUP_ONLY(s);
@@ -935,6 +1147,62 @@ int fdtd_add_probe(fdtd_sim* s, int i, int j, int k, int* id) {
   p.k = k;
   s->probes.push_back(p);
   *id = (int)s->probes.size() - 1;
+  return 0;
+}
+
+int fdtd_add_tfsf(fdtd_sim* s, const fdtd_tfsf* p, int* id) {
+  GUARD(s);
+  SETUP_ONLY(s);
+  if (!p || !id) return set_error(FDTD_ERR_ARG, "null argument");
+  const int n[3] = {s->nx, s->ny, s->nz};
+  if (p->axis < 0 || p->axis > 2 || p->comp < 0 || p->comp > 2 || p->comp == p->axis)
+    return set_error(FDTD_ERR_ARG, "bad TFSF axis/component");
+  const int N = n[p->axis];
+  if (p->lo < 2 || p->lo > N || (p->hi != 0 && (p->hi < p->lo || p->hi >= N)) || p->npml < 0 ||
+      2 * p->npml + 2 >= N || p->src_index <= p->npml || p->src_index > N - p->npml || !(p->eps > 0) ||
+      (p->npml > 0 && !(p->R0 > 0 && p->R0 < 1)) || !(p->pulse.st != 0.0))
+    return set_error(FDTD_ERR_ARG, "bad TFSF planes or line");
+  if (s->tfsfs.size() >= 4) return set_error(FDTD_ERR_ARG, "too many TFSF waves");
+  Tfsf t;
+  t.p = *p;
+  t.n = N;
+  t.hc = 3 - p->axis - p->comp;
+  // orientation: +1 when (comp, axis) is (x,y), (y,z) or (z,x)
+  t.sg = ((p->comp + 1) % 3 == p->axis) ? 1 : -1;
+  s->tfsfs.push_back(t);
+  *id = (int)s->tfsfs.size() - 1;
+  return 0;
+}
+
+int fdtd_add_tfsf_probe(fdtd_sim* s, int tid, int index, int* id) {
+  GUARD(s);
+  SETUP_ONLY(s);
+  if (!id || tid < 0 || tid >= (int)s->tfsfs.size() || index < 0 || index > s->tfsfs[tid].n + 1)
+    return set_error(FDTD_ERR_ARG, "bad TFSF probe");
+  if (s->probes.size() >= 64) return set_error(FDTD_ERR_ARG, "too many probes");
+  int nl = 0;
+  for (const auto& q : s->probes) nl += q.line == tid;
+  if (nl >= 4) return set_error(FDTD_ERR_ARG, "at most 4 probes per TFSF line");
+  Probe p;
+  p.i = 0;
+  p.j = index;
+  p.k = 0;
+  p.line = tid;
+  s->probes.push_back(p);
+  *id = (int)s->probes.size() - 1;
+  return 0;
+}
+
+int fdtd_get_tfsf_line(fdtd_sim* s, int tid, double* e, double* h) {
+  GUARD(s);
+  if (!e || !h || tid < 0 || tid >= (int)s->tfsfs.size()) return set_error(FDTD_ERR_ARG, "bad argument");
+  int rc = ensure_committed(s);
+  if (rc) return rc;
+  const Tfsf& t = s->tfsfs[tid];
+  const size_t n = (size_t)t.n + 2;
+  FDTD_HIP(hipMemcpyAsync(e, t.d_line, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  FDTD_HIP(hipMemcpyAsync(h, t.d_line + n, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  FDTD_HIP(hipStreamSynchronize(s->stream));
   return 0;
 }
 
@@ -1128,7 +1396,7 @@ int fdtd_algorithmic_bytes(fdtd_sim* s, int kernel, double* bytes) {
     // other field (3), read/write 6 splits (12), write the main field (3).
     if (kernel == FDTD_K_E || kernel == FDTD_K_H) b = w * (9.0 * (N - s->n_pml) + 18.0 * s->n_pml);
     if (kernel == FDTD_K_ADE)
-      for (auto& g : s->groups) b += w * (6.0 + 2.0 * 3.0 * g.nstate) * (double)g.ncells;
+      for (auto& g : s->groups) b += (w * 6.0 + (g.kind == 1 ? 8.0 : w) * 2.0 * 3.0 * g.nstate) * (double)g.ncells;
   } else {
     const int P = s->pml_mode == FDTD_PML_REF_Y ? s->pml.npml[1] : 0;
     const double Nm = (double)s->nx * (s->ny - 2 * P) * s->nz;
@@ -1137,7 +1405,7 @@ int fdtd_algorithmic_bytes(fdtd_sim* s, int kernel, double* bytes) {
     if (kernel == FDTD_K_MAT) b = w * 6.0 * N;
     if (kernel == FDTD_K_PML_E || kernel == FDTD_K_PML_H) b = w * 24.0 * Ns;
     if (kernel == FDTD_K_ADE)
-      for (auto& g : s->groups) b += w * (6.0 + 2.0 * 3.0 * g.nstate) * (double)g.ncells;
+      for (auto& g : s->groups) b += (w * 6.0 + (g.kind == 1 ? 8.0 : w) * 2.0 * 3.0 * g.nstate) * (double)g.ncells;
   }
   *bytes = b;
   return 0;

@@ -50,15 +50,35 @@ struct AdeCoef {
   real r0, r1, r2;
 };
 
+// Coefficients of one group of Lorentz-Jp cells (E form, see
+// fdtd_lorentz_jp).  kuga = Ku'*ga with Ku' = dt*c0/ds; xi and xi3 are
+// divided by c0 in the kernel, in the reference's operation order.
+template <typename real>
+struct LjpCoef {
+  int32_t npols, jmask, nl, ezjx;
+  real kuga, ga, ds, dt, c0, z0;
+  real xi[4], g1[4], g2[4];
+  real xi3, ng1, ng2;
+};
+
 // Up to this many plane sources per handle.
 constexpr int kMaxSrc = 8;
 
+// A plane source adds val, or, when dval is set, dscale * *dval read on the
+// device (the TFSF consistency terms read the incident line that way).
 template <typename real>
 struct SrcArgs {
   int32_t n;
   int32_t axis[kMaxSrc], index[kMaxSrc], comp[kMaxSrc];
   real val[kMaxSrc];
+  const double* dval[kMaxSrc];
+  double dscale[kMaxSrc];
 };
+
+template <typename real>
+__device__ __forceinline__ real src_value(const SrcArgs<real>& s, int q) {
+  return s.dval[q] ? (real)(s.dscale[q] * *s.dval[q]) : s.val[q];
+}
 
 // split-field components
 enum { S_EXY = 0, S_EXZ, S_EYZ, S_EYX, S_EZX, S_EZY, S_HXY, S_HXZ, S_HYZ, S_HYX, S_HZX, S_HZY, S_N };
@@ -69,11 +89,12 @@ struct DevBuf {
 };
 
 struct HostMaterial {
-  int type;  // 0 dielectric, 1 drude2cp, 2 graphene
+  int type;  // 0 dielectric, 1 drude2cp, 2 graphene, 3 Lorentz-Jp
   fdtd_region reg;
   double eps_r;
   fdtd_drude2cp d2;
   fdtd_graphene gr;
+  fdtd_lorentz_jp lj;
   int writes[3];
   int group;  // ADE group index (dispersive only)
 };
@@ -102,11 +123,24 @@ struct DftPlane {
 
 struct Probe {
   int i, j, k;
+  int line = -1;  // TFSF id of an incident-line probe (j = line cell)
   std::vector<double> rec;
 };
 
+// TFSF plane wave: the incident 1D line and its PML (device, FP64):
+// e, h, pe, ph, f1, f2, g1, g2, each n+2 doubles (cells 0..n+1).
+struct Tfsf {
+  fdtd_tfsf p{};
+  int n = 0, hc = 0, sg = 1;
+  double* d_line = nullptr;
+  int32_t* d_lp = nullptr;  // (probe index, line cell) pairs of its probes
+  int nlp = 0;
+};
+
 struct AdeGroup {
+  int kind = 0;          // 0 Drude/CP/Raman (coef), 1 Lorentz-Jp (lj)
   AdeCoef<double> coef;  // FP64 master copy
+  LjpCoef<double> lj{};
   double eps_inf;
   int nstate;                 // state words per component
   int64_t ncells = 0;
@@ -129,6 +163,7 @@ struct Sim {
   std::vector<HostSource> srcs;
   std::vector<DftPlane> dfts;
   std::vector<Probe> probes;
+  std::vector<Tfsf> tfsfs;
   bool committed = false;
   int n_done = 0;
   int device = 0;

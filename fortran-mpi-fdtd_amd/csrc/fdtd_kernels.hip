@@ -159,10 +159,16 @@ __device__ __forceinline__ void fused_e_cell(const FieldArgs<real>& a, const Pml
     if (sa[2] == 1) ezy = ezy + sv[2] * r;
     p.s[S_EYX][s] = eyx; p.s[S_EZX][s] = ezx; p.s[S_EXY][s] = exy;
     p.s[S_EZY][s] = ezy; p.s[S_EXZ][s] = exz; p.s[S_EYZ][s] = eyz;
-    a.e[0][c] = exy + exz;
-    a.e[1][c] = eyz + eyx;
-    a.e[2][c] = ezx + ezy;
-    return;  // periodic axes carry no PML: no ghosts to refresh here
+    const real ex = exy + exz, ey = eyz + eyx, ez = ezx + ezy;
+    a.e[0][c] = ex;
+    a.e[1][c] = ey;
+    a.e[2][c] = ez;
+    // a periodic axis without PML of its own still meets the slabs of the
+    // other axes at its boundary
+    put_e_ghost(a.e[0], ex, i, j, k, c, a);
+    put_e_ghost(a.e[1], ey, i, j, k, c, a);
+    put_e_ghost(a.e[2], ez, i, j, k, c, a);
+    return;
   }
   // E_q += Ku*gax_q*curl_q (+ gax_q * source) for every component a
   // dielectric writes
@@ -195,9 +201,10 @@ __device__ __forceinline__ void plane_sources(const SrcArgs<real>& src, int nsrc
     const int pos = ax == 0 ? i : (ax == 1 ? j : k);
     if (pos == src.index[q]) {
       const int cq = src.comp[q];
-      if (cq == 0) { sv[0] += src.val[q]; sa[0] = ax; }
-      else if (cq == 1) { sv[1] += src.val[q]; sa[1] = ax; }
-      else { sv[2] += src.val[q]; sa[2] = ax; }
+      const real v = src_value(src, q);
+      if (cq == 0) { sv[0] += v; sa[0] = ax; }
+      else if (cq == 1) { sv[1] += v; sa[1] = ax; }
+      else { sv[2] += v; sa[2] = ax; }
     }
   }
 }
@@ -263,9 +270,13 @@ __global__ __launch_bounds__(512) void k_fused_h(FieldArgs<real> a, PmlArgs<real
     real hyz = p.g1[2][k] * p.s[S_HYZ][s] + p.g2[2][k] * (ex + (-ex_kp));
     p.s[S_HYX][s] = hyx; p.s[S_HZX][s] = hzx; p.s[S_HXY][s] = hxy;
     p.s[S_HZY][s] = hzy; p.s[S_HXZ][s] = hxz; p.s[S_HYZ][s] = hyz;
-    a.h[0][c] = hxy + hxz;
-    a.h[1][c] = hyz + hyx;
-    a.h[2][c] = hzx + hzy;
+    const real hx = hxy + hxz, hy = hyz + hyx, hz = hzx + hzy;
+    a.h[0][c] = hx;
+    a.h[1][c] = hy;
+    a.h[2][c] = hz;
+    put_h_ghost(a.h[0], hx, i, j, k, c, a);
+    put_h_ghost(a.h[1], hy, i, j, k, c, a);
+    put_h_ghost(a.h[2], hz, i, j, k, c, a);
     return;
   }
   const real curleyz = ey_kp - ey, curlezy = -ez_jp + ez;
@@ -308,7 +319,7 @@ __global__ __launch_bounds__(256) void k_ade(FieldArgs<real> a, AdeCoef<real> co
     cur[2] = (hy - a.h[1][c - 1]) + (-hx + a.h[0][c - a.px]);
     for (int q = 0; q < nsrc; ++q) {
       const int pos = src.axis[q] == 0 ? i : (src.axis[q] == 1 ? j : k);
-      if (pos == src.index[q]) sv[src.comp[q]] += src.val[q];
+      if (pos == src.index[q]) sv[src.comp[q]] += src_value(src, q);
     }
   }
   const int w0 = FUSED ? 1 : 0;  // word offset of I
@@ -371,6 +382,84 @@ __global__ __launch_bounds__(256) void k_ade(FieldArgs<real> a, AdeCoef<real> co
 }
 
 // ---------------------------------------------------------------------------
+// Lorentz-Jp cells, E form (mod3/goldLorentzJp_mod.f90:37-91 and the family:
+// grapheneLorentzJp_mod.f90:42-101 adds the third-order oscillator and
+// drives Ez with JpT_x).  The pole state is FP64 in both precisions (an FP32
+// P would lose the small per-step increments dt*Jp of slow poles); the E
+// update is evaluated in FP64 and stored in field precision.  State SoA,
+// word w of component q for cell e at state[(w*3+q)*n + e]; words: P_l
+// (l < npols), Jp_l, [Q, G].  JpT is the running sum of the Jp_l, summed in
+// pole order as the modules accumulate it.
+template <typename real>
+__global__ __launch_bounds__(256) void k_ljp(FieldArgs<real> a, LjpCoef<double> co, const int32_t* cells,
+                                             const uint8_t* masks, double* state, int64_t n, SrcArgs<real> src,
+                                             int nsrc) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int64_t c = cells[e];
+  const int mask = masks[e];
+  const int k = (int)(c / a.pxy);
+  const int64_t rem = c - (int64_t)k * a.pxy;
+  const int j = (int)(rem / a.px);
+  const int i = (int)(rem - (int64_t)j * a.px);
+  const real hx = a.h[0][c], hy = a.h[1][c], hz = a.h[2][c];
+  const real cur[3] = {(hz - a.h[2][c - a.px]) + (-hy + a.h[1][c - a.pxy]),   // curlhzy + curlhyz
+                       (hx - a.h[0][c - a.pxy]) + (-hz + a.h[2][c - 1]),     // curlhxz + curlhzx
+                       (hy - a.h[1][c - 1]) + (-hx + a.h[0][c - a.px])};     // curlhyx + curlhxy
+  real sv[3] = {0, 0, 0};
+  bool hit[3] = {false, false, false};
+  for (int q = 0; q < nsrc; ++q) {
+    const int pos = src.axis[q] == 0 ? i : (src.axis[q] == 1 ? j : k);
+    if (pos == src.index[q]) {
+      sv[src.comp[q]] += src_value(src, q);
+      hit[src.comp[q]] = true;
+    }
+  }
+  const int np = co.npols;
+  const int wq = 2 * np;  // word of Q
+  double jt[3] = {0, 0, 0};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    if (!((co.jmask >> q) & 1)) continue;
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      if (l >= np) break;
+      jt[q] = jt[q] + state[(int64_t)((np + l) * 3 + q) * n + e];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    if (!(mask & (1 << q))) continue;
+    const double jd = (q == 2 && co.ezjx) ? jt[0] : jt[q];
+    double ev = (double)a.e[q][c] + co.kuga * ((double)cur[q] - co.ds * jd);
+    if (hit[q]) ev = ev + co.ga * (double)sv[q];
+    const real er = (real)ev;
+    a.e[q][c] = er;
+    put_e_ghost(a.e[q], er, i, j, k, c, a);
+    double Qv = 0;
+    if (co.nl) Qv = state[(int64_t)(wq * 3 + q) * n + e];
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      if (l >= np) break;
+      double* Pp = state + (int64_t)(l * 3 + q) * n + e;
+      double* Jp = state + (int64_t)((np + l) * 3 + q) * n + e;
+      double p0 = (co.xi[l] * ev) / co.c0;
+      if (l == 0 && co.nl) p0 = p0 + ((co.xi3 * Qv) * ev) / co.c0;
+      const double jn = co.g1[l] * *Jp + co.g2[l] * (p0 - *Pp);
+      *Jp = jn;
+      *Pp = *Pp + co.dt * jn;
+    }
+    if (co.nl) {
+      double* Gp = state + (int64_t)((wq + 1) * 3 + q) * n + e;
+      const double ze = co.z0 * ev;
+      const double gn = co.ng1 * *Gp + co.ng2 * (ze * ze - Qv);
+      *Gp = gn;
+      state[(int64_t)(wq * 3 + q) * n + e] = Qv + co.dt * gn;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // STAGED engine
 // updateD, sim3/include/myfdtd_func.f90:19-41, over rows [j0, j1]
 template <typename real>
@@ -392,9 +481,12 @@ __global__ __launch_bounds__(256) void k_staged_d(FieldArgs<real> a, int j0, int
   a.d[2][c] = a.d[2][c] + a.ku * (curlhyx + curlhxy);
 }
 
-// plane source, sim3/program.f90:396-406
+// plane source, sim3/program.f90:396-406 (dval: value dscale * *dval read
+// here, the TFSF consistency terms)
 template <typename real>
-__global__ void k_staged_src(FieldArgs<real> a, int axis, int index, int comp, real val) {
+__global__ void k_staged_src(FieldArgs<real> a, int axis, int index, int comp, real val, const double* dval,
+                             double dscale) {
+  if (dval) val = (real)(dscale * *dval);
   const int u = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int v = blockIdx.y + 1;
   int p[3];
@@ -646,7 +738,7 @@ __global__ __launch_bounds__(256) void k_dft_accum(const real* samp, int64_t np,
 template <typename real>
 __global__ void k_probe(FieldArgs<real> a, const int32_t* pos, int np, double* out, double tn) {
   const int p = threadIdx.x;
-  if (p >= np) return;
+  if (p >= np || pos[3 * p] < 0) return;  // incident-line probes are written by k_tfsf_h
   const int64_t c = pos[3 * p] + a.px * pos[3 * p + 1] + a.pxy * pos[3 * p + 2];
   double* r = out + 7 * p;
   r[0] = tn;
@@ -654,6 +746,136 @@ __global__ void k_probe(FieldArgs<real> a, const int32_t* pos, int np, double* o
     r[1 + q] = (double)a.e[q][c];
     r[4 + q] = (double)a.h[q][c];
   }
+}
+
+// ---------------------------------------------------------------------------
+// TFSF incident line (sim1/program.f90:217-262, mod1/BERpmly_ExHz_mod.f90).
+// One workgroup walks the line; the PML object's own fields pe, ph live on
+// cells 1..npml and n-npml+1..n and hand over at the slab edges.
+struct TfsfLine {
+  double *e, *h, *pe, *ph;
+  const double *f1, *f2, *g1, *g2;
+  int32_t n, npml, src, lo, hi, comp, hc, axis, sg;
+  double ck, ku;
+};
+
+inline TfsfLine tfsf_line(const Sim* s, const Tfsf& t) {
+  TfsfLine L;
+  const int64_t m = t.n + 2;
+  L.e = t.d_line;
+  L.h = t.d_line + m;
+  L.pe = t.d_line + 2 * m;
+  L.ph = t.d_line + 3 * m;
+  L.f1 = t.d_line + 4 * m;
+  L.f2 = t.d_line + 5 * m;
+  L.g1 = t.d_line + 6 * m;
+  L.g2 = t.d_line + 7 * m;
+  L.n = t.n;
+  L.npml = t.p.npml;
+  L.src = t.p.src_index;
+  L.lo = t.p.lo;
+  L.hi = t.p.hi;
+  L.comp = t.p.comp;
+  L.hc = t.hc;
+  L.axis = t.p.axis;
+  L.sg = t.sg;
+  L.ck = s->ku * (1.0 / t.p.eps);
+  L.ku = s->ku;
+  return L;
+}
+
+// E half step of the line: ex_inc update (:222-224), the pulse (:228-229),
+// mypmly_inc%updateE (:231; updateE_pmy_low/high + setboundaryEa/Eb)
+__global__ __launch_bounds__(256) void k_tfsf_line_e(TfsfLine L, double g) {
+  const int N = L.n, P = L.npml, hi = N - P + 1;
+  for (int j = P + 1 + (int)threadIdx.x; j <= hi - 1; j += blockDim.x) {
+    double v = L.e[j] + L.ck * (L.h[j] - L.h[j - 1]);
+    if (j == L.src) v = v + g;
+    L.e[j] = v;
+  }
+  if (P > 0) {
+    for (int j = 1 + (int)threadIdx.x; j <= P; j += blockDim.x)
+      L.pe[j] = L.f1[j] * L.pe[j] + L.f2[j] * (L.ph[j] - L.ph[j - 1]);
+    for (int j = hi + (int)threadIdx.x; j <= N; j += blockDim.x)
+      L.pe[j] = L.f1[j] * L.pe[j] + L.f2[j] * (L.ph[j] - L.ph[j - 1]);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      L.pe[P + 1] = L.e[P + 1];
+      L.e[P] = L.pe[P];
+      L.pe[hi - 1] = L.e[hi - 1];
+      L.e[hi] = L.pe[hi];
+    }
+  }
+}
+
+// H half step of the TFSF: block 0 advances the line (hz_inc update :252-254,
+// mypmly_inc%updateH :255) and records its probes; the other blocks apply
+// the consistency terms on H (:260-262 and, two-sided, the far plane):
+// H_h(lo-1) -= s*Ku*e(lo), H_h(hi) += s*Ku*e(hi+1).  ALL_FACES PML cells on
+// a plane take the term into the split along the line's axis.
+template <typename real>
+__global__ __launch_bounds__(256) void k_tfsf_h(FieldArgs<real> a, PmlArgs<real> p, int all_faces, TfsfLine L,
+                                                const int32_t* lp, int nlp, double* rec, double tn, int n1, int n2) {
+  if (blockIdx.x == 0) {
+    const int N = L.n, P = L.npml, hi = N - P + 1;
+    for (int j = P + 1 + (int)threadIdx.x; j <= hi - 1; j += blockDim.x) L.h[j] = L.h[j] + L.ku * (L.e[j + 1] - L.e[j]);
+    if (P > 0) {
+      for (int j = 1 + (int)threadIdx.x; j <= P; j += blockDim.x)
+        L.ph[j] = L.g1[j] * L.ph[j] + L.g2[j] * (L.pe[j + 1] - L.pe[j]);
+      for (int j = hi + (int)threadIdx.x; j <= N; j += blockDim.x)
+        L.ph[j] = L.g1[j] * L.ph[j] + L.g2[j] * (L.pe[j + 1] - L.pe[j]);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        L.ph[P + 1] = L.h[P + 1];
+        L.h[P] = L.ph[P];
+        L.ph[hi - 1] = L.h[hi - 1];
+        L.h[hi] = L.ph[hi];
+      }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nlp) {  // sim1/program.f90:272-273
+      double* r = rec + 7 * lp[2 * threadIdx.x];
+      const int j = lp[2 * threadIdx.x + 1];
+      r[0] = tn;
+      for (int q = 1; q < 7; ++q) r[q] = 0.0;
+      r[1 + L.comp] = L.e[j];
+      r[4 + L.hc] = L.sg * L.h[j];
+    }
+    return;
+  }
+  const int64_t u = (int64_t)(blockIdx.x - 1) * blockDim.x + threadIdx.x;
+  const int64_t np = (int64_t)n1 * n2;
+  const int nplanes = L.hi > 0 ? 2 : 1;
+  if (u >= np * nplanes) return;
+  const int pl = (int)(u / np);
+  const int64_t w = u - (int64_t)pl * np;
+  const int index = pl == 0 ? L.lo - 1 : L.hi;
+  const double ev = pl == 0 ? L.e[L.lo] : L.e[L.hi + 1];
+  const double val = (pl == 0 ? -(L.sg * L.ku) : (L.sg * L.ku)) * ev;
+  const int a1 = L.axis == 0 ? 1 : 0, a2 = L.axis == 2 ? 1 : 2;
+  int c3[3];
+  c3[L.axis] = index;
+  c3[a1] = (int)(w % n1) + 1;
+  c3[a2] = (int)(w / n1) + 1;
+  const int i = c3[0], j = c3[1], k = c3[2];
+  const int c = i + a.px * j + a.pxy * k;
+  const int q = L.hc;
+  real hv;
+  if (all_faces && in_pml(p.P, i, j, k, a.nx, a.ny, a.nz)) {
+    // split H_q along the axis, then H_q = its two splits summed as
+    // k_fused_h sums them (Hxy+Hxz, Hyz+Hyx, Hzx+Hzy)
+    const int sa = q == 0 ? (L.axis == 1 ? S_HXY : S_HXZ)
+                          : (q == 1 ? (L.axis == 0 ? S_HYX : S_HYZ) : (L.axis == 0 ? S_HZX : S_HZY));
+    const int s0 = q == 0 ? S_HXY : (q == 1 ? S_HYZ : S_HZX);
+    const int s1 = q == 0 ? S_HXZ : (q == 1 ? S_HYX : S_HZY);
+    const int sp = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
+    p.s[sa][sp] = p.s[sa][sp] + (real)val;
+    hv = p.s[s0][sp] + p.s[s1][sp];
+  } else {
+    hv = a.h[q][c] + (real)val;
+  }
+  a.h[q][c] = hv;
+  put_h_ghost(a.h[q], hv, i, j, k, c, a);
 }
 
 // Flux of a DFT plane, one block per frequency, fixed-order tree reduction.
@@ -848,7 +1070,29 @@ SrcArgs<real> source_args(Sim* s, int n, int* nsrc) {
     sa.index[q] = src.index;
     sa.comp[q] = src.comp;
     sa.val[q] = (real)g;
+    sa.dval[q] = nullptr;
     ++q;
+  }
+  // TFSF consistency terms on D, sim1/program.f90:236-238: D(lo) -= Ku*h(lo-1)
+  // and, two-sided, D(hi+1) += Ku*h(hi), h read from the line on the device
+  for (const auto& t : s->tfsfs) {
+    const double* h = t.d_line + (t.n + 2);
+    sa.axis[q] = t.p.axis;
+    sa.index[q] = t.p.lo;
+    sa.comp[q] = t.p.comp;
+    sa.val[q] = 0;
+    sa.dval[q] = h + (t.p.lo - 1);
+    sa.dscale[q] = -s->ku;
+    ++q;
+    if (t.p.hi > 0) {
+      sa.axis[q] = t.p.axis;
+      sa.index[q] = t.p.hi + 1;
+      sa.comp[q] = t.p.comp;
+      sa.val[q] = 0;
+      sa.dval[q] = h + t.p.hi;
+      sa.dscale[q] = s->ku;
+      ++q;
+    }
   }
   *nsrc = q;
   return sa;
@@ -896,6 +1140,33 @@ int step_impl(Sim* s) {
     c.r2 = (real)g.coef.r2;
     return c;
   };
+  // TFSF lines: E half step after the main E stage (the D terms above read
+  // h(lo-1) before this step's line H update); H half step, consistency
+  // terms on H and the line probes after the main H stage
+  auto tfsf_e = [&](hipStream_t sq) {
+    for (const auto& t : s->tfsfs) {
+      const fdtd_pulse& pu = t.p.pulse;
+      const double tt = (double)(n - 1) * s->dt;
+      const double g = pu.amplitude * (std::exp(-0.5 * ((tt - pu.t0) / pu.st) * ((tt - pu.t0) / pu.st)) *
+                                       std::sin(pu.w0 * tt));
+      Timer tm(s, FDTD_K_MAT);
+      hipLaunchKernelGGL(k_tfsf_line_e, dim3(1), dim3(256), 0, sq, tfsf_line(s, t), g);
+    }
+  };
+  auto tfsf_h = [&](hipStream_t sq, PmlArgs<real> pa, int all_faces) {
+    const int np = (int)s->probes.size();
+    double* row = np ? s->d_probe + (int64_t)7 * np * (s->probe_row0 + (n - 1 - s->call_base)) : nullptr;
+    for (const auto& t : s->tfsfs) {
+      const int ax = t.p.axis;
+      const int nn[3] = {s->nx, s->ny, s->nz};
+      const int n1 = nn[ax == 0 ? 1 : 0], n2 = nn[ax == 2 ? 1 : 2];
+      const int64_t cells = (int64_t)n1 * n2 * (t.p.hi > 0 ? 2 : 1);
+      Timer tm(s, FDTD_K_MAT);
+      hipLaunchKernelGGL((k_tfsf_h<real>), dim3((unsigned)(1 + (cells + 255) / 256)), dim3(256), 0, sq, a, pa,
+                         all_faces, tfsf_line(s, t), (const int32_t*)t.d_lp, row ? t.nlp : 0, row,
+                         (double)n * s->dt, n1, n2);
+    }
+  };
   if (s->engine == FDTD_ENGINE_FUSED) {
     PmlArgs<real> p = pml_args<real>(s);
     {
@@ -911,13 +1182,19 @@ int step_impl(Sim* s) {
     for (auto& g : s->groups) {
       if (!g.ncells) continue;
       Timer tm(s, FDTD_K_ADE);
-      hipLaunchKernelGGL((k_ade<real, true>), dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, a,
-                         ade_coef(g), g.d_cells, g.d_mask, (real*)g.d_state, g.ncells, src, nsrc);
+      if (g.kind == 1)
+        hipLaunchKernelGGL(k_ljp<real>, dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, a, g.lj,
+                           g.d_cells, g.d_mask, (double*)g.d_state, g.ncells, src, nsrc);
+      else
+        hipLaunchKernelGGL((k_ade<real, true>), dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, a,
+                           ade_coef(g), g.d_cells, g.d_mask, (real*)g.d_state, g.ncells, src, nsrc);
     }
+    tfsf_e(st);
     {
       Timer tm(s, FDTD_K_H);
       hipLaunchKernelGGL(k_fused_h<real>, tile_grid(a), blk, 0, st, a, p);
     }
+    tfsf_h(st, p, 1);
   } else {
     const bool refy = s->pml_mode == FDTD_PML_REF_Y;
     const int P = refy ? s->pml.npml[1] : 0;
@@ -932,7 +1209,7 @@ int step_impl(Sim* s) {
       const int nn[3] = {s->nx, s->ny, s->nz};
       Timer tm(s, FDTD_K_MAT);
       hipLaunchKernelGGL(k_staged_src<real>, dim3((unsigned)((nn[a1] + 63) / 64), (unsigned)nn[a2]), dim3(64), 0, st,
-                         a, ax, src.index[q], src.comp[q], src.val[q]);
+                         a, ax, src.index[q], src.comp[q], src.val[q], src.dval[q], src.dscale[q]);
     }
     {
       Timer tm(s, FDTD_K_MAT);
@@ -941,8 +1218,12 @@ int step_impl(Sim* s) {
     for (auto& g : s->groups) {
       if (!g.ncells) continue;
       Timer tm(s, FDTD_K_ADE);
-      hipLaunchKernelGGL((k_ade<real, false>), dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, a,
-                         ade_coef(g), g.d_cells, g.d_mask, (real*)g.d_state, g.ncells, src, 0);
+      if (g.kind == 1)  // E form from curl H: the plane sources enter here, not through D
+        hipLaunchKernelGGL(k_ljp<real>, dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, a, g.lj,
+                           g.d_cells, g.d_mask, (double*)g.d_state, g.ncells, src, nsrc);
+      else
+        hipLaunchKernelGGL((k_ade<real, false>), dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, a,
+                           ade_coef(g), g.d_cells, g.d_mask, (real*)g.d_state, g.ncells, src, 0);
     }
     RefPmlArgs<real> rp{};
     if (refy) {
@@ -950,6 +1231,7 @@ int step_impl(Sim* s) {
       Timer tm(s, FDTD_K_PML_E);
       hipLaunchKernelGGL(k_ref_pml_e<real>, cell_grid(s->nx, 2 * P, s->nz, blk), blk, 0, st, a, rp);
     }
+    tfsf_e(st);
     {
       Timer tm(s, FDTD_K_H);
       hipLaunchKernelGGL(k_staged_h<real>, cell_grid(s->nx, j1 - j0 + 1, s->nz, blk), blk, 0, st, a, j0, j1);
@@ -958,6 +1240,7 @@ int step_impl(Sim* s) {
       Timer tm(s, FDTD_K_PML_H);
       hipLaunchKernelGGL(k_ref_pml_h<real>, cell_grid(s->nx, 2 * P, s->nz, blk), blk, 0, st, a, rp);
     }
+    tfsf_h(st, PmlArgs<real>{}, 0);
   }
   // monitors: buffer this step's plane samples, fold a full block
   (void)t;

@@ -102,6 +102,28 @@ def graphene_ref_dict():
             "wNL": 0.0, "dnuNL": 0.0, "xi3": 0.0}
 
 
+def lorentz_jp(d, binding=None):
+    """fdtd_lorentz_jp from a dict: {"preset": name} (gold, si, titanium,
+    film_gold, graphene, graphene3_1nm, graphene_nd: the Lorentz-Jp modules'
+    tables) and/or explicit fields, which override the preset's."""
+    b = binding or library()
+    p = abi.LorentzJp()
+    if "preset" in d:
+        b.check(b.lorentz_jp_preset(abi.LJP_KINDS[d["preset"]], C.byref(p)))
+    for name in ("npols", "jp_mask", "nonlinear", "ez_uses_jx"):
+        if name in d:
+            setattr(p, name, int(d[name]))
+    for name in ("eps_inf", "wNL", "dnuNL", "xi3"):
+        if name in d:
+            setattr(p, name, float(d[name]))
+    for name in ("xi", "w0", "dnu"):
+        if name in d:
+            arr = getattr(p, name)
+            for l, v in enumerate(d[name]):
+                arr[l] = float(v)
+    return p
+
+
 def _drude2cp(d):
     p = abi.Drude2CP()
     p.eps_inf, p.gamma, p.omega_d = d["eps_inf"], d["gamma"], d["omega_d"]
@@ -192,6 +214,10 @@ class Simulation:
         p = _graphene(params)
         self.b.check(self.b.add_graphene(self.h, C.byref(_region(region)), C.byref(p)))
 
+    def add_lorentz_jp(self, region, params):
+        p = lorentz_jp(params, self.b)
+        self.b.check(self.b.add_lorentz_jp(self.h, C.byref(_region(region)), C.byref(p)))
+
     def add_plane_source(self, axis, index, comp, t0, st, w0, amplitude=1.0):
         p = abi.Pulse(t0, st, w0, amplitude)
         self.b.check(self.b.add_plane_source(self.h, int(axis), int(index), int(comp), C.byref(p)))
@@ -211,6 +237,31 @@ class Simulation:
         n2 = n[1] if axis == 2 else n[2]
         self._dfts.append((n1, n2, len(om)))
         return i.value
+
+    def add_tfsf(self, axis, comp, lo, src_index, t0, st, w0, amplitude=1.0, hi=0, npml=10, eps=1.0, order=3.0,
+                 R0=math.exp(-16.0)):
+        """TFSF plane wave along +axis (fdtd_tfsf); returns its id."""
+        t = abi.Tfsf()
+        t.axis, t.comp, t.lo, t.hi, t.src_index, t.npml = int(axis), int(comp), int(lo), int(hi), int(src_index), int(npml)
+        t.eps, t.order, t.R0 = float(eps), float(order), float(R0)
+        t.pulse.t0, t.pulse.st, t.pulse.w0, t.pulse.amplitude = t0, st, w0, amplitude
+        out = C.c_int()
+        self.b.check(self.b.add_tfsf(self.h, C.byref(t), C.byref(out)))
+        self._tfsf_n = getattr(self, "_tfsf_n", {})
+        self._tfsf_n[out.value] = self.shape[int(axis)]
+        return out.value
+
+    def add_tfsf_probe(self, tfsf_id, index):
+        out = C.c_int()
+        self.b.check(self.b.add_tfsf_probe(self.h, int(tfsf_id), int(index), C.byref(out)))
+        return out.value
+
+    def tfsf_line(self, tfsf_id):
+        n = self._tfsf_n[tfsf_id] + 2
+        e = np.empty(n, dtype=np.float64)
+        h = np.empty(n, dtype=np.float64)
+        self.b.check(self.b.get_tfsf_line(self.h, int(tfsf_id), e.ctypes.data_as(abi._PD), h.ctypes.data_as(abi._PD)))
+        return e, h
 
     def add_probe(self, i, j, k):
         r = C.c_int()
@@ -337,6 +388,8 @@ def apply_spec(sim, spec):
             sim.add_drude2cp(m["region"], m.get("params") or gold_params_dict())
         elif m["type"] == "graphene":
             sim.add_graphene(m["region"], m["params"])
+        elif m["type"] == "lorentz_jp":
+            sim.add_lorentz_jp(m["region"], m["params"])
         else:
             raise ValueError(m["type"])
     for s in spec.get("sources", []):
@@ -346,6 +399,12 @@ def apply_spec(sim, spec):
         ids.append(sim.add_dft_plane(d["axis"], d["index"], np.array(d["omega"]), d.get("comps"),
                                      d.get("n_begin", 1), d.get("n_end", 0)))
     pids = [sim.add_probe(*pr) for pr in spec.get("probes", [])]
+    tids = []
+    for t in spec.get("tfsf", []):
+        tids.append(sim.add_tfsf(t["axis"], t["comp"], t["lo"], t["src_index"], t["t0"], t["st"], t["w0"],
+                                 t.get("amplitude", 1.0), t.get("hi", 0), t.get("npml", 10), t.get("eps", 1.0),
+                                 t.get("order", 3.0), t.get("R0", math.exp(-16.0))))
+    pids += [sim.add_tfsf_probe(tids[q], idx) for q, idx in spec.get("tfsf_probes", [])]
     return ids, pids
 
 

@@ -94,9 +94,45 @@ class Graphene(C.Structure):
     ]
 
 
+class LorentzJp(C.Structure):
+    _fields_ = [
+        ("npols", C.c_int32),
+        ("jp_mask", C.c_int32),
+        ("nonlinear", C.c_int32),
+        ("ez_uses_jx", C.c_int32),
+        ("eps_inf", C.c_double),
+        ("xi", C.c_double * 4),
+        ("w0", C.c_double * 4),
+        ("dnu", C.c_double * 4),
+        ("wNL", C.c_double),
+        ("dnuNL", C.c_double),
+        ("xi3", C.c_double),
+    ]
+
+
+LJP_GOLD, LJP_SI, LJP_TITANIUM, LJP_FILM_GOLD, LJP_GRAPHENE, LJP_GRAPHENE3_1NM, LJP_GRAPHENE_ND = range(7)
+LJP_KINDS = {"gold": LJP_GOLD, "si": LJP_SI, "titanium": LJP_TITANIUM, "film_gold": LJP_FILM_GOLD,
+             "graphene": LJP_GRAPHENE, "graphene3_1nm": LJP_GRAPHENE3_1NM, "graphene_nd": LJP_GRAPHENE_ND}
+
+
 class Pulse(C.Structure):
     _fields_ = [("t0", C.c_double), ("st", C.c_double), ("w0", C.c_double),
                 ("amplitude", C.c_double)]
+
+
+class Tfsf(C.Structure):
+    _fields_ = [
+        ("axis", C.c_int32),
+        ("comp", C.c_int32),
+        ("lo", C.c_int32),
+        ("hi", C.c_int32),
+        ("src_index", C.c_int32),
+        ("npml", C.c_int32),
+        ("eps", C.c_double),
+        ("order", C.c_double),
+        ("R0", C.c_double),
+        ("pulse", Pulse),
+    ]
 
 
 _P = C.c_void_p
@@ -114,9 +150,14 @@ PROTOTYPES = {
     "add_dielectric": (_I, [_P, C.POINTER(Region), _D]),
     "add_drude2cp": (_I, [_P, C.POINTER(Region), C.POINTER(Drude2CP)]),
     "add_graphene": (_I, [_P, C.POINTER(Region), C.POINTER(Graphene)]),
+    "add_lorentz_jp": (_I, [_P, C.POINTER(Region), C.POINTER(LorentzJp)]),
+    "lorentz_jp_preset": (_I, [C.c_int32, C.POINTER(LorentzJp)]),
     "add_plane_source": (_I, [_P, _I, _I, _I, C.POINTER(Pulse)]),
     "add_dft_plane": (_I, [_P, _I, _I, _I, _PD, _I, _I, _I, C.POINTER(_I)]),
     "add_probe": (_I, [_P, _I, _I, _I, C.POINTER(_I)]),
+    "add_tfsf": (_I, [_P, C.POINTER(Tfsf), C.POINTER(_I)]),
+    "add_tfsf_probe": (_I, [_P, _I, _I, C.POINTER(_I)]),
+    "get_tfsf_line": (_I, [_P, _I, _PD, _PD]),
     "commit": (_I, [_P]),
     "step": (_I, [_P, _I]),
     "sync": (_I, [_P]),

@@ -107,6 +107,159 @@ def ref_graphene_small(nsteps=900):
     }
 
 
+def ref_ljp_metals(nsteps=1000, titanium=False):
+    """Lorentz-Jp metals (gold antenna, Si and film-gold slabs) between air
+    and SiO2 along y, y-PML 10 with x/z grading depth 2, periodic x/z.
+    Every cell has exactly one writer (no reference driver calls
+    update_EJp, so the case keeps E-form and D-form regions apart).
+    titanium=True puts the titanium table (mod3/titaniumJp_mod.f90, negative
+    damping on pole 1: the recursion grows without bound) in row 50."""
+    nx, ny, nz = 8, 72, 6
+    ds = 3.999 * NANO
+    P = 10
+    lam0 = 633.0 * NANO
+    st = 0.5e-15
+    arg = reference_frequencies(500e-9, 800e-9, 16)
+    ljp = lambda name: {"preset": name}  # noqa: E731
+    mats = [
+        {"type": "dielectric", "region": box((1, P + 1, 1), (nx, 40, nz)), "eps": 1.0},
+        {"type": "dielectric", "region": box((1, 41, 1), (1, 43, nz)), "eps": 1.0},
+        {"type": "lorentz_jp", "region": box((2, 41, 1), (nx - 1, 43, nz)), "params": ljp("gold")},
+        {"type": "dielectric", "region": box((nx, 41, 1), (nx, 43, nz)), "eps": 1.0},
+        {"type": "dielectric", "region": box((1, 44, 1), (nx, 45, nz)), "eps": 1.0},
+        {"type": "lorentz_jp", "region": box((1, 46, 1), (nx, 48, nz)), "params": ljp("si")},
+        {"type": "dielectric", "region": box((1, 49, 1), (nx, 49, nz)), "eps": 1.0},
+        {"type": "lorentz_jp", "region": box((1, 50, 1), (nx, 50, nz)), "params": ljp("titanium")} if titanium
+        else {"type": "dielectric", "region": box((1, 50, 1), (nx, 50, nz)), "eps": 1.0},
+        {"type": "lorentz_jp", "region": box((1, 51, 1), (nx, 52, nz)), "params": ljp("film_gold")},
+        {"type": "dielectric", "region": box((1, 53, 1), (nx, ny - P, nz)), "eps": 1.46 ** 2},
+    ]
+    return {
+        "name": "ref_ljp_titanium" if titanium else "ref_ljp_metals",
+        "grid": {"nx": nx, "ny": ny, "nz": nz, "ds": ds, "courant": 0.5, "periodic": [1, 0, 1]},
+        "pml": {"mode": "ref_y", "npml": [2, P, 2], "order": 3.0, "R0": math.exp(-16.0),
+                "eps_low": 1.0, "eps_high": 1.46 ** 2},
+        "materials": mats,
+        "sources": [{"axis": 1, "index": P + 2, "comp": EX, "t0": 4.0 * st, "st": st,
+                     "w0": 2.0 * PI * C0 / lam0, "amplitude": 1.0}],
+        "dft": [
+            {"axis": 1, "index": 16, "omega": list(arg), "comps": [EX, EZ, HX, HZ], "n_begin": 1, "n_end": 0},
+            {"axis": 1, "index": 56, "omega": list(arg), "comps": [EX, EZ, HX, HZ], "n_begin": 1, "n_end": 0},
+        ],
+        "probes": [[4, 42, 3], [4, 47, 3], [4, 51, 3], [4, 56, 3]],
+        "nsteps": nsteps,
+        "lam": [500e-9, 800e-9],
+    }
+
+
+def ref_ljp_graphene(nsteps=900):
+    """The three Lorentz-Jp graphene sheets (one cell thick, normal y) with
+    their third-order oscillator, in air over SiO2, y-PML, periodic x/z; the
+    4-pole sheet drives Ez with JpT_x as mod3/grapheneLorentzJp_mod.f90:68
+    does, which couples the Ex-polarised pulse into Ez."""
+    nx, ny, nz = 6, 60, 6
+    ds = 2.0 * NANO
+    P = 10
+    lam0 = 600.0 * NANO
+    st = 0.4e-15
+    arg = reference_frequencies(450e-9, 900e-9, 12)
+    nl = {"wNL": 2.976e14, "dnuNL": 3.0e12, "xi3": 2e-7}
+    mats = [
+        {"type": "dielectric", "region": box((1, P + 1, 1), (nx, 29, nz)), "eps": 1.0},
+        {"type": "lorentz_jp", "region": box((1, 30, 1), (nx, 30, nz)), "params": {"preset": "graphene", **nl}},
+        {"type": "dielectric", "region": box((1, 31, 1), (nx, 32, nz)), "eps": 1.0},
+        {"type": "lorentz_jp", "region": box((1, 33, 1), (nx, 33, nz)),
+         "params": {"preset": "graphene3_1nm", **nl}},
+        {"type": "dielectric", "region": box((1, 34, 1), (nx, 35, nz)), "eps": 1.0},
+        {"type": "lorentz_jp", "region": box((1, 36, 1), (nx, 36, nz)), "params": {"preset": "graphene_nd", **nl}},
+        {"type": "dielectric", "region": box((1, 37, 1), (nx, 40, nz)), "eps": 1.0},
+        {"type": "dielectric", "region": box((1, 41, 1), (nx, ny - P, nz)), "eps": 2.1316},
+    ]
+    return {
+        "name": "ref_ljp_graphene",
+        "grid": {"nx": nx, "ny": ny, "nz": nz, "ds": ds, "courant": 0.5, "periodic": [1, 0, 1]},
+        "pml": {"mode": "ref_y", "npml": [2, P, 2], "order": 3.0, "R0": math.exp(-16.0),
+                "eps_low": 1.0, "eps_high": 2.1316},
+        "materials": mats,
+        "sources": [{"axis": 1, "index": P + 2, "comp": EX, "t0": 4.0 * st, "st": st,
+                     "w0": 2.0 * PI * C0 / lam0, "amplitude": 1.0}],
+        "dft": [
+            {"axis": 1, "index": 16, "omega": list(arg), "comps": [EX, EZ, HX, HZ], "n_begin": 1, "n_end": 0},
+            {"axis": 1, "index": 45, "omega": list(arg), "comps": [EX, EZ, HX, HZ], "n_begin": 1, "n_end": 0},
+        ],
+        "probes": [[3, 30, 3], [3, 33, 3], [3, 36, 3], [3, 45, 3]],
+        "nsteps": nsteps,
+        "lam": [450e-9, 900e-9],
+    }
+
+
+def sim1_tfsf_shipped(nsteps=None):
+    """The reference's 1D TFSF run (sim1/program.f90:18-216) as a 3D grid of
+    1 x npos x 1 cells, periodic in x and z: ds = 1 nm, Ku = 0.99, air /
+    SiO2 / Si along y, a one-sided TFSF boundary at TFSF_bd = 25 fed by an
+    incident line in air with its own PML (:134, the line's PML is the 1D
+    module's) and the pulse at spos = 20 (beta = 2), the reflection probe at
+    monitor_ptr = 30 and the incident-line probe at monitor_pti = 20.
+    Parameters computed as the program computes them, integer truncations
+    included.  The main grid's y PML (10 cells, air below, Si above,
+    :133) is the ALL_FACES split PML over media that fill it: the 3D
+    reference slab PML keeps its split ghosts zero, which in a one-cell-wide
+    periodic grid couples the transverse components, and Ku = 0.99 is above
+    the 3D limit, so that form does not reduce to the 1D run."""
+    lam0 = 633.0 * NANO
+    f0 = C0 / lam0
+    ds = 1.0 * NANO
+    Ku = 0.99
+    dt = Ku * ds / C0
+    npos = math.floor((20.0 * lam0 + 344.0 * NANO) / ds)
+    P = 10
+    jc = int(0.5 * npos)
+    st = 2.0 * (1.0 / f0)
+    t0 = 4.0 * st
+    spos = P + math.floor(10.0 * NANO / ds)
+    bd = spos + math.floor(5.0 * NANO / ds)
+    pti, ptr = bd - math.floor(5.0 * NANO / ds), bd + math.floor(5.0 * NANO / ds)
+    tfin = math.floor((ds * float(jc)) / (C0 / 1.0) / dt)
+    Nt = 6 * tfin + 3 * math.floor(t0 / dt)
+    air = (P + 1, P + 1 + math.floor(20.0 * lam0 / ds) - 1)
+    sio2 = (air[1] + 1, air[1] + 1 + math.floor(284.0 * NANO / ds) - 1)
+    si = (sio2[1] + 1, npos - P)
+    return {
+        "name": "sim1_tfsf_shipped",
+        "grid": {"nx": 1, "ny": npos, "nz": 1, "ds": ds, "courant": Ku, "periodic": [1, 0, 1]},
+        "pml": {"mode": "all_faces", "npml": [0, P, 0], "order": 3.0, "R0": math.exp(-16.0)},
+        "materials": [
+            {"type": "dielectric", "region": box((1, 1, 1), (1, air[1], 1)), "eps": 1.0},
+            {"type": "dielectric", "region": box((1, sio2[0], 1), (1, sio2[1], 1)), "eps": 1.46 ** 2},
+            {"type": "dielectric", "region": box((1, si[0], 1), (1, npos, 1)), "eps": 3.772 ** 2},
+        ],
+        "tfsf": [{"axis": 1, "comp": EX, "lo": bd, "hi": 0, "src_index": spos, "npml": P, "eps": 1.0,
+                  "order": 3.0, "R0": math.exp(-16.0), "t0": t0, "st": st, "w0": 2.0 * PI * f0, "amplitude": 1.0}],
+        "probes": [[1, ptr, 1]],
+        "tfsf_probes": [[0, pti]],
+        "freq": list(reference_frequencies(200e-9, 1000e-9, 100)),
+        "tfinStop": tfin,
+        "nsteps": Nt if nsteps is None else nsteps,
+    }
+
+
+def running_spectrum(trace, omega, dt, ds, e_col, h_col):
+    """sim1/program.f90:270-326 from a probe trace: re/im += cos/sin(w t)*F*dt
+    with t = (n-1)*dt, then S = (-(reH*reE) - (imH*imE))*ds*z0."""
+    z0 = math.sqrt((PI * 4e-7) / 8.85418781762039e-12)  # misc/constants_mod.f90
+    omega = np.asarray(omega)
+    n = trace.shape[0]
+    t = np.arange(n, dtype=np.float64) * dt
+    re_e = np.zeros(len(omega)); im_e = np.zeros(len(omega))
+    re_h = np.zeros(len(omega)); im_h = np.zeros(len(omega))
+    for q in range(n):
+        c, s_ = np.cos(omega * t[q]), np.sin(omega * t[q])
+        e, h = trace[q, e_col], trace[q, h_col]
+        re_e = re_e + c * e * dt; im_e = im_e + s_ * e * dt
+        re_h = re_h + c * h * dt; im_h = im_h + s_ * h * dt
+    return (-(re_h * re_e) - (im_h * im_e)) * ds * z0
+
+
 def ref_c1_twin(nsteps=2000):
     """Reference-envelope twin of the 96^3 gold-sphere case: y-PML only,
     periodic x/z, a gold cube of the sphere's volume, Dx source on a y plane,

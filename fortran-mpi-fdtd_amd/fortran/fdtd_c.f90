@@ -49,6 +49,23 @@ module fdtd_c
         real(c_double) :: t0, st, w0, amplitude
     end type fdtd_pulse
 
+    ! the Lorentz-Jp family (mod3/goldLorentzJp_mod.f90 and siblings)
+    type, bind(c) :: fdtd_lorentz_jp
+        integer(c_int32_t) :: npols, jp_mask, nonlinear, ez_uses_jx
+        real(c_double) :: eps_inf
+        real(c_double) :: xi(4), w0(4), dnu(4)
+        real(c_double) :: wNL, dnuNL, xi3
+    end type fdtd_lorentz_jp
+    ! TFSF plane wave (sim1/program.f90:86-91,217-262)
+    type, bind(c) :: fdtd_tfsf
+        integer(c_int32_t) :: axis, comp, lo, hi, src_index, npml
+        real(c_double) :: eps, order, R0
+        type(fdtd_pulse) :: pulse
+    end type fdtd_tfsf
+    integer(c_int32_t), parameter :: FDTD_LJP_GOLD = 0, FDTD_LJP_SI = 1, FDTD_LJP_TITANIUM = 2
+    integer(c_int32_t), parameter :: FDTD_LJP_FILM_GOLD = 3, FDTD_LJP_GRAPHENE = 4
+    integer(c_int32_t), parameter :: FDTD_LJP_GRAPHENE3_1NM = 5, FDTD_LJP_GRAPHENE_ND = 6
+
     interface
         subroutine fdtd_drude2cp_gold(p) bind(c, name="fdtd_drude2cp_gold")
             import :: fdtd_drude2cp
@@ -92,6 +109,35 @@ module fdtd_c
             type(c_ptr), value :: sim
             type(fdtd_region), intent(in) :: region
             type(fdtd_graphene), intent(in) :: p
+        end function
+        integer(c_int) function fdtd_add_lorentz_jp(sim, region, p) bind(c, name="fdtd_add_lorentz_jp")
+            import :: c_int, c_ptr, fdtd_region, fdtd_lorentz_jp
+            type(c_ptr), value :: sim
+            type(fdtd_region), intent(in) :: region
+            type(fdtd_lorentz_jp), intent(in) :: p
+        end function
+        integer(c_int) function fdtd_lorentz_jp_preset(kind, p) bind(c, name="fdtd_lorentz_jp_preset")
+            import :: c_int, c_int32_t, fdtd_lorentz_jp
+            integer(c_int32_t), value :: kind
+            type(fdtd_lorentz_jp), intent(out) :: p
+        end function
+        integer(c_int) function fdtd_add_tfsf(sim, t, id) bind(c, name="fdtd_add_tfsf")
+            import :: c_int, c_ptr, fdtd_tfsf
+            type(c_ptr), value :: sim
+            type(fdtd_tfsf), intent(in) :: t
+            integer(c_int), intent(out) :: id
+        end function
+        integer(c_int) function fdtd_add_tfsf_probe(sim, tid, idx, id) bind(c, name="fdtd_add_tfsf_probe")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: sim
+            integer(c_int), value :: tid, idx
+            integer(c_int), intent(out) :: id
+        end function
+        integer(c_int) function fdtd_get_tfsf_line(sim, tid, e, h) bind(c, name="fdtd_get_tfsf_line")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: sim
+            integer(c_int), value :: tid
+            real(c_double), intent(out) :: e(*), h(*)
         end function
         integer(c_int) function fdtd_add_plane_source(sim, axis, idx, comp, p) bind(c, name="fdtd_add_plane_source")
             import :: c_int, c_ptr, fdtd_pulse

@@ -8,7 +8,10 @@
  * (paths relative to the reference checkout):
  *
  *   sim3/  = FDTDsimulations/3D/air-AuAntennasSiO2-Si/
+ *   sim2/  = FDTDsimulations/2D/air-nanoAntennas_Si02-Sisubstrate/
+ *   sim1/  = FDTDsimulations/1D/TFSF-air_SLG_SiO2_Sidisp/
  *   mod3/  = modules/src/3D/
+ *   mod1/  = modules/src/1D/
  *   misc/  = modules/src/misc/
  *
  * Conventions shared with the reference
@@ -253,6 +256,47 @@ int fdtd_add_graphene(fdtd_sim* sim, const fdtd_region* region,
  * D update and before E = f(D).  axis = plane normal, index 1-based.       */
 int fdtd_add_plane_source(fdtd_sim* sim, int axis, int index, int comp,
                           const fdtd_pulse* pulse);
+
+/* Total-field/scattered-field plane wave at normal incidence,
+ * sim1/program.f90:86-91,217-262 (the one-sided form; the two-sided form
+ * follows the dormant sim3/include/myTFSF_incEzHx_3Dfunc.f90 and
+ * sim2/include/myTFSF_2Dfunc.f90).  An auxiliary 1D line along `axis`
+ * (cells 1..n_axis, eps uniform, its own split-free PML of npml cells at
+ * both ends as mod1/BERpmly_ExHz_mod.f90, graded with order and R0) carries
+ * the incident wave: e(j) += Ku/eps*(h(j) - h(j-1)), the pulse added to
+ * e(src_index), h(j) += Ku*(e(j+1) - e(j)).  Planes lo..hi (hi = 0: to the
+ * end of the grid) along `axis` are total field, the rest scattered field:
+ *   after the D/E update   D_comp(lo)   -= Ku*h(lo-1),  D_comp(hi+1) += Ku*h(hi)
+ *   after the H update     H_h(lo-1)    -= s*Ku*e(lo),  H_h(hi) += s*Ku*e(hi+1)
+ * where h is the third axis and s = +1 when (comp, axis) is (x,y), (y,z) or
+ * (z,x), else -1 (the incident H is s*h).  The correction planes lie outside
+ * any PML along `axis`; transverse ALL_FACES PML cells take them into the
+ * split along `axis`.  With FDTD_PML_REF_Y the axis must be y.  Each TFSF
+ * uses two of the 8 source slots (one when one-sided).                     */
+typedef struct {
+  int32_t axis;           /* propagation direction +axis                        */
+  int32_t comp;           /* E component of the incident wave (!= axis)         */
+  int32_t lo;             /* first total-field plane (TFSF_bd, :87)             */
+  int32_t hi;             /* last total-field plane; 0: one-sided               */
+  int32_t src_index;      /* line cell receiving the pulse (spos, :86)          */
+  int32_t npml;           /* line PML cells per end (npml, :37)                 */
+  double eps;             /* relative permittivity of the line (air_ep, :134)   */
+  double order;           /* line PML grading order (:130)                      */
+  double R0;              /* line PML design reflection (:131)                  */
+  fdtd_pulse pulse;       /* e(src_index) += g(t), t = (n-1)*dt (:228-229)      */
+} fdtd_tfsf;
+
+/* Adds a TFSF plane wave (id: 0, 1, ...).                                   */
+int fdtd_add_tfsf(fdtd_sim* sim, const fdtd_tfsf* tfsf, int* id);
+
+/* Probe of the incident line at cell `index`, sim1/program.f90:272-273: a
+ * point probe (read with fdtd_get_probe) whose records hold (n*dt, the
+ * incident E in its component's slot, the incident H in its slot, zeros). */
+int fdtd_add_tfsf_probe(fdtd_sim* sim, int tfsf_id, int index, int* id);
+
+/* The incident line after the last step: e and h, n_axis + 2 doubles each
+ * (cells 0..n_axis+1).                                                       */
+int fdtd_get_tfsf_line(fdtd_sim* sim, int tfsf_id, double* e, double* h);
 
 /* Running DFT on a plane, sim3/program.f90:441-540: at step n with
  * n_begin <= n <= n_end (n_end <= 0: no end), for every selected component

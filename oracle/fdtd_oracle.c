@@ -55,7 +55,7 @@ static int o_fail(int code, const char* msg) {
 }
 const char* ofdtd_last_error(void) { return o_err; }
 
-enum { M_DIEL = 0, M_DRUDE2CP = 1, M_GRAPHENE = 2 };
+enum { M_DIEL = 0, M_DRUDE2CP = 1, M_GRAPHENE = 2, M_LJP = 3 };
 enum { S_EXY = 0, S_EXZ, S_EYZ, S_EYX, S_EZX, S_EZY, S_HXY, S_HXZ, S_HYZ, S_HYX, S_HZX, S_HZY, S_N };
 
 typedef struct {
@@ -76,11 +76,17 @@ typedef struct {
   double *Ia[3], *S[3];
   double complex* g[4][3];
   double *sn[3], *snm1[3], *snm2[3];
+  /* Lorentz-Jp (goldLorentzJp_mod.f90 family): poles, per comp P, Jp, JpT,
+   * nonlinear Q, G over the box */
+  fdtd_lorentz_jp lj;
+  double lj_ga;
+  double *P[4][3], *Jp[4][3], *JpT[3], *Q[3], *G[3];
 } omat;
 
 typedef struct {
   int axis, index, comp;
   fdtd_pulse p;
+  double g_now; /* value added this step (read by the E-form Lorentz-Jp cells) */
 } osrc;
 
 typedef struct {
@@ -93,9 +99,20 @@ typedef struct {
 
 typedef struct {
   int i, j, k;
+  int line; /* -1: grid probe; else TFSF id, j = line cell */
   int nrec, cap;
   double* rec;
 } oprobe;
+
+/* TFSF plane wave: the incident 1D line (sim1/program.f90:217-262) with its
+ * own PML object (mod1/BERpmly_ExHz_mod.f90), cells 0..n+1 */
+typedef struct {
+  fdtd_tfsf p;
+  int n, hc, sg;            /* line length, H component, orientation sign */
+  int bdlo, bdhi;           /* yBDlow = npml, yBDhigh = n - npml + 1       */
+  double *e, *h, *pe, *ph;  /* line fields and the PML object's own fields */
+  double *f1, *f2, *g1, *g2;
+} otfsf;
 
 /* one y slab of the reference's pmly object (multi-rank: low slab on rank
  * ycord=0, high slab on rank ycord=Ymax-1; Xmax=Zmax=1) */
@@ -127,13 +144,19 @@ struct osim {
   int ndfts;
   oprobe* probes;
   int nprobes;
+  otfsf* tfsfs;
+  int ntfsfs;
   double* eps_cell; /* energy weights */
+  double* Eprev[3]; /* E at the start of the material pass (Lorentz-Jp only) */
   int committed;
   int n_done;
 };
 typedef struct osim osim;
 
 #define IDX(s, i, j, k) ((long)(i) + (s)->sx * ((long)(j) + (s)->sy * (long)(k)))
+
+static double pulse_at(const fdtd_pulse* p, double t);
+static int in_pml_all(const osim* s, int i, int j, int k);
 #define CIDX(s, i, j, k) ((long)((i)-1) + (long)(s)->nx * ((long)((j)-1) + (long)(s)->ny * (long)((k)-1)))
 
 static double* zalloc(long n) { return (double*)calloc((size_t)n, sizeof(double)); }
@@ -205,9 +228,11 @@ int ofdtd_destroy(osim* s) {
     free(mt->inside);
     for (int c = 0; c < 3; ++c) {
       free(mt->Ia[c]); free(mt->S[c]); free(mt->sn[c]); free(mt->snm1[c]); free(mt->snm2[c]);
-      for (int p = 0; p < 4; ++p) free(mt->g[p][c]);
+      for (int p = 0; p < 4; ++p) { free(mt->g[p][c]); free(mt->P[p][c]); free(mt->Jp[p][c]); }
+      free(mt->JpT[c]); free(mt->Q[c]); free(mt->G[c]);
     }
   }
+  for (int c = 0; c < 3; ++c) free(s->Eprev[c]);
   free(s->mats);
   free(s->srcs);
   for (int d = 0; d < s->ndfts; ++d) {
@@ -217,6 +242,11 @@ int ofdtd_destroy(osim* s) {
   free(s->dfts);
   for (int p = 0; p < s->nprobes; ++p) free(s->probes[p].rec);
   free(s->probes);
+  for (int q = 0; q < s->ntfsfs; ++q) {
+    otfsf* t = &s->tfsfs[q];
+    free(t->e); free(t->h); free(t->pe); free(t->ph); free(t->f1); free(t->f2); free(t->g1); free(t->g2);
+  }
+  free(s->tfsfs);
   free(s->eps_cell);
   free(s);
   return 0;
@@ -407,6 +437,73 @@ int ofdtd_add_graphene(osim* s, const fdtd_region* r, const fdtd_graphene* p) {
   return 0;
 }
 
+/* <material>_LJp%initialize_material, mod3/goldLorentzJp_mod.f90:93-163:
+ * ga = 1/epinf over the box, state zero.  Nothing else is precomputed: the
+ * update evaluates g1, g2 inline as filmGoldLorentzJp_mod.f90:72-83 does
+ * (the same expressions initialize_GoldVars stores, :141-150).           */
+int ofdtd_add_lorentz_jp(osim* s, const fdtd_region* r, const fdtd_lorentz_jp* p) {
+  if (!s || !r || !p) return o_fail(FDTD_ERR_ARG, "null argument");
+  if (s->committed) return o_fail(FDTD_ERR_STATE, "set-up after commit");
+  if (p->npols < 1 || p->npols > 4 || p->jp_mask < 1 || p->jp_mask > 7 || !(p->eps_inf > 0))
+    return o_fail(FDTD_ERR_ARG, "bad Lorentz-Jp parameters");
+  omat* m = new_mat(s, r, M_LJP);
+  if (!m) return o_fail(FDTD_ERR_ARG, "bad region");
+  m->lj = *p;
+  m->lj_ga = 1.0 / p->eps_inf;
+  m->eps_inf = p->eps_inf;
+  long nb = (long)m->bx * m->by * m->bz;
+  if (nb < 1) nb = 1;
+  for (int c = 0; c < 3; ++c) {
+    for (int l = 0; l < p->npols; ++l) { m->P[l][c] = zalloc(nb); m->Jp[l][c] = zalloc(nb); }
+    m->JpT[c] = zalloc(nb);
+    m->Q[c] = zalloc(nb);
+    m->G[c] = zalloc(nb);
+  }
+  return 0;
+}
+
+/* pole tables of the Lorentz-Jp modules (w0, dnu in 2*pi*c0/micro) */
+int ofdtd_lorentz_jp_preset(int kind, fdtd_lorentz_jp* p) {
+  static const int npols[7] = {3, 3, 4, 2, 4, 3, 3};
+  static const int mask[7] = {7, 7, 7, 7, 5, 5, 5};
+  static const double einf[7] = {1.053787, 1.0, 1.1, 3.497071, 1.070682, 1.023242, 1.023242};
+  static const double xi[7][4] = {{2.862890, -0.0722398, 367.7668, 0},       /* goldLorentzJp_mod.f90:24 */
+                                  {8.0, 2.85, -0.107, 0},                    /* siLorentzJp_mod.f90:24 */
+                                  {-12.74873, 0.9605442, 67.66233, 55.0525}, /* titaniumJp_mod.f90:24 */
+                                  {2.301113, 400.0, 0, 0},                   /* filmGoldLorentzJp_mod.f90:23 */
+                                  {-0.163673, 0.843432, -0.170024, -0.121134}, /* grapheneLorentzJp_mod.f90:29 */
+                                  {4.957366, 4.455092, 1.905429, 0},         /* graphene3LorentzJp1nm_mod.f90:29 */
+                                  {0, 0, 0, 0}};                             /* NodispersiveGraphene.f90 */
+  static const double w0[7][4] = {{-2.445679, 1.480982, 0.3371279, 0},
+                                  {3.64, 2.76, 1.73, 0},
+                                  {4.36014, -1.261863, 0.4063879, -2.176176},
+                                  {-2.393546, 0.3441309, 0, 0},
+                                  {1.642784, 0.807691, 0.934364, 0.525964},
+                                  {0.573075, 1.713164, 4.598546, 0},
+                                  {0.573075, 1.713164, 4.598546, 0}};
+  static const double dnu[7][4] = {{0.2170955, 0.1429645, 0.02285188, 0},
+                                   {0.0, 0.063, 2.5, 0},
+                                   {-5.825585, 0.2087635, 1.246245, 8.059584},
+                                   {0.2460773, 0.02533959, 0, 0},
+                                   {2.267351, 1.018179, 0.621508, 0.033048},
+                                   {0.368926, 2.42099, 1.35581, 0},
+                                   {0.368926, 2.42099, 1.35581, 0}};
+  if (!p || kind < 0 || kind > 6) return o_fail(FDTD_ERR_ARG, "unknown preset");
+  const double unit = 2.0 * O_PI * O_C0 / 1e-6; /* 2.d0*pi*c0/micro */
+  memset(p, 0, sizeof *p);
+  p->npols = npols[kind];
+  p->jp_mask = mask[kind];
+  p->nonlinear = kind >= 4;
+  p->ez_uses_jx = kind == 4;
+  p->eps_inf = einf[kind];
+  for (int l = 0; l < p->npols; ++l) {
+    p->xi[l] = xi[kind][l];
+    p->w0[l] = unit * w0[kind][l];
+    p->dnu[l] = unit * dnu[kind][l];
+  }
+  return 0;
+}
+
 int ofdtd_add_plane_source(osim* s, int axis, int index, int comp, const fdtd_pulse* p) {
   if (!s || !p) return o_fail(FDTD_ERR_ARG, "null argument");
   if (s->committed) return o_fail(FDTD_ERR_STATE, "set-up after commit");
@@ -454,8 +551,176 @@ int ofdtd_add_probe(osim* s, int i, int j, int k, int* id) {
   oprobe* p = &s->probes[s->nprobes];
   memset(p, 0, sizeof *p);
   p->i = i; p->j = j; p->k = k;
+  p->line = -1;
   *id = s->nprobes++;
   return 0;
+}
+
+/* ---- TFSF ------------------------------------------------------------------ */
+/* pmly_ExHz%initialize / allocate_vars / initialize_vars,
+ * mod1/BERpmly_ExHz_mod.f90:156-255, for the two end slabs of one line (the
+ * low slab of rank ycoord = 0 and the high slab of rank Ymax-1), with
+ * eplow = ephigh = eps*ep0 (sim1/program.f90:134). */
+static void tfsf_line_profiles(const osim* s, otfsf* t) {
+  const int N = t->n, np = t->p.npml;
+  const double dt = s->dt, ds = s->ds;
+  const double z0 = sqrt(O_MU0 / O_EP0);
+  const double Ku = dt * O_C0 / ds;
+  const double ep = t->p.eps * O_EP0, epr = ep / O_EP0;
+  const double order = t->p.order;
+  const double sigmam = (order + 1.0) * O_C0 * log(1.0 / t->p.R0) / (2.0 * np * ds); /* sim1/program.f90:132 */
+  for (int j = 0; j <= N + 1; ++j) {
+    t->f1[j] = 1.0; t->f2[j] = Ku / epr; t->g1[j] = 1.0; t->g2[j] = Ku;
+  }
+  if (np <= 0) return;
+  for (int j = 1; j <= t->bdlo; ++j) {
+    double sE = ep * sigmam * pow(((double)(np - j) + 0.5) / (double)np, order);
+    double aE = sE * dt / ep;
+    t->f1[j] = exp(-aE);
+    t->f2[j] = (1.0 - t->f1[j]) / (z0 * sE * ds);
+    if (j < np) {
+      double sH = O_MU0 * sigmam * pow((double)(np - j) / (double)np, order);
+      double aH = sH * dt / O_MU0;
+      t->g1[j] = exp(-aH);
+      t->g2[j] = z0 * (1.0 - t->g1[j]) / (sH * ds);
+    }
+  }
+  int jj = np;
+  for (int j = t->bdhi; j <= N; ++j) {
+    if (jj < np) {
+      double sE = ep * sigmam * pow((double)(np - jj) / (double)np, order);
+      double aE = sE * dt / ep;
+      t->f1[j] = exp(-aE);
+      t->f2[j] = (1.0 - t->f1[j]) / (z0 * sE * ds);
+    }
+    double sH = O_MU0 * sigmam * pow(((double)(np - jj) + 0.5) / (double)np, order);
+    double aH = sH * dt / O_MU0;
+    t->g1[j] = exp(-aH);
+    t->g2[j] = z0 * (1.0 - t->g1[j]) / (sH * ds);
+    jj = jj - 1;
+  }
+}
+
+int ofdtd_add_tfsf(osim* s, const fdtd_tfsf* p, int* id) {
+  if (!s || !p || !id) return o_fail(FDTD_ERR_ARG, "null argument");
+  if (s->committed) return o_fail(FDTD_ERR_STATE, "set-up after commit");
+  int n[3] = {s->nx, s->ny, s->nz};
+  if (p->axis < 0 || p->axis > 2 || p->comp < 0 || p->comp > 2 || p->comp == p->axis)
+    return o_fail(FDTD_ERR_ARG, "bad TFSF axis/component");
+  int N = n[p->axis];
+  if (p->lo < 2 || p->lo > N || (p->hi != 0 && (p->hi < p->lo || p->hi >= N)) || p->src_index < 1 ||
+      p->src_index > N || p->npml < 0 || 2 * p->npml + 2 >= N || !(p->eps > 0))
+    return o_fail(FDTD_ERR_ARG, "bad TFSF planes");
+  s->tfsfs = (otfsf*)realloc(s->tfsfs, sizeof(otfsf) * (size_t)(s->ntfsfs + 1));
+  otfsf* t = &s->tfsfs[s->ntfsfs];
+  memset(t, 0, sizeof *t);
+  t->p = *p;
+  t->n = N;
+  t->hc = 3 - p->axis - p->comp;
+  t->sg = ((p->comp == 0 && p->axis == 1) || (p->comp == 1 && p->axis == 2) || (p->comp == 2 && p->axis == 0)) ? 1
+                                                                                                           : -1;
+  t->bdlo = p->npml;
+  t->bdhi = N - p->npml + 1;
+  t->e = zalloc(N + 2); t->h = zalloc(N + 2); t->pe = zalloc(N + 2); t->ph = zalloc(N + 2);
+  t->f1 = zalloc(N + 2); t->f2 = zalloc(N + 2); t->g1 = zalloc(N + 2); t->g2 = zalloc(N + 2);
+  tfsf_line_profiles(s, t);
+  *id = s->ntfsfs++;
+  return 0;
+}
+
+int ofdtd_add_tfsf_probe(osim* s, int tid, int index, int* id) {
+  if (!s || !id || tid < 0 || tid >= s->ntfsfs || index < 0 || index > s->tfsfs[tid].n + 1)
+    return o_fail(FDTD_ERR_ARG, "bad TFSF probe");
+  s->probes = (oprobe*)realloc(s->probes, sizeof(oprobe) * (size_t)(s->nprobes + 1));
+  oprobe* p = &s->probes[s->nprobes];
+  memset(p, 0, sizeof *p);
+  p->line = tid;
+  p->j = index;
+  *id = s->nprobes++;
+  return 0;
+}
+
+int ofdtd_get_tfsf_line(osim* s, int tid, double* e, double* h) {
+  if (!s || !e || !h || tid < 0 || tid >= s->ntfsfs) return o_fail(FDTD_ERR_ARG, "bad argument");
+  otfsf* t = &s->tfsfs[tid];
+  memcpy(e, t->e, sizeof(double) * (size_t)(t->n + 2));
+  memcpy(h, t->h, sizeof(double) * (size_t)(t->n + 2));
+  return 0;
+}
+
+/* line E half step: ex_inc update on the main cells, the pulse, then
+ * mypmly_inc%updateE (updateE_pmy_low/high + setboundaryEa/Eb),
+ * sim1/program.f90:222-231, mod1/BERpmly_ExHz_mod.f90:37-46,60-80,110-130 */
+static void tfsf_line_E(osim* s, otfsf* t, double tnow) {
+  const int N = t->n, lo = t->bdlo, hi = t->bdhi;
+  const double ck = s->ku * (1.0 / t->p.eps);
+  for (int j = lo + 1; j <= hi - 1; ++j) t->e[j] = t->e[j] + ck * (t->h[j] - t->h[j - 1]);
+  t->e[t->p.src_index] = t->e[t->p.src_index] + pulse_at(&t->p.pulse, tnow);
+  if (t->p.npml <= 0) return;
+  for (int j = 1; j <= lo; ++j) t->pe[j] = t->f1[j] * t->pe[j] + t->f2[j] * (t->ph[j] - t->ph[j - 1]);
+  t->pe[lo + 1] = t->e[lo + 1];
+  t->e[lo] = t->pe[lo];
+  for (int j = hi; j <= N; ++j) t->pe[j] = t->f1[j] * t->pe[j] + t->f2[j] * (t->ph[j] - t->ph[j - 1]);
+  t->pe[hi - 1] = t->e[hi - 1];
+  t->e[hi] = t->pe[hi];
+}
+
+/* line H half step: hz_inc update, then mypmly_inc%updateH,
+ * sim1/program.f90:252-255, mod1/BERpmly_ExHz_mod.f90:48-57,82-102,117-137 */
+static void tfsf_line_H(osim* s, otfsf* t) {
+  const int N = t->n, lo = t->bdlo, hi = t->bdhi;
+  for (int j = lo + 1; j <= hi - 1; ++j) t->h[j] = t->h[j] + s->ku * (t->e[j + 1] - t->e[j]);
+  if (t->p.npml <= 0) return;
+  for (int j = 1; j <= lo; ++j) t->ph[j] = t->g1[j] * t->ph[j] + t->g2[j] * (t->pe[j + 1] - t->pe[j]);
+  t->ph[lo + 1] = t->h[lo + 1];
+  t->h[lo] = t->ph[lo];
+  for (int j = hi; j <= N; ++j) t->ph[j] = t->g1[j] * t->ph[j] + t->g2[j] * (t->pe[j + 1] - t->pe[j]);
+  t->ph[hi - 1] = t->h[hi - 1];
+  t->h[hi] = t->ph[hi];
+}
+
+/* add v to component q (E: field 0, H: field 1) of every cell of plane
+ * `index` normal to `axis`; in ALL_FACES PML cells the value goes into the
+ * split along `axis` (E scaled by the cell's PML medium, as the sources) */
+static void plane_add(osim* s, int field, int q, int axis, int index, double v) {
+  static const int sidxE[3][3] = {{-1, S_EXY, S_EXZ}, {S_EYX, -1, S_EYZ}, {S_EZX, S_EZY, -1}};
+  static const int pairE[3][2] = {{S_EXY, S_EXZ}, {S_EYZ, S_EYX}, {S_EZX, S_EZY}};
+  static const int sidxH[3][3] = {{-1, S_HXY, S_HXZ}, {S_HYX, -1, S_HYZ}, {S_HZX, S_HZY, -1}};
+  static const int pairH[3][2] = {{S_HXY, S_HXZ}, {S_HYZ, S_HYX}, {S_HZX, S_HZY}};
+  int n[3] = {s->nx, s->ny, s->nz};
+  int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+  for (int w = 1; w <= n[a2]; ++w)
+    for (int u = 1; u <= n[a1]; ++u) {
+      int p[3];
+      p[axis] = index; p[a1] = u; p[a2] = w;
+      long c = IDX(s, p[0], p[1], p[2]);
+      if (s->pml_mode == FDTD_PML_ALL_FACES && in_pml_all(s, p[0], p[1], p[2])) {
+        if (field == 0) {
+          int si = sidxE[q][axis];
+          s->s[si][c] = s->s[si][c] + v * s->pml_gax[c];
+          s->E[q][c] = s->s[pairE[q][0]][c] + s->s[pairE[q][1]][c];
+        } else {
+          int si = sidxH[q][axis];
+          s->s[si][c] = s->s[si][c] + v;
+          s->H[q][c] = s->s[pairH[q][0]][c] + s->s[pairH[q][1]][c];
+        }
+      } else if (field == 0) {
+        s->D[q][c] = s->D[q][c] + v;
+      } else {
+        s->H[q][c] = s->H[q][c] + v;
+      }
+    }
+}
+
+/* consistency conditions, sim1/program.f90:236-238 (D) and 260-262 (H),
+ * plus the far plane of the two-sided form */
+static void tfsf_corr_D(osim* s, otfsf* t) {
+  plane_add(s, 0, t->p.comp, t->p.axis, t->p.lo, -s->ku * t->h[t->p.lo - 1]);
+  if (t->p.hi > 0) plane_add(s, 0, t->p.comp, t->p.axis, t->p.hi + 1, s->ku * t->h[t->p.hi]);
+}
+static void tfsf_corr_H(osim* s, otfsf* t) {
+  plane_add(s, 1, t->hc, t->p.axis, t->p.lo - 1, -(t->sg * s->ku) * t->e[t->p.lo]);
+  if (t->p.hi > 0) plane_add(s, 1, t->hc, t->p.axis, t->p.hi, (t->sg * s->ku) * t->e[t->p.hi + 1]);
 }
 
 /* ---- PML set-up ------------------------------------------------------------ */
@@ -665,7 +930,7 @@ static void update_D(osim* s, int x0, int x1, int y0, int y1, int z0, int z1, in
   double *hx = s->H[0], *hy = s->H[1], *hz = s->H[2];
   double *dx = s->D[0], *dy = s->D[1], *dz = s->D[2];
   double Ku = s->ku;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->n3 > 65536 && s->nz >= 8)
   for (int k = z0; k <= z1; ++k)
     for (int j = y0; j <= y1; ++j)
       for (int i = x0; i <= x1; ++i) {
@@ -688,7 +953,7 @@ static void update_H(osim* s, int x0, int x1, int y0, int y1, int z0, int z1, in
   double *hx = s->H[0], *hy = s->H[1], *hz = s->H[2];
   double *ex = s->E[0], *ey = s->E[1], *ez = s->E[2];
   double Ku = s->ku;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->n3 > 65536 && s->nz >= 8)
   for (int k = z0; k <= z1; ++k)
     for (int j = y0; j <= y1; ++j)
       for (int i = x0; i <= x1; ++i) {
@@ -717,6 +982,7 @@ static void apply_sources(osim* s, double t) {
   for (int q = 0; q < s->nsrcs; ++q) {
     osrc* sr = &s->srcs[q];
     double g = pulse_at(&sr->p, t);
+    sr->g_now = g;
     int n[3] = {s->nx, s->ny, s->nz};
     int a1 = sr->axis == 0 ? 1 : 0, a2 = sr->axis == 2 ? 1 : 2;
     for (int v = 1; v <= n[a2]; ++v)
@@ -739,13 +1005,77 @@ static void apply_sources(osim* s, double t) {
   }
 }
 
+/* update_EJp of the Lorentz-Jp family for one cell,
+ * mod3/goldLorentzJp_mod.f90:47-88 (and grapheneLorentzJp_mod.f90:52-100 for
+ * the nonlinear term and the Ez/JpxT coupling).  The E form reads E as it
+ * was before this material pass (Eprev), so an earlier writer of the same
+ * cell does not feed it; components outside jp_mask are vacuum (E = D). */
+static void ljp_cell(osim* s, omat* mt, long b, int i, int j, int k, long c) {
+  const fdtd_lorentz_jp* p = &mt->lj;
+  double **Hx = &s->H[0], **Hy = &s->H[1], **Hz = &s->H[2];
+  const double dt = s->dt;
+  const double z0 = sqrt(O_MU0 / O_EP0);
+  const double Ku = s->dt * O_C0 / s->ds;
+  double curlhzy = (*Hz)[c] - (*Hz)[IDX(s, i, j - 1, k)];
+  double curlhyz = -(*Hy)[c] + (*Hy)[IDX(s, i, j, k - 1)];
+  double curlhxz = (*Hx)[c] - (*Hx)[IDX(s, i, j, k - 1)];
+  double curlhzx = -(*Hz)[c] + (*Hz)[IDX(s, i - 1, j, k)];
+  double curlhyx = (*Hy)[c] - (*Hy)[IDX(s, i - 1, j, k)];
+  double curlhxy = -(*Hx)[c] + (*Hx)[IDX(s, i, j - 1, k)];
+  const double cur[3] = {curlhzy + curlhyz, curlhxz + curlhzx, curlhyx + curlhxy};
+  double e[3];
+  for (int q = 0; q < 3; ++q) {
+    if (!((p->jp_mask >> q) & 1)) {
+      e[q] = s->D[q][c] * 1.0;
+      s->E[q][c] = e[q];
+      continue;
+    }
+    double jt = (q == 2 && p->ez_uses_jx) ? mt->JpT[0][b] : mt->JpT[q][b];
+    e[q] = s->Eprev[q][c] + Ku * mt->lj_ga * (cur[q] - s->ds * jt);
+    /* a plane source through the cell enters as D does elsewhere: ga*g */
+    double sv = 0.0;
+    int hit = 0;
+    for (int r = 0; r < s->nsrcs; ++r) {
+      const osrc* sr = &s->srcs[r];
+      const int pos = sr->axis == 0 ? i : (sr->axis == 1 ? j : k);
+      if (sr->comp == q && pos == sr->index) { sv += sr->g_now; hit = 1; }
+    }
+    if (hit) e[q] = e[q] + mt->lj_ga * sv;
+    s->E[q][c] = e[q];
+  }
+  for (int q = 0; q < 3; ++q) {
+    if (!((p->jp_mask >> q) & 1)) continue;
+    mt->JpT[q][b] = 0.0;
+    for (int l = 0; l < p->npols; ++l) {
+      double P0 = p->xi[l] * e[q] / O_C0;
+      if (l == 0 && p->nonlinear) P0 = P0 + p->xi3 * mt->Q[q][b] * e[q] / O_C0;
+      mt->Jp[l][q][b] = ((1.0 - p->dnu[l] * dt) / (1.0 + p->dnu[l] * dt)) * mt->Jp[l][q][b] +
+                        ((p->w0[l] * p->w0[l]) * dt / (1.0 + p->dnu[l] * dt)) * (P0 - mt->P[l][q][b]);
+      mt->P[l][q][b] = mt->P[l][q][b] + dt * mt->Jp[l][q][b];
+      mt->JpT[q][b] = mt->JpT[q][b] + mt->Jp[l][q][b];
+    }
+    if (p->nonlinear) {
+      mt->G[q][b] = ((1.0 - p->dnuNL * dt) / (1.0 + p->dnuNL * dt)) * mt->G[q][b] +
+                    ((p->wNL * p->wNL) * dt / (1.0 + p->dnuNL * dt)) * ((z0 * e[q]) * (z0 * e[q]) - mt->Q[q][b]);
+      mt->Q[q][b] = mt->Q[q][b] + dt * mt->G[q][b];
+    }
+  }
+}
+
 /* update_E of every material in call order (sim3/program.f90:411-414) */
 static void update_materials(osim* s) {
   int skip_pml = s->pml_mode == FDTD_PML_ALL_FACES;
+  int any_ljp = 0;
+  for (int m = 0; m < s->nmats; ++m) any_ljp |= s->mats[m].type == M_LJP;
+  if (any_ljp)
+    for (int q = 0; q < 3; ++q) {
+      if (!s->Eprev[q]) s->Eprev[q] = zalloc(s->n3);
+      memcpy(s->Eprev[q], s->E[q], sizeof(double) * (size_t)s->n3);
+    }
   for (int m = 0; m < s->nmats; ++m) {
     omat* mt = &s->mats[m];
     if (mt->bx == 0 || mt->by == 0 || mt->bz == 0) continue;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->n3 > 65536 && s->nz >= 8)
     for (int k = mt->z0; k <= mt->z1; ++k)
       for (int j = mt->y0; j <= mt->y1; ++j)
         for (int i = mt->x0; i <= mt->x1; ++i) {
@@ -756,6 +1086,8 @@ static void update_materials(osim* s) {
           if (mt->type == M_DIEL) {
             /* simpledielectric_mod.f90:181-183 */
             for (int q = 0; q < 3; ++q) s->E[q][c] = s->D[q][c] * mt->gax;
+          } else if (mt->type == M_LJP) {
+            ljp_cell(s, mt, b, i, j, k, c);
           } else if (mt->type == M_DRUDE2CP) {
             /* gold_mod.f90:61-83 */
             for (int q = 0; q < 3; ++q) {
@@ -793,7 +1125,7 @@ static void update_materials(osim* s) {
 /* updateEs, BERpmly_mod.f90:117-159 */
 static void ref_updateEs(osim* s, oslab* sl, int yi, int yf) {
   double **S = sl->s;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->n3 > 65536 && s->nz >= 8)
   for (int k = 1; k <= s->nz; ++k)
     for (int j = yi; j <= yf; ++j)
       for (int i = 1; i <= s->nx; ++i) {
@@ -823,7 +1155,7 @@ static void ref_updateEs(osim* s, oslab* sl, int yi, int yf) {
 /* updateHs, BERpmly_mod.f90:161-195 */
 static void ref_updateHs(osim* s, oslab* sl, int yi, int yf) {
   double **S = sl->s;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->n3 > 65536 && s->nz >= 8)
   for (int k = 1; k <= s->nz; ++k)
     for (int j = yi; j <= yf; ++j)
       for (int i = 1; i <= s->nx; ++i) {
@@ -898,7 +1230,7 @@ static void all_pml_E(osim* s) {
   double **S = s->s, *hx = s->H[0], *hy = s->H[1], *hz = s->H[2];
   double *f1x = s->f1E[0], *f2x = s->f2E[0], *f1y = s->f1E[1], *f2y = s->f2E[1];
   double *f1z = s->f1E[2], *f2z = s->f2E[2];
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->n3 > 65536 && s->nz >= 8)
   for (int k = 1; k <= s->nz; ++k)
     for (int j = 1; j <= s->ny; ++j)
       for (int i = 1; i <= s->nx; ++i) {
@@ -927,7 +1259,7 @@ static void all_pml_H(osim* s) {
   double **S = s->s, *ex = s->E[0], *ey = s->E[1], *ez = s->E[2];
   double *g1x = s->g1H[0], *g2x = s->g2H[0], *g1y = s->g1H[1], *g2y = s->g2H[1];
   double *g1z = s->g1H[2], *g2z = s->g2H[2];
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->n3 > 65536 && s->nz >= 8)
   for (int k = 1; k <= s->nz; ++k)
     for (int j = 1; j <= s->ny; ++j)
       for (int i = 1; i <= s->nx; ++i) {
@@ -994,7 +1326,7 @@ static void monitors(osim* s, int n) {
       if (!(df->mask & (1 << c))) continue;
       double* F = field_ptr(s, c);
       /* sim3/program.f90:456-472 */
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->n3 > 65536)
       for (int m = 0; m < df->nfreq; ++m) {
         double cw = cos(df->omega[m] * t), sw = sin(df->omega[m] * t);
         for (int v = 1; v <= df->n2; ++v)
@@ -1015,9 +1347,16 @@ static void monitors(osim* s, int n) {
       pr->cap = pr->cap ? 2 * pr->cap : 1024;
       pr->rec = (double*)realloc(pr->rec, sizeof(double) * 7 * (size_t)pr->cap);
     }
-    long c = IDX(s, pr->i, pr->j, pr->k);
     double* r = pr->rec + 7 * (long)pr->nrec++;
     r[0] = (double)n * s->dt; /* sim3/program.f90:445 writes n*dt */
+    if (pr->line >= 0) { /* sim1/program.f90:272-273: the incident line */
+      const otfsf* t = &s->tfsfs[pr->line];
+      for (int q = 1; q < 7; ++q) r[q] = 0.0;
+      r[1 + t->p.comp] = t->e[pr->j];
+      r[4 + t->hc] = t->sg * t->h[pr->j];
+      continue;
+    }
+    long c = IDX(s, pr->i, pr->j, pr->k);
     for (int q = 0; q < 3; ++q) { r[1 + q] = s->E[q][c]; r[4 + q] = s->H[q][c]; }
   }
 }
@@ -1034,27 +1373,33 @@ int ofdtd_step(osim* s, int nsteps) {
       /* set_index: edge ranks exclude the y-PML rows (mympi_func.f90:162-169) */
       update_D(s, 1, nx, P + 1, ny - P, 1, nz, 0);
       apply_sources(s, t);
+      for (int q = 0; q < s->ntfsfs; ++q) { tfsf_line_E(s, &s->tfsfs[q], t); tfsf_corr_D(s, &s->tfsfs[q]); }
       update_materials(s);
       ref_pml_E(s);
       exchange(s, s->E, 1);
       update_H(s, 1, nx, P + 1, ny - P, 1, nz, 0);
       ref_pml_H(s);
+      for (int q = 0; q < s->ntfsfs; ++q) { tfsf_line_H(s, &s->tfsfs[q]); tfsf_corr_H(s, &s->tfsfs[q]); }
       exchange(s, s->H, 0);
     } else if (s->pml_mode == FDTD_PML_ALL_FACES) {
       update_D(s, 1, nx, 1, ny, 1, nz, 1);
       all_pml_E(s);
       apply_sources(s, t);
+      for (int q = 0; q < s->ntfsfs; ++q) { tfsf_line_E(s, &s->tfsfs[q], t); tfsf_corr_D(s, &s->tfsfs[q]); }
       update_materials(s);
       exchange(s, s->E, 1);
       update_H(s, 1, nx, 1, ny, 1, nz, 1);
       all_pml_H(s);
+      for (int q = 0; q < s->ntfsfs; ++q) { tfsf_line_H(s, &s->tfsfs[q]); tfsf_corr_H(s, &s->tfsfs[q]); }
       exchange(s, s->H, 0);
     } else {
       update_D(s, 1, nx, 1, ny, 1, nz, 0);
       apply_sources(s, t);
+      for (int q = 0; q < s->ntfsfs; ++q) { tfsf_line_E(s, &s->tfsfs[q], t); tfsf_corr_D(s, &s->tfsfs[q]); }
       update_materials(s);
       exchange(s, s->E, 1);
       update_H(s, 1, nx, 1, ny, 1, nz, 0);
+      for (int q = 0; q < s->ntfsfs; ++q) { tfsf_line_H(s, &s->tfsfs[q]); tfsf_corr_H(s, &s->tfsfs[q]); }
       exchange(s, s->H, 0);
     }
     monitors(s, n);

@@ -14,6 +14,10 @@
 !   - the y-PML is two pmly objects, one with ycord = 0 and one with
 !     ycord = Ymax-1 (Ymax = 2): the multi-rank semantics of the reference;
 !   - materials are applied in the order given (last writer wins);
+!   - the Lorentz-Jp family (mod3/*LorentzJp*_mod.f90, titaniumJp_mod.f90,
+!     NodispersiveGraphene.f90), which no reference driver calls, is applied
+!     through its update_EJp in the same material pass: E form from curl H,
+!     so those cells ignore D (test cases keep them apart from the others);
 !   - source, DFT gating, probes and flux follow sim3/program.f90:396-685.
 ! Usage: ref3d <config.nml> <outdir>
 program ref3d
@@ -24,13 +28,20 @@ program ref3d
     use gold_mod
     use graphene_mod
     use BERpmly_mod
+    use goldLorentzJp_mod
+    use siLorentzJp_mod
+    use titaniumJp_mod
+    use filmgoldLorentzJp_mod
+    use grapheneLorentzJp_mod
+    use graphene3LorentzJp1nm_mod
+    use nondispersivegrapheneJp_mod
     implicit none
     integer :: nx, ny, nz, nsteps
     real*8 :: ds, Ku, dt
     logical :: pbcx, pbcz
     integer :: npmlx, npmly, npmlz
     real*8 :: order, R0, eps_low, eps_high, sigmam
-    integer, parameter :: MAXM = 8
+    integer, parameter :: MAXM = 12
     integer :: nmat, mtype(MAXM), mbox(6, MAXM)
     real*8 :: meps(MAXM), gwNL(MAXM), gdnuNL(MAXM), gxi3(MAXM)
     logical :: s_on
@@ -38,7 +49,7 @@ program ref3d
     real*8 :: s_t0, s_st, s_w0, s_amp
     integer :: ndft, nfreq, d_index(3), d_nb(3), d_ne(3)
     real*8 :: lam_init, lam_end
-    integer :: nprobe, p_ijk(3, 3)
+    integer :: nprobe, p_ijk(3, 8)
     namelist /grid/ nx, ny, nz, ds, Ku, nsteps, pbcx, pbcz
     namelist /pml/ npmlx, npmly, npmlz, order, R0, eps_low, eps_high
     namelist /mats/ nmat, mtype, mbox, meps, gwNL, gdnuNL, gxi3
@@ -51,6 +62,14 @@ program ref3d
     type(ConstantDielectric) :: diel(MAXM)
     type(Gold) :: au(MAXM)
     type(Graphene) :: gr(MAXM)
+    ! the Lorentz-Jp family (mtype 4..10 = FDTD_LJP_* + 4), E form
+    type(gold_LJp) :: lgold(MAXM)
+    type(si_LJp) :: lsi(MAXM)
+    type(Ti_LJp) :: lti(MAXM)
+    type(filmgold_LJp) :: lfilm(MAXM)
+    type(SLG_LJp) :: lgr(MAXM)
+    type(SLG_3LJp1nm) :: lgr3(MAXM)
+    type(SLG_ndJp) :: lgrnd(MAXM)
     type(pmly) :: lowp, highp
     type(Irange) :: absIx, absIy, absIz
     real*8, allocatable :: dre(:,:,:,:,:), dim_(:,:,:,:,:), arg(:), prb(:,:,:), Sy(:,:)
@@ -115,6 +134,37 @@ program ref3d
             call gr(q)%initialize_material(absIx, absIy, absIz, (/ nx, ny, nz /))
             print *, 'graphene gax(1st cell)=', gr(q)%getRgax(gr(q)%xrange%init, gr(q)%yrange%init, &
                 gr(q)%zrange%init), gr(q)%getIgax(gr(q)%xrange%init, gr(q)%yrange%init, gr(q)%zrange%init)
+        case (4)
+            call setbox(lgold(q)%material, mbox(:, q))
+            lgold(q)%dt = dt; lgold(q)%ds = ds
+            call lgold(q)%initialize_material(absIx, absIy, absIz, (/ nx, ny, nz /))
+        case (5)
+            call setbox(lsi(q)%material, mbox(:, q))
+            lsi(q)%dt = dt; lsi(q)%ds = ds
+            call lsi(q)%initialize_material(absIx, absIy, absIz, (/ nx, ny, nz /))
+        case (6)
+            call setbox(lti(q)%material, mbox(:, q))
+            lti(q)%dt = dt; lti(q)%ds = ds
+            call lti(q)%initialize_material(absIx, absIy, absIz, (/ nx, ny, nz /))
+        case (7)
+            call setbox(lfilm(q)%material, mbox(:, q))
+            lfilm(q)%dt = dt; lfilm(q)%ds = ds
+            call lfilm(q)%initialize_material(absIx, absIy, absIz, (/ nx, ny, nz /))
+        case (8)
+            call setbox(lgr(q)%material, mbox(:, q))
+            lgr(q)%dt = dt; lgr(q)%ds = ds
+            lgr(q)%wNL = gwNL(q); lgr(q)%dnuNL = gdnuNL(q); lgr(q)%xi3 = gxi3(q)
+            call lgr(q)%initialize_material(absIx, absIy, absIz, (/ nx, ny, nz /))
+        case (9)
+            call setbox(lgr3(q)%material, mbox(:, q))
+            lgr3(q)%dt = dt; lgr3(q)%ds = ds
+            lgr3(q)%wNL = gwNL(q); lgr3(q)%dnuNL = gdnuNL(q); lgr3(q)%xi3 = gxi3(q)
+            call lgr3(q)%initialize_material(absIx, absIy, absIz, (/ nx, ny, nz /))
+        case (10)
+            call setbox(lgrnd(q)%material, mbox(:, q))
+            lgrnd(q)%dt = dt; lgrnd(q)%ds = ds
+            lgrnd(q)%wNL = gwNL(q); lgrnd(q)%dnuNL = gdnuNL(q); lgrnd(q)%xi3 = gxi3(q)
+            call lgrnd(q)%initialize_material(absIx, absIy, absIz, (/ nx, ny, nz /))
         end select
     end do
 
@@ -159,6 +209,20 @@ program ref3d
                 call au(q)%update_E(PEx, PEy, PEz, PDx, PDy, PDz)
             case (3)
                 call gr(q)%update_E(PEx, PEy, PEz, PDx, PDy, PDz)
+            case (4)
+                call lgold(q)%update_EJp(PEx, PEy, PEz, PHx, PHy, PHz)
+            case (5)
+                call lsi(q)%update_EJp(PEx, PEy, PEz, PHx, PHy, PHz)
+            case (6)
+                call lti(q)%update_EJp(PEx, PEy, PEz, PHx, PHy, PHz)
+            case (7)
+                call lfilm(q)%update_EJp(PEx, PEy, PEz, PHx, PHy, PHz)
+            case (8)
+                call lgr(q)%update_EJp(PEx, PEy, PEz, PHx, PHy, PHz)
+            case (9)
+                call lgr3(q)%update_EJp(PEx, PEy, PEz, PHx, PHy, PHz)
+            case (10)
+                call lgrnd(q)%update_EJp(PEx, PEy, PEz, PHx, PHy, PHz)
             end select
         end do
         call lowp%updateE(PEx, PEy, PEz, PHx, PHy, PHz)

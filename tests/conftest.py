@@ -37,3 +37,13 @@ def golden():
         return spec, z
 
     return load
+
+
+@pytest.fixture(scope="session")
+def golden_npz():
+    """A golden fixture without an embedded workload spec."""
+
+    def load(name):
+        return np.load(os.path.join(GOLDEN, name + ".npz"))
+
+    return load

@@ -24,6 +24,10 @@ from fdtd_amd import workloads  # noqa: E402
 
 REF_EXE = os.path.join(REPO, "oracle", "_ref", "ref3d")
 MTYPE = {"dielectric": 1, "drude2cp": 2, "graphene": 3}
+# the Lorentz-Jp modules: mtype 4 + FDTD_LJP_* (gold, si, titanium, film gold,
+# graphene, graphene3 1nm, non-dispersive graphene); only their own tables
+LJP_MTYPE = {"gold": 4, "si": 5, "titanium": 6, "film_gold": 7, "graphene": 8, "graphene3_1nm": 9,
+             "graphene_nd": 10}
 
 
 def namelist(spec):
@@ -41,7 +45,15 @@ def namelist(spec):
         r = m["region"]
         assert r["kind"] == "box"
         b = [r["lo"][0], r["hi"][0], r["lo"][1], r["hi"][1], r["lo"][2], r["hi"][2]]
-        lines.append(f" mtype({q})={MTYPE[m['type']]}, mbox(1:6,{q})={','.join(str(v) for v in b)},")
+        if m["type"] == "lorentz_jp":
+            lp = m["params"]
+            assert set(lp) <= {"preset", "wNL", "dnuNL", "xi3"}, "the modules take only their own tables"
+            mt = LJP_MTYPE[lp["preset"]]
+            if mt >= 8:
+                lines.append(f" gwNL({q})={lp['wNL']!r}, gdnuNL({q})={lp['dnuNL']!r}, gxi3({q})={lp['xi3']!r},")
+        else:
+            mt = MTYPE[m["type"]]
+        lines.append(f" mtype({q})={mt}, mbox(1:6,{q})={','.join(str(v) for v in b)},")
         if m["type"] == "dielectric":
             lines.append(f" meps({q})={m['eps']!r},")
         if m["type"] == "graphene":
@@ -126,6 +138,9 @@ CASES = {
     "ref_box_small": (workloads.ref_box_small, True),
     "ref_graphene_small": (workloads.ref_graphene_small, True),
     "ref_c1_twin": (workloads.ref_c1_twin, False),
+    "ref_ljp_metals": (workloads.ref_ljp_metals, True),
+    "ref_ljp_graphene": (workloads.ref_ljp_graphene, True),
+    "ref_ljp_titanium": (lambda: workloads.ref_ljp_metals(nsteps=300, titanium=True), True),
 }
 
 if __name__ == "__main__":

@@ -3,6 +3,7 @@ reference printed in its own run log, closed forms, convergence to analytic
 models, invariants and limits.  Each test names the reference routine the
 oracle restates and the mistake class it would catch.
 """
+import ctypes as C
 import json
 import math
 import os
@@ -176,6 +177,74 @@ def test_gold_ade_converges_to_the_analytic_drude_2cp():
     assert (errs[1] < 0.035).all()
     ratio = errs[0] / errs[1]
     assert ((ratio > 1.7) & (ratio < 2.3)).all()  # first order in dt
+
+
+def _lorentz_eps(w, p):
+    """eps(w) = eps_inf + sum_l xi_l w0_l^2 / (w0_l^2 - w^2 - 2 i dnu_l w),
+    exp(-i w t): the continuous model P'' + 2 dnu P' + w0^2 P = w0^2 xi E/c0
+    whose Jp = P' recursion the Lorentz-Jp modules discretise."""
+    eps = p.eps_inf + 0j
+    for l in range(p.npols):
+        eps = eps + p.xi[l] * p.w0[l] ** 2 / (p.w0[l] ** 2 - w * w - 2j * p.dnu[l] * w)
+    return eps
+
+
+def _impulse_ljp(ds, nsteps, params):
+    """0D Lorentz-Jp cell: E += ga*g for the source value g, so the same
+    two-pulse D impulse as _impulse gives E_n = the material's response."""
+    from fdtd_amd import box
+
+    sim = _sim(1, 1, 1, ds, 0.5, periodic=(1, 1, 1))
+    dt = sim.dt
+    sim.add_lorentz_jp(box((1, 1, 1), (1, 1, 1)), params)
+    w0 = 3 * PI / (4 * dt)
+    sim.add_plane_source(1, 1, 0, dt, dt / 10, w0, math.sqrt(2.0))
+    sim.add_plane_source(1, 1, 0, 2 * dt, dt / 10, w0, 1.0)
+    pid = sim.add_probe(1, 1, 1)
+    sim.step(nsteps)
+    return dt, sim.probe(pid)[:, 1]
+
+
+@pytest.mark.parametrize("preset", ["gold", "film_gold", "graphene3_1nm"])
+def test_lorentz_jp_converges_to_the_analytic_lorentz_sum(preset):
+    """update_EJp (mod3/goldLorentzJp_mod.f90:59-86 and siblings): the
+    discrete permittivity of a 0D cell converges at second order in dt (the
+    Jp recursion is centred: Crank-Nicolson damping, leapfrog P) to the
+    closed-form Lorentz sum of its pole table.  Catches a wrong g1/g2, a
+    dropped pole, a wrong sign of the ds*JpT term or of xi."""
+    from fdtd_amd import lorentz_jp
+
+    p = lorentz_jp({"preset": preset, "nonlinear": 0}, _oracle().binding())
+    w = 2 * PI * C0 / np.array([500e-9, 650e-9, 800e-9])
+    analytic = _lorentz_eps(w, p)
+    slowest = min(p.dnu[l] for l in range(p.npols))
+    errs = []
+    for ds in (2e-9, 1e-9):
+        dt = 0.5 * ds / C0
+        dt, e = _impulse_ljp(ds, int(30.0 / slowest / dt), {"preset": preset, "nonlinear": 0})
+        assert abs(e[-1]) < 1e-9 * np.abs(e).max()
+        errs.append(np.abs(_eps_from_impulse(dt, e, w) - analytic) / np.abs(analytic))
+    assert (errs[1] < 1e-3).all(), errs
+    ratio = errs[0] / errs[1]
+    assert ((ratio > 3.5) & (ratio < 4.5)).all(), ratio  # second order in dt
+
+
+def test_lorentz_jp_presets_agree_with_the_library():
+    """The oracle's restated pole tables and the library's
+    fdtd_lorentz_jp_preset give the same structs (the oracle's tables are
+    pinned to the modules by the ref_ljp_* golden cases)."""
+    from fdtd_amd import abi, library
+
+    try:
+        lib = library()
+    except OSError:
+        pytest.skip("HIP library not built")
+    for kind in range(7):
+        a, b = abi.LorentzJp(), abi.LorentzJp()
+        assert lib.lorentz_jp_preset(kind, C.byref(a)) == 0
+        assert _oracle().binding().lorentz_jp_preset(kind, C.byref(b)) == 0
+        assert bytes(a) == bytes(b), kind
+    assert lib.lorentz_jp_preset(7, C.byref(a)) != 0
 
 
 def test_kubo_graphene_matches_the_intraband_conductivity():
