@@ -36,6 +36,7 @@ def run_gpu(spec, precision, steps, warmup):
     t0 = time.time()
     sim, ids, pids = fdtd_amd.make_simulation(spec, precision=precision)
     sim.commit()
+    tuned = sim.autotune(3) if os.environ.get("FDTD_AUTOTUNE", "1") != "0" else None
     setup_s = time.time() - t0
     sim.step(warmup)
     sim.sync()
@@ -52,19 +53,28 @@ def run_gpu(spec, precision, steps, warmup):
         if n:
             kern[abi.KERNEL_NAMES[k]] = {"mean_ms": ms / n, "launches": n,
                                          "bytes_per_launch": sim.algorithmic_bytes(k) if k != abi.K_MON else None}
+    # the one-pass step (E and H of a step in one kernel) times everything
+    # under the E timer; the two-kernel step has separate E and H kernels
     e = kern["e_update"]
+    onepass = "h_update" not in kern
     ach = e["bytes_per_launch"] / (e["mean_ms"] * 1e-3) / 1e9
-    h = kern["h_update"]
-    ach_h = h["bytes_per_launch"] / (h["mean_ms"] * 1e-3) / 1e9
+    h_upd = None
+    if not onepass:
+        h = kern["h_update"]
+        ach_h = h["bytes_per_launch"] / (h["mean_ms"] * 1e-3) / 1e9
+        h_upd = {"achieved_GBs": ach_h, "frac": ach_h / HBM_PEAK_GBS, "mean_launch_ms": h["mean_ms"]}
     out = {
         "value": cells * steps / elapsed / 1e6,
         "ms_per_step": elapsed * 1e3 / steps,
         "setup_s": setup_s,
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "k_fused_e",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_onepass" if onepass else "k_fused_e",
                      "bytes_per_launch": e["bytes_per_launch"], "mean_launch_ms": e["mean_ms"]},
-        "h_update": {"achieved_GBs": ach_h, "frac": ach_h / HBM_PEAK_GBS, "mean_launch_ms": h["mean_ms"]},
+        "h_update": h_upd,
         "kernels": kern,
+        "step_form": None if tuned is None else ("two-kernel" if tuned[0] == 0 else
+                                                 {"onepass_tile": [63, tuned[0] - 1], "z_chunk": tuned[1]}),
         "device_bytes": sim.memory_bytes(),
         "cell_counts": dict(zip(("pml", "ade"), sim.cell_counts())),
     }
@@ -74,20 +84,26 @@ def run_gpu(spec, precision, steps, warmup):
     return out
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_fused_e_gold_dimer_512.md")
+PROFILES = os.path.join(ROOT, "profiles")
 
 
 def profiled_traffic(kernel, prec):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
-    (FETCH_SIZE and WRITE_SIZE runs, each scaled by a 1 GiB copy's reading)."""
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 summary
+    (FETCH_SIZE and WRITE_SIZE runs, each scaled by a 1 GiB copy's reading):
+    (bytes, summary file) or (None, None)."""
     try:
-        for line in open(TRAFFIC_FILE):
+        names = sorted(os.listdir(PROFILES))
+    except OSError:
+        return None, None
+    for name in names:
+        if not name.endswith(".md"):
+            continue
+        path = os.path.join(PROFILES, name)
+        for line in open(path):
             w = line.split()
             if len(w) == 4 and w[0] == "traffic_bytes_per_launch" and w[1] == kernel and w[2] == prec:
-                return float(w[3])
-    except OSError:
-        pass
-    return None
+                return float(w[3]), os.path.relpath(path, ROOT)
+    return None, None
 
 
 def cpu_baseline(spec, steps):
@@ -262,9 +278,10 @@ def main():
               file=sys.stderr, flush=True)
     spec_err = fp32_spectra_error()
     if args.n == 512 and args.seed == 0:  # the workload the summary profiles
-        r32["roofline"]["traffic"] = profiled_traffic("k_fused_e", "fp32")
-        if r32["roofline"]["traffic"] is not None:
-            r32["roofline"]["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
+        tb, src = profiled_traffic(r32["roofline"]["kernel"], "fp32")
+        r32["roofline"]["traffic"] = tb
+        if tb is not None:
+            r32["roofline"]["traffic_source"] = src
     line = {
         "metric": METRIC,
         "value": r32["value"],
@@ -290,8 +307,10 @@ def main():
         "p99_job_latency_ms": None if srv is None else srv["p99_ms"],
         "served_stream": srv,
         "batched_2d": b2,
-        "detail": {"fp32": {k: r32[k] for k in ("kernels", "h_update", "device_bytes", "cell_counts", "setup_s")},
-                   "fp64": None if r64 is None else {k: r64[k] for k in ("kernels", "h_update", "device_bytes")}},
+        "detail": {"fp32": {k: r32[k] for k in ("kernels", "h_update", "step_form", "device_bytes", "cell_counts",
+                                                 "setup_s")},
+                   "fp64": None if r64 is None else {k: r64[k] for k in ("kernels", "h_update", "step_form",
+                                                                        "device_bytes")}},
     }
     print(json.dumps(line))
 

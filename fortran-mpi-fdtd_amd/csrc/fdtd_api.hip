@@ -57,6 +57,17 @@ void* dev_alloc(Sim* s, size_t bytes) {
   return p;
 }
 
+// Stream-ordered release of one dev_alloc block (base pointer).
+void dev_free(Sim* s, void* p) {
+  for (size_t q = 0; q < s->allocs.size(); ++q)
+    if (s->allocs[q].p == p) {
+      (void)hipFreeAsync(p, s->stream);
+      s->dev_bytes -= s->allocs[q].bytes;
+      s->allocs.erase(s->allocs.begin() + (long)q);
+      return;
+    }
+}
+
 double* scratch(Sim* s, size_t bytes) {
   if ((size_t)s->red_cap < bytes) {
     s->d_red = (double*)dev_alloc(s, bytes);
@@ -380,6 +391,10 @@ int setup_all_faces(Sim* s) {
   s->d_prof = s->prec == FDTD_FP32 ? upload<float>(s, prof) : upload<double>(s, prof);
   s->d_split = dev_alloc(s, (size_t)S_N * (size_t)std::max<int64_t>(s->npml_cells, 1) * s->wsize);
   if (!s->d_prof || !s->d_split) return set_error(FDTD_ERR_NOMEM, "PML allocation");
+  if (s->onepass) {
+    s->d_split2 = dev_alloc(s, (size_t)S_N * (size_t)std::max<int64_t>(s->npml_cells, 1) * s->wsize);
+    if (!s->d_split2) return set_error(FDTD_ERR_NOMEM, "PML allocation");
+  }
   return 0;
 }
 
@@ -456,6 +471,7 @@ void* upload_table(Sim* s) {
       coef[q].v[c] = (real)m.coef[c];
       gaxe[q].v[c] = (real)m.gax[c];
       if (m.act[c] == ACT_DIEL) fl |= 1u << c;
+      if (m.act[c] == ACT_ADE) fl |= 16u << c;  // written by a dispersive kernel
     }
     coef[q].v[3] = (real)m.pgax;
     gaxe[q].v[3] = (real)m.eps;
@@ -512,12 +528,38 @@ int commit_impl(Sim* s) {
   if (s->ncell_alloc + le >= ((int64_t)1 << 31)) return set_error(FDTD_ERR_ARG, "grid too large for one handle");
   const size_t fbytes = (size_t)(s->ncell_alloc + le) * s->wsize;
   const size_t shift = (size_t)(le - 1) * s->wsize;
+  // one-pass step: FUSED engine without Lorentz-Jp materials (their E form
+  // reads the previous E in place); FDTD_ONEPASS=0 keeps the two-kernel step
+  s->onepass = 0;
+  if (s->engine == FDTD_ENGINE_FUSED) {
+    bool ljp = false;
+    for (const auto& m : s->mats) ljp |= m.type == 3;
+    // the one-pass step stays off unless asked for (FDTD_ONEPASS=1) or
+    // picked by fdtd_autotune, which then allocates the second copies
+    const char* env = std::getenv("FDTD_ONEPASS");
+    s->onepass = !ljp && env && std::atoi(env) == 1;
+    if (const char* e = std::getenv("FDTD_ONEPASS_BY")) {
+      const int v = std::atoi(e);
+      if (v == 4 || v == 8 || v == 16) s->op_by = v;
+    }
+    if (const char* e = std::getenv("FDTD_ONEPASS_KC")) {
+      const int v = std::atoi(e);
+      if (v >= 4 && v <= 4096) s->op_kc = v;
+    }
+  }
   for (int q = 0; q < 3; ++q) {
     char* e = (char*)dev_alloc(s, fbytes);
     char* h = (char*)dev_alloc(s, fbytes);
     if (!e || !h) return set_error(FDTD_ERR_NOMEM, "field allocation");
     s->E[q] = e + shift;
     s->H[q] = h + shift;
+    if (s->onepass) {
+      char* e2 = (char*)dev_alloc(s, fbytes);
+      char* h2 = (char*)dev_alloc(s, fbytes);
+      if (!e2 || !h2) return set_error(FDTD_ERR_NOMEM, "field allocation");
+      s->E2[q] = e2 + shift;
+      s->H2[q] = h2 + shift;
+    }
     if (s->engine == FDTD_ENGINE_STAGED) {
       char* d = (char*)dev_alloc(s, fbytes);
       if (!d) return set_error(FDTD_ERR_NOMEM, "field allocation");
@@ -857,14 +899,18 @@ size_t estimate_bytes(const Sim* s) {
   const int64_t ncell = px * (s->ny + 2) * (s->nz + 2);
   int eng = s->g.engine;
   if (eng == FDTD_ENGINE_AUTO) eng = s->pml_mode == FDTD_PML_REF_Y ? FDTD_ENGINE_STAGED : FDTD_ENGINE_FUSED;
-  size_t b = (size_t)(eng == FDTD_ENGINE_STAGED ? 9 : 6) * (size_t)(ncell + le) * w;
+  bool ljp = false;
+  for (const auto& m : s->mats) ljp |= m.type == 3;
+  const char* env = std::getenv("FDTD_ONEPASS");
+  const bool two = eng == FDTD_ENGINE_FUSED && !ljp && env && std::atoi(env) == 1;  // one-pass copies
+  size_t b = (size_t)(eng == FDTD_ENGINE_STAGED ? 9 : (two ? 12 : 6)) * (size_t)(ncell + le) * w;
   b += (size_t)ncell + (size_t)(s->ny + 2) * (s->nz + 2) * 8;  // ids, row ids, mixed rows
   const int n[3] = {s->nx, s->ny, s->nz};
   if (s->pml_mode == FDTD_PML_ALL_FACES) {
     const int Px = s->pml.npml[0], Py = s->pml.npml[1], Pz = s->pml.npml[2];
     const int64_t np = (int64_t)2 * Pz * s->nx * s->ny + (int64_t)(s->nz - 2 * Pz) * 2 * Py * s->nx +
                        (int64_t)(s->nz - 2 * Pz) * (s->ny - 2 * Py) * 2 * Px;
-    b += (size_t)S_N * (size_t)std::max<int64_t>(np, 1) * w + (size_t)4 * (n[0] + n[1] + n[2] + 6) * w;
+    b += (size_t)(two ? 2 : 1) * S_N * (size_t)std::max<int64_t>(np, 1) * w + (size_t)4 * (n[0] + n[1] + n[2] + 6) * w;
   } else if (s->pml_mode == FDTD_PML_REF_Y) {
     b += (size_t)2 * S_N * (size_t)(s->nx + 2) * (s->pml.npml[1] + 2) * (s->nz + 2) * w;
   }
@@ -1211,6 +1257,112 @@ int fdtd_commit(fdtd_sim* s) {
   return ensure_committed(s);
 }
 
+// second field and split copies of the one-pass step (zero-filled)
+static int alloc_onepass_copies(Sim* s) {
+  const int64_t le = 128 / (int64_t)s->wsize;
+  const size_t fbytes = (size_t)(s->ncell_alloc + le) * s->wsize;
+  const size_t shift = (size_t)(le - 1) * s->wsize;
+  for (int q = 0; q < 3; ++q) {
+    char* e2 = (char*)dev_alloc(s, fbytes);
+    char* h2 = (char*)dev_alloc(s, fbytes);
+    if (!e2 || !h2) return set_error(FDTD_ERR_NOMEM, "one-pass field copies");
+    s->E2[q] = e2 + shift;
+    s->H2[q] = h2 + shift;
+  }
+  if (s->d_split) {
+    s->d_split2 = dev_alloc(s, (size_t)S_N * (size_t)std::max<int64_t>(s->npml_cells, 1) * s->wsize);
+    if (!s->d_split2) return set_error(FDTD_ERR_NOMEM, "one-pass split copies");
+  }
+  return 0;
+}
+
+static void free_onepass_copies(Sim* s) {
+  const size_t shift = (size_t)(128 / (int64_t)s->wsize - 1) * s->wsize;
+  for (int q = 0; q < 3; ++q) {
+    if (s->E2[q]) dev_free(s, (char*)s->E2[q] - shift);
+    if (s->H2[q]) dev_free(s, (char*)s->H2[q] - shift);
+    s->E2[q] = s->H2[q] = nullptr;
+  }
+  if (s->d_split2) dev_free(s, s->d_split2);
+  s->d_split2 = nullptr;
+}
+
+int fdtd_autotune(fdtd_sim* s, int steps, int* out_by, int* out_kc) {
+  GUARD(s);
+  if (steps < 1) return set_error(FDTD_ERR_ARG, "autotune needs at least one timed step");
+  int rc = ensure_committed(s);
+  if (rc) return rc;
+  if (s->n_done != 0 || s->fields_set)
+    return set_error(FDTD_ERR_STATE, "autotune runs before the first step, on untouched fields");
+  FDTD_HIP(hipSetDevice(s->device));
+  // candidates: the two-kernel step (by = 0) and one-pass tiles of 64 x by
+  // threads marching kc planes
+  std::vector<std::pair<int, int>> cand;
+  bool ljp = false;
+  for (const auto& m : s->mats) ljp |= m.type == 3;
+  bool added = false;  // second copies allocated here for the trial
+  if (s->engine == FDTD_ENGINE_FUSED && !ljp && !s->E2[0]) {
+    rc = alloc_onepass_copies(s);
+    if (rc) return rc;
+    added = true;
+  }
+  if (s->engine == FDTD_ENGINE_FUSED && s->E2[0]) {
+    cand.push_back({0, 0});
+    for (int by : {4, 8, 16})
+      for (int kc : {16, 32, 64, 128}) {
+        if (kc > s->nz && kc != 16) continue;
+        cand.push_back({by, kc});
+      }
+  }
+  if (cand.empty()) {
+    if (out_by) *out_by = 0;
+    if (out_kc) *out_kc = 0;
+    return 0;
+  }
+  hipEvent_t e0, e1;
+  FDTD_HIP(hipEventCreate(&e0));
+  FDTD_HIP(hipEventCreate(&e1));
+  const bool prof = s->profiling;
+  s->profiling = false;
+  s->dry = true;
+  float best = -1.f;
+  std::pair<int, int> pick = cand[0];
+  for (const auto& c : cand) {
+    s->onepass = c.first > 0;
+    if (c.first > 0) {
+      s->op_by = c.first;
+      s->op_kc = c.second;
+    }
+    for (int w = 0; w < 2 && rc == 0; ++w) rc = launch_step(s);
+    if (rc == 0 && hipEventRecord(e0, s->stream) != hipSuccess) rc = set_error(FDTD_ERR_HIP, "event");
+    for (int it = 0; it < steps && rc == 0; ++it) rc = launch_step(s);
+    if (rc == 0 && hipEventRecord(e1, s->stream) != hipSuccess) rc = set_error(FDTD_ERR_HIP, "event");
+    float ms = 0.f;
+    if (rc == 0 && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      rc = set_error(FDTD_ERR_HIP, "event timing");
+    if (rc) break;
+    if (best < 0.f || ms < best) {
+      best = ms;
+      pick = c;
+    }
+  }
+  s->dry = false;
+  s->profiling = prof;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  s->onepass = pick.first > 0;
+  if (pick.first > 0) {
+    s->op_by = pick.first;
+    s->op_kc = pick.second;
+  } else if (added) {
+    free_onepass_copies(s);
+  }
+  if (out_by) *out_by = pick.first;
+  if (out_kc) *out_kc = pick.second;
+  return 0;
+}
+
 int fdtd_step(fdtd_sim* s, int nsteps) {
   GUARD(s);
   if (nsteps < 0) return set_error(FDTD_ERR_ARG, "negative step count");
@@ -1303,6 +1455,7 @@ int fdtd_set_field(fdtd_sim* s, int comp, const double* src) {
   if (!f) return set_error(FDTD_ERR_UNSUPPORTED, "D is kept only by the STAGED engine");
   rc = s->prec == FDTD_FP32 ? write_field<float>(s, f, src) : write_field<double>(s, f, src);
   if (rc) return rc;
+  s->fields_set = true;
   rc = launch_set_ghosts(s, comp);
   if (rc) return rc;
   FDTD_HIP(hipStreamSynchronize(s->stream));
@@ -1394,7 +1547,13 @@ int fdtd_algorithmic_bytes(fdtd_sim* s, int kernel, double* bytes) {
   if (s->engine == FDTD_ENGINE_FUSED) {
     // per non-PML cell: read H(3) + read/write E(6); per PML cell: read the
     // other field (3), read/write 6 splits (12), write the main field (3).
-    if (kernel == FDTD_K_E || kernel == FDTD_K_H) b = w * (9.0 * (N - s->n_pml) + 18.0 * s->n_pml);
+    if (s->onepass) {
+      // one pass per step: read E, H and write E, H (12 words); a PML cell
+      // reads H, reads and writes 12 splits, writes E and H (33 words)
+      if (kernel == FDTD_K_E) b = w * (12.0 * (N - s->n_pml) + 33.0 * s->n_pml);
+    } else if (kernel == FDTD_K_E || kernel == FDTD_K_H) {
+      b = w * (9.0 * (N - s->n_pml) + 18.0 * s->n_pml);
+    }
     if (kernel == FDTD_K_ADE)
       for (auto& g : s->groups) b += (w * 6.0 + (g.kind == 1 ? 8.0 : w) * 2.0 * 3.0 * g.nstate) * (double)g.ncells;
   } else {

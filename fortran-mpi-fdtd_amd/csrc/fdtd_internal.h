@@ -206,6 +206,19 @@ struct Sim {
   // whether fused kernels deal tiles to XCDs in contiguous runs
   int bx = 64, by = 4;
   int xcdmap = 1;
+  // one-pass step (FUSED engine): E and H of a step in one z-marching
+  // kernel over ping-pong buffers; E2/H2/d_split2 hold the other copy.
+  // Tiles are 64 x op_by threads (63 x (op_by-1) cells owned), op_kc planes
+  // deep.
+  int onepass = 0;
+  int op_by = 16, op_kc = 64;
+  // dry steps (autotuner): no sources, TFSF or monitors, no step count; run
+  // only while every field is still zero, so they leave the state as it was
+  bool dry = false;
+  bool fields_set = false;  // set_field was called
+  void* E2[3] = {nullptr, nullptr, nullptr};
+  void* H2[3] = {nullptr, nullptr, nullptr};
+  void* d_split2 = nullptr;
   int32_t* d_rowmat = nullptr;  // uniform material id of a (j,k) row, or -1
   int32_t* d_mixrows = nullptr;  // rows j + (ny+2)*k whose material ids differ
   int nmix = 0;

@@ -294,6 +294,256 @@ __global__ __launch_bounds__(512) void k_fused_h(FieldArgs<real> a, PmlArgs<real
 }
 
 // ---------------------------------------------------------------------------
+// One-pass step (FUSED engine): the E half step and the H half step of one
+// time step in a single kernel, so each field crosses HBM once per step
+// (read E, H, write E, H: 12 words per interior cell instead of 18).
+//
+// Each block owns a tile of 63 x (BY-1) cells of an xy plane range and
+// marches a chunk of kc planes in z.  Its 64 x BY threads compute the new E
+// on the tile plus one halo column (i+1) and one halo row (j+1), and on the
+// plane after the chunk, because the new H of a cell needs the new E at i+1,
+// j+1 and k+1.  Halo values are computed, never stored: the fields, the PML
+// splits and the dispersive state live in two copies (old: read, new:
+// written), so a cell's neighbours read old values however the blocks are
+// ordered.  Per plane k the block computes E(k), stores it, puts Ex and Ez
+// in LDS (three rotating plane buffers, one barrier per plane) and then the
+// H of plane k-1: E(k-1) at i+1 by a lane shuffle, at j+1 from LDS, E(k)
+// and the old H(k-1) from registers (the register queue of the z march).
+//
+// The arithmetic of every cell is that of k_fused_e / k_fused_h term for
+// term.  Dispersive components (flag bit 4+q) were written into the new E
+// by k_ade before this kernel; components nobody writes keep their value.
+template <typename real>
+struct OnePassArgs {
+  const real* eo[3];  // E after the previous step
+  const real* ho[3];  // H after the previous step
+  real* sn[S_N];      // split fields written by this step (p.s: read)
+  int32_t kc, ntx, nty, nch, tper, ntab;
+};
+
+// Workgroup barrier for LDS only: this kernel never reads global data that
+// another thread of the launch writes (old and new copies are distinct), so
+// the loads of the next plane stay in flight across it.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// Old fields around one cell of one plane, loaded a plane ahead of use.
+template <typename real>
+struct PlaneLoad {
+  real hx, hy, hz, hz_jm, hx_jm, hz_im, hy_im, ex, ey, ez;
+  uint32_t id;
+};
+
+template <typename real>
+__device__ __forceinline__ void load_plane(const OnePassArgs<real>& o, const FieldArgs<real>& a, int c,
+                                           PlaneLoad<real>& L) {
+  L.hx = o.ho[0][c];
+  L.hy = o.ho[1][c];
+  L.hz = o.ho[2][c];
+  L.hz_jm = o.ho[2][c - a.px];
+  L.hx_jm = o.ho[0][c - a.px];
+  L.hz_im = o.ho[2][c - 1];
+  L.hy_im = o.ho[1][c - 1];
+  L.ex = o.eo[0][c];
+  L.ey = o.eo[1][c];
+  L.ez = o.eo[2][c];
+  L.id = a.matid[c];
+}
+
+template <typename real, int BY>
+__global__ __launch_bounds__(64 * BY) void k_onepass(FieldArgs<real> a, PmlArgs<real> p, OnePassArgs<real> o,
+                                                     SrcArgs<real> src, int nsrc) {
+  constexpr int kTab = 64;  // material-table entries kept in LDS
+  __shared__ real lds[3][2][BY][64];  // [plane k % 3][Ex, Ez][row][lane]
+  __shared__ Vec4<real> tcoef[kTab], tgaxe[kTab];
+  __shared__ uint32_t tflags[kTab];
+  const int lane = threadIdx.x, row = threadIdx.y;
+  const int b = blockIdx.x;
+  const int t = a.xcdmap ? (b & 7) * o.tper + (b >> 3) : b;
+  if (t >= o.ntx * o.nty * o.nch) return;  // whole (padding) block
+  const bool lds_tab = o.ntab <= kTab;
+  if (lds_tab)
+    for (int q = lane + 64 * row; q < o.ntab; q += 64 * BY) {
+      tcoef[q] = a.coef[q];
+      tgaxe[q] = a.gaxe[q];
+      tflags[q] = a.flags[q];
+    }
+  __syncthreads();
+  const int tx = t % o.ntx;
+  const int r1 = t / o.ntx;
+  const int ty = r1 % o.nty;
+  const int tz = r1 / o.nty;
+  const int i0 = tx * 63 + lane + 1;
+  const int j0 = ty * (BY - 1) + __builtin_amdgcn_readfirstlane(row) + 1;  // a wavefront is one row
+  const int k0 = tz * o.kc + 1;
+  const int k1 = min(k0 + o.kc - 1, a.nz);
+  const bool own = lane < 63 && row < BY - 1 && i0 <= a.nx && j0 <= a.ny;
+  // physical cell of this lane: the halo column/row past the last cell wraps
+  // on a periodic axis and is a zero ghost otherwise
+  int i = i0, j = j0;
+  bool live = true;
+  if (i0 > a.nx) {
+    if (i0 == a.nx + 1 && a.per[0]) i = 1;
+    else live = false;
+  }
+  if (j0 > a.ny) {
+    if (j0 == a.ny + 1 && a.per[1]) j = 1;
+    else live = false;
+  }
+  const bool pmlxy = i <= p.P[0] || i > a.nx - p.P[0] || j <= p.P[1] || j > a.ny - p.P[1];
+  // plane index of march step k (wrapped past nz on a periodic z axis) and
+  // whether this lane computes a cell there
+  auto plane = [&](int k, int& kk) {
+    kk = k;
+    if (k > a.nz) {
+      if (!a.per[2]) return false;
+      kk = 1;
+    }
+    return live;
+  };
+  real hxp = 0, hyp = 0, hzp = 0;  // old H of plane k-1 at this cell
+  real exq = 0, eyq = 0, ezq = 0;  // new E of plane k-1 at this cell
+  if (live) {
+    const int c = i + a.px * j + a.pxy * (k0 - 1);
+    hxp = o.ho[0][c];
+    hyp = o.ho[1][c];
+    hzp = o.ho[2][c];
+  }
+  // one march step: E of plane k from `cur` (loaded a step earlier), the
+  // loads of plane k+1 into `nxt`, H of plane k-1.  Two register sets swap
+  // roles from step to step (no copies, which would wait for the loads).
+  auto march = [&](int k, const PlaneLoad<real>& cur, PlaneLoad<real>& nxt) __attribute__((always_inline)) {
+    int kk;
+    const bool lk = plane(k, kk);
+    int kn;
+    if (k + 1 <= k1 + 1 && plane(k + 1, kn)) load_plane(o, a, i + a.px * j + a.pxy * kn, nxt);  // z register queue
+    real ex = 0, ey = 0, ez = 0;
+    const bool store_e = own && k <= k1;
+    if (lk) {
+      const int c = i + a.px * j + a.pxy * kk;
+      real sv[3] = {0, 0, 0};
+      int sa[3] = {-1, -1, -1};
+      plane_sources(src, nsrc, i, j, kk, sv, sa);
+      const real hx = cur.hx, hy = cur.hy, hz = cur.hz;
+      const real hz_jm = cur.hz_jm, hx_jm = cur.hx_jm, hz_im = cur.hz_im, hy_im = cur.hy_im;
+      const real hy_km = hyp, hx_km = hxp;
+      const unsigned id = cur.id;
+      const uint32_t fl = lds_tab ? tflags[id] : a.flags[id];
+      const Vec4<real> cf = lds_tab ? tcoef[id] : a.coef[id];
+      if (pmlxy || kk <= p.P[2] || kk > a.nz - p.P[2]) {
+        // mod3/BERpmly_mod.f90:150-155 with the splits driven by total H
+        const int s = pml_index(p.P, i, j, kk, a.nx, a.ny, a.nz);
+        const real r = cf.v[3];
+        real eyx = p.f1[0][i] * p.s[S_EYX][s] + (p.f2[0][i] * r) * (-hz + hz_im);
+        real ezx = p.f1[0][i] * p.s[S_EZX][s] + (p.f2[0][i] * r) * (hy + (-hy_im));
+        real exy = p.f1[1][j] * p.s[S_EXY][s] + (p.f2[1][j] * r) * (hz + (-hz_jm));
+        real ezy = p.f1[1][j] * p.s[S_EZY][s] + (p.f2[1][j] * r) * (-hx + hx_jm);
+        real exz = p.f1[2][kk] * p.s[S_EXZ][s] + (p.f2[2][kk] * r) * (-hy + hy_km);
+        real eyz = p.f1[2][kk] * p.s[S_EYZ][s] + (p.f2[2][kk] * r) * (hx + (-hx_km));
+        if (sa[0] == 1) exy = exy + sv[0] * r;
+        if (sa[0] == 2) exz = exz + sv[0] * r;
+        if (sa[1] == 0) eyx = eyx + sv[1] * r;
+        if (sa[1] == 2) eyz = eyz + sv[1] * r;
+        if (sa[2] == 0) ezx = ezx + sv[2] * r;
+        if (sa[2] == 1) ezy = ezy + sv[2] * r;
+        if (store_e) {
+          o.sn[S_EYX][s] = eyx; o.sn[S_EZX][s] = ezx; o.sn[S_EXY][s] = exy;
+          o.sn[S_EZY][s] = ezy; o.sn[S_EXZ][s] = exz; o.sn[S_EYZ][s] = eyz;
+        }
+        ex = exy + exz;
+        ey = eyz + eyx;
+        ez = ezx + ezy;
+        if (store_e) {
+          a.e[0][c] = ex; a.e[1][c] = ey; a.e[2][c] = ez;
+          put_e_ghost(a.e[0], ex, i, j, kk, c, a);
+          put_e_ghost(a.e[1], ey, i, j, kk, c, a);
+          put_e_ghost(a.e[2], ez, i, j, kk, c, a);
+        }
+      } else {
+        const real e0[3] = {cur.ex, cur.ey, cur.ez};
+        const bool any_src = sa[0] >= 0 || sa[1] >= 0 || sa[2] >= 0;
+        Vec4<real> gx;
+        if (any_src) gx = lds_tab ? tgaxe[id] : a.gaxe[id];
+        const real cu[3] = {(hz - hz_jm) + (-hy + hy_km),   // curlhzy + curlhyz
+                            (hx - hx_km) + (-hz + hz_im),   // curlhxz + curlhzx
+                            (hy - hy_im) + (-hx + hx_jm)};  // curlhyx + curlhxy
+        real ev[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          if (fl & (1u << q)) {
+            ev[q] = e0[q] + cf.v[q] * cu[q];
+            if (sa[q] >= 0) ev[q] = ev[q] + gx.v[q] * sv[q];
+          } else if (fl & (16u << q)) {
+            ev[q] = a.e[q][c];  // written by k_ade for this step
+            // wait here, in the rare dispersive branch, rather than where
+            // the branches merge (which would drain the plane queue)
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+          } else {
+            ev[q] = e0[q];
+          }
+          if (store_e && !(fl & (16u << q))) {
+            a.e[q][c] = ev[q];
+            put_e_ghost(a.e[q], ev[q], i, j, kk, c, a);
+          }
+        }
+        ex = ev[0];
+        ey = ev[1];
+        ez = ev[2];
+      }
+    }
+    lds[k % 3][0][row][lane] = ex;
+    lds[k % 3][1][row][lane] = ez;
+    // new E of plane k-1 at i+1 (the next lane; lane 63 owns no H)
+    const real ey_ip = __shfl_down(eyq, 1);
+    const real ez_ip = __shfl_down(ezq, 1);
+    lds_barrier();
+    if (k > k0 && own) {
+      const int kh = k - 1;
+      const int c = i + a.px * j + a.pxy * kh;
+      const real ex_jp = lds[(k - 1) % 3][0][row + 1][lane];
+      const real ez_jp = lds[(k - 1) % 3][1][row + 1][lane];
+      const real ey_kp = ey, ex_kp = ex;
+      real nhx, nhy, nhz;
+      if (pmlxy || kh <= p.P[2] || kh > a.nz - p.P[2]) {
+        // mod3/BERpmly_mod.f90:186-191 with the splits driven by total E
+        const int s = pml_index(p.P, i, j, kh, a.nx, a.ny, a.nz);
+        const real hyx = p.g1[0][i] * p.s[S_HYX][s] + p.g2[0][i] * (-ezq + ez_ip);
+        const real hzx = p.g1[0][i] * p.s[S_HZX][s] + p.g2[0][i] * (eyq + (-ey_ip));
+        const real hxy = p.g1[1][j] * p.s[S_HXY][s] + p.g2[1][j] * (ezq + (-ez_jp));
+        const real hzy = p.g1[1][j] * p.s[S_HZY][s] + p.g2[1][j] * (-exq + ex_jp);
+        const real hxz = p.g1[2][kh] * p.s[S_HXZ][s] + p.g2[2][kh] * (-eyq + ey_kp);
+        const real hyz = p.g1[2][kh] * p.s[S_HYZ][s] + p.g2[2][kh] * (exq + (-ex_kp));
+        o.sn[S_HYX][s] = hyx; o.sn[S_HZX][s] = hzx; o.sn[S_HXY][s] = hxy;
+        o.sn[S_HZY][s] = hzy; o.sn[S_HXZ][s] = hxz; o.sn[S_HYZ][s] = hyz;
+        nhx = hxy + hxz;
+        nhy = hyz + hyx;
+        nhz = hzx + hzy;
+      } else {
+        nhx = hxp + a.ku * ((ey_kp - eyq) + (-ez_jp + ezq));
+        nhy = hyp + a.ku * ((ez_ip - ezq) + (-ex_kp + exq));
+        nhz = hzp + a.ku * ((ex_jp - exq) + (-ey_ip + eyq));
+      }
+      a.h[0][c] = nhx;
+      a.h[1][c] = nhy;
+      a.h[2][c] = nhz;
+      put_h_ghost(a.h[0], nhx, i, j, kh, c, a);
+      put_h_ghost(a.h[1], nhy, i, j, kh, c, a);
+      put_h_ghost(a.h[2], nhz, i, j, kh, c, a);
+    }
+    exq = ex; eyq = ey; ezq = ez;
+    hxp = cur.hx; hyp = cur.hy; hzp = cur.hz;
+  };
+  PlaneLoad<real> A{}, B{};
+  {
+    int kk;
+    if (plane(k0, kk)) load_plane(o, a, i + a.px * j + a.pxy * kk, A);
+  }
+  for (int k = k0; k <= k1 + 1; k += 2) {
+    march(k, A, B);
+    if (k + 1 <= k1 + 1) march(k + 1, B, A);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Dispersive cells: Drude + up to 4 critical points + Raman
 // (mod3/gold_mod.f90:61-83, mod3/graphene_mod.f90:70-100).  State SoA:
 // word w of component q for cell e at state[(w*3+q)*n + e];
@@ -407,12 +657,12 @@ __global__ __launch_bounds__(256) void k_ljp(FieldArgs<real> a, LjpCoef<double> 
                        (hx - a.h[0][c - a.pxy]) + (-hz + a.h[2][c - 1]),     // curlhxz + curlhzx
                        (hy - a.h[1][c - 1]) + (-hx + a.h[0][c - a.px])};     // curlhyx + curlhxy
   real sv[3] = {0, 0, 0};
-  bool hit[3] = {false, false, false};
+  int hit = 0;  // bit q: a source plane through this cell feeds E_q
   for (int q = 0; q < nsrc; ++q) {
     const int pos = src.axis[q] == 0 ? i : (src.axis[q] == 1 ? j : k);
     if (pos == src.index[q]) {
       sv[src.comp[q]] += src_value(src, q);
-      hit[src.comp[q]] = true;
+      hit |= 1 << src.comp[q];
     }
   }
   const int np = co.npols;
@@ -432,7 +682,7 @@ __global__ __launch_bounds__(256) void k_ljp(FieldArgs<real> a, LjpCoef<double> 
     if (!(mask & (1 << q))) continue;
     const double jd = (q == 2 && co.ezjx) ? jt[0] : jt[q];
     double ev = (double)a.e[q][c] + co.kuga * ((double)cur[q] - co.ds * jd);
-    if (hit[q]) ev = ev + co.ga * (double)sv[q];
+    if ((hit >> q) & 1) ev = ev + co.ga * (double)sv[q];
     const real er = (real)ev;
     a.e[q][c] = er;
     put_e_ghost(a.e[q], er, i, j, k, c, a);
@@ -1118,6 +1368,7 @@ int step_impl(Sim* s) {
   FieldArgs<real> a = field_args<real>(s);
   int nsrc = 0;
   SrcArgs<real> src = source_args<real>(s, n, &nsrc);
+  if (s->dry) nsrc = 0;
   const dim3 blk((unsigned)s->bx, (unsigned)s->by, 1);
   hipStream_t st = s->stream;
   auto ade_coef = [&](const AdeGroup& g) {
@@ -1144,6 +1395,7 @@ int step_impl(Sim* s) {
   // h(lo-1) before this step's line H update); H half step, consistency
   // terms on H and the line probes after the main H stage
   auto tfsf_e = [&](hipStream_t sq) {
+    if (s->dry) return;
     for (const auto& t : s->tfsfs) {
       const fdtd_pulse& pu = t.p.pulse;
       const double tt = (double)(n - 1) * s->dt;
@@ -1154,6 +1406,7 @@ int step_impl(Sim* s) {
     }
   };
   auto tfsf_h = [&](hipStream_t sq, PmlArgs<real> pa, int all_faces) {
+    if (s->dry) return;
     const int np = (int)s->probes.size();
     double* row = np ? s->d_probe + (int64_t)7 * np * (s->probe_row0 + (n - 1 - s->call_base)) : nullptr;
     for (const auto& t : s->tfsfs) {
@@ -1167,7 +1420,56 @@ int step_impl(Sim* s) {
                          (double)n * s->dt, n1, n2);
     }
   };
-  if (s->engine == FDTD_ENGINE_FUSED) {
+  if (s->engine == FDTD_ENGINE_FUSED && s->onepass) {
+    // new fields go to the second copies, then the copies swap roles
+    FieldArgs<real> an = a;
+    for (int q = 0; q < 3; ++q) {
+      an.e[q] = (real*)s->E2[q];
+      an.h[q] = (real*)s->H2[q];
+    }
+    for (auto& g : s->groups) {
+      if (!g.ncells) continue;
+      Timer tm(s, FDTD_K_ADE);
+      FieldArgs<real> ad = a;  // D form from the old H into the new E
+      for (int q = 0; q < 3; ++q) ad.e[q] = an.e[q];
+      hipLaunchKernelGGL((k_ade<real, true>), dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, ad,
+                         ade_coef(g), g.d_cells, g.d_mask, (real*)g.d_state, g.ncells, src, nsrc);
+    }
+    PmlArgs<real> p = pml_args<real>(s);
+    OnePassArgs<real> o{};
+    for (int q = 0; q < 3; ++q) {
+      o.eo[q] = a.e[q];
+      o.ho[q] = a.h[q];
+    }
+    for (int q = 0; q < S_N; ++q)
+      o.sn[q] = s->d_split2 ? (real*)s->d_split2 + (int64_t)q * s->npml_cells : nullptr;
+    const int BY = s->op_by;
+    o.kc = s->op_kc;
+    o.ntx = (s->nx + 62) / 63;
+    o.nty = (s->ny + BY - 2) / (BY - 1);
+    o.nch = (s->nz + o.kc - 1) / o.kc;
+    o.ntab = s->ntab;
+    const int tiles = o.ntx * o.nty * o.nch;
+    o.tper = (tiles + 7) / 8;
+    const dim3 grid((unsigned)(an.xcdmap ? 8 * o.tper : tiles));
+    {
+      Timer tm(s, FDTD_K_E);
+      if (BY == 16)
+        hipLaunchKernelGGL((k_onepass<real, 16>), grid, dim3(64, 16), 0, st, an, p, o, src, nsrc);
+      else if (BY == 8)
+        hipLaunchKernelGGL((k_onepass<real, 8>), grid, dim3(64, 8), 0, st, an, p, o, src, nsrc);
+      else
+        hipLaunchKernelGGL((k_onepass<real, 4>), grid, dim3(64, 4), 0, st, an, p, o, src, nsrc);
+    }
+    for (int q = 0; q < 3; ++q) {
+      std::swap(s->E[q], s->E2[q]);
+      std::swap(s->H[q], s->H2[q]);
+    }
+    std::swap(s->d_split, s->d_split2);
+    a = field_args<real>(s);
+    tfsf_e(st);
+    tfsf_h(st, pml_args<real>(s), 1);
+  } else if (s->engine == FDTD_ENGINE_FUSED) {
     PmlArgs<real> p = pml_args<real>(s);
     {
       Timer tm(s, FDTD_K_E);
@@ -1244,6 +1546,7 @@ int step_impl(Sim* s) {
   }
   // monitors: buffer this step's plane samples, fold a full block
   (void)t;
+  if (s->dry) return 0;
   for (auto& d : s->dfts) {
     if (n < d.n_begin) continue;
     if (d.n_end > 0 && n > d.n_end) continue;

@@ -273,6 +273,13 @@ class Simulation:
         self.b.check(self.b.commit(self.h))
 
     # ---- time loop
+    def autotune(self, steps=5):
+        """fdtd_autotune: time the FUSED step forms on this handle, keep the
+        fastest; returns (by, kc), by = 0 for the two-kernel step."""
+        by, kc = C.c_int(), C.c_int()
+        self.b.check(self.b.autotune(self.h, int(steps), C.byref(by), C.byref(kc)))
+        return by.value, kc.value
+
     def step(self, n=1):
         self.b.check(self.b.step(self.h, int(n)))
 

@@ -180,6 +180,7 @@ GPU_PROTOTYPES = {
     "algorithmic_bytes": (_I, [_P, _I, _PD]),
     "cell_counts": (_I, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "estimate_bytes": (_I, [_P, C.POINTER(C.c_size_t)]),
+    "autotune": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I)]),
     "pool_create": (_I, [C.POINTER(_P), C.c_size_t, _I, _I]),
     "pool_destroy": (_I, [_P]),
     "pool_submit": (_I, [_P, _P, _I, C.POINTER(_I)]),

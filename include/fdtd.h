@@ -315,6 +315,16 @@ int fdtd_add_probe(fdtd_sim* sim, int i, int j, int k, int* id);
  * by the first call that needs it).                                        */
 int fdtd_commit(fdtd_sim* sim);
 
+/* Optional, no reference counterpart: pick the FUSED engine's step form by
+ * timing `steps` steps of each candidate on this handle -- the two-kernel
+ * step and the one-pass step with tiles of 64 x by threads marching kc
+ * planes (by in {4, 8, 16}, kc in {16, 32, 64, 128}).  The timed steps run
+ * without sources or monitors on the untouched (zero) fields and leave them
+ * zero; call it after set-up and before the first fdtd_step (and before any
+ * fdtd_set_field).  *by = 0 reports the two-kernel step.  Other engines:
+ * returns 0 with *by = *kc = 0.                                             */
+int fdtd_autotune(fdtd_sim* sim, int steps, int* by, int* kc);
+
 /* ---- time loop ----------------------------------------------------------- */
 /* nsteps iterations of the body of sim3/program.f90:382-540:
  * updateD, source, material update_E, pmly%updateE, updateH, pmly%updateH,

@@ -1,0 +1,116 @@
+"""The one-pass step (k_onepass: E and H of a step in one z-marching kernel
+over ping-pong buffers) against the two-kernel FUSED step (k_fused_e +
+k_fused_h, FDTD_ONEPASS=0) on the same inputs: the per-cell arithmetic is
+the same term for term, so FP64 results agree to rounding (1e-11
+normwise after a few hundred steps; only the compiler's FMA contraction
+differs between the kernels) and FP32 to 1e-4 (measured 1.1e-5).  Cases cover partial tiles and chunks (tile height and chunk
+depth varied), periodic wrap of the halo column, row and plane, ALL_FACES
+PML on some or all faces, dispersive cells (written by k_ade before the
+pass), plane sources through PML cells, and TFSF terms.  Fields are
+compared as E and H vectors (normwise over the three components, as the
+other parity tests do: a small parasitic component carries the rounding
+differences of the large ones)."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def _periodic_box():
+    from fdtd_amd import box, sphere
+    from fdtd_amd.workloads import pulse_for_band
+
+    n = (130, 31, 9)
+    return {"grid": {"nx": n[0], "ny": n[1], "nz": n[2], "ds": 2e-9, "courant": 0.5, "periodic": [1, 1, 1]},
+            "materials": [{"type": "dielectric", "region": box((1, 1, 1), n), "eps": 1.0},
+                          {"type": "dielectric", "region": sphere((40, 15, 5), 6.0), "eps": 2.25}],
+            "sources": [dict(axis=0, index=7, comp=1, amplitude=1.0, **pulse_for_band(400e-9, 900e-9, 600e-9))],
+            "probes": [[1, 1, 1], [130, 31, 9], [60, 16, 5]],
+            "nsteps": 300}
+
+
+def _cases():
+    from test_gpu_parity import _small_sphere
+    from test_gpu_tfsf import _tfsf_through_pml
+    from fdtd_amd import workloads
+
+    graphene = workloads.c3_graphene_disks(nsteps=120, nx=48, ny=48, nz=40, nfreq=8)
+    graphene["pml"]["npml"] = [6, 6, 6]
+    for m in graphene["materials"]:
+        if m["type"] == "drude2cp":
+            m["region"]["radius"] = 3.0
+            m["region"]["half_length"] = 2.0
+    graphene["sources"][0]["index"] = 8
+    graphene["dft"] = [{"axis": 2, "index": 32, "omega": graphene["dft"][0]["omega"], "comps": [0, 1, 3, 4]}]
+    return {"gold_sphere": _small_sphere(nsteps=200), "periodic_box": _periodic_box(),
+            "graphene_disks": graphene, "tfsf_pml": _tfsf_through_pml(nsteps=200)}
+
+
+def _run(spec, precision, env, monkeypatch):
+    import fdtd_amd
+
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    sim, ids, pids = fdtd_amd.make_simulation(spec, precision=precision, engine="fused")
+    sim.step(spec["nsteps"])
+    return sim, ids, pids
+
+
+@pytest.mark.parametrize("case", ["gold_sphere", "periodic_box", "graphene_disks", "tfsf_pml"])
+@pytest.mark.parametrize("by,kc", [(16, 64), (8, 7), (4, 5)])
+def test_onepass_equals_two_kernel_step_fp64(case, by, kc, monkeypatch):
+    spec = _cases()[case]
+    a, ia, pa = _run(spec, "fp64", {"FDTD_ONEPASS": "0"}, monkeypatch)
+    b, ib, pb = _run(spec, "fp64", {"FDTD_ONEPASS": "1", "FDTD_ONEPASS_BY": str(by), "FDTD_ONEPASS_KC": str(kc)},
+                     monkeypatch)
+    for comps in ((0, 1, 2), (3, 4, 5)):
+        assert relerr(np.stack([b.field(c) for c in comps]), np.stack([a.field(c) for c in comps])) < 1e-11, case
+    for x, y in zip(ia, ib):
+        assert relerr(b.flux(y), a.flux(x)) < 1e-11
+    for x, y in zip(pa, pb):
+        assert relerr(b.probe(y), a.probe(x)) < 1e-11
+    assert max(np.abs(a.field(c)).max() for c in range(3)) > 1e-4  # something propagated
+
+
+@pytest.mark.parametrize("case", ["gold_sphere", "periodic_box"])
+def test_onepass_equals_two_kernel_step_fp32(case, monkeypatch):
+    spec = _cases()[case]
+    a, _, pa = _run(spec, "fp32", {"FDTD_ONEPASS": "0"}, monkeypatch)
+    b, _, pb = _run(spec, "fp32", {"FDTD_ONEPASS": "1"}, monkeypatch)
+    for comps in ((0, 1, 2), (3, 4, 5)):
+        assert relerr(np.stack([b.field(c) for c in comps]), np.stack([a.field(c) for c in comps])) < 1e-4, case
+
+
+def test_autotune_leaves_the_state_untouched(monkeypatch):
+    """fdtd_autotune times the step forms with dry steps on the zero fields:
+    the run that follows equals an untuned run (to rounding: the tuned form
+    may be another kernel), the step count stays 0, and tuning after a step
+    or after set_field is refused."""
+    import fdtd_amd
+    from fdtd_amd.abi import FdtdError
+
+    spec = _cases()["gold_sphere"]
+    monkeypatch.setenv("FDTD_ONEPASS", "1")
+    a, ia, _ = fdtd_amd.make_simulation(spec, engine="fused")
+    by, kc = a.autotune(2)
+    assert by in (0, 4, 8, 16) and (by == 0 or kc in (16, 32, 64, 128))
+    assert a.steps_done == 0
+    for c in range(6):
+        assert not np.abs(a.field(c)).any()
+    a.step(spec["nsteps"])
+    b, ib, _ = fdtd_amd.make_simulation(spec, engine="fused")
+    b.step(spec["nsteps"])
+    for comps in ((0, 1, 2), (3, 4, 5)):
+        assert relerr(np.stack([a.field(c) for c in comps]), np.stack([b.field(c) for c in comps])) < 1e-11
+    assert relerr(a.flux(ia[0]), b.flux(ib[0])) < 1e-11
+    with pytest.raises(FdtdError):
+        a.autotune(1)
+    c_, _, _ = fdtd_amd.make_simulation(spec, engine="fused")
+    c_.commit()
+    c_.set_field(0, np.zeros(c_.shape))
+    with pytest.raises(FdtdError):
+        c_.autotune(1)
