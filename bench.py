@@ -227,6 +227,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-2d", action="store_true")
+    ap.add_argument("--no-spectra", action="store_true", help="skip the FP32-vs-FP64 spectra check")
     ap.add_argument("--b2-count", type=int, default=64, help="batched 2D sweep: instances")
     ap.add_argument("--b2-steps", type=int, default=1000, help="batched 2D sweep: timed steps")
     ap.add_argument("--jobs", type=int, default=24)
@@ -276,7 +277,7 @@ def main():
         b2 = batched_2d(args)
         print(f"[bench] batched 2d {b2['batched_mcells_per_s']:.0f} Mcell/s ({b2['speedup']:.2f}x one at a time)",
               file=sys.stderr, flush=True)
-    spec_err = fp32_spectra_error()
+    spec_err = None if args.no_spectra else fp32_spectra_error()
     if args.n == 512 and args.seed == 0:  # the workload the summary profiles
         tb, src = profiled_traffic(r32["roofline"]["kernel"], "fp32")
         r32["roofline"]["traffic"] = tb

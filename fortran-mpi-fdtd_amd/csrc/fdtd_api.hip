@@ -544,7 +544,7 @@ int commit_impl(Sim* s) {
     }
     if (const char* e = std::getenv("FDTD_ONEPASS_KC")) {
       const int v = std::atoi(e);
-      if (v >= 4 && v <= 4096) s->op_kc = v;
+      if (v >= 2 && v <= 4096) s->op_kc = v;
     }
   }
   for (int q = 0; q < 3; ++q) {
@@ -1309,8 +1309,8 @@ int fdtd_autotune(fdtd_sim* s, int steps, int* out_by, int* out_kc) {
   if (s->engine == FDTD_ENGINE_FUSED && s->E2[0]) {
     cand.push_back({0, 0});
     for (int by : {4, 8, 16})
-      for (int kc : {16, 32, 64, 128}) {
-        if (kc > s->nz && kc != 16) continue;
+      for (int kc : {2, 4, 16, 32, 64, 128}) {
+        if (kc > s->nz && kc > 16) continue;
         cand.push_back({by, kc});
       }
   }

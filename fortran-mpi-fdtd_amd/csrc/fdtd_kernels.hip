@@ -543,6 +543,212 @@ __global__ __launch_bounds__(64 * BY) void k_onepass(FieldArgs<real> a, PmlArgs<
   }
 }
 
+// Short-chunk form of the one-pass step: each thread owns a column of KC
+// planes, issues every load of its KC+1 planes up front, computes their new
+// E (the last plane only for the H below it), meets the block once at an
+// LDS barrier and computes the KC new H.  The extra plane costs 1/KC more E
+// and H reads; the loads of a column are all in flight together, as in the
+// two-kernel step.  Same per-cell arithmetic as k_onepass.
+template <typename real>
+struct NewE {
+  real x, y, z;
+};
+
+template <typename real>
+__device__ __forceinline__ NewE<real> onepass_e(const FieldArgs<real>& a, const PmlArgs<real>& p,
+                                                const OnePassArgs<real>& o, const SrcArgs<real>& src, int nsrc,
+                                                const PlaneLoad<real>& L, real hx_km, real hy_km, int i, int j,
+                                                int kk, bool store_e, bool pmlxy, const Vec4<real>* tcoef,
+                                                const Vec4<real>* tgaxe, const uint32_t* tflags, bool lds_tab) {
+  const int c = i + a.px * j + a.pxy * kk;
+  real sv[3] = {0, 0, 0};
+  int sa[3] = {-1, -1, -1};
+  plane_sources(src, nsrc, i, j, kk, sv, sa);
+  const real hx = L.hx, hy = L.hy, hz = L.hz;
+  const unsigned id = L.id;
+  const uint32_t fl = lds_tab ? tflags[id] : a.flags[id];
+  const Vec4<real> cf = lds_tab ? tcoef[id] : a.coef[id];
+  NewE<real> r;
+  if (pmlxy || kk <= p.P[2] || kk > a.nz - p.P[2]) {
+    // mod3/BERpmly_mod.f90:150-155 with the splits driven by total H
+    const int s = pml_index(p.P, i, j, kk, a.nx, a.ny, a.nz);
+    const real m = cf.v[3];
+    real eyx = p.f1[0][i] * p.s[S_EYX][s] + (p.f2[0][i] * m) * (-hz + L.hz_im);
+    real ezx = p.f1[0][i] * p.s[S_EZX][s] + (p.f2[0][i] * m) * (hy + (-L.hy_im));
+    real exy = p.f1[1][j] * p.s[S_EXY][s] + (p.f2[1][j] * m) * (hz + (-L.hz_jm));
+    real ezy = p.f1[1][j] * p.s[S_EZY][s] + (p.f2[1][j] * m) * (-hx + L.hx_jm);
+    real exz = p.f1[2][kk] * p.s[S_EXZ][s] + (p.f2[2][kk] * m) * (-hy + hy_km);
+    real eyz = p.f1[2][kk] * p.s[S_EYZ][s] + (p.f2[2][kk] * m) * (hx + (-hx_km));
+    if (sa[0] == 1) exy = exy + sv[0] * m;
+    if (sa[0] == 2) exz = exz + sv[0] * m;
+    if (sa[1] == 0) eyx = eyx + sv[1] * m;
+    if (sa[1] == 2) eyz = eyz + sv[1] * m;
+    if (sa[2] == 0) ezx = ezx + sv[2] * m;
+    if (sa[2] == 1) ezy = ezy + sv[2] * m;
+    r.x = exy + exz;
+    r.y = eyz + eyx;
+    r.z = ezx + ezy;
+    if (store_e) {
+      o.sn[S_EYX][s] = eyx; o.sn[S_EZX][s] = ezx; o.sn[S_EXY][s] = exy;
+      o.sn[S_EZY][s] = ezy; o.sn[S_EXZ][s] = exz; o.sn[S_EYZ][s] = eyz;
+      a.e[0][c] = r.x; a.e[1][c] = r.y; a.e[2][c] = r.z;
+      put_e_ghost(a.e[0], r.x, i, j, kk, c, a);
+      put_e_ghost(a.e[1], r.y, i, j, kk, c, a);
+      put_e_ghost(a.e[2], r.z, i, j, kk, c, a);
+    }
+    return r;
+  }
+  const real e0[3] = {L.ex, L.ey, L.ez};
+  const bool any_src = sa[0] >= 0 || sa[1] >= 0 || sa[2] >= 0;
+  Vec4<real> gx;
+  if (any_src) gx = lds_tab ? tgaxe[id] : a.gaxe[id];
+  const real cu[3] = {(hz - L.hz_jm) + (-hy + hy_km),   // curlhzy + curlhyz
+                      (hx - hx_km) + (-hz + L.hz_im),   // curlhxz + curlhzx
+                      (hy - L.hy_im) + (-hx + L.hx_jm)};  // curlhyx + curlhxy
+  real ev[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    if (fl & (1u << q)) {
+      ev[q] = e0[q] + cf.v[q] * cu[q];
+      if (sa[q] >= 0) ev[q] = ev[q] + gx.v[q] * sv[q];
+    } else if (fl & (16u << q)) {
+      ev[q] = a.e[q][c];  // written by k_ade for this step
+      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): wait in this rare branch only
+    } else {
+      ev[q] = e0[q];
+    }
+    if (store_e && !(fl & (16u << q))) {
+      a.e[q][c] = ev[q];
+      put_e_ghost(a.e[q], ev[q], i, j, kk, c, a);
+    }
+  }
+  r.x = ev[0];
+  r.y = ev[1];
+  r.z = ev[2];
+  return r;
+}
+
+template <typename real, int BY, int KC>
+__global__ __launch_bounds__(64 * BY) void k_onepass_short(FieldArgs<real> a, PmlArgs<real> p, OnePassArgs<real> o,
+                                                           SrcArgs<real> src, int nsrc) {
+  constexpr int kTab = 64;
+  __shared__ real lds[KC][2][BY][64];  // new Ex, Ez of the KC owned planes
+  __shared__ Vec4<real> tcoef[kTab], tgaxe[kTab];
+  __shared__ uint32_t tflags[kTab];
+  const int lane = threadIdx.x, row = threadIdx.y;
+  const int b = blockIdx.x;
+  const int t = a.xcdmap ? (b & 7) * o.tper + (b >> 3) : b;
+  if (t >= o.ntx * o.nty * o.nch) return;  // whole (padding) block
+  const bool lds_tab = o.ntab <= kTab;
+  if (lds_tab)
+    for (int q = lane + 64 * row; q < o.ntab; q += 64 * BY) {
+      tcoef[q] = a.coef[q];
+      tgaxe[q] = a.gaxe[q];
+      tflags[q] = a.flags[q];
+    }
+  const int tx = t % o.ntx;
+  const int r1 = t / o.ntx;
+  const int ty = r1 % o.nty;
+  const int tz = r1 / o.nty;
+  const int i0 = tx * 63 + lane + 1;
+  const int j0 = ty * (BY - 1) + __builtin_amdgcn_readfirstlane(row) + 1;
+  const int k0 = tz * KC + 1;
+  const int np = min(KC, a.nz - k0 + 1);  // owned planes k0..k0+np-1
+  const bool own = lane < 63 && row < BY - 1 && i0 <= a.nx && j0 <= a.ny;
+  int i = i0, j = j0;
+  bool live = true;
+  if (i0 > a.nx) {
+    if (i0 == a.nx + 1 && a.per[0]) i = 1;
+    else live = false;
+  }
+  if (j0 > a.ny) {
+    if (j0 == a.ny + 1 && a.per[1]) j = 1;
+    else live = false;
+  }
+  const bool pmlxy = i <= p.P[0] || i > a.nx - p.P[0] || j <= p.P[1] || j > a.ny - p.P[1];
+  auto plane = [&](int k, int& kk) {
+    kk = k;
+    if (k > a.nz) {
+      if (!a.per[2]) return false;
+      kk = 1;
+    }
+    return live;
+  };
+  // every load of the column first
+  PlaneLoad<real> L[KC + 1];
+  bool lk[KC + 1];
+  int kq[KC + 1];
+  real hxb = 0, hyb = 0;  // old Hx, Hy of plane k0-1
+  if (live) {
+    const int c = i + a.px * j + a.pxy * (k0 - 1);
+    hxb = o.ho[0][c];
+    hyb = o.ho[1][c];
+  }
+#pragma unroll
+  for (int q = 0; q <= KC; ++q) {
+    lk[q] = q <= np && plane(k0 + q, kq[q]);
+    if (lk[q]) load_plane(o, a, i + a.px * j + a.pxy * kq[q], L[q]);
+  }
+  __syncthreads();  // the material table
+  // new E of planes k0..k0+np (the last one only for the H below it)
+  NewE<real> E[KC + 1];
+#pragma unroll
+  for (int q = 0; q <= KC; ++q) {
+    E[q] = NewE<real>{0, 0, 0};
+    if (lk[q])
+      E[q] = onepass_e(a, p, o, src, nsrc, L[q], q == 0 ? hxb : L[q - 1].hx, q == 0 ? hyb : L[q - 1].hy, i, j, kq[q],
+                       own && q < np, pmlxy, tcoef, tgaxe, tflags, lds_tab);
+    if (q < KC) {
+      lds[q][0][row][lane] = E[q].x;
+      lds[q][1][row][lane] = E[q].z;
+    }
+  }
+  real ey_ip[KC], ez_ip[KC];
+#pragma unroll
+  for (int q = 0; q < KC; ++q) {
+    ey_ip[q] = __shfl_down(E[q].y, 1);
+    ez_ip[q] = __shfl_down(E[q].z, 1);
+  }
+  lds_barrier();
+  if (!own) return;
+#pragma unroll
+  for (int q = 0; q < KC; ++q) {
+    if (q >= np) break;
+    const int kh = k0 + q;
+    const int c = i + a.px * j + a.pxy * kh;
+    const real ex_jp = lds[q][0][row + 1][lane];
+    const real ez_jp = lds[q][1][row + 1][lane];
+    const real exq = E[q].x, eyq = E[q].y, ezq = E[q].z;
+    const real ey_kp = E[q + 1].y, ex_kp = E[q + 1].x;
+    real nhx, nhy, nhz;
+    if (pmlxy || kh <= p.P[2] || kh > a.nz - p.P[2]) {
+      // mod3/BERpmly_mod.f90:186-191 with the splits driven by total E
+      const int s = pml_index(p.P, i, j, kh, a.nx, a.ny, a.nz);
+      const real hyx = p.g1[0][i] * p.s[S_HYX][s] + p.g2[0][i] * (-ezq + ez_ip[q]);
+      const real hzx = p.g1[0][i] * p.s[S_HZX][s] + p.g2[0][i] * (eyq + (-ey_ip[q]));
+      const real hxy = p.g1[1][j] * p.s[S_HXY][s] + p.g2[1][j] * (ezq + (-ez_jp));
+      const real hzy = p.g1[1][j] * p.s[S_HZY][s] + p.g2[1][j] * (-exq + ex_jp);
+      const real hxz = p.g1[2][kh] * p.s[S_HXZ][s] + p.g2[2][kh] * (-eyq + ey_kp);
+      const real hyz = p.g1[2][kh] * p.s[S_HYZ][s] + p.g2[2][kh] * (exq + (-ex_kp));
+      o.sn[S_HYX][s] = hyx; o.sn[S_HZX][s] = hzx; o.sn[S_HXY][s] = hxy;
+      o.sn[S_HZY][s] = hzy; o.sn[S_HXZ][s] = hxz; o.sn[S_HYZ][s] = hyz;
+      nhx = hxy + hxz;
+      nhy = hyz + hyx;
+      nhz = hzx + hzy;
+    } else {
+      nhx = L[q].hx + a.ku * ((ey_kp - eyq) + (-ez_jp + ezq));
+      nhy = L[q].hy + a.ku * ((ez_ip[q] - ezq) + (-ex_kp + exq));
+      nhz = L[q].hz + a.ku * ((ex_jp - exq) + (-ey_ip[q] + eyq));
+    }
+    a.h[0][c] = nhx;
+    a.h[1][c] = nhy;
+    a.h[2][c] = nhz;
+    put_h_ghost(a.h[0], nhx, i, j, kh, c, a);
+    put_h_ghost(a.h[1], nhy, i, j, kh, c, a);
+    put_h_ghost(a.h[2], nhz, i, j, kh, c, a);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Dispersive cells: Drude + up to 4 critical points + Raman
 // (mod3/gold_mod.f90:61-83, mod3/graphene_mod.f90:70-100).  State SoA:
@@ -1454,7 +1660,15 @@ int step_impl(Sim* s) {
     const dim3 grid((unsigned)(an.xcdmap ? 8 * o.tper : tiles));
     {
       Timer tm(s, FDTD_K_E);
-      if (BY == 16)
+      if (o.kc == 2 || o.kc == 4) {
+        auto go = [&](auto kern, int by) { hipLaunchKernelGGL(kern, grid, dim3(64, by), 0, st, an, p, o, src, nsrc); };
+        if (o.kc == 2)
+          BY == 16 ? go(k_onepass_short<real, 16, 2>, 16) : BY == 8 ? go(k_onepass_short<real, 8, 2>, 8)
+                                                           : go(k_onepass_short<real, 4, 2>, 4);
+        else
+          BY == 16 ? go(k_onepass_short<real, 16, 4>, 16) : BY == 8 ? go(k_onepass_short<real, 8, 4>, 8)
+                                                           : go(k_onepass_short<real, 4, 4>, 4);
+      } else if (BY == 16)
         hipLaunchKernelGGL((k_onepass<real, 16>), grid, dim3(64, 16), 0, st, an, p, o, src, nsrc);
       else if (BY == 8)
         hipLaunchKernelGGL((k_onepass<real, 8>), grid, dim3(64, 8), 0, st, an, p, o, src, nsrc);

@@ -1,5 +1,5 @@
 """The one-pass step (k_onepass: E and H of a step in one z-marching kernel
-over ping-pong buffers) against the two-kernel FUSED step (k_fused_e +
+over ping-pong buffers; k_onepass_short for chunks of 2 or 4 planes) against the two-kernel FUSED step (k_fused_e +
 k_fused_h, FDTD_ONEPASS=0) on the same inputs: the per-cell arithmetic is
 the same term for term, so FP64 results agree to rounding (1e-11
 normwise after a few hundred steps; only the compiler's FMA contraction
@@ -61,7 +61,7 @@ def _run(spec, precision, env, monkeypatch):
 
 
 @pytest.mark.parametrize("case", ["gold_sphere", "periodic_box", "graphene_disks", "tfsf_pml"])
-@pytest.mark.parametrize("by,kc", [(16, 64), (8, 7), (4, 5)])
+@pytest.mark.parametrize("by,kc", [(16, 64), (8, 7), (4, 5), (8, 2), (16, 4), (4, 4)])
 def test_onepass_equals_two_kernel_step_fp64(case, by, kc, monkeypatch):
     spec = _cases()[case]
     a, ia, pa = _run(spec, "fp64", {"FDTD_ONEPASS": "0"}, monkeypatch)
