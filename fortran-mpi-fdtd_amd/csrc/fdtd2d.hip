@@ -1,4 +1,5 @@
-// fdtd2d.hip -- gfx950 2D TEz (Hz, Ex, Ey) stepper behind include/fdtd2d.h.
+// fdtd2d.hip -- gfx950 2D stepper behind include/fdtd2d.h: TEz (Hz, Ex,
+// Ey), the reference's 2D driver, and TMz (Ez, Hx, Hy).
 //
 // One code path steps one simulation or a batch of many.  A step is five
 // launches over every simulation of the batch: the E half step over a
@@ -22,6 +23,19 @@
 //      periodic x: ghost columns of Hz.
 // Cells are independent within each half step, so one thread per cell
 // reproduces the reference's loops exactly.
+//
+// TMz, per simulation, follows the 3D FUSED engine's order for fields with
+// no z variation (sim3/include/myfdtd_func.f90:19-64 with the z differences
+// zero; mod3/BERpmly_mod.f90:117-195 on the x and y faces):
+//   E: non-PML cells Dz += Ku ((Hy(i)-Hy(i-1)) + (Hx(j-1)-Hx(j))), line
+//      sources on Dz, Ez = Dz*gax (dispersive cells: their own launch);
+//      PML cells Ezx, Ezy split updates scaled by the cell's PML medium, the
+//      source on Ezy, Ez = Ezx + Ezy; periodic x: ghost columns of Ez.
+//   H: non-PML cells Hx -= Ku (Ez(j+1)-Ez(j)), Hy += Ku (Ez(i+1)-Ez(i));
+//      PML cells Hx = g1y Hx + g2y (Ez(j)-Ez(j+1)), Hy = g1x Hy + g2x
+//      (Ez(i+1)-Ez(i)) (their z splits stay zero); ghost columns of Hy.
+// A batch mixes both polarisations: a tile belongs to one simulation, so
+// the branch is uniform over a wavefront.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -54,6 +68,8 @@ struct Mat {
   real gax[2];
   int32_t act[2];  // ACT_KEEP / ACT_DIEL / ACT_ADE per component (Ex, Ey)
 };
+// TMz: act[0], gax[0] act on Ez; gax[1] is the PML medium (1/eps of the last
+// dielectric written to the cell, 1 where none); act[1] is unused.
 
 template <typename real>
 struct AdeDev {
@@ -100,6 +116,12 @@ struct Inst {
   double* probe_rec;           // [n-1][probe][3]
   int32_t probe_cap;           // steps the buffer holds
   int32_t* ctl;                // [0] steps done, [1] target
+  // TMz: fields, Ez splits (PML cells only), profiles f1 f2 g1 g2 [nx+2]
+  // along x then [ny+2] along y, PML depths Px (x) and P (y, above)
+  int32_t tmz, Px;
+  real *ez, *hx, *hy, *dz;
+  real *ezx, *ezy;
+  const real* tprof;
 };
 
 // ---- host state ------------------------------------------------------------------
@@ -165,6 +187,11 @@ struct Sim {
   size_t dev_bytes = 0;
   Batch* solo = nullptr;  // batch of one behind fdtd2d_step
   Batch* owner = nullptr;
+  // TMz: F holds Ez, Hx, Hy, Dz; Ez splits and profiles of its PML
+  bool tmz = false;
+  void* d_ezx = nullptr;
+  void* d_ezy = nullptr;
+  void* d_tprof = nullptr;
 };
 
 struct Batch {
@@ -211,6 +238,82 @@ void* upload(std::vector<DevBuf>& list, size_t* total, const void* data, size_t 
 __device__ __forceinline__ int tile_inst_of(const int32_t* tile_inst, int t) { return tile_inst[t]; }
 
 template <typename real>
+__device__ __forceinline__ bool in_pml_tmz(const Inst<real>& S, int i, int j) {
+  return (S.Px > 0 && (i <= S.Px || i > S.nx - S.Px)) || (S.P > 0 && (j <= S.P || j > S.ny - S.P));
+}
+
+// TMz E half step of cell (i, j); loads first (the pointers may alias as far
+// as the compiler knows)
+template <typename real>
+__device__ __forceinline__ void e_tmz(const Inst<real>& S, int i, int j) {
+  const int px = S.px;
+  const int c = i + px * j;
+  bool src = false;
+  real sv = 0;
+  for (int s = 0; s < S.nsrc; ++s)
+    if (j == S.src_j[s]) {
+      src = true;
+      sv = S.src_val[s];
+    }
+  const bool pml = in_pml_tmz(S, i, j);
+  const real hxc = S.hx[c], hxj = S.hx[c - px], hyc = S.hy[c], hyi = S.hy[c - 1];
+  const Mat<real> m = S.tab[S.matid[c]];
+  real ez = 0;
+  bool wrote = false;
+  if (!pml) {
+    real dz = S.dz[c];
+    dz = dz + S.ku * ((hyc - hyi) + (-hxc + hxj));
+    if (src) dz = dz + sv;
+    S.dz[c] = dz;
+    if (m.act[0] == ACT_DIEL) {
+      ez = dz * m.gax[0];
+      wrote = true;
+    }
+  } else {
+    const real r = m.gax[1];
+    const real* fx = S.tprof;
+    const real* fy = S.tprof + 4 * (S.nx + 2);
+    const real ox = S.ezx[c], oy = S.ezy[c];
+    const real zx = fx[i] * ox + (fx[(S.nx + 2) + i] * r) * (hyc + -hyi);
+    real zy = fy[j] * oy + (fy[(S.ny + 2) + j] * r) * (-hxc + hxj);
+    if (src) zy = zy + sv * r;
+    S.ezx[c] = zx;
+    S.ezy[c] = zy;
+    ez = zx + zy;
+    wrote = true;
+  }
+  if (wrote) {
+    S.ez[c] = ez;
+    if (S.per && i == 1) S.ez[c + S.nx] = ez;
+    if (S.per && i == S.nx) S.ez[c - S.nx] = ez;
+  }
+}
+
+// TMz H half step of cell (i, j)
+template <typename real>
+__device__ __forceinline__ void h_tmz(const Inst<real>& S, int i, int j) {
+  const int px = S.px;
+  const int c = i + px * j;
+  const bool pml = in_pml_tmz(S, i, j);
+  const real ezc = S.ez[c], ezi = S.ez[c + 1], ezj = S.ez[c + px];
+  const real hxc = S.hx[c], hyc = S.hy[c];
+  real hx, hy;
+  if (!pml) {
+    hx = hxc + S.ku * (-ezj + ezc);
+    hy = hyc + S.ku * (ezi - ezc);
+  } else {
+    const real* fx = S.tprof;
+    const real* fy = S.tprof + 4 * (S.nx + 2);
+    hx = fy[2 * (S.ny + 2) + j] * hxc + fy[3 * (S.ny + 2) + j] * (ezc + -ezj);
+    hy = fx[2 * (S.nx + 2) + i] * hyc + fx[3 * (S.nx + 2) + i] * (-ezc + ezi);
+  }
+  S.hx[c] = hx;
+  S.hy[c] = hy;
+  if (S.per && i == 1) S.hy[c + S.nx] = hy;
+  if (S.per && i == S.nx) S.hy[c - S.nx] = hy;
+}
+
+template <typename real>
 __global__ __launch_bounds__(256) void k2_e(const Inst<real>* __restrict__ I, const int32_t* __restrict__ tile_inst,
                                             int ntiles) {
   const int lane = threadIdx.x & 63;
@@ -223,6 +326,10 @@ __global__ __launch_bounds__(256) void k2_e(const Inst<real>* __restrict__ I, co
     const int i = (lt % S.ntx) * kTX + lane + 1;
     const int j = (lt / S.ntx) * kTY + row + 1;
     if (i > S.nx || j > S.ny) continue;
+    if (S.tmz) {
+      e_tmz(S, i, j);
+      continue;
+    }
     const int px = S.px;
     const int c = i + px * j;
     const bool main = j >= S.j0 && j <= S.j1;
@@ -294,6 +401,10 @@ __global__ __launch_bounds__(256) void k2_h(const Inst<real>* __restrict__ I, co
     const int i = (lt % S.ntx) * kTX + lane + 1;
     const int j = (lt / S.ntx) * kTY + row + 1;
     if (i > S.nx || j > S.ny) continue;
+    if (S.tmz) {
+      h_tmz(S, i, j);
+      continue;
+    }
     const int px = S.px;
     const int c = i + px * j;
     const bool main = j >= S.j0 && j <= S.j1;
@@ -356,7 +467,7 @@ __global__ __launch_bounds__(256) void k2_ade(const Inst<real>* __restrict__ I, 
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
     if (!(mask & (1 << q))) continue;
-    const real dv = q == 0 ? S.dx[c] : S.dy[c];
+    const real dv = S.tmz ? S.dz[c] : (q == 0 ? S.dx[c] : S.dy[c]);
     real* Ip = G.state + (int64_t)(0 * 2 + q) * n + e;
     real* Sp = G.state + (int64_t)(1 * 2 + q) * n + e;
     const real Iv = *Ip, Sv = *Sp;
@@ -382,10 +493,11 @@ __global__ __launch_bounds__(256) void k2_ade(const Inst<real>* __restrict__ I, 
     }
     const real dr = co.gar + s1, di = co.gai;
     const real ev = (xr * dr + xi * di) / (dr * dr + di * di);
-    real* E = q == 0 ? S.ex : S.ey;
+    real* E = S.tmz ? S.ez : (q == 0 ? S.ex : S.ey);
     E[c] = ev;
-    if (q == 1 && S.per && i == 1) E[c + S.nx] = ev;
-    if (q == 1 && S.per && i == S.nx) E[c - S.nx] = ev;
+    const bool wrap = S.per && (q == 1 || S.tmz);  // Ey (TEz) and Ez (TMz) wrap in x
+    if (wrap && i == 1) E[c + S.nx] = ev;
+    if (wrap && i == S.nx) E[c - S.nx] = ev;
     if (co.raman) {  // graphene_mod.f90:74-77 (Sn_new; shift)
       const real ns = co.r1 * s1 + co.r2 * s2 + co.r0 * (ev * ev);
       *s2p = s1;
@@ -413,13 +525,13 @@ __global__ __launch_bounds__(256) void k2_mon(const Inst<real>* __restrict__ I, 
   const Inst<real>& S = I[wk.x];
   if (S.ctl[0] >= S.ctl[1]) return;
   const int n = S.ctl[0] + 1;
-  if (wk.y < 0) {  // probes: (n*dt, Ex, Hz), sim2/program.f90:376,397,417
+  if (wk.y < 0) {  // probes: (n*dt, Ex, Hz), sim2/program.f90:376,397,417; TMz (n*dt, Ez, Hx)
     const int p = threadIdx.x;
     if (p < S.nprobe && n - 1 < S.probe_cap) {
       double* r = S.probe_rec + ((int64_t)(n - 1) * S.nprobe + p) * 3;
       r[0] = (double)n * S.dt;
-      r[1] = (double)S.ex[S.probe_c[p]];
-      r[2] = (double)S.hz[S.probe_c[p]];
+      r[1] = (double)(S.tmz ? S.ez : S.ex)[S.probe_c[p]];
+      r[2] = (double)(S.tmz ? S.hx : S.hz)[S.probe_c[p]];
     }
     return;
   }
@@ -432,6 +544,11 @@ __global__ __launch_bounds__(256) void k2_mon(const Inst<real>* __restrict__ I, 
   const int c = i + S.px * D.j;
   real* samp = (real*)D.samp + (int64_t)b * D.nslots * S.nx;
   int slot = 0;
+  if (S.tmz) {
+    if (D.mask & FDTD2D_DFT_EZ) samp[(int64_t)(slot++) * S.nx + (i - 1)] = S.ez[c];
+    if (D.mask & FDTD2D_DFT_HX) samp[(int64_t)slot * S.nx + (i - 1)] = S.hx[c];
+    return;
+  }
   if (D.mask & FDTD2D_DFT_EX) samp[(int64_t)(slot++) * S.nx + (i - 1)] = S.ex[c];
   if (D.mask & FDTD2D_DFT_HZ) samp[(int64_t)slot * S.nx + (i - 1)] = S.hz[c];
 }
@@ -697,12 +814,51 @@ int commit_impl(Sim* s) {
   if (s->ncell + le >= ((int64_t)1 << 31)) return set_error(FDTD_ERR_ARG, "grid too large");
   const size_t shift = (size_t)(le - 1) * s->wsize;
   const size_t fbytes = (size_t)(s->ncell + le) * s->wsize;
-  for (int f = 0; f < 5; ++f) {
+  for (int f = 0; f < (s->tmz ? 4 : 5); ++f) {
     char* p = (char*)alloc_in(s->allocs, &s->dev_bytes, fbytes);
     if (!p) return set_error(FDTD_ERR_NOMEM, "field allocation");
     s->F[f] = p + shift;
   }
-  if (s->has_pml) {
+  if (s->tmz) {
+    // the 3D engine's ALL_FACES PML on the x and y faces (vacuum grading,
+    // defaults f1 = g1 = 1, f2 = g2 = Ku)
+    const Consts& K = consts();
+    s->Px = s->has_pml ? s->pml.npml[0] : 0;
+    s->P = s->has_pml ? s->pml.npml[1] : 0;
+    s->j0 = 1;
+    s->j1 = ny;
+    s->ja = 0;
+    s->jb = ny + 1;
+    const double ku_pml = s->dt * K.c0 / s->ds;
+    std::vector<double> prof;
+    const int n[2] = {nx, ny}, P[2] = {s->Px, s->P};
+    for (int a = 0; a < 2; ++a) {
+      std::vector<double> f1(n[a] + 2, 1.0), f2(n[a] + 2, ku_pml), g1(n[a] + 2, 1.0), g2(n[a] + 2, ku_pml);
+      if (P[a] > 0) {
+        const double sigmam = s->pml.sigmam > 0 ? s->pml.sigmam
+                                                : (s->pml.order + 1.0) * K.c0 * std::log(1.0 / s->pml.R0) /
+                                                      (2.0 * P[a] * s->ds);  // sim3/program.f90:102
+        pml_grade(s->dt, s->ds, s->pml.order, sigmam, P[a], K.ep0, n[a], f1.data(), f2.data(), g1.data(),
+                  g2.data());
+      }
+      for (auto* v : {&f1, &f2, &g1, &g2}) prof.insert(prof.end(), v->begin(), v->end());
+    }
+    s->d_tprof = s->prec == FDTD_FP32 ? upload_cast<float>(s, prof) : upload_cast<double>(s, prof);
+    if (!s->d_tprof) return set_error(FDTD_ERR_NOMEM, "PML profiles");
+    if (s->has_pml) {
+      char* a = (char*)alloc_in(s->allocs, &s->dev_bytes, fbytes);
+      char* b = (char*)alloc_in(s->allocs, &s->dev_bytes, fbytes);
+      if (!a || !b) return set_error(FDTD_ERR_NOMEM, "PML allocation");
+      s->d_ezx = a + shift;
+      s->d_ezy = b + shift;
+    }
+    // rasterised below with key[0] = last writer of Ez, key[1] = last
+    // dielectric (the PML medium)
+    for (auto& m : s->mats) {
+      m.writes[0] = 1;
+      m.writes[1] = m.type == 0 ? 1 : 0;
+    }
+  } else if (s->has_pml) {
     const int P = s->pml.npml[1];
     s->P = P;
     s->Px = s->pml.npml[0];
@@ -737,6 +893,11 @@ int commit_impl(Sim* s) {
     AdeCoef<double> c;
     if (m.type == 1) {
       drude2cp_coef(s, m.d2, c);
+    } else if (s->tmz) {  // the 3D module's graphene coefficients
+      double einf = 0;
+      int rc = graphene3d_coef(s->dt, s->ds, m.gr, c, &einf);
+      if (rc) return rc;
+      c.ku = s->ku;
     } else {
       int rc = graphene_coef(s, m.gr, c);
       if (rc) return rc;
@@ -786,7 +947,20 @@ int commit_impl(Sim* s) {
       }
   }
   std::vector<Mat<double>> tab(combos.size());
-  for (size_t q = 0; q < combos.size(); ++q)
+  if (s->tmz) {
+    for (size_t q = 0; q < combos.size(); ++q) {
+      tab[q].act[0] = tab[q].act[1] = ACT_KEEP;
+      tab[q].gax[0] = tab[q].gax[1] = 1.0;
+      const int w = combos[q][0], d = combos[q][1];
+      if (w >= 0) {
+        const HostMat& m = s->mats[w];
+        tab[q].act[0] = m.type == 0 ? ACT_DIEL : ACT_ADE;
+        if (m.type == 0) tab[q].gax[0] = m.eps_r == 0.0 ? 0.0 : 1.0 / m.eps_r;
+      }
+      if (d >= 0) tab[q].gax[1] = s->mats[d].eps_r == 0.0 ? 0.0 : 1.0 / s->mats[d].eps_r;
+    }
+  }
+  for (size_t q = 0; q < combos.size() && !s->tmz; ++q)
     for (int c = 0; c < 2; ++c) {
       tab[q].act[c] = ACT_KEEP;
       tab[q].gax[c] = 1.0;
@@ -823,11 +997,17 @@ int commit_impl(Sim* s) {
       const int64_t c = i + s->px * j;
       const Key& key = combos[cid[(size_t)c]];
       int gm[2] = {-1, -1};
-      for (int q = 0; q < 2; ++q)
-        if (key[q] >= 0 && s->mats[key[q]].type != 0 && !(s->mats[key[q]].type == 2 && q == 1))
-          gm[q] = s->mats[key[q]].group;
+      if (s->tmz) {
+        if (key[0] >= 0 && s->mats[key[0]].type != 0) gm[0] = s->mats[key[0]].group;
+      } else {
+        for (int q = 0; q < 2; ++q)
+          if (key[q] >= 0 && s->mats[key[q]].type != 0 && !(s->mats[key[q]].type == 2 && q == 1))
+            gm[q] = s->mats[key[q]].group;
+      }
       if (gm[0] < 0 && gm[1] < 0) continue;
-      if (s->has_pml && (j <= s->ja || j >= s->jb))
+      if (s->tmz && ((s->Px > 0 && (i <= s->Px || i > nx - s->Px)) || (s->P > 0 && (j <= s->P || j > ny - s->P))))
+        return set_error(FDTD_ERR_UNSUPPORTED, "dispersive cells inside the PML");
+      if (!s->tmz && s->has_pml && (j <= s->ja || j >= s->jb))
         return set_error(FDTD_ERR_UNSUPPORTED, "dispersive cells inside a PML slab row");
       for (size_t g = 0; g < s->groups.size(); ++g) {
         uint8_t mk = 0;
@@ -850,7 +1030,8 @@ int commit_impl(Sim* s) {
   }
   // monitors
   for (auto& d : s->dfts) {
-    d.nslots = ((d.mask & FDTD2D_DFT_EX) ? 1 : 0) + ((d.mask & FDTD2D_DFT_HZ) ? 1 : 0);
+    d.nslots = s->tmz ? ((d.mask & FDTD2D_DFT_EZ) ? 1 : 0) + ((d.mask & FDTD2D_DFT_HX) ? 1 : 0)
+                      : ((d.mask & FDTD2D_DFT_EX) ? 1 : 0) + ((d.mask & FDTD2D_DFT_HZ) ? 1 : 0);
     d.d_omega = (double*)upload(s->allocs, &s->dev_bytes, d.omega.data(), d.omega.size() * sizeof(double));
     d.d_acc = (double*)alloc_in(s->allocs, &s->dev_bytes, sizeof(double) * 2 * (size_t)d.nslots * d.nfreq * nx);
     d.d_samp = alloc_in(s->allocs, &s->dev_bytes, s->wsize * (size_t)kBlockSteps * d.nslots * nx);
@@ -884,7 +1065,15 @@ int ensure_probe_cap(Sim* s, int steps) {
 template <typename real>
 Inst<real> make_inst(Sim* s, int32_t* ctl, int tile0) {
   Inst<real> d{};
-  d.ex = (real*)s->F[0]; d.ey = (real*)s->F[1]; d.hz = (real*)s->F[2]; d.dx = (real*)s->F[3]; d.dy = (real*)s->F[4];
+  if (s->tmz) {
+    d.tmz = 1;
+    d.ez = (real*)s->F[0]; d.hx = (real*)s->F[1]; d.hy = (real*)s->F[2]; d.dz = (real*)s->F[3];
+    d.ezx = (real*)s->d_ezx; d.ezy = (real*)s->d_ezy;
+    d.tprof = (const real*)s->d_tprof;
+    d.Px = s->Px;
+  } else {
+    d.ex = (real*)s->F[0]; d.ey = (real*)s->F[1]; d.hz = (real*)s->F[2]; d.dx = (real*)s->F[3]; d.dy = (real*)s->F[4];
+  }
   d.matid = s->d_matid;
   d.tab = (const Mat<real>*)s->d_tab;
   d.px = (int32_t)s->px;
@@ -1099,6 +1288,15 @@ int sync_sim(Sim* s) {
 }
 
 void* field_ptr(Sim* s, int comp) {
+  if (s->tmz) {
+    switch (comp) {
+      case FDTD2D_EZ: return s->F[0];
+      case FDTD2D_HX: return s->F[1];
+      case FDTD2D_HY: return s->F[2];
+      case FDTD2D_DZ: return s->F[3];
+      default: return nullptr;
+    }
+  }
   switch (comp) {
     case FDTD2D_EX: return s->F[0];
     case FDTD2D_EY: return s->F[1];
@@ -1133,6 +1331,7 @@ int fdtd2d_create(fdtd2d_sim** out, const fdtd2d_grid* g) {
   if (!out || !g) return set_error(FDTD_ERR_ARG, "null argument");
   if (g->nx < 2 || g->ny < 2 || !(g->ds > 0) || !(g->courant > 0)) return set_error(FDTD_ERR_ARG, "bad grid");
   if (g->precision != FDTD_FP64 && g->precision != FDTD_FP32) return set_error(FDTD_ERR_ARG, "bad precision");
+  if (g->polarization != FDTD2D_TEZ && g->polarization != FDTD2D_TMZ) return set_error(FDTD_ERR_ARG, "bad polarization");
   int dev = 0;
   FDTD_HIP(hipGetDevice(&dev));
   fdtd2d_sim* s = new fdtd2d_sim();
@@ -1144,6 +1343,7 @@ int fdtd2d_create(fdtd2d_sim** out, const fdtd2d_grid* g) {
   s->dt = s->ku * s->ds / consts().c0;  // sim2/program.f90:45
   s->prec = g->precision;
   s->wsize = g->precision == FDTD_FP32 ? 4 : 8;
+  s->tmz = g->polarization == FDTD2D_TMZ;
   s->device = dev;
   hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -1170,6 +1370,16 @@ int fdtd2d_set_pml(fdtd2d_sim* s, const fdtd2d_pml* p) {
   GUARD2(s);
   SETUP2(s);
   if (!p) return set_error(FDTD_ERR_ARG, "null argument");
+  if (s->tmz) {
+    if (p->npml[0] < 0 || p->npml[1] < 0 || 2 * p->npml[0] > s->nx || 2 * p->npml[1] > s->ny)
+      return set_error(FDTD_ERR_ARG, "bad npml");
+    if (p->npml[0] > 0 && s->g.periodic_x) return set_error(FDTD_ERR_ARG, "PML on a periodic axis");
+    if ((p->npml[0] > 0 || p->npml[1] > 0) && (!(p->order > 0) || (!(p->sigmam > 0) && !(p->R0 > 0 && p->R0 < 1))))
+      return set_error(FDTD_ERR_ARG, "bad grading");
+    s->pml = *p;
+    s->has_pml = p->npml[0] > 0 || p->npml[1] > 0;
+    return 0;
+  }
   if (p->npml[1] < 1 || p->npml[0] < 0 || 2 * p->npml[1] + 2 > s->ny || 2 * p->npml[0] > s->nx)
     return set_error(FDTD_ERR_ARG, "bad npml");
   if (p->npml[1] >= s->ny / 2 - 1) return set_error(FDTD_ERR_ARG, "y PML reaches the middle of the domain");
@@ -1240,9 +1450,8 @@ int fdtd2d_add_dft_line(fdtd2d_sim* s, int j, int nfreq, const double* omega, in
   GUARD2(s);
   SETUP2(s);
   if (!omega || !id) return set_error(FDTD_ERR_ARG, "null argument");
-  if (j < 1 || j > s->ny || nfreq < 1 || !(mask & (FDTD2D_DFT_EX | FDTD2D_DFT_HZ)) ||
-      (mask & ~(FDTD2D_DFT_EX | FDTD2D_DFT_HZ)))
-    return set_error(FDTD_ERR_ARG, "bad DFT line");
+  const int all = s->tmz ? (FDTD2D_DFT_EZ | FDTD2D_DFT_HX) : (FDTD2D_DFT_EX | FDTD2D_DFT_HZ);
+  if (j < 1 || j > s->ny || nfreq < 1 || !(mask & all) || (mask & ~all)) return set_error(FDTD_ERR_ARG, "bad DFT line");
   if ((int)s->dfts.size() >= kMaxDft) return set_error(FDTD_ERR_UNSUPPORTED, "too many DFT lines");
   HostDft d{};
   d.j = j;
@@ -1342,7 +1551,7 @@ int fdtd2d_set_field(fdtd2d_sim* s, int comp, const double* src) {
   if (rc) return rc;
   std::vector<char> h((size_t)s->ncell * s->wsize);
   FDTD_HIP(hipMemcpy(h.data(), f, h.size(), hipMemcpyDeviceToHost));
-  const bool wrap = s->g.periodic_x && (comp == FDTD2D_EY || comp == FDTD2D_HZ);
+  const bool wrap = s->g.periodic_x && (comp == FDTD2D_EY || comp == FDTD2D_HZ || comp == FDTD2D_EZ || comp == FDTD2D_HY);
   for (int j = 1; j <= s->ny; ++j)
     for (int i = 1; i <= s->nx; ++i) {
       const double v = src[(size_t)(i - 1) + (size_t)s->nx * (j - 1)];
@@ -1367,8 +1576,10 @@ int fdtd2d_get_dft(fdtd2d_sim* s, int id, int comp, double* re, double* im) {
   if (rc) return rc;
   const HostDft& d = s->dfts[id];
   int slot = -1;
-  if (comp == FDTD2D_EX && (d.mask & FDTD2D_DFT_EX)) slot = 0;
-  if (comp == FDTD2D_HZ && (d.mask & FDTD2D_DFT_HZ)) slot = (d.mask & FDTD2D_DFT_EX) ? 1 : 0;
+  const int e_bit = s->tmz ? FDTD2D_DFT_EZ : FDTD2D_DFT_EX, h_bit = s->tmz ? FDTD2D_DFT_HX : FDTD2D_DFT_HZ;
+  const int e_comp = s->tmz ? FDTD2D_EZ : FDTD2D_EX, h_comp = s->tmz ? FDTD2D_HX : FDTD2D_HZ;
+  if (comp == e_comp && (d.mask & e_bit)) slot = 0;
+  if (comp == h_comp && (d.mask & h_bit)) slot = (d.mask & e_bit) ? 1 : 0;
   if (slot < 0) return set_error(FDTD_ERR_ARG, "component not recorded");
   rc = sync_sim(s);
   if (rc) return rc;
@@ -1382,13 +1593,22 @@ int fdtd2d_get_flux(fdtd2d_sim* s, int id, double* S) {
   GUARD2(s);
   if (!S || id < 0 || id >= (int)s->dfts.size()) return set_error(FDTD_ERR_ARG, "bad argument");
   const HostDft& d = s->dfts[id];
-  if ((d.mask & (FDTD2D_DFT_EX | FDTD2D_DFT_HZ)) != (FDTD2D_DFT_EX | FDTD2D_DFT_HZ))
-    return set_error(FDTD_ERR_ARG, "flux needs Ex and Hz");
+  const int both = s->tmz ? (FDTD2D_DFT_EZ | FDTD2D_DFT_HX) : (FDTD2D_DFT_EX | FDTD2D_DFT_HZ);
+  if ((d.mask & both) != both) return set_error(FDTD_ERR_ARG, "flux needs both recorded components");
   const size_t n = (size_t)d.nfreq * s->nx;
   std::vector<double> er(n), ei(n), hr(n), hi(n);
-  int rc = fdtd2d_get_dft(s, id, FDTD2D_EX, er.data(), ei.data());
-  if (!rc) rc = fdtd2d_get_dft(s, id, FDTD2D_HZ, hr.data(), hi.data());
+  int rc = fdtd2d_get_dft(s, id, s->tmz ? FDTD2D_EZ : FDTD2D_EX, er.data(), ei.data());
+  if (!rc) rc = fdtd2d_get_dft(s, id, s->tmz ? FDTD2D_HX : FDTD2D_HZ, hr.data(), hi.data());
   if (rc) return rc;
+  if (s->tmz) {  // sim3/program.f90:613-685, the Hx*Ez terms
+    for (int m = 0; m < d.nfreq; ++m) {
+      double sy = 0.0;
+      const size_t o = (size_t)m * s->nx;
+      for (int i = 0; i < s->nx; ++i) sy = sy + ((hr[o + i] * er[o + i]) + (hi[o + i] * ei[o + i])) * s->ds;
+      S[m] = sy;
+    }
+    return 0;
+  }
   // sim2/program.f90:479-529, the reference's summation order
   for (int m = 0; m < d.nfreq; ++m) {
     double sy = 0.0;

@@ -152,15 +152,14 @@ void drude2cp_coef(const Sim* s, const fdtd_drude2cp& p, AdeCoef<double>& c, dou
   *eps_inf = p.eps_inf;
 }
 
-int graphene_coef(const Sim* s, const fdtd_graphene& p, AdeCoef<double>& c, double* eps_inf) {
+int graphene_coef_dt(double dt, double ds, const fdtd_graphene& p, AdeCoef<double>& c, double* eps_inf) {
   const Consts& K = consts();
-  const double dt = s->dt;
   const cplx I(0.0, 1.0);
   std::memset(&c, 0, sizeof c);
   c.ncp = 4;
   if (p.model == FDTD_GRAPHENE_CP4) {
     // initialize_GrapheneVars + scaleParams, mod3/graphene_mod.f90:202-269
-    const double scale = p.thickness / s->ds;
+    const double scale = p.thickness / ds;
     const double wD = std::sqrt(scale) * p.omega_d;
     c.ha = std::exp(-p.gamma * dt);
     c.hb = (wD * wD) * dt / p.gamma;
@@ -185,7 +184,7 @@ int graphene_coef(const Sim* s, const fdtd_graphene& p, AdeCoef<double>& c, doub
     const double kT = K.kb * p.temperature;
     const double ef = p.fermi_ev * K.echarge;
     const double Dw = 2.0 * K.echarge * K.echarge * kT / (K.hbar * K.hbar) * std::log(2.0 * std::cosh(ef / (2.0 * kT)));
-    const double wp2 = Dw / (K.pi * K.ep0 * s->ds);
+    const double wp2 = Dw / (K.pi * K.ep0 * ds);
     const double gam = 1.0 / p.tau;
     c.ha = std::exp(-gam * dt);
     c.hb = wp2 * dt / gam;
@@ -294,36 +293,36 @@ void tfsf_profiles(const Sim* s, const Tfsf& t, double* out) {
 
 // ---- PML profiles (reference formulas, one axis at a time) ----------------
 struct Prof {
-  const Sim* s;
+  double dt, ds, order;
   double sigmam;
   int P;
   void e_low(double ep, int idx, double& f1, double& f2) const {
     const Consts& K = consts();
-    const double sig = ep * sigmam * std::pow(((double)(P - idx) + 0.5) / (double)P, s->pml.order);
-    const double alpha = sig * s->dt / ep;
+    const double sig = ep * sigmam * std::pow(((double)(P - idx) + 0.5) / (double)P, order);
+    const double alpha = sig * dt / ep;
     f1 = std::exp(-alpha);
-    f2 = (1.0 - f1) / (K.z0 * sig * s->ds);
+    f2 = (1.0 - f1) / (K.z0 * sig * ds);
   }
   void h_low(int idx, double& g1, double& g2) const {
     const Consts& K = consts();
-    const double sig = K.mu0 * sigmam * std::pow(((double)(P - idx)) / (double)P, s->pml.order);
-    const double alpha = sig * s->dt / K.mu0;
+    const double sig = K.mu0 * sigmam * std::pow(((double)(P - idx)) / (double)P, order);
+    const double alpha = sig * dt / K.mu0;
     g1 = std::exp(-alpha);
-    g2 = K.z0 * (1.0 - g1) / (sig * s->ds);
+    g2 = K.z0 * (1.0 - g1) / (sig * ds);
   }
   void e_high(double ep, int q, double& f1, double& f2) const {
     const Consts& K = consts();
-    const double sig = ep * sigmam * std::pow(((double)(P - q)) / (double)P, s->pml.order);
-    const double alpha = sig * s->dt / ep;
+    const double sig = ep * sigmam * std::pow(((double)(P - q)) / (double)P, order);
+    const double alpha = sig * dt / ep;
     f1 = std::exp(-alpha);
-    f2 = (1.0 - f1) / (K.z0 * sig * s->ds);
+    f2 = (1.0 - f1) / (K.z0 * sig * ds);
   }
   void h_high(int q, double& g1, double& g2) const {
     const Consts& K = consts();
-    const double sig = K.mu0 * sigmam * std::pow(((double)(P - q) + 0.5) / (double)P, s->pml.order);
-    const double alpha = sig * s->dt / K.mu0;
+    const double sig = K.mu0 * sigmam * std::pow(((double)(P - q) + 0.5) / (double)P, order);
+    const double alpha = sig * dt / K.mu0;
     g1 = std::exp(-alpha);
-    g2 = K.z0 * (1.0 - g1) / (sig * s->ds);
+    g2 = K.z0 * (1.0 - g1) / (sig * ds);
   }
   // init_varx1 then init_varx2 shape on indices 1..n
   void grade(double ep, int n, double* f1, double* f2, double* g1, double* g2) const {
@@ -380,7 +379,7 @@ int setup_all_faces(Sim* s) {
   for (int a = 0; a < 3; ++a) {
     std::vector<double> f1(n[a] + 2, 1.0), f2(n[a] + 2, s->ku_pml), g1(n[a] + 2, 1.0), g2(n[a] + 2, s->ku_pml);
     if (s->P[a] > 0) {
-      Prof pr{s, sigmam_for(s, s->P[a]), s->P[a]};
+      Prof pr{s->dt, s->ds, s->pml.order, sigmam_for(s, s->P[a]), s->P[a]};
       pr.grade(consts().ep0, n[a], f1.data(), f2.data(), g1.data(), g2.data());
     }
     prof.insert(prof.end(), f1.begin(), f1.end());
@@ -431,8 +430,8 @@ int setup_ref_y(Sim* s) {
       g2x[x] = g2y[x] = g2z[x] = s->ku_pml;
     }
     const double ep = low ? eplow : ephigh;
-    if (npx > 0) Prof{s, sigmam, npx}.grade(ep, s->nx, f1x, f2x, g1x, g2x);  // :630-656
-    Prof py{s, sigmam, P};                                                   // :674-714
+    if (npx > 0) Prof{s->dt, s->ds, s->pml.order, sigmam, npx}.grade(ep, s->nx, f1x, f2x, g1x, g2x);  // :630-656
+    Prof py{s->dt, s->ds, s->pml.order, sigmam, P};                                                   // :674-714
     if (low) {
       for (int j = 1; j <= P; ++j) {
         py.e_low(eplow, j, f1y[j], f2y[j]);
@@ -446,7 +445,7 @@ int setup_ref_y(Sim* s) {
         --jj;
       }
     }
-    if (npz > 0) Prof{s, sigmam, npz}.grade(ep, s->nz, f1z, f2z, g1z, g2z);  // :719-745
+    if (npz > 0) Prof{s->dt, s->ds, s->pml.order, sigmam, npz}.grade(ep, s->nz, f1z, f2z, g1z, g2z);  // :719-745
   }
   s->d_ref_prof = s->prec == FDTD_FP32 ? upload<float>(s, all) : upload<double>(s, all);
   if (!s->d_slab || !s->d_ref_prof) return set_error(FDTD_ERR_NOMEM, "PML allocation");
@@ -604,7 +603,7 @@ int commit_impl(Sim* s) {
     if (m.type == 1) {
       drude2cp_coef(s, m.d2, c, &einf);
     } else {
-      int rc = graphene_coef(s, m.gr, c, &einf);
+      int rc = graphene_coef_dt(s->dt, s->ds, m.gr, c, &einf);
       if (rc) return rc;
     }
     int gi = -1;
@@ -890,6 +889,15 @@ void* field_ptr(Sim* s, int comp) {
 }
 
 }  // namespace
+
+int graphene3d_coef(double dt, double ds, const fdtd_graphene& p, AdeCoef<double>& c, double* eps_inf) {
+  return graphene_coef_dt(dt, ds, p, c, eps_inf);
+}
+
+void pml_grade(double dt, double ds, double order, double sigmam, int P, double ep, int n, double* f1, double* f2,
+               double* g1, double* g2) {
+  Prof{dt, ds, order, sigmam, P}.grade(ep, n, f1, f2, g1, g2);
+}
 
 // Upper bound of the device bytes commit_impl allocates (admission control).
 size_t estimate_bytes(const Sim* s) {

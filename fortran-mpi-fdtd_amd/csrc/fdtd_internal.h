@@ -254,6 +254,15 @@ int launch_set_ghosts(Sim* s, int comp);
 
 size_t estimate_bytes(const Sim* s);  // device bytes commit would allocate
 
+// graphene coefficients of mod3/graphene_mod.f90 (CP4 or Kubo, exact-
+// exponential Raman) for time step dt and cell size ds (fdtd_api.hip)
+int graphene3d_coef(double dt, double ds, const fdtd_graphene& p, AdeCoef<double>& c, double* eps_inf);
+
+// split-field PML grading of one axis, mod3/BERpmly_mod.f90:612-874: cells
+// 1..P graded low (E at P-x+1/2, H at P-x), n-P+1..n high; others untouched
+void pml_grade(double dt, double ds, double order, double sigmam, int P, double ep, int n, double* f1, double* f2,
+               double* g1, double* g2);
+
 }  // namespace fdtd
 
 // The opaque handle of the C ABI is the simulation state itself.

@@ -1,17 +1,18 @@
-"""ctypes mirror of include/fdtd2d.h (2D TEz stepper and batches)."""
+"""ctypes mirror of include/fdtd2d.h (2D TEz / TMz stepper and batches)."""
 import ctypes as C
 
 from .abi import Drude2CP, FdtdError, Graphene, Pulse  # noqa: F401
 
-EX2, EY2, HZ2, DX2, DY2 = range(5)
-COMP_NAMES_2D = ("Ex", "Ey", "Hz", "Dx", "Dy")
+TEZ, TMZ = 0, 1
+EX2, EY2, HZ2, DX2, DY2, EZ2, HX2, HY2, DZ2 = range(9)
+COMP_NAMES_2D = ("Ex", "Ey", "Hz", "Dx", "Dy", "Ez", "Hx", "Hy", "Dz")
 REGION_BOX, REGION_DISK = 0, 1
-DFT_EX, DFT_HZ = 1, 4
+DFT_EX, DFT_HZ, DFT_EZ, DFT_HX = 1, 4, 8, 16
 
 
 class Grid2D(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("precision", C.c_int32), ("periodic_x", C.c_int32),
-                ("ds", C.c_double), ("courant", C.c_double)]
+                ("ds", C.c_double), ("courant", C.c_double), ("polarization", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Pml2D(C.Structure):

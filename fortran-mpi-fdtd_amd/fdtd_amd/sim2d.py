@@ -14,7 +14,7 @@ import numpy as np
 from . import C0, NANO, PI, _drude2cp, _graphene, gold_params_dict
 from . import abi
 from . import abi2d
-from .abi2d import DFT_EX, DFT_HZ, DX2, DY2, EX2, EY2, HZ2  # noqa: F401
+from .abi2d import DFT_EX, DFT_EZ, DFT_HX, DFT_HZ, DX2, DY2, DZ2, EX2, EY2, EZ2, HX2, HY2, HZ2, TEZ, TMZ  # noqa: F401
 
 _binding2d = None
 
@@ -51,15 +51,18 @@ def _region2d(d):
 
 
 class Simulation2D:
-    """One 2D TEz domain (HIP library by default, or the oracle's binding)."""
+    """One 2D domain, TEz (the reference's: Hz, Ex, Ey) or TMz (Ez, Hx, Hy),
+    on the HIP library by default or on the oracle's binding."""
 
-    def __init__(self, nx, ny, ds, courant, precision="fp64", periodic_x=True, binding=None):
+    def __init__(self, nx, ny, ds, courant, precision="fp64", periodic_x=True, binding=None, polarization="tez"):
         self.b = binding or library2d()
         g = abi2d.Grid2D()
         g.nx, g.ny = int(nx), int(ny)
         g.precision = abi.FP32 if precision in ("fp32", abi.FP32) else abi.FP64
         g.periodic_x = int(bool(periodic_x))
         g.ds, g.courant = float(ds), float(courant)
+        g.polarization = TMZ if polarization in ("tmz", TMZ) else TEZ
+        self.tmz = g.polarization == TMZ
         self.grid = g
         self.shape = (int(nx), int(ny))
         self.h = C.c_void_p()
@@ -98,7 +101,10 @@ class Simulation2D:
         p = abi.Pulse(t0, st, w0, amplitude)
         self.b.check(self.b.add_line_source(self.h, int(j), C.byref(p)))
 
-    def add_dft_line(self, j, omega, mask=DFT_EX | DFT_HZ, n_begin=1, n_end=0):
+    def add_dft_line(self, j, omega, mask=None, n_begin=1, n_end=0):
+        """mask None: both components of the polarisation (Ex, Hz or Ez, Hx)."""
+        if mask is None:
+            mask = (DFT_EZ | DFT_HX) if self.tmz else (DFT_EX | DFT_HZ)
         om = np.ascontiguousarray(omega, dtype=np.float64)
         i = C.c_int()
         self.b.check(self.b.add_dft_line(self.h, int(j), len(om), om.ctypes.data_as(abi._PD), int(mask),
@@ -211,7 +217,7 @@ def apply_spec2d(sim, spec):
             raise ValueError(m["type"])
     for s in spec.get("sources", []):
         sim.add_line_source(s["j"], s["t0"], s["st"], s["w0"], s.get("amplitude", 1.0))
-    ids = [sim.add_dft_line(d["j"], np.array(d["omega"]), d.get("mask", DFT_EX | DFT_HZ), d.get("n_begin", 1),
+    ids = [sim.add_dft_line(d["j"], np.array(d["omega"]), d.get("mask"), d.get("n_begin", 1),
                             d.get("n_end", 0)) for d in spec.get("dft", [])]
     pids = [sim.add_probe(*pr) for pr in spec.get("probes", [])]
     return ids, pids
@@ -220,7 +226,8 @@ def apply_spec2d(sim, spec):
 def make_simulation2d(spec, precision="fp64", binding=None):
     g = spec["grid"]
     sim = Simulation2D(g["nx"], g["ny"], g["ds"], g["courant"], precision=precision,
-                       periodic_x=g.get("periodic_x", True), binding=binding)
+                       periodic_x=g.get("periodic_x", True), binding=binding,
+                       polarization=g.get("polarization", "tez"))
     ids, pids = apply_spec2d(sim, spec)
     return sim, ids, pids
 
@@ -264,6 +271,43 @@ def sweep_instance(rng, side=None, nsteps=None, ds=1e-9, npml=16, nfreq=32):
         "nsteps": steps,
         "radius_nm": r * ds / 1e-9,
         "fermi_ev": ef,
+    }
+
+
+def tmz_case(periodic_x=False, nx=40, ny=48, nsteps=300, graphene="cp4", precision_hint=None):
+    """A small TMz (Ez, Hx, Hy) case: PML on the y faces (and the x faces
+    when x is not periodic), dielectric layers below (over the source row)
+    and above, a gold disk, a graphene line
+    above it (CP4 with the Raman term, or Kubo), an Ez line source, DFT lines
+    of Ez and Hx below and above the particle, two probes."""
+    from .workloads import pulse_for_band
+    from . import graphene_ref_dict
+
+    P = (0 if periodic_x else 6, 8)
+    ds = 2e-9
+    cx, cy = (nx + 1) / 2.0, ny / 2.0
+    if graphene == "kubo":
+        gr = {"model": "kubo", "normal_axis": 1, "thickness": 3.4e-10, "eps_inf": 1.0, "fermi_ev": 0.4,
+              "tau": 1e-13, "temperature": 300.0, "wNL": 2.976e14, "dnuNL": 3.0e12, "xi3": 0.05}
+    else:
+        gr = dict(graphene_ref_dict())
+        gr.update(ref_init_order=0, wNL=2.976e14, dnuNL=3.0e12, xi3=0.05)
+    pul = pulse_for_band(400e-9, 900e-9, 600e-9)
+    om = 2.0 * PI * C0 / np.linspace(400e-9, 900e-9, 12)
+    g_row = int(round(cy + 6)) + 2
+    return {
+        "name": f"tmz_{'per' if periodic_x else 'pml'}_{graphene}",
+        "grid": {"nx": nx, "ny": ny, "ds": ds, "courant": 0.6, "periodic_x": periodic_x, "polarization": "tmz"},
+        "pml": {"npml": list(P), "order": 3.0, "R0": math.exp(-16.0)},
+        "materials": [{"type": "dielectric", "region": box2((1, 1), (nx, ny)), "eps": 1.0},
+                      {"type": "dielectric", "region": box2((1, 1), (nx, P[1] + 4)), "eps": 1.44},
+                      {"type": "dielectric", "region": box2((1, ny - P[1] - 9), (nx, ny)), "eps": 2.25},
+                      {"type": "drude2cp", "region": disk((cx, cy), 6.0)},
+                      {"type": "graphene", "region": box2((P[0] + 1, g_row), (nx - P[0], g_row)), "params": gr}],
+        "sources": [dict(j=P[1] + 3, **pul)],
+        "dft": [{"j": P[1] + 5, "omega": om}, {"j": ny - P[1] - 3, "omega": om, "n_begin": 40}],
+        "probes": [[int(cx), P[1] + 5], [3, g_row]],
+        "nsteps": nsteps,
     }
 
 

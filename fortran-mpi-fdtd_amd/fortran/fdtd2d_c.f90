@@ -6,13 +6,16 @@ module fdtd2d_c
     use fdtd_c
     implicit none
 
+    integer(c_int), parameter :: FDTD2D_TEZ = 0, FDTD2D_TMZ = 1
     integer(c_int), parameter :: FDTD2D_EX = 0, FDTD2D_EY = 1, FDTD2D_HZ = 2, FDTD2D_DX = 3, FDTD2D_DY = 4
+    integer(c_int), parameter :: FDTD2D_EZ = 5, FDTD2D_HX = 6, FDTD2D_HY = 7, FDTD2D_DZ = 8
     integer(c_int), parameter :: FDTD2D_REGION_BOX = 0, FDTD2D_REGION_DISK = 1
-    integer(c_int), parameter :: FDTD2D_DFT_EX = 1, FDTD2D_DFT_HZ = 4
+    integer(c_int), parameter :: FDTD2D_DFT_EX = 1, FDTD2D_DFT_HZ = 4, FDTD2D_DFT_EZ = 8, FDTD2D_DFT_HX = 16
 
     type, bind(c) :: fdtd2d_grid
         integer(c_int32_t) :: nx, ny, precision, periodic_x
         real(c_double) :: ds, courant
+        integer(c_int32_t) :: polarization = FDTD2D_TEZ, reserved = 0
     end type fdtd2d_grid
 
     type, bind(c) :: fdtd2d_pml

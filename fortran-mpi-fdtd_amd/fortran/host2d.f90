@@ -72,6 +72,7 @@ program host2d
     grid%periodic_x = 1                   ! PBC_x (sim2/program.f90:12)
     grid%ds = ds
     grid%courant = Ku
+    grid%polarization = FDTD2D_TEZ        ! the reference's 2D driver is TEz
     call fdtd_check(fdtd2d_create(sim, grid), 'create')
     call fdtd_check(fdtd2d_get_dt(sim, dt), 'get_dt')
 

@@ -25,6 +25,16 @@
  * coincide with the multi-rank run for every PML cell.
  * Extensions: disks next to boxes; graphene KUBO (Kubo intraband Drude
  * weight, the same form as the 3D oracle's).
+ *
+ * TMz (Ez, Hx, Hy; the reference's 2D driver has none) restates the 3D
+ * oracle's FUSED stepping (fdtd_oracle.c, ALL_FACES PML) for fields that do
+ * not vary along z: the 3D updates of sim3/include/myfdtd_func.f90:19-64
+ * with every z difference zero, the split-field PML of
+ * mod3/BERpmly_mod.f90:117-195 on the x and y faces (Ez = Ezx + Ezy; the
+ * Hxz and Hyz splits stay zero, so Hx and Hy are their y and x splits),
+ * gold on Ez (mod3/gold_mod.f90:50-88) and graphene on Ez with the 3D
+ * module's coefficients (mod3/graphene_mod.f90:202-283).  It is pinned
+ * against the 3D oracle run with nz = 1 and periodic z.
  */
 #include <complex.h>
 #include <math.h>
@@ -105,6 +115,12 @@ struct o2sim {
   o2probe* probes;
   int nprobes;
   int committed, n_done;
+  /* TMz */
+  int tmz;
+  double *ez, *hx, *hy, *dz;
+  double *ezx, *ezy, *rz;         /* PML splits of Ez, PML medium 1/eps */
+  int tp[2];                      /* PML depth along x, y */
+  double *tf1[2], *tf2[2], *tg1[2], *tg2[2]; /* profiles per axis, 0..n+1 */
 };
 typedef struct o2sim o2sim;
 
@@ -117,6 +133,7 @@ int ofdtd2d_create(o2sim** out, const fdtd2d_grid* g) {
   if (!out || !g) return o2_fail(FDTD_ERR_ARG, "null argument");
   if (g->nx < 2 || g->ny < 2) return o2_fail(FDTD_ERR_ARG, "grid too small");
   if (!(g->ds > 0) || !(g->courant > 0)) return o2_fail(FDTD_ERR_ARG, "ds and courant must be > 0");
+  if (g->polarization != FDTD2D_TEZ && g->polarization != FDTD2D_TMZ) return o2_fail(FDTD_ERR_ARG, "bad polarization");
   o2sim* s = (o2sim*)calloc(1, sizeof(o2sim));
   s->g = *g;
   s->nx = g->nx;
@@ -127,6 +144,8 @@ int ofdtd2d_create(o2sim** out, const fdtd2d_grid* g) {
   s->ku = g->courant;
   s->dt = g->courant * g->ds / O2_C0; /* sim2/program.f90:45 */
   s->ex = z2(s->n2); s->ey = z2(s->n2); s->hz = z2(s->n2); s->dx = z2(s->n2); s->dy = z2(s->n2);
+  s->tmz = g->polarization == FDTD2D_TMZ;
+  if (s->tmz) { s->ez = z2(s->n2); s->hx = z2(s->n2); s->hy = z2(s->n2); s->dz = z2(s->n2); }
   *out = s;
   return 0;
 }
@@ -137,6 +156,8 @@ int ofdtd2d_destroy(o2sim* s) {
   free(s->pex); free(s->pey); free(s->hzx); free(s->hzy);
   free(s->f1x); free(s->f2x); free(s->f1y); free(s->f2y);
   free(s->g1x); free(s->g2x); free(s->g1y); free(s->g2y);
+  free(s->ez); free(s->hx); free(s->hy); free(s->dz); free(s->ezx); free(s->ezy); free(s->rz);
+  for (int a = 0; a < 2; ++a) { free(s->tf1[a]); free(s->tf2[a]); free(s->tg1[a]); free(s->tg2[a]); }
   for (int q = 0; q < s->nmats; ++q) {
     o2mat* m = &s->mats[q];
     free(m->inside); free(m->Ix); free(m->Sx); free(m->Iy); free(m->Sy);
@@ -159,6 +180,14 @@ int ofdtd2d_destroy(o2sim* s) {
 int ofdtd2d_set_pml(o2sim* s, const fdtd2d_pml* p) {
   if (!s || !p) return o2_fail(FDTD_ERR_ARG, "null argument");
   if (s->committed) return o2_fail(FDTD_ERR_STATE, "set-up after commit");
+  if (s->tmz) {
+    if (p->npml[0] < 0 || p->npml[1] < 0 || 2 * p->npml[0] > s->nx || 2 * p->npml[1] > s->ny)
+      return o2_fail(FDTD_ERR_ARG, "bad npml");
+    if (p->npml[0] > 0 && s->g.periodic_x) return o2_fail(FDTD_ERR_ARG, "PML on a periodic axis");
+    s->pml = *p;
+    s->has_pml = p->npml[0] > 0 || p->npml[1] > 0;
+    return 0;
+  }
   if (p->npml[1] < 1 || p->npml[0] < 0 || 2 * p->npml[1] + 2 > s->ny || 2 * p->npml[0] > s->nx)
     return o2_fail(FDTD_ERR_ARG, "bad npml");
   if (!(p->eps_low > 0) || !(p->eps_high > 0)) return o2_fail(FDTD_ERR_ARG, "eps_low/eps_high must be > 0");
@@ -250,7 +279,55 @@ int ofdtd2d_add_drude2cp(o2sim* s, const fdtd2d_region* r, const fdtd_drude2cp* 
     m->gc[q] = dt * 2.0 * I * p->A[q] * p->Omega[q] * cexp(-I * p->phi[q]);
   }
   m->ga = p->eps_inf - m->gc[0] - m->gc[1];
-  alloc_state(m, 2);
+  alloc_state(m, s->tmz ? 1 : 2); /* TMz: Ez in the x slots */
+  return 0;
+}
+
+/* TMz graphene: the 3D module's coefficients, mod3/graphene_mod.f90:202-283
+ * (CP4 with the init-order quirk when ref_init_order is set, exact-exponential
+ * Raman), or the Kubo Drude weight. */
+static int graphene_tmz(o2sim* s, o2mat* m, const fdtd_graphene* p) {
+  const double dt = s->dt;
+  if (p->model == FDTD_GRAPHENE_CP4) {
+    const double scale = p->thickness / s->ds;
+    const double wD = sqrt(scale) * p->omega_d;
+    m->ha = exp(-p->gamma * dt);
+    m->hb = (wD * wD) * dt / p->gamma;
+    for (int q = 0; q < 4; ++q) {
+      double complex expi = cexp(-I * p->phi[q]);
+      double complex ex = cexp((-p->Gamma[q] + I * p->Omega[q]) * dt);
+      if (p->ref_init_order) { expi = 0; ex = 0; }
+      m->gb[q] = ex;
+      m->gc[q] = dt * 2.0 * I * (scale * p->A[q]) * p->Omega[q] * expi;
+    }
+    m->ga = p->eps_inf - m->gc[0] - m->gc[1] - m->gc[2] - m->gc[3];
+  } else if (p->model == FDTD_GRAPHENE_KUBO) {
+    if (!(p->tau > 0) || !(p->temperature > 0)) return o2_fail(FDTD_ERR_ARG, "tau and T must be > 0");
+    const double hbar = O2_H / (2.0 * O2_PI);
+    const double kT = O2_KB * p->temperature;
+    const double ef = p->fermi_ev * O2_Q;
+    const double Dw = 2.0 * O2_Q * O2_Q * kT / (hbar * hbar) * log(2.0 * cosh(ef / (2.0 * kT)));
+    const double wp2 = Dw / (O2_PI * O2_EP0 * s->ds);
+    const double gam = 1.0 / p->tau;
+    m->ha = exp(-gam * dt);
+    m->hb = wp2 * dt / gam;
+    for (int q = 0; q < 4; ++q) { m->gb[q] = 0; m->gc[q] = 0; }
+    m->ga = p->eps_inf;
+  } else {
+    return o2_fail(FDTD_ERR_ARG, "bad graphene model");
+  }
+  m->raman0 = m->raman1 = m->raman2 = 0.0;
+  if (p->wNL > 0) { /* graphene_mod.f90:216-218,256-258 */
+    const double alfaR = 0.5 * p->dnuNL;
+    const double betaR = sqrt((p->wNL * p->wNL) - (alfaR * alfaR));
+    const double omegaR = (p->wNL * p->wNL) / betaR;
+    m->raman0 = p->xi3 * omegaR * dt * exp(-alfaR * dt) * sin(betaR * dt);
+    m->raman1 = 2.0 * exp(-alfaR * dt) * cos(betaR * dt);
+    m->raman2 = -exp(-2.0 * alfaR * dt);
+  }
+  alloc_state(m, 1);
+  const long nb = (long)m->bx * m->by;
+  m->sn = z2(nb); m->snm1 = z2(nb); m->snm2 = z2(nb);
   return 0;
 }
 
@@ -262,6 +339,7 @@ int ofdtd2d_add_graphene(o2sim* s, const fdtd2d_region* r, const fdtd_graphene* 
   if (!m) return o2_fail(FDTD_ERR_ARG, "bad region");
   const double dt = s->dt;
   m->ncp = 4;
+  if (s->tmz) return graphene_tmz(s, m, p);
   if (p->model == FDTD_GRAPHENE_CP4) {
     /* setexp, scaleParams, initialize_GrapheneVars: graphene_mod.f90:163-236 */
     const double scale = p->thickness / s->ds;
@@ -314,8 +392,8 @@ int ofdtd2d_add_line_source(o2sim* s, int j, const fdtd_pulse* p) {
 int ofdtd2d_add_dft_line(o2sim* s, int j, int nfreq, const double* omega, int mask, int n_begin, int n_end, int* id) {
   if (!s || !omega || !id) return o2_fail(FDTD_ERR_ARG, "null argument");
   if (s->committed) return o2_fail(FDTD_ERR_STATE, "set-up after commit");
-  if (j < 1 || j > s->ny || nfreq < 1 || !(mask & (FDTD2D_DFT_EX | FDTD2D_DFT_HZ)) ||
-      (mask & ~(FDTD2D_DFT_EX | FDTD2D_DFT_HZ)))
+  const int all = s->tmz ? (FDTD2D_DFT_EZ | FDTD2D_DFT_HX) : (FDTD2D_DFT_EX | FDTD2D_DFT_HZ);
+  if (j < 1 || j > s->ny || nfreq < 1 || !(mask & all) || (mask & ~all))
     return o2_fail(FDTD_ERR_ARG, "bad DFT line");
   s->dfts = (o2dft*)realloc(s->dfts, sizeof(o2dft) * (size_t)(s->ndfts + 1));
   o2dft* d = &s->dfts[s->ndfts];
@@ -430,9 +508,86 @@ static void pml_init(o2sim* s) {
   }
 }
 
+/* TMz PML: init_all_faces of the 3D oracle for the x and y axes (profiles
+ * of mod3/BERpmly_mod.f90:612-874 with vacuum grading: E at d = P-x+1/2,
+ * H at d = P-x below, mirrored above), defaults f1 = g1 = 1, f2 = g2 = Ku;
+ * the PML medium is 1/eps of the last dielectric written to the cell. */
+static int tmz_init(o2sim* s) {
+  const double z0 = sqrt(O2_MU0 / O2_EP0), ep = O2_EP0, mu0 = O2_MU0;
+  const double dt = s->dt, ds = s->ds, order = s->pml.order;
+  const double ku_pml = dt * O2_C0 / ds;
+  const int n[2] = {s->nx, s->ny};
+  for (int a = 0; a < 2; ++a) {
+    const int P = s->has_pml ? s->pml.npml[a] : 0;
+    s->tp[a] = P;
+    s->tf1[a] = z2(n[a] + 2); s->tf2[a] = z2(n[a] + 2); s->tg1[a] = z2(n[a] + 2); s->tg2[a] = z2(n[a] + 2);
+    for (int x = 0; x < n[a] + 2; ++x) { s->tf1[a][x] = 1.0; s->tf2[a][x] = ku_pml; s->tg1[a][x] = 1.0; s->tg2[a][x] = ku_pml; }
+    if (P == 0) continue;
+    const double sigmam = s->pml.sigmam > 0 ? s->pml.sigmam
+                                            : (order + 1.0) * O2_C0 * log(1.0 / s->pml.R0) / (2.0 * P * ds);
+    for (int x = 1; x <= P; ++x) {
+      const double sE = ep * sigmam * pow(((double)(P - x) + 0.5) / (double)P, order);
+      s->tf1[a][x] = exp(-(sE * dt / ep));
+      s->tf2[a][x] = (1.0 - s->tf1[a][x]) / (z0 * sE * ds);
+      if (x < P) {
+        const double sH = mu0 * sigmam * pow(((double)(P - x)) / (double)P, order);
+        s->tg1[a][x] = exp(-(sH * dt / mu0));
+        s->tg2[a][x] = z0 * (1.0 - s->tg1[a][x]) / (sH * ds);
+      }
+    }
+    int q = P;
+    for (int x = n[a] - P + 1; x <= n[a]; ++x) {
+      if (q < P) {
+        const double sE = ep * sigmam * pow(((double)(P - q)) / (double)P, order);
+        s->tf1[a][x] = exp(-(sE * dt / ep));
+        s->tf2[a][x] = (1.0 - s->tf1[a][x]) / (z0 * sE * ds);
+      }
+      const double sH = mu0 * sigmam * pow(((double)(P - q) + 0.5) / (double)P, order);
+      s->tg1[a][x] = exp(-(sH * dt / mu0));
+      s->tg2[a][x] = z0 * (1.0 - s->tg1[a][x]) / (sH * ds);
+      --q;
+    }
+  }
+  s->ezx = z2(s->n2); s->ezy = z2(s->n2); s->rz = z2(s->n2);
+  for (long c = 0; c < s->n2; ++c) s->rz[c] = 1.0;
+  for (int q = 0; q < s->nmats; ++q) {
+    const o2mat* m = &s->mats[q];
+    for (int j = m->y0; j <= m->y1; ++j)
+      for (int i = m->x0; i <= m->x1; ++i) {
+        if (!m->inside[(i - m->x0) + (long)m->bx * (j - m->y0)]) continue;
+        const int in_pml = (s->tp[0] > 0 && (i <= s->tp[0] || i > s->nx - s->tp[0])) ||
+                           (s->tp[1] > 0 && (j <= s->tp[1] || j > s->ny - s->tp[1]));
+        if (!in_pml) continue;
+        if (m->type == M2_DIEL) s->rz[F2(s, i, j)] = m->gax;
+      }
+  }
+  /* a dispersive cell in the PML would be left to the PML medium there (the
+   * 3D engine's rule); refuse it, as the GPU library does */
+  for (int j = 1; j <= s->ny; ++j)
+    for (int i = 1; i <= s->nx; ++i) {
+      const int in_pml = (s->tp[0] > 0 && (i <= s->tp[0] || i > s->nx - s->tp[0])) ||
+                         (s->tp[1] > 0 && (j <= s->tp[1] || j > s->ny - s->tp[1]));
+      if (!in_pml) continue;
+      int last = -1;
+      for (int q = 0; q < s->nmats; ++q) {
+        const o2mat* m = &s->mats[q];
+        if (i < m->x0 || i > m->x1 || j < m->y0 || j > m->y1) continue;
+        if (m->inside[(i - m->x0) + (long)m->bx * (j - m->y0)]) last = q;
+      }
+      if (last >= 0 && s->mats[last].type != M2_DIEL) return o2_fail(FDTD_ERR_UNSUPPORTED, "dispersive cells inside the PML");
+    }
+  return 0;
+}
+
 int ofdtd2d_commit(o2sim* s) {
   if (!s) return o2_fail(FDTD_ERR_ARG, "null argument");
   if (s->committed) return 0;
+  if (s->tmz) {
+    int rc = tmz_init(s);
+    if (rc) return rc;
+    s->committed = 1;
+    return 0;
+  }
   if (s->has_pml) {
     s->sigmam = s->pml.sigmam > 0 ? s->pml.sigmam
                                   : (s->pml.order + 1.0) * O2_C0 * log(1.0 / s->pml.R0) /
@@ -556,6 +711,11 @@ static void wrap_x(o2sim* s, double* f) {
   for (int j = 0; j <= s->ny + 1; ++j) f[F2(s, 0, j)] = f[F2(s, s->nx, j)];
 }
 
+/* mask bit of DFT slot c (0: Ex or Ez, 1: Hz or Hx) */
+static int dft_bit(const o2sim* s, int c) {
+  return s->tmz ? (c == 0 ? FDTD2D_DFT_EZ : FDTD2D_DFT_HX) : (c == 0 ? FDTD2D_DFT_EX : FDTD2D_DFT_HZ);
+}
+
 static void monitors(o2sim* s, int n) {
   const double t = (double)(n - 1) * s->dt;
   for (int q = 0; q < s->ndfts; ++q) {
@@ -566,8 +726,8 @@ static void monitors(o2sim* s, int n) {
     for (int m = 0; m < d->nfreq; ++m) {
       const double cs = cos(d->omega[m] * t), sn = sin(d->omega[m] * t);
       for (int c = 0; c < 2; ++c) {
-        if (!(d->mask & (c == 0 ? FDTD2D_DFT_EX : FDTD2D_DFT_HZ))) continue;
-        const double* f = c == 0 ? s->ex : s->hz;
+        if (!(d->mask & dft_bit(s, c))) continue;
+        const double* f = s->tmz ? (c == 0 ? s->ez : s->hx) : (c == 0 ? s->ex : s->hz);
         double* re = d->re[c] + (long)m * s->nx;
         double* im = d->im[c] + (long)m * s->nx;
         for (int i = 1; i <= s->nx; ++i) {
@@ -586,10 +746,111 @@ static void monitors(o2sim* s, int n) {
     }
     const long c = F2(s, p->i, p->j);
     p->rec[3 * p->nrec + 0] = (double)n * s->dt;
-    p->rec[3 * p->nrec + 1] = s->ex[c];
-    p->rec[3 * p->nrec + 2] = s->hz[c];
+    p->rec[3 * p->nrec + 1] = s->tmz ? s->ez[c] : s->ex[c];
+    p->rec[3 * p->nrec + 2] = s->tmz ? s->hx[c] : s->hz[c];
     p->nrec++;
   }
+}
+
+/* TMz materials, non-PML cells, in call order: dielectric (E = D*gax), gold
+ * (mod3/gold_mod.f90:61-83) and graphene (mod3/graphene_mod.f90:86-100) on Ez */
+static int tmz_in_pml(const o2sim* s, int i, int j) {
+  return (s->tp[0] > 0 && (i <= s->tp[0] || i > s->nx - s->tp[0])) ||
+         (s->tp[1] > 0 && (j <= s->tp[1] || j > s->ny - s->tp[1]));
+}
+
+static void tmz_materials(o2sim* s) {
+  for (int q = 0; q < s->nmats; ++q) {
+    o2mat* m = &s->mats[q];
+    for (int j = m->y0; j <= m->y1; ++j)
+      for (int i = m->x0; i <= m->x1; ++i) {
+        const long b = (i - m->x0) + (long)m->bx * (j - m->y0);
+        if (!m->inside[b] || tmz_in_pml(s, i, j)) continue;
+        const long c = F2(s, i, j);
+        if (m->type == M2_DIEL) {
+          s->ez[c] = s->dz[c] * m->gax;
+        } else if (m->type == M2_DRUDE2CP) {
+          const double complex num =
+              (s->dz[c] - m->Ix[b] + m->ha * m->Sx[b]) + m->gb[0] * m->gx[0][b] + m->gb[1] * m->gx[1][b];
+          const double e = creal(num / m->ga);
+          s->ez[c] = e;
+          m->Ix[b] = m->Ix[b] + m->hb * e;
+          m->Sx[b] = m->ha * m->Sx[b] + m->hb * e;
+          m->gx[0][b] = m->gb[0] * m->gx[0][b] + m->gc[0] * e;
+          m->gx[1][b] = m->gb[1] * m->gx[1][b] + m->gc[1] * e;
+        } else {
+          const double complex num = (s->dz[c] - m->Ix[b] + m->ha * m->Sx[b]) + m->gb[0] * m->gx[0][b] +
+                                     m->gb[1] * m->gx[1][b] + m->gb[2] * m->gx[2][b] + m->gb[3] * m->gx[3][b];
+          const double e = creal(num / (m->ga + m->sn[b]));
+          s->ez[c] = e;
+          m->sn[b] = m->raman1 * m->snm1[b] + m->raman2 * m->snm2[b] + m->raman0 * (e * e);
+          m->snm2[b] = m->snm1[b];
+          m->snm1[b] = m->sn[b];
+          m->Ix[b] = m->Ix[b] + m->hb * e;
+          m->Sx[b] = m->ha * m->Sx[b] + m->hb * e;
+          for (int p = 0; p < 4; ++p) m->gx[p][b] = m->gb[p] * m->gx[p][b] + m->gc[p] * e;
+        }
+      }
+  }
+}
+
+static void monitors(o2sim* s, int n);
+static void wrap_x(o2sim* s, double* f);
+
+/* One TMz step in the 3D FUSED order: D update of non-PML cells
+ * (myfdtd_func.f90:19-41, dz row), split-field E in the PML
+ * (BERpmly_mod.f90:117-159, Ezx and Ezy), sources, materials, periodic
+ * ghosts of Ez, H update of non-PML cells (:45-64), split-field H in the
+ * PML (:161-195), ghosts of Hy, monitors. */
+static void tmz_step(o2sim* s, int n) {
+  const int nx = s->nx, ny = s->ny;
+  const long sx = s->sx;
+  const double ku = s->ku, t = (double)(n - 1) * s->dt;
+  double *ez = s->ez, *hx = s->hx, *hy = s->hy, *dz = s->dz;
+  const double *f1x = s->tf1[0], *f2x = s->tf2[0], *f1y = s->tf1[1], *f2y = s->tf2[1];
+  const double *g1x = s->tg1[0], *g2x = s->tg2[0], *g1y = s->tg1[1], *g2y = s->tg2[1];
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) {
+      const long c = F2(s, i, j);
+      if (!tmz_in_pml(s, i, j)) {
+        const double curlhyx = hy[c] - hy[c - 1];
+        const double curlhxy = -hx[c] + hx[c - sx];
+        dz[c] = dz[c] + ku * (curlhyx + curlhxy);
+      } else {
+        const double r = s->rz[c];
+        s->ezx[c] = f1x[i] * s->ezx[c] + (f2x[i] * r) * (hy[c] + -hy[c - 1]);
+        s->ezy[c] = f1y[j] * s->ezy[c] + (f2y[j] * r) * (-hx[c] + hx[c - sx]);
+        ez[c] = s->ezx[c] + s->ezy[c];
+      }
+    }
+  for (int q = 0; q < s->nsrcs; ++q) {
+    const double g = pulse_at(&s->srcs[q].p, t);
+    const int j = s->srcs[q].j;
+    for (int i = 1; i <= nx; ++i) {
+      const long c = F2(s, i, j);
+      if (tmz_in_pml(s, i, j)) {
+        s->ezy[c] = s->ezy[c] + g * s->rz[c];
+        ez[c] = s->ezx[c] + s->ezy[c];
+      } else {
+        dz[c] = dz[c] + g;
+      }
+    }
+  }
+  tmz_materials(s);
+  if (s->g.periodic_x) wrap_x(s, ez);
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) {
+      const long c = F2(s, i, j);
+      if (!tmz_in_pml(s, i, j)) {
+        hx[c] = hx[c] + ku * (-ez[c + sx] + ez[c]);
+        hy[c] = hy[c] + ku * (ez[c + 1] - ez[c]);
+      } else {
+        hx[c] = g1y[j] * hx[c] + g2y[j] * (ez[c] + -ez[c + sx]);
+        hy[c] = g1x[i] * hy[c] + g2x[i] * (-ez[c] + ez[c + 1]);
+      }
+    }
+  if (s->g.periodic_x) wrap_x(s, hy);
+  monitors(s, n);
 }
 
 int ofdtd2d_step(o2sim* s, int nsteps) {
@@ -597,6 +858,13 @@ int ofdtd2d_step(o2sim* s, int nsteps) {
   if (!s->committed) {
     int rc = ofdtd2d_commit(s);
     if (rc) return rc;
+  }
+  if (s->tmz) {
+    for (int it = 0; it < nsteps; ++it) {
+      tmz_step(s, s->n_done + 1);
+      s->n_done++;
+    }
+    return 0;
   }
   const int nx = s->nx, ny = s->ny;
   const int j0 = s->has_pml ? s->npy : 1, j1 = s->has_pml ? ny - s->npy : ny;
@@ -650,6 +918,15 @@ int ofdtd2d_get_dt(o2sim* s, double* dt) {
 }
 
 static double* field2(o2sim* s, int comp) {
+  if (s->tmz) {
+    switch (comp) {
+      case FDTD2D_EZ: return s->ez;
+      case FDTD2D_HX: return s->hx;
+      case FDTD2D_HY: return s->hy;
+      case FDTD2D_DZ: return s->dz;
+      default: return NULL;
+    }
+  }
   switch (comp) {
     case FDTD2D_EX: return s->ex;
     case FDTD2D_EY: return s->ey;
@@ -679,15 +956,17 @@ int ofdtd2d_set_field(o2sim* s, int comp, const double* src) {
   }
   for (int j = 1; j <= s->ny; ++j)
     for (int i = 1; i <= s->nx; ++i) f[F2(s, i, j)] = src[C2(s, i, j)];
-  if (s->g.periodic_x && (comp == FDTD2D_EY || comp == FDTD2D_HZ)) wrap_x(s, f);
+  if (s->g.periodic_x && (comp == FDTD2D_EY || comp == FDTD2D_HZ || comp == FDTD2D_EZ || comp == FDTD2D_HY))
+    wrap_x(s, f);
   return 0;
 }
 
 int ofdtd2d_get_dft(o2sim* s, int id, int comp, double* re, double* im) {
   if (!s || !re || !im || id < 0 || id >= s->ndfts) return o2_fail(FDTD_ERR_ARG, "bad argument");
   const o2dft* d = &s->dfts[id];
-  const int c = comp == FDTD2D_EX ? 0 : (comp == FDTD2D_HZ ? 1 : -1);
-  if (c < 0 || !(d->mask & (c == 0 ? FDTD2D_DFT_EX : FDTD2D_DFT_HZ))) return o2_fail(FDTD_ERR_ARG, "component not recorded");
+  const int c = s->tmz ? (comp == FDTD2D_EZ ? 0 : (comp == FDTD2D_HX ? 1 : -1))
+                       : (comp == FDTD2D_EX ? 0 : (comp == FDTD2D_HZ ? 1 : -1));
+  if (c < 0 || !(d->mask & dft_bit(s, c))) return o2_fail(FDTD_ERR_ARG, "component not recorded");
   memcpy(re, d->re[c], sizeof(double) * (size_t)s->nx * d->nfreq);
   memcpy(im, d->im[c], sizeof(double) * (size_t)s->nx * d->nfreq);
   return 0;
@@ -696,8 +975,18 @@ int ofdtd2d_get_dft(o2sim* s, int id, int comp, double* re, double* im) {
 int ofdtd2d_get_flux(o2sim* s, int id, double* S) {
   if (!s || !S || id < 0 || id >= s->ndfts) return o2_fail(FDTD_ERR_ARG, "bad argument");
   const o2dft* d = &s->dfts[id];
-  if ((d->mask & (FDTD2D_DFT_EX | FDTD2D_DFT_HZ)) != (FDTD2D_DFT_EX | FDTD2D_DFT_HZ))
-    return o2_fail(FDTD_ERR_ARG, "flux needs Ex and Hz");
+  if ((d->mask & (dft_bit(s, 0) | dft_bit(s, 1))) != (dft_bit(s, 0) | dft_bit(s, 1)))
+    return o2_fail(FDTD_ERR_ARG, "flux needs both recorded components");
+  if (s->tmz) { /* sim3/program.f90:613-685: the Hx*Ez terms */
+    for (int m = 0; m < d->nfreq; ++m) {
+      double sy = 0.0;
+      const long o = (long)m * s->nx;
+      for (int i = 0; i < s->nx; ++i)
+        sy = sy + ((d->re[1][o + i] * d->re[0][o + i]) + (d->im[1][o + i] * d->im[0][o + i])) * s->ds;
+      S[m] = sy;
+    }
+    return 0;
+  }
   for (int m = 0; m < d->nfreq; ++m) {
     double sy = 0.0;
     const long o = (long)m * s->nx;
