@@ -147,17 +147,15 @@ def served(args):
     return out
 
 
-def batched_2d(args):
-    """Batched 2D TEz sweep: `--b2-count` instances drawn from the seed
-    (sides 256-2048 cells, gold disk r 10-80 nm, Kubo graphene line E_F
-    0.1-0.6 eV, 16-cell PML, 32 wavelengths on 2 lines, FP32), all stepped
-    `--b2-steps` steps in one batch, then the same instances one at a time."""
+def _batch_vs_separate(specs, steps):
+    """Time `steps` steps of `specs` stepped as one batch (fdtd2d_batch_run),
+    then the same simulations as separate handles (one fdtd2d_step call per
+    handle, each on its own stream, all enqueued before the first sync)."""
     import numpy as np
     from fdtd_amd import sim2d
 
-    specs = sim2d.sweep_stream(seed=args.seed, count=args.b2_count, nsteps=args.b2_steps)
     cells = sum(s["grid"]["nx"] * s["grid"]["ny"] for s in specs)
-    work = cells * args.b2_steps
+    work = cells * steps
 
     def build():
         sims = [sim2d.make_simulation2d(s, precision="fp32") for s in specs]
@@ -170,7 +168,7 @@ def batched_2d(args):
     batch.run([8] * len(sims))  # graph capture and first touch, untimed
     batch.sync()
     t0 = time.perf_counter()
-    batch.run([args.b2_steps] * len(sims))
+    batch.run([steps] * len(sims))
     batch.sync()
     tb = time.perf_counter() - t0
     fb = [s.flux(ids[1]) for s, ids, _ in sims]
@@ -183,18 +181,44 @@ def batched_2d(args):
         s.sync()
     t0 = time.perf_counter()
     for s, _, _ in sims:
-        s.step(args.b2_steps)
+        s.step(steps)
     for s, _, _ in sims:
         s.sync()
     ts = time.perf_counter() - t0
     same = all(np.array_equal(a, s.flux(ids[1])) for a, (s, ids, _) in zip(fb, sims))
     for s, _, _ in sims:
         s.close()
-    # algorithmic words per main cell-update: E 7 (Hz, Dx, Dy rw, Ex, Ey), H 4
-    return {"instances": len(specs), "cells": cells, "steps": args.b2_steps, "dtype": "f32",
-            "batched_mcells_per_s": work / tb / 1e6, "one_at_a_time_mcells_per_s": work / ts / 1e6,
-            "speedup": ts / tb, "achieved_GBs_all_in": work * 11 * 4 / tb / 1e9,
+    # algorithmic words per main cell-update: TEz E 7 (Hz, Dx, Dy rw, Ex, Ey),
+    # H 4; TMz E 5 (Hx, Hy, Dz rw, Ez), H 5 (Ez, Hx rw, Hy rw): 11 and 10
+    words = sum((10 if s["grid"].get("polarization") == "tmz" else 11) * s["grid"]["nx"] * s["grid"]["ny"]
+                for s in specs) / cells
+    n_tmz = sum(1 for s in specs if s["grid"].get("polarization") == "tmz")
+    return {"instances": len(specs), "tmz_instances": n_tmz, "cells": cells, "steps": steps, "dtype": "f32",
+            "batched_mcells_per_s": work / tb / 1e6, "separate_handles_mcells_per_s": work / ts / 1e6,
+            "speedup": ts / tb, "achieved_GBs_all_in": work * words * 4 / tb / 1e9,
             "results_identical_batched_and_alone": bool(same)}
+
+
+def batched_2d(args):
+    """Batched 2D sweep: `--b2-count` TEz and TMz instances drawn from the
+    seed (sides 256-2048 cells, gold disk r 10-80 nm, Kubo graphene line E_F
+    0.1-0.6 eV, 16-cell PML, 32 wavelengths on 2 lines, FP32), all stepped
+    `--b2-steps` steps in one batch, then as separate handles."""
+    from fdtd_amd import sim2d
+
+    specs = sim2d.sweep_stream(seed=args.seed, count=args.b2_count, nsteps=args.b2_steps)
+    return _batch_vs_separate(specs, args.b2_steps)
+
+
+def batched_2d_small(args):
+    """The same sweep draws at a fixed side of `--b2s-side` cells: instances
+    too small to fill the GPU alone, where one batch replaces many launches."""
+    from fdtd_amd import sim2d
+
+    specs = sim2d.sweep_stream(seed=args.seed, count=args.b2_count, nsteps=args.b2s_steps, side=args.b2s_side)
+    out = _batch_vs_separate(specs, args.b2s_steps)
+    out["side"] = args.b2s_side
+    return out
 
 
 def fp32_spectra_error():
@@ -228,8 +252,10 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-2d", action="store_true")
     ap.add_argument("--no-spectra", action="store_true", help="skip the FP32-vs-FP64 spectra check")
-    ap.add_argument("--b2-count", type=int, default=64, help="batched 2D sweep: instances")
-    ap.add_argument("--b2-steps", type=int, default=1000, help="batched 2D sweep: timed steps")
+    ap.add_argument("--b2-count", type=int, default=256, help="batched 2D sweep: instances")
+    ap.add_argument("--b2-steps", type=int, default=500, help="batched 2D sweep: timed steps")
+    ap.add_argument("--b2s-side", type=int, default=128, help="small batched 2D sweep: grid side")
+    ap.add_argument("--b2s-steps", type=int, default=1000, help="small batched 2D sweep: timed steps")
     ap.add_argument("--jobs", type=int, default=24)
     ap.add_argument("--rate", type=float, default=20.0, help="served-stream arrival rate [jobs/s]")
     ap.add_argument("--max-active", type=int, default=16)
@@ -272,11 +298,14 @@ def main():
         srv = served(args)
         print(f"[bench] served {srv['mcells_per_s']:.0f} Mcell/s p50 {srv['p50_ms']:.0f} ms p99 {srv['p99_ms']:.0f} ms",
               file=sys.stderr, flush=True)
-    b2 = None
+    b2 = b2s = None
     if not args.no_2d:
         b2 = batched_2d(args)
-        print(f"[bench] batched 2d {b2['batched_mcells_per_s']:.0f} Mcell/s ({b2['speedup']:.2f}x one at a time)",
+        print(f"[bench] batched 2d {b2['batched_mcells_per_s']:.0f} Mcell/s ({b2['speedup']:.2f}x separate handles)",
               file=sys.stderr, flush=True)
+        b2s = batched_2d_small(args)
+        print(f"[bench] batched 2d small {b2s['batched_mcells_per_s']:.0f} Mcell/s "
+              f"({b2s['speedup']:.2f}x separate handles)", file=sys.stderr, flush=True)
     spec_err = None if args.no_spectra else fp32_spectra_error()
     if args.n == 512 and args.seed == 0:  # the workload the summary profiles
         tb, src = profiled_traffic(r32["roofline"]["kernel"], "fp32")
@@ -308,6 +337,7 @@ def main():
         "p99_job_latency_ms": None if srv is None else srv["p99_ms"],
         "served_stream": srv,
         "batched_2d": b2,
+        "batched_2d_small": b2s,
         "detail": {"fp32": {k: r32[k] for k in ("kernels", "h_update", "step_form", "device_bytes", "cell_counts",
                                                  "setup_s")},
                    "fp64": None if r64 is None else {k: r64[k] for k in ("kernels", "h_update", "step_form",

@@ -41,6 +41,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -1196,7 +1197,13 @@ template <typename real>
 void enqueue_block(Batch* b) {
   hipStream_t st = b->stream;
   const Inst<real>* I = (const Inst<real>*)b->d_inst;
-  const unsigned grid = (unsigned)std::min(b->ntiles, 8 * 256 * 4);
+  // blocks of the E and H launches; each strides over the packed tile list
+  int tpb = 8;
+  if (const char* e = std::getenv("FDTD2D_TILES_PER_BLOCK")) {
+    const int v = std::atoi(e);
+    if (v >= 1 && v <= 4096) tpb = v;
+  }
+  const unsigned grid = (unsigned)std::min(b->ntiles, std::max(8192, (b->ntiles + tpb - 1) / tpb));
   const int count = (int)b->sims.size();
   for (int s = 0; s < kBlockSteps; ++s) {
     hipLaunchKernelGGL(k2_e<real>, dim3(grid), dim3(kTX * kTY), 0, st, I, b->d_tile_inst, b->ntiles);

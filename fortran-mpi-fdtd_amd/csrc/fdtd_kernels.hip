@@ -1792,6 +1792,10 @@ int step_impl(Sim* s) {
 }  // namespace
 
 int launch_step(Sim* s) {
+  // launch errors are read back with hipGetLastError, which reports the
+  // last failing call of this thread: clear what an earlier, unrelated call
+  // left so it is not taken for a launch failure of this step
+  (void)hipGetLastError();
   int rc = s->prec == FDTD_FP32 ? step_impl<float>(s) : step_impl<double>(s);
   if (rc) return rc;
   hipError_t e = hipGetLastError();

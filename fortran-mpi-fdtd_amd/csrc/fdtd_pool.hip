@@ -129,6 +129,7 @@ int fdtd_pool_create(fdtd_pool** out, size_t hbm_budget, int max_active, int chu
 
 int fdtd_pool_destroy(fdtd_pool* p) {
   if (!p) return 0;
+  // events of jobs never released (their handles must still be alive)
   for (auto& j : p->jobs)
     for (int e = 0; e < 2; ++e)
       if (j.ev[e]) {
@@ -227,6 +228,16 @@ int fdtd_pool_release(fdtd_pool* p, int ticket) {
     p->used -= j.bytes;
     j.counted = false;
   }
+  // the chunk events go now, while the job's stream still exists: the
+  // caller may destroy the handle next, and an event must not outlive the
+  // stream it was recorded on
+  for (int e = 0; e < 2; ++e)
+    if (j.ev[e]) {
+      (void)hipEventSynchronize(j.ev[e]);
+      (void)hipEventDestroy(j.ev[e]);
+      j.ev[e] = nullptr;
+    }
+  j.inflight = 0;
   return 0;
 }
 

@@ -233,21 +233,25 @@ def make_simulation2d(spec, precision="fp64", binding=None):
 
 
 # ---- workloads -------------------------------------------------------------------
-def sweep_instance(rng, side=None, nsteps=None, ds=1e-9, npml=16, nfreq=32):
+def sweep_instance(rng, side=None, nsteps=None, ds=1e-9, npml=16, nfreq=32, polarization="tez"):
     """One simulation of the batched 2D sweep, drawn from `rng` in this order:
     grid side (uniform 256-2048 cells), gold cylinder radius (uniform
     10-80 nm), graphene Fermi level (uniform 0.1-0.6 eV), step count (uniform
-    3000-8000).  Layout along y: 16-cell PML, Dx line source, reflection
-    line, the gold disk at mid-height with a Kubo graphene line (tau 0.1 ps,
-    300 K) two cells above it, transmission line, PML; x periodic with the
-    reference's x grading inside the slabs.  `side`/`nsteps` override the
-    drawn values (the draws still happen, so the stream stays aligned)."""
+    3000-8000), polarisation (TEz or TMz, even odds).  Layout along y:
+    16-cell PML, line source, reflection line, the gold disk at mid-height
+    with a Kubo graphene line (tau 0.1 ps, 300 K) two cells above it,
+    transmission line, PML; x periodic (TEz: the reference's x grading
+    inside the slabs; TMz: y slabs only).  `side`/`nsteps` override the
+    drawn values and `polarization` ("tez", "tmz" or "drawn") the drawn
+    polarisation (the draws still happen, so the stream stays aligned)."""
     from .workloads import pulse_for_band
 
     s_draw = int(rng.integers(256, 2049))
     r_nm = float(rng.uniform(10.0, 80.0))
     ef = float(rng.uniform(0.1, 0.6))
     n_draw = int(rng.integers(3000, 8001))
+    tm_draw = bool(rng.random() < 0.5)
+    tm = tm_draw if polarization == "drawn" else polarization == "tmz"
     n = side or s_draw
     steps = nsteps or n_draw
     P = npml
@@ -259,9 +263,10 @@ def sweep_instance(rng, side=None, nsteps=None, ds=1e-9, npml=16, nfreq=32):
     pul = pulse_for_band(400e-9, 900e-9, 600e-9)
     om = 2.0 * PI * C0 / np.linspace(400e-9, 900e-9, nfreq)
     return {
-        "name": f"sweep2d_{n}",
-        "grid": {"nx": n, "ny": n, "ds": ds, "courant": 0.5, "periodic_x": True},
-        "pml": {"npml": [P, P], "order": 3.0, "R0": math.exp(-16.0), "eps_low": 1.0, "eps_high": 1.0},
+        "name": f"sweep2d_{'tmz' if tm else 'tez'}_{n}",
+        "grid": {"nx": n, "ny": n, "ds": ds, "courant": 0.5, "periodic_x": True,
+                 "polarization": "tmz" if tm else "tez"},
+        "pml": {"npml": [0 if tm else P, P], "order": 3.0, "R0": math.exp(-16.0), "eps_low": 1.0, "eps_high": 1.0},
         "materials": [{"type": "dielectric", "region": box2((1, P), (n, n - P)), "eps": 1.0},
                       {"type": "drude2cp", "region": disk((c, c), r)},
                       {"type": "graphene", "region": box2((1, g_row), (n, g_row)), "params": gr}],
@@ -311,10 +316,10 @@ def tmz_case(periodic_x=False, nx=40, ny=48, nsteps=300, graphene="cp4", precisi
     }
 
 
-def sweep_stream(seed=0, count=256, side=None, nsteps=None):
-    """`count` sweep instances from one seed."""
+def sweep_stream(seed=0, count=256, side=None, nsteps=None, polarization="drawn"):
+    """`count` sweep instances from one seed (TEz and TMz drawn by default)."""
     rng = np.random.default_rng(seed)
-    return [sweep_instance(rng, side=side, nsteps=nsteps) for _ in range(count)]
+    return [sweep_instance(rng, side=side, nsteps=nsteps, polarization=polarization) for _ in range(count)]
 
 
 def ref2d_frequencies(lam_init, lam_end, nfreq):

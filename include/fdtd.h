@@ -397,6 +397,9 @@ int fdtd_pool_progress(fdtd_pool* pool, int* n_running, int* n_waiting);
 int fdtd_pool_status(fdtd_pool* pool, int ticket, int* state, double* wait_ms, double* run_ms);
 /* Waiting: never runs.  Running: stops after its in-flight chunk. */
 int fdtd_pool_cancel(fdtd_pool* pool, int ticket);
+/* Frees the job's budget share and its chunk events; afterwards its handle
+ * may be destroyed.  Destroy handles only after releasing their jobs (or
+ * after fdtd_pool_destroy).                                                */
 int fdtd_pool_release(fdtd_pool* pool, int ticket);
 int fdtd_pool_usage(fdtd_pool* pool, size_t* used, size_t* budget);
 

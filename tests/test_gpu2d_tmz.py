@@ -103,3 +103,23 @@ def test_tmz_argument_errors():
     with pytest.raises(abi.FdtdError):
         s.commit()  # dispersive cells inside the PML
     s.close()
+
+
+def test_tmz_sweep_instance_fp64_matches_the_oracle():
+    """A TMz member of the batched sweep (gold disk, Kubo graphene line,
+    periodic x, y PML only)."""
+    import oracle
+    from fdtd_amd import sim2d
+
+    spec = sim2d.sweep_instance(np.random.default_rng(4), side=128, nsteps=1500, polarization="tmz")
+    gs, gi, gp = sim2d.make_simulation2d(spec, precision="fp64")
+    os_, oi, op = oracle.make_simulation2d(spec)
+    gs.step(spec["nsteps"])
+    os_.step(spec["nsteps"])
+    for a, b in zip(gi, oi):
+        assert _max_rel(gs.flux(a), os_.flux(b)) <= 1e-12
+    for comp in (sim2d.EZ2, sim2d.HX2, sim2d.HY2):
+        assert relerr(gs.field(comp), os_.field(comp)) <= 1e-11
+    assert _max_rel(gs.probe(gp[0])[:, 1:], os_.probe(op[0])[:, 1:]) <= 1e-11
+    gs.close()
+    os_.close()
