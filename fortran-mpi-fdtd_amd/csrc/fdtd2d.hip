@@ -57,7 +57,9 @@ constexpr int kMaxLine = 4;     // line sources per simulation
 constexpr int kMaxDft = 4;     // DFT lines per simulation
 constexpr int kMaxProbe = 8;   // probes per simulation
 constexpr int kMaxGroup = 4;   // dispersive coefficient groups per simulation
-constexpr int kTX = 64, kTY = 4;  // tile: one wavefront per row, 4 rows
+// tile: 64 x 16 cells, four wavefronts of 64 lanes, each marching through
+// four consecutive rows (the row below / above stays in registers)
+constexpr int kTX = 64, kTY = 16, kWaves = 4, kRowsPerWave = kTY / kWaves;
 constexpr int kChunk = 256;    // cells per work item of the list kernels
 constexpr int kBlockSteps = 32;  // steps per captured graph (one DFT block)
 
@@ -246,7 +248,7 @@ __device__ __forceinline__ bool in_pml_tmz(const Inst<real>& S, int i, int j) {
 // TMz E half step of cell (i, j); loads first (the pointers may alias as far
 // as the compiler knows)
 template <typename real>
-__device__ __forceinline__ void e_tmz(const Inst<real>& S, int i, int j) {
+__device__ __forceinline__ void e_tmz(const Inst<real>& S, int i, int j, real& carry, bool use_carry) {
   const int px = S.px;
   const int c = i + px * j;
   bool src = false;
@@ -257,7 +259,8 @@ __device__ __forceinline__ void e_tmz(const Inst<real>& S, int i, int j) {
       sv = S.src_val[s];
     }
   const bool pml = in_pml_tmz(S, i, j);
-  const real hxc = S.hx[c], hxj = S.hx[c - px], hyc = S.hy[c], hyi = S.hy[c - 1];
+  const real hxc = S.hx[c], hxj = use_carry ? carry : S.hx[c - px], hyc = S.hy[c], hyi = S.hy[c - 1];
+  carry = hxc;  // Hx(j) is the next row's Hx(j-1)
   const Mat<real> m = S.tab[S.matid[c]];
   real ez = 0;
   bool wrote = false;
@@ -292,11 +295,12 @@ __device__ __forceinline__ void e_tmz(const Inst<real>& S, int i, int j) {
 
 // TMz H half step of cell (i, j)
 template <typename real>
-__device__ __forceinline__ void h_tmz(const Inst<real>& S, int i, int j) {
+__device__ __forceinline__ void h_tmz(const Inst<real>& S, int i, int j, real& carry, bool use_carry) {
   const int px = S.px;
   const int c = i + px * j;
   const bool pml = in_pml_tmz(S, i, j);
-  const real ezc = S.ez[c], ezi = S.ez[c + 1], ezj = S.ez[c + px];
+  const real ezc = use_carry ? carry : S.ez[c], ezi = S.ez[c + 1], ezj = S.ez[c + px];
+  carry = ezj;  // Ez(j+1) is the next row's Ez(j)
   const real hxc = S.hx[c], hyc = S.hy[c];
   real hx, hy;
   if (!pml) {
@@ -314,134 +318,160 @@ __device__ __forceinline__ void h_tmz(const Inst<real>& S, int i, int j) {
   if (S.per && i == S.nx) S.hy[c - S.nx] = hy;
 }
 
+// TEz E half step of cell (i, j); carry: Hz of the row below when use_carry
 template <typename real>
-__global__ __launch_bounds__(256) void k2_e(const Inst<real>* __restrict__ I, const int32_t* __restrict__ tile_inst,
-                                            int ntiles) {
+__device__ __forceinline__ void e_tez(const Inst<real>& S, int i, int j, real& carry, bool use_carry) {
+  const int px = S.px;
+  const int c = i + px * j;
+  const bool main = j >= S.j0 && j <= S.j1;
+  bool src = false;
+  real sv = 0;
+  for (int s = 0; s < S.nsrc; ++s)
+    if (j == S.src_j[s]) {
+      src = true;
+      sv = S.src_val[s];
+    }
+  const bool slab = S.has_pml && (j <= S.ja || j >= S.jb);
+  // loads first: the field pointers may alias as far as the compiler knows
+  const real hzc = S.hz[c];
+  real dxv = S.dx[c], dyv = S.dy[c];
+  const real hzi = main ? S.hz[c - 1] : real(0);
+  const real hzj = main ? (use_carry ? carry : S.hz[c - px]) : real(0);
+  carry = hzc;
+  const Mat<real> m = S.tab[S.matid[c]];
+  if (main) {
+    dxv = dxv + S.ku * (hzc - hzj);  // dx + Ku*curlhy
+    dyv = dyv - S.ku * (hzc - hzi);  // dy - Ku*curlhx
+  }
+  if (src) dxv = dxv + sv;
+  if (main) S.dy[c] = dyv;
+  if (main || src) S.dx[c] = dxv;
+  // dielectric writers (graphene's Ey = Dy is one with gax 1)
+  if (m.act[1] == ACT_DIEL) {
+    const real ey = dyv * m.gax[1];
+    S.ey[c] = ey;
+    if (S.per && i == 1) S.ey[c + S.nx] = ey;
+    if (S.per && i == S.nx) S.ey[c - S.nx] = ey;
+  }
+  if (slab) {
+    const int s = j <= S.ja ? 0 : 1;
+    const int r = j - S.prow0[s];
+    const int sc = i + px * r;
+    real* pex = S.pml[s][0];
+    real* pey = S.pml[s][1];
+    const real* hzx = S.pml[s][2];
+    const real* hzy = S.pml[s][3];
+    const real* py = S.prof[s];
+    const real* pxp = py + 4 * (S.P + 2);
+    const real curlHzx_p = -hzx[sc] - hzy[sc];
+    const real curlHzx_m = hzx[sc - 1] + hzy[sc - 1];
+    real curlHzy_p = hzx[sc] + hzy[sc];
+    real curlHzy_m = -hzx[sc - px] - hzy[sc - px];
+    if (j == S.ja) curlHzy_p = hzc;
+    if (j == S.jb) curlHzy_m = -S.hz[c - px];
+    const real ex = py[r] * pex[sc] + py[(S.P + 2) + r] * (curlHzy_p + curlHzy_m);
+    const real ey = pxp[i] * pey[sc] + pxp[(S.nx + 2) + i] * (curlHzx_p + curlHzx_m);
+    pex[sc] = ex;
+    pey[sc] = ey;
+    S.ex[c] = ex;  // setbounday{Eya,Eyb}: Ex of every slab row
+  } else if (m.act[0] == ACT_DIEL) {
+    S.ex[c] = dxv * m.gax[0];
+  }
+}
+
+// TEz H half step of cell (i, j); carry: Ex of the row above when use_carry
+template <typename real>
+__device__ __forceinline__ void h_tez(const Inst<real>& S, int i, int j, real& carry, bool use_carry) {
+  const int px = S.px;
+  const int c = i + px * j;
+  const bool main = j >= S.j0 && j <= S.j1;
+  const bool slab = S.has_pml && (j <= S.ja || j >= S.jb);
+  real hz = 0;
+  bool wrote = false;
+  const real exj = S.ex[c + px];
+  if (main) {
+    const real exc = use_carry ? carry : S.ex[c];
+    const real curlE = S.ey[c + 1] - S.ey[c] - exj + exc;
+    hz = S.hz[c] - S.ku * curlE;
+    wrote = true;
+  }
+  carry = exj;  // Ex(j+1) is the next row's Ex(j)
+  if (slab) {
+    const int s = j <= S.ja ? 0 : 1;
+    const int r = j - S.prow0[s];
+    const int sc = i + px * r;
+    const real* pex = S.pml[s][0];
+    const real* pey = S.pml[s][1];
+    real* hzx = S.pml[s][2];
+    real* hzy = S.pml[s][3];
+    const real* py = S.prof[s];
+    const real* pxp = py + 4 * (S.P + 2);
+    const real curlExy = pex[sc + px] - pex[sc];
+    const real curlEyx = -pey[sc + 1] + pey[sc];
+    const real ny_ = py[2 * (S.P + 2) + r] * hzy[sc] + py[3 * (S.P + 2) + r] * curlExy;
+    const real nx_ = pxp[2 * (S.nx + 2) + i] * hzx[sc] + pxp[3 * (S.nx + 2) + i] * curlEyx;
+    hzy[sc] = ny_;
+    hzx[sc] = nx_;
+    if (j <= S.ja - 1 || j >= S.jb) {  // setbounday{Hya,Hyb}
+      hz = ny_ + nx_;
+      wrote = true;
+    }
+  }
+  if (wrote) {
+    S.hz[c] = hz;
+    if (S.per && i == 1) S.hz[c + S.nx] = hz;
+    if (S.per && i == S.nx) S.hz[c - S.nx] = hz;
+  }
+}
+
+// E and H over the packed tile list: block = one tile at a time, wavefront
+// w marches rows 4w..4w+3 of the tile (j uniform per wavefront).
+template <typename real>
+__global__ __launch_bounds__(kTX * kWaves) void k2_e(const Inst<real>* __restrict__ I,
+                                                     const int32_t* __restrict__ tile_inst, int ntiles) {
   const int lane = threadIdx.x & 63;
-  const int row = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const int q = tile_inst[t];
     const Inst<real>& S = I[q];
     if (S.ctl[0] >= S.ctl[1]) continue;  // finished: nothing to do this step
     const int lt = t - S.tile0;
     const int i = (lt % S.ntx) * kTX + lane + 1;
-    const int j = (lt / S.ntx) * kTY + row + 1;
-    if (i > S.nx || j > S.ny) continue;
-    if (S.tmz) {
-      e_tmz(S, i, j);
-      continue;
-    }
-    const int px = S.px;
-    const int c = i + px * j;
-    const bool main = j >= S.j0 && j <= S.j1;
-    bool src = false;
-    real sv = 0;
-    for (int s = 0; s < S.nsrc; ++s)
-      if (j == S.src_j[s]) {
-        src = true;
-        sv = S.src_val[s];
-      }
-    const bool slab = S.has_pml && (j <= S.ja || j >= S.jb);
-    // loads first: the field pointers may alias as far as the compiler knows
-    const real hzc = S.hz[c];
-    real dxv = S.dx[c], dyv = S.dy[c];
-    const real hzi = main ? S.hz[c - 1] : real(0);
-    const real hzj = main ? S.hz[c - px] : real(0);
-    const Mat<real> m = S.tab[S.matid[c]];
-    if (main) {
-      dxv = dxv + S.ku * (hzc - hzj);  // dx + Ku*curlhy
-      dyv = dyv - S.ku * (hzc - hzi);  // dy - Ku*curlhx
-    }
-    if (src) dxv = dxv + sv;
-    if (main) S.dy[c] = dyv;
-    if (main || src) S.dx[c] = dxv;
-    // dielectric writers (graphene's Ey = Dy is one with gax 1)
-    if (m.act[1] == ACT_DIEL) {
-      const real ey = dyv * m.gax[1];
-      S.ey[c] = ey;
-      if (S.per && i == 1) S.ey[c + S.nx] = ey;
-      if (S.per && i == S.nx) S.ey[c - S.nx] = ey;
-    }
-    if (slab) {
-      const int s = j <= S.ja ? 0 : 1;
-      const int r = j - S.prow0[s];
-      const int sc = i + px * r;
-      real* pex = S.pml[s][0];
-      real* pey = S.pml[s][1];
-      const real* hzx = S.pml[s][2];
-      const real* hzy = S.pml[s][3];
-      const real* py = S.prof[s];
-      const real* pxp = py + 4 * (S.P + 2);
-      const real curlHzx_p = -hzx[sc] - hzy[sc];
-      const real curlHzx_m = hzx[sc - 1] + hzy[sc - 1];
-      real curlHzy_p = hzx[sc] + hzy[sc];
-      real curlHzy_m = -hzx[sc - px] - hzy[sc - px];
-      if (j == S.ja) curlHzy_p = hzc;
-      if (j == S.jb) curlHzy_m = -S.hz[c - px];
-      const real ex = py[r] * pex[sc] + py[(S.P + 2) + r] * (curlHzy_p + curlHzy_m);
-      const real ey = pxp[i] * pey[sc] + pxp[(S.nx + 2) + i] * (curlHzx_p + curlHzx_m);
-      pex[sc] = ex;
-      pey[sc] = ey;
-      S.ex[c] = ex;  // setbounday{Eya,Eyb}: Ex of every slab row
-    } else if (m.act[0] == ACT_DIEL) {
-      S.ex[c] = dxv * m.gax[0];
+    const int j0 = (lt / S.ntx) * kTY + wave * kRowsPerWave + 1;
+    if (i > S.nx) continue;
+    real carry = 0;
+    for (int r = 0; r < kRowsPerWave; ++r) {
+      const int j = j0 + r;
+      if (j > S.ny) break;
+      if (S.tmz)
+        e_tmz(S, i, j, carry, r > 0);
+      else
+        e_tez(S, i, j, carry, r > 0);
     }
   }
 }
 
 template <typename real>
-__global__ __launch_bounds__(256) void k2_h(const Inst<real>* __restrict__ I, const int32_t* __restrict__ tile_inst,
-                                            int ntiles) {
+__global__ __launch_bounds__(kTX * kWaves) void k2_h(const Inst<real>* __restrict__ I,
+                                                     const int32_t* __restrict__ tile_inst, int ntiles) {
   const int lane = threadIdx.x & 63;
-  const int row = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const int q = tile_inst[t];
     const Inst<real>& S = I[q];
     if (S.ctl[0] >= S.ctl[1]) continue;
     const int lt = t - S.tile0;
     const int i = (lt % S.ntx) * kTX + lane + 1;
-    const int j = (lt / S.ntx) * kTY + row + 1;
-    if (i > S.nx || j > S.ny) continue;
-    if (S.tmz) {
-      h_tmz(S, i, j);
-      continue;
-    }
-    const int px = S.px;
-    const int c = i + px * j;
-    const bool main = j >= S.j0 && j <= S.j1;
-    const bool slab = S.has_pml && (j <= S.ja || j >= S.jb);
-    real hz = 0;
-    bool wrote = false;
-    if (main) {
-      const real curlE = S.ey[c + 1] - S.ey[c] - S.ex[c + px] + S.ex[c];
-      hz = S.hz[c] - S.ku * curlE;
-      wrote = true;
-    }
-    if (slab) {
-      const int s = j <= S.ja ? 0 : 1;
-      const int r = j - S.prow0[s];
-      const int sc = i + px * r;
-      const real* pex = S.pml[s][0];
-      const real* pey = S.pml[s][1];
-      real* hzx = S.pml[s][2];
-      real* hzy = S.pml[s][3];
-      const real* py = S.prof[s];
-      const real* pxp = py + 4 * (S.P + 2);
-      const real curlExy = pex[sc + px] - pex[sc];
-      const real curlEyx = -pey[sc + 1] + pey[sc];
-      const real ny_ = py[2 * (S.P + 2) + r] * hzy[sc] + py[3 * (S.P + 2) + r] * curlExy;
-      const real nx_ = pxp[2 * (S.nx + 2) + i] * hzx[sc] + pxp[3 * (S.nx + 2) + i] * curlEyx;
-      hzy[sc] = ny_;
-      hzx[sc] = nx_;
-      if (j <= S.ja - 1 || j >= S.jb) {  // setbounday{Hya,Hyb}
-        hz = ny_ + nx_;
-        wrote = true;
-      }
-    }
-    if (wrote) {
-      S.hz[c] = hz;
-      if (S.per && i == 1) S.hz[c + S.nx] = hz;
-      if (S.per && i == S.nx) S.hz[c - S.nx] = hz;
+    const int j0 = (lt / S.ntx) * kTY + wave * kRowsPerWave + 1;
+    if (i > S.nx) continue;
+    real carry = 0;
+    for (int r = 0; r < kRowsPerWave; ++r) {
+      const int j = j0 + r;
+      if (j > S.ny) break;
+      if (S.tmz)
+        h_tmz(S, i, j, carry, r > 0);
+      else
+        h_tez(S, i, j, carry, r > 0);
     }
   }
 }
@@ -1198,7 +1228,7 @@ void enqueue_block(Batch* b) {
   hipStream_t st = b->stream;
   const Inst<real>* I = (const Inst<real>*)b->d_inst;
   // blocks of the E and H launches; each strides over the packed tile list
-  int tpb = 8;
+  int tpb = 2;
   if (const char* e = std::getenv("FDTD2D_TILES_PER_BLOCK")) {
     const int v = std::atoi(e);
     if (v >= 1 && v <= 4096) tpb = v;
@@ -1206,9 +1236,9 @@ void enqueue_block(Batch* b) {
   const unsigned grid = (unsigned)std::min(b->ntiles, std::max(8192, (b->ntiles + tpb - 1) / tpb));
   const int count = (int)b->sims.size();
   for (int s = 0; s < kBlockSteps; ++s) {
-    hipLaunchKernelGGL(k2_e<real>, dim3(grid), dim3(kTX * kTY), 0, st, I, b->d_tile_inst, b->ntiles);
+    hipLaunchKernelGGL(k2_e<real>, dim3(grid), dim3(kTX * kWaves), 0, st, I, b->d_tile_inst, b->ntiles);
     if (b->nade) hipLaunchKernelGGL(k2_ade<real>, dim3((unsigned)b->nade), dim3(kChunk), 0, st, I, b->d_ade, b->nade);
-    hipLaunchKernelGGL(k2_h<real>, dim3(grid), dim3(kTX * kTY), 0, st, I, b->d_tile_inst, b->ntiles);
+    hipLaunchKernelGGL(k2_h<real>, dim3(grid), dim3(kTX * kWaves), 0, st, I, b->d_tile_inst, b->ntiles);
     if (b->nmon) hipLaunchKernelGGL(k2_mon<real>, dim3((unsigned)b->nmon), dim3(kChunk), 0, st, I, b->d_mon, b->nmon, s);
     hipLaunchKernelGGL(k2_adv<real>, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, st, I, count, 1);
   }
