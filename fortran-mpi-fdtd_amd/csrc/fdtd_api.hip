@@ -545,6 +545,10 @@ int commit_impl(Sim* s) {
       const int v = std::atoi(e);
       if (v >= 2 && v <= 4096) s->op_kc = v;
     }
+    if (const char* e = std::getenv("FDTD_ZMARCH")) {
+      const int v = std::atoi(e);
+      if (v == 1 || v == 2 || v == 4 || v == 8) s->zm = v;
+    }
   }
   for (int q = 0; q < 3; ++q) {
     char* e = (char*)dev_alloc(s, fbytes);
@@ -1315,7 +1319,7 @@ int fdtd_autotune(fdtd_sim* s, int steps, int* out_by, int* out_kc) {
     added = true;
   }
   if (s->engine == FDTD_ENGINE_FUSED && s->E2[0]) {
-    cand.push_back({0, 0});
+    for (int zm : {1, 2, 4, 8}) cand.push_back({0, zm});  // two-kernel step, zm planes per tile
     for (int by : {4, 8, 16})
       for (int kc : {2, 4, 16, 32, 64, 128}) {
         if (kc > s->nz && kc > 16) continue;
@@ -1340,6 +1344,8 @@ int fdtd_autotune(fdtd_sim* s, int steps, int* out_by, int* out_kc) {
     if (c.first > 0) {
       s->op_by = c.first;
       s->op_kc = c.second;
+    } else {
+      s->zm = c.second;
     }
     for (int w = 0; w < 2 && rc == 0; ++w) rc = launch_step(s);
     if (rc == 0 && hipEventRecord(e0, s->stream) != hipSuccess) rc = set_error(FDTD_ERR_HIP, "event");
@@ -1363,8 +1369,9 @@ int fdtd_autotune(fdtd_sim* s, int steps, int* out_by, int* out_kc) {
   if (pick.first > 0) {
     s->op_by = pick.first;
     s->op_kc = pick.second;
-  } else if (added) {
-    free_onepass_copies(s);
+  } else {
+    s->zm = pick.second;
+    if (added) free_onepass_copies(s);
   }
   if (out_by) *out_by = pick.first;
   if (out_kc) *out_kc = pick.second;

@@ -212,6 +212,9 @@ struct Sim {
   // deep.
   int onepass = 0;
   int op_by = 16, op_kc = 64;
+  // two-kernel step: E and H tiles span zm planes (1, 2, 4 or 8), each
+  // thread marching its column and carrying the neighbour plane in registers
+  int zm = 1;
   // dry steps (autotuner): no sources, TFSF or monitors, no step count; run
   // only while every field is still zero, so they leave the state as it was
   bool dry = false;

@@ -129,14 +129,22 @@ __device__ __forceinline__ int pml_index(const int32_t* P, int i, int j, int k, 
 // id `rm`, whose table entry is read with scalar loads; otherwise the entry
 // is gathered per cell.  The two forms are separate code paths so that no
 // merge of scalar and vector values makes the compiler wait for the loads.
+//
+// z march: with use_carry, Hy and Hx of plane k-1 come from cy, cx (the
+// previous plane of the same thread) instead of memory; the cell leaves its
+// own Hy, Hx there for plane k+1.
 template <typename real, bool UNI>
 __device__ __forceinline__ void fused_e_cell(const FieldArgs<real>& a, const PmlArgs<real>& p, const real* sv,
-                                             const int* sa, int rm, int i, int j, int k) {
+                                             const int* sa, int rm, int i, int j, int k, real& cy, real& cx,
+                                             bool use_carry, const real* H0, const real* H1, const real* H2,
+                                             real* E0, real* E1, real* E2) {
   const int c = i + a.px * j + a.pxy * k;
-  const real hx = a.h[0][c], hy = a.h[1][c], hz = a.h[2][c];
-  const real hz_jm = a.h[2][c - a.px], hx_jm = a.h[0][c - a.px];
-  const real hy_km = a.h[1][c - a.pxy], hx_km = a.h[0][c - a.pxy];
-  const real hz_im = a.h[2][c - 1], hy_im = a.h[1][c - 1];
+  const real hx = H0[c], hy = H1[c], hz = H2[c];
+  const real hz_jm = H2[c - a.px], hx_jm = H0[c - a.px];
+  const real hy_km = use_carry ? cy : H1[c - a.pxy], hx_km = use_carry ? cx : H0[c - a.pxy];
+  cy = hy;
+  cx = hx;
+  const real hz_im = H2[c - 1], hy_im = H1[c - 1];
   const unsigned id = UNI ? (unsigned)__builtin_amdgcn_readfirstlane(rm) : (unsigned)a.matid[c];
   const uint32_t fl = a.flags[id];
   const Vec4<real> cf = a.coef[id];
@@ -160,19 +168,19 @@ __device__ __forceinline__ void fused_e_cell(const FieldArgs<real>& a, const Pml
     p.s[S_EYX][s] = eyx; p.s[S_EZX][s] = ezx; p.s[S_EXY][s] = exy;
     p.s[S_EZY][s] = ezy; p.s[S_EXZ][s] = exz; p.s[S_EYZ][s] = eyz;
     const real ex = exy + exz, ey = eyz + eyx, ez = ezx + ezy;
-    a.e[0][c] = ex;
-    a.e[1][c] = ey;
-    a.e[2][c] = ez;
+    E0[c] = ex;
+    E1[c] = ey;
+    E2[c] = ez;
     // a periodic axis without PML of its own still meets the slabs of the
     // other axes at its boundary
-    put_e_ghost(a.e[0], ex, i, j, k, c, a);
-    put_e_ghost(a.e[1], ey, i, j, k, c, a);
-    put_e_ghost(a.e[2], ez, i, j, k, c, a);
+    put_e_ghost(E0, ex, i, j, k, c, a);
+    put_e_ghost(E1, ey, i, j, k, c, a);
+    put_e_ghost(E2, ez, i, j, k, c, a);
     return;
   }
   // E_q += Ku*gax_q*curl_q (+ gax_q * source) for every component a
   // dielectric writes
-  const real e0[3] = {a.e[0][c], a.e[1][c], a.e[2][c]};
+  const real e0[3] = {E0[c], E1[c], E2[c]};
   const bool any_src = sa[0] >= 0 || sa[1] >= 0 || sa[2] >= 0;
   Vec4<real> gx;
   if (any_src) gx = a.gaxe[id];
@@ -184,8 +192,9 @@ __device__ __forceinline__ void fused_e_cell(const FieldArgs<real>& a, const Pml
     if (!(fl & (1u << q))) continue;
     real ev = e0[q] + cf.v[q] * cur[q];
     if (sa[q] >= 0) ev = ev + gx.v[q] * sv[q];
-    a.e[q][c] = ev;
-    put_e_ghost(a.e[q], ev, i, j, k, c, a);
+    real* Eq = q == 0 ? E0 : (q == 1 ? E1 : E2);
+    Eq[c] = ev;
+    put_e_ghost(Eq, ev, i, j, k, c, a);
   }
 }
 
@@ -209,9 +218,39 @@ __device__ __forceinline__ void plane_sources(const SrcArgs<real>& src, int nsrc
   }
 }
 
+// The z march of one (i,j) column over planes tz*KZ+1 .. tz*KZ+KZ.  The
+// field arrays come in as restrict pointers: no H array is written here and
+// no E array is read through another pointer, so the compiler may issue the
+// loads of the next plane before the stores of this one.
+template <typename real, int KZ>
+__device__ __forceinline__ void e_column(const FieldArgs<real>& a, const PmlArgs<real>& p, const SrcArgs<real>& src,
+                                         int nsrc, int i, int j, int tz, const real* __restrict__ H0,
+                                         const real* __restrict__ H1, const real* __restrict__ H2,
+                                         real* __restrict__ E0, real* __restrict__ E1, real* __restrict__ E2) {
+  real cy = 0, cx = 0;
+  bool have = false;
+#pragma unroll
+  for (int r = 0; r < KZ; ++r) {
+    const int k = tz * KZ + r + 1;
+    if (k > a.nz) break;
+    const int rm = a.rowmat[__builtin_amdgcn_readfirstlane(j + (a.ny + 2) * k)];
+    if (rm < 0) {  // a mixed row: k_fused_e_mixed updates it
+      have = false;
+      continue;
+    }
+    real sv[3] = {0, 0, 0};
+    int sa[3] = {-1, -1, -1};
+    plane_sources(src, nsrc, i, j, k, sv, sa);
+    fused_e_cell<real, true>(a, p, sv, sa, rm, i, j, k, cy, cx, have, H0, H1, H2, E0, E1, E2);
+    have = true;
+  }
+}
+
 // Rows with one material id (all but the rows that cross a particle
-// surface): the table entry is a wave-uniform scalar load.
-template <typename real>
+// surface): the table entry is a wave-uniform scalar load.  A tile spans KZ
+// planes: each thread marches its (i,j) column through them, carrying H of
+// the plane below in registers (KZ = 1: one plane, nothing carried).
+template <typename real, int KZ>
 __global__ __launch_bounds__(512) void k_fused_e(FieldArgs<real> a, PmlArgs<real> p, SrcArgs<real> src,
                                                  int nsrc) {
   int tx, ty, tz;
@@ -220,14 +259,8 @@ __global__ __launch_bounds__(512) void k_fused_e(FieldArgs<real> a, PmlArgs<real
   // blockDim.x is a multiple of 64 (set at commit): a wavefront lies in one
   // (j,k) row, so j and k are wave-uniform and row data are scalar loads
   const int j = ty * blockDim.y + __builtin_amdgcn_readfirstlane((int)threadIdx.y) + 1;
-  const int k = tz + 1;
   if (i > a.nx || j > a.ny) return;
-  const int rm = a.rowmat[__builtin_amdgcn_readfirstlane(j + (a.ny + 2) * k)];
-  if (rm < 0) return;  // a mixed row: k_fused_e_mixed updates it
-  real sv[3] = {0, 0, 0};
-  int sa[3] = {-1, -1, -1};
-  plane_sources(src, nsrc, i, j, k, sv, sa);
-  fused_e_cell<real, true>(a, p, sv, sa, rm, i, j, k);
+  e_column<real, KZ>(a, p, src, nsrc, i, j, tz, a.h[0], a.h[1], a.h[2], a.e[0], a.e[1], a.e[2]);
 }
 
 // Rows that mix material ids: block b updates x tile b % ntx of row
@@ -242,23 +275,26 @@ __global__ __launch_bounds__(256) void k_fused_e_mixed(FieldArgs<real> a, PmlArg
   real sv[3] = {0, 0, 0};
   int sa[3] = {-1, -1, -1};
   plane_sources(src, nsrc, i, j, k, sv, sa);
-  fused_e_cell<real, false>(a, p, sv, sa, -1, i, j, k);
+  real cy = 0, cx = 0;
+  fused_e_cell<real, false>(a, p, sv, sa, -1, i, j, k, cy, cx, false, a.h[0], a.h[1], a.h[2], a.e[0], a.e[1],
+                            a.e[2]);
 }
 
-// FUSED engine: H half step (myfdtd_func.f90:45-64 + split-field H).
+// FUSED engine: H half step (myfdtd_func.f90:45-64 + split-field H) of one
+// cell.  z march: with use_carry, Ex and Ey of plane k come from cex, cey
+// (loaded as plane k+1 by the previous plane of the same thread); the cell
+// leaves Ex, Ey of plane k+1 there.
 template <typename real>
-__global__ __launch_bounds__(512) void k_fused_h(FieldArgs<real> a, PmlArgs<real> p) {
-  int tx, ty, tz;
-  if (!tile_of(a, tx, ty, tz)) return;
-  const int i = tx * blockDim.x + threadIdx.x + 1;
-  const int j = ty * blockDim.y + __builtin_amdgcn_readfirstlane((int)threadIdx.y) + 1;  // wave-uniform row
-  const int k = tz + 1;
-  if (i > a.nx || j > a.ny) return;
+__device__ __forceinline__ void fused_h_cell(const FieldArgs<real>& a, const PmlArgs<real>& p, int i, int j, int k,
+                                             real& cex, real& cey, bool use_carry, const real* E0,
+                                             const real* E1, const real* E2, real* H0, real* H1, real* H2) {
   const int c = i + a.px * j + a.pxy * k;
-  const real ex = a.e[0][c], ey = a.e[1][c], ez = a.e[2][c];
-  const real ey_kp = a.e[1][c + a.pxy], ex_kp = a.e[0][c + a.pxy];
-  const real ez_jp = a.e[2][c + a.px], ex_jp = a.e[0][c + a.px];
-  const real ez_ip = a.e[2][c + 1], ey_ip = a.e[1][c + 1];
+  const real ex = use_carry ? cex : E0[c], ey = use_carry ? cey : E1[c], ez = E2[c];
+  const real ey_kp = E1[c + a.pxy], ex_kp = E0[c + a.pxy];
+  const real ez_jp = E2[c + a.px], ex_jp = E0[c + a.px];
+  const real ez_ip = E2[c + 1], ey_ip = E1[c + 1];
+  cex = ex_kp;
+  cey = ey_kp;
   if (in_pml(p.P, i, j, k, a.nx, a.ny, a.nz)) {
     // mod3/BERpmly_mod.f90:186-191 with the splits driven by total E
     const int s = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
@@ -271,26 +307,52 @@ __global__ __launch_bounds__(512) void k_fused_h(FieldArgs<real> a, PmlArgs<real
     p.s[S_HYX][s] = hyx; p.s[S_HZX][s] = hzx; p.s[S_HXY][s] = hxy;
     p.s[S_HZY][s] = hzy; p.s[S_HXZ][s] = hxz; p.s[S_HYZ][s] = hyz;
     const real hx = hxy + hxz, hy = hyz + hyx, hz = hzx + hzy;
-    a.h[0][c] = hx;
-    a.h[1][c] = hy;
-    a.h[2][c] = hz;
-    put_h_ghost(a.h[0], hx, i, j, k, c, a);
-    put_h_ghost(a.h[1], hy, i, j, k, c, a);
-    put_h_ghost(a.h[2], hz, i, j, k, c, a);
+    H0[c] = hx;
+    H1[c] = hy;
+    H2[c] = hz;
+    put_h_ghost(H0, hx, i, j, k, c, a);
+    put_h_ghost(H1, hy, i, j, k, c, a);
+    put_h_ghost(H2, hz, i, j, k, c, a);
     return;
   }
   const real curleyz = ey_kp - ey, curlezy = -ez_jp + ez;
   const real curlezx = ez_ip - ez, curlexz = -ex_kp + ex;
   const real curlexy = ex_jp - ex, curleyx = -ey_ip + ey;
-  const real hx = a.h[0][c] + a.ku * (curleyz + curlezy);
-  const real hy = a.h[1][c] + a.ku * (curlezx + curlexz);
-  const real hz = a.h[2][c] + a.ku * (curlexy + curleyx);
-  a.h[0][c] = hx;
-  a.h[1][c] = hy;
-  a.h[2][c] = hz;
-  put_h_ghost(a.h[0], hx, i, j, k, c, a);
-  put_h_ghost(a.h[1], hy, i, j, k, c, a);
-  put_h_ghost(a.h[2], hz, i, j, k, c, a);
+  const real hx = H0[c] + a.ku * (curleyz + curlezy);
+  const real hy = H1[c] + a.ku * (curlezx + curlexz);
+  const real hz = H2[c] + a.ku * (curlexy + curleyx);
+  H0[c] = hx;
+  H1[c] = hy;
+  H2[c] = hz;
+  put_h_ghost(H0, hx, i, j, k, c, a);
+  put_h_ghost(H1, hy, i, j, k, c, a);
+  put_h_ghost(H2, hz, i, j, k, c, a);
+}
+
+// H over tiles of KZ planes, each thread marching its (i,j) column; restrict
+// pointers as in e_column (no E array is written here).
+template <typename real, int KZ>
+__device__ __forceinline__ void h_column(const FieldArgs<real>& a, const PmlArgs<real>& p, int i, int j, int tz,
+                                         const real* __restrict__ E0, const real* __restrict__ E1,
+                                         const real* __restrict__ E2, real* __restrict__ H0, real* __restrict__ H1,
+                                         real* __restrict__ H2) {
+  real cex = 0, cey = 0;
+#pragma unroll
+  for (int r = 0; r < KZ; ++r) {
+    const int k = tz * KZ + r + 1;
+    if (k > a.nz) break;
+    fused_h_cell(a, p, i, j, k, cex, cey, r > 0, E0, E1, E2, H0, H1, H2);
+  }
+}
+
+template <typename real, int KZ>
+__global__ __launch_bounds__(512) void k_fused_h(FieldArgs<real> a, PmlArgs<real> p) {
+  int tx, ty, tz;
+  if (!tile_of(a, tx, ty, tz)) return;
+  const int i = tx * blockDim.x + threadIdx.x + 1;
+  const int j = ty * blockDim.y + __builtin_amdgcn_readfirstlane((int)threadIdx.y) + 1;  // wave-uniform row
+  if (i > a.nx || j > a.ny) return;
+  h_column<real, KZ>(a, p, i, j, tz, a.e[0], a.e[1], a.e[2], a.h[0], a.h[1], a.h[2]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1685,9 +1747,21 @@ int step_impl(Sim* s) {
     tfsf_h(st, pml_args<real>(s), 1);
   } else if (s->engine == FDTD_ENGINE_FUSED) {
     PmlArgs<real> p = pml_args<real>(s);
+    // tiles of zm planes, each thread marching its column through them
+    FieldArgs<real> az = a;
+    az.ntz = (s->nz + s->zm - 1) / s->zm;
+    az.tper = (az.ntx * az.nty * az.ntz + 7) / 8;
+    const dim3 gz = tile_grid(az);
     {
       Timer tm(s, FDTD_K_E);
-      hipLaunchKernelGGL(k_fused_e<real>, tile_grid(a), blk, 0, st, a, p, src, nsrc);
+      if (s->zm == 8)
+        hipLaunchKernelGGL((k_fused_e<real, 8>), gz, blk, 0, st, az, p, src, nsrc);
+      else if (s->zm == 4)
+        hipLaunchKernelGGL((k_fused_e<real, 4>), gz, blk, 0, st, az, p, src, nsrc);
+      else if (s->zm == 2)
+        hipLaunchKernelGGL((k_fused_e<real, 2>), gz, blk, 0, st, az, p, src, nsrc);
+      else
+        hipLaunchKernelGGL((k_fused_e<real, 1>), gz, blk, 0, st, az, p, src, nsrc);
       if (s->nmix) {
         const int bm = std::min(256, (s->nx + 63) / 64 * 64);
         const int ntx = (s->nx + bm - 1) / bm;
@@ -1708,7 +1782,14 @@ int step_impl(Sim* s) {
     tfsf_e(st);
     {
       Timer tm(s, FDTD_K_H);
-      hipLaunchKernelGGL(k_fused_h<real>, tile_grid(a), blk, 0, st, a, p);
+      if (s->zm == 8)
+        hipLaunchKernelGGL((k_fused_h<real, 8>), gz, blk, 0, st, az, p);
+      else if (s->zm == 4)
+        hipLaunchKernelGGL((k_fused_h<real, 4>), gz, blk, 0, st, az, p);
+      else if (s->zm == 2)
+        hipLaunchKernelGGL((k_fused_h<real, 2>), gz, blk, 0, st, az, p);
+      else
+        hipLaunchKernelGGL((k_fused_h<real, 1>), gz, blk, 0, st, az, p);
     }
     tfsf_h(st, p, 1);
   } else {

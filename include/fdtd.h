@@ -317,12 +317,14 @@ int fdtd_commit(fdtd_sim* sim);
 
 /* Optional, no reference counterpart: pick the FUSED engine's step form by
  * timing `steps` steps of each candidate on this handle -- the two-kernel
- * step and the one-pass step with tiles of 64 x by threads marching kc
+ * step with tiles of kc = 1, 2, 4 or 8 planes (each thread marching its
+ * column) and the one-pass step with tiles of 64 x by threads marching kc
  * planes (by in {4, 8, 16}, kc in {2, 4, 16, 32, 64, 128}; kc 2 and 4 use a
  * variant that issues a chunk's loads before its first store).  The timed
  * steps run without sources or monitors on the untouched (zero) fields and
  * leave them zero; call it after set-up and before the first fdtd_step (and
- * before any fdtd_set_field).  *by = 0 reports the two-kernel step.  Other engines:
+ * before any fdtd_set_field).  *by = 0 reports the two-kernel step (*kc its
+ * planes per tile).  Other engines:
  * returns 0 with *by = *kc = 0.                                             */
 int fdtd_autotune(fdtd_sim* sim, int steps, int* by, int* kc);
 
