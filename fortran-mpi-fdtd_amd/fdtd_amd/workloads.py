@@ -260,11 +260,11 @@ def running_spectrum(trace, omega, dt, ds, e_col, h_col):
     return (-(re_h * re_e) - (im_h * im_e)) * ds * z0
 
 
-def ref_c1_twin(nsteps=2000):
+def ref_c1_twin(nsteps=2000, n=96):
     """Reference-envelope twin of the 96^3 gold-sphere case: y-PML only,
     periodic x/z, a gold cube of the sphere's volume, Dx source on a y plane,
-    a transmission DFT plane (16 frequencies, reference spacing)."""
-    n = 96
+    a transmission DFT plane (16 frequencies, reference spacing).  `n`
+    resizes the cube grid (same cell size, PML and particle)."""
     ds = 2.0 * NANO
     P = 10
     pul = pulse_for_band(400e-9, 900e-9, 600e-9)
@@ -272,7 +272,7 @@ def ref_c1_twin(nsteps=2000):
     side = int(round((4.0 / 3.0 * PI * 10 ** 3) ** (1.0 / 3.0)))  # 16 cells
     lo = (n - side) // 2 + 1
     return {
-        "name": "ref_c1_twin",
+        "name": "ref_c1_twin" if n == 96 else f"ref_c1_twin_{n}",
         "grid": {"nx": n, "ny": n, "nz": n, "ds": ds, "courant": COURANT_3D, "periodic": [1, 0, 1]},
         "pml": {"mode": "ref_y", "npml": [P, P, P], "order": 3.0, "R0": 1e-6, "eps_low": 1.0, "eps_high": 1.0},
         "materials": [

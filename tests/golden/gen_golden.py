@@ -23,61 +23,8 @@ sys.path.insert(0, os.path.join(REPO, "fortran-mpi-fdtd_amd"))
 from fdtd_amd import workloads  # noqa: E402
 
 REF_EXE = os.path.join(REPO, "oracle", "_ref", "ref3d")
-MTYPE = {"dielectric": 1, "drude2cp": 2, "graphene": 3}
-# the Lorentz-Jp modules: mtype 4 + FDTD_LJP_* (gold, si, titanium, film gold,
-# graphene, graphene3 1nm, non-dispersive graphene); only their own tables
-LJP_MTYPE = {"gold": 4, "si": 5, "titanium": 6, "film_gold": 7, "graphene": 8, "graphene3_1nm": 9,
-             "graphene_nd": 10}
-
-
-def namelist(spec):
-    g, p = spec["grid"], spec["pml"]
-    mats = spec["materials"]
-    lines = ["&grid",
-             f" nx={g['nx']}, ny={g['ny']}, nz={g['nz']}, ds={g['ds']!r}, Ku={g['courant']!r},",
-             f" nsteps={spec['nsteps']}, pbcx={'.true.' if g['periodic'][0] else '.false.'},"
-             f" pbcz={'.true.' if g['periodic'][2] else '.false.'}", "/",
-             "&pml",
-             f" npmlx={p['npml'][0]}, npmly={p['npml'][1]}, npmlz={p['npml'][2]}, order={p['order']!r},"
-             f" R0={p['R0']!r}, eps_low={p['eps_low']!r}, eps_high={p['eps_high']!r}", "/",
-             "&mats", f" nmat={len(mats)},"]
-    for q, m in enumerate(mats, start=1):
-        r = m["region"]
-        assert r["kind"] == "box"
-        b = [r["lo"][0], r["hi"][0], r["lo"][1], r["hi"][1], r["lo"][2], r["hi"][2]]
-        if m["type"] == "lorentz_jp":
-            lp = m["params"]
-            assert set(lp) <= {"preset", "wNL", "dnuNL", "xi3"}, "the modules take only their own tables"
-            mt = LJP_MTYPE[lp["preset"]]
-            if mt >= 8:
-                lines.append(f" gwNL({q})={lp['wNL']!r}, gdnuNL({q})={lp['dnuNL']!r}, gxi3({q})={lp['xi3']!r},")
-        else:
-            mt = MTYPE[m["type"]]
-        lines.append(f" mtype({q})={mt}, mbox(1:6,{q})={','.join(str(v) for v in b)},")
-        if m["type"] == "dielectric":
-            lines.append(f" meps({q})={m['eps']!r},")
-        if m["type"] == "graphene":
-            gp = m["params"]
-            assert gp["model"] == "cp4" and gp["ref_init_order"] == 1 and gp["normal_axis"] == 1
-            lines.append(f" gwNL({q})={gp['wNL']!r}, gdnuNL({q})={gp['dnuNL']!r}, gxi3({q})={gp['xi3']!r},")
-    lines.append("/")
-    s = spec["sources"][0]
-    assert s["axis"] == 1
-    lines += ["&src", f" s_on=.true., s_index={s['index']}, s_comp={s['comp'] + 1}, s_t0={s['t0']!r},"
-              f" s_st={s['st']!r}, s_w0={s['w0']!r}, s_amp={s['amplitude']!r}", "/"]
-    d = spec["dft"]
-    assert all(x["axis"] == 1 for x in d)
-    lines += ["&dft", f" ndft={len(d)}, nfreq={len(d[0]['omega'])},"
-              f" d_index(1:{len(d)})={','.join(str(x['index']) for x in d)},"
-              f" d_nb(1:{len(d)})={','.join(str(x['n_begin']) for x in d)},"
-              f" d_ne(1:{len(d)})={','.join(str(x['n_end']) for x in d)},"
-              f" lam_init={spec['lam'][0]!r}, lam_end={spec['lam'][1]!r}", "/"]
-    pr = spec["probes"]
-    lines += ["&probes", f" nprobe={len(pr)},"]
-    for q, (i, j, k) in enumerate(pr, start=1):
-        lines.append(f" p_ijk(1:3,{q})={i},{j},{k},")
-    lines.append("/")
-    return "\n".join(lines).replace("e-", "d-").replace("e+", "d+") + "\n"
+sys.path.insert(0, REPO)
+from oracle import ref3d_namelist as namelist  # noqa: E402
 
 
 def run_reference(spec):
