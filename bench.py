@@ -252,22 +252,64 @@ def batched_2d_small(args):
     return out
 
 
-def fp32_spectra_error():
-    """FP32 against FP64 spectra on the 96^3 gold sphere (all-faces PML,
-    2000 steps, 16 wavelengths): max over wavelengths of |S32 - S64| / max|S64|."""
+def c3_mixed_precision(args):
+    """The 384x384x256 graphene-sheet + gold-disk case (Kubo graphene with
+    the Raman term, 16 jittered disks, 12-cell PML, 128 wavelengths on two
+    planes) in FP32 and in FP64 for `--c3-steps` steps: throughput of each
+    and the FP32 spectra's error against the FP64 run (the FP64 GPU path
+    agrees with the FP64 oracle to 1e-11 on the test cases)."""
     import numpy as np
     import fdtd_amd
     from fdtd_amd import workloads
 
-    spec = workloads.c1_gold_sphere()
-    out = {}
-    for prec in ("fp64", "fp32"):
+    spec = workloads.c3_graphene_disks(seed=args.seed, nsteps=args.c3_steps)
+    out = {"workload": spec["name"], "steps": args.c3_steps, "nfreq": len(spec["dft"][0]["omega"])}
+    flux = {}
+    for prec in ("fp32", "fp64"):
         sim, ids, _ = fdtd_amd.make_simulation(spec, precision=prec)
-        sim.step(spec["nsteps"])
-        out[prec] = sim.flux(ids[0])
+        sim.commit()
+        sim.step(1)
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.step(args.c3_steps - 1)
+        sim.sync()
+        el = time.perf_counter() - t0
+        nx, ny, nz = sim.shape
+        out[f"{prec}_mcells_per_s"] = nx * ny * nz * (args.c3_steps - 1) / el / 1e6
+        flux[prec] = [sim.flux(i) for i in ids]
         sim.close()
-    return {"workload": "gold_sphere_96 (2000 steps, 16 wavelengths)",
-            "max_rel_err": float(np.max(np.abs(out["fp32"] - out["fp64"])) / np.max(np.abs(out["fp64"])))}
+    errs = [float(np.max(np.abs(a - b)) / np.max(np.abs(b))) for a, b in zip(flux["fp32"], flux["fp64"])]
+    out["fp32_over_fp64"] = out["fp32_mcells_per_s"] / out["fp64_mcells_per_s"]
+    out["fp32_spectra_max_rel_err"] = errs
+    return out
+
+
+def fp32_spectra_error(args):
+    """FP32 against FP64 spectra, max over wavelengths and planes of
+    |S32 - S64| / max|S64|: the 96^3 gold sphere (all-faces PML, 2000 steps,
+    16 wavelengths) and the full 512^3 gold dimer run (4000 steps, 64
+    wavelengths on 3 planes).  The FP64 GPU path agrees with the FP64 oracle
+    to 1e-11 on the test cases."""
+    import numpy as np
+    import fdtd_amd
+    from fdtd_amd import workloads
+
+    out = {}
+    cases = [("gold_sphere_96", workloads.c1_gold_sphere())]
+    if args.c2_spectra_steps > 0:
+        cases.append((f"gold_dimer_{args.n}", workloads.c2_gold_dimer(seed=args.seed, n=args.n,
+                                                                         nsteps=args.c2_spectra_steps)))
+    for name, spec in cases:
+        flux = {}
+        for prec in ("fp64", "fp32"):
+            sim, ids, _ = fdtd_amd.make_simulation(spec, precision=prec)
+            sim.step(spec["nsteps"])
+            flux[prec] = [sim.flux(i) for i in ids]
+            sim.close()
+        errs = [float(np.max(np.abs(a - b)) / np.max(np.abs(b))) for a, b in zip(flux["fp32"], flux["fp64"])]
+        out[name] = {"steps": spec["nsteps"], "nfreq": len(spec["dft"][0]["omega"]), "max_rel_err": max(errs),
+                     "per_plane": errs}
+    return out
 
 
 def main():
@@ -283,6 +325,10 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-2d", action="store_true")
     ap.add_argument("--no-spectra", action="store_true", help="skip the FP32-vs-FP64 spectra check")
+    ap.add_argument("--no-c3", action="store_true", help="skip the graphene + gold-disk mixed-precision case")
+    ap.add_argument("--c3-steps", type=int, default=6000)
+    ap.add_argument("--c2-spectra-steps", type=int, default=4000,
+                    help="steps of the 512^3 FP32-vs-FP64 spectra comparison (0: skip it)")
     ap.add_argument("--b2-count", type=int, default=256, help="batched 2D sweep: instances")
     ap.add_argument("--b2-steps", type=int, default=500, help="batched 2D sweep: timed steps")
     ap.add_argument("--b2s-side", type=int, default=128, help="small batched 2D sweep: grid side")
@@ -341,7 +387,15 @@ def main():
         b2s = batched_2d_small(args)
         print(f"[bench] batched 2d small {b2s['batched_mcells_per_s']:.0f} Mcell/s "
               f"({b2s['speedup']:.2f}x separate handles)", file=sys.stderr, flush=True)
-    spec_err = None if args.no_spectra else fp32_spectra_error()
+    spec_err = None if args.no_spectra else fp32_spectra_error(args)
+    if spec_err:
+        print("[bench] fp32 spectra error " + ", ".join(f"{k} {v['max_rel_err']:.1e}" for k, v in spec_err.items()),
+              file=sys.stderr, flush=True)
+    c3 = None
+    if not args.no_c3:
+        c3 = c3_mixed_precision(args)
+        print(f"[bench] c3 fp32 {c3['fp32_mcells_per_s']:.0f} fp64 {c3['fp64_mcells_per_s']:.0f} Mcell/s, "
+              f"spectra err {max(c3['fp32_spectra_max_rel_err']):.2e}", file=sys.stderr, flush=True)
     if args.n == 512 and args.seed == 0:  # the workload the summary profiles
         tb, src = profiled_traffic(r32["roofline"]["kernel"], "fp32")
         r32["roofline"]["traffic"] = tb
@@ -368,6 +422,7 @@ def main():
                                           "dtype": "f64", "roofline": r64["roofline"]},
         "fp32_over_fp64": None if r64 is None else r32["value"] / r64["value"],
         "fp32_spectra_error": spec_err,
+        "graphene_disks_mixed_precision": c3,
         "cpu_baseline": cpu,
         "cpu_reference_path": cpu_ref,
         "p99_job_latency_ms": None if srv is None else srv["p99_ms"],

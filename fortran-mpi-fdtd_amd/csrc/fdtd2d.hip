@@ -74,13 +74,16 @@ struct Mat {
 // TMz: act[0], gax[0] act on Ez; gax[1] is the PML medium (1/eps of the last
 // dielectric written to the cell, 1 where none); act[1] is unused.
 
+// Dispersive state and coefficients are FP64 in both precisions: inside a
+// metal E is the small difference of the large D and I, which FP32 would
+// cancel away.
 template <typename real>
 struct AdeDev {
   const int32_t* cells;  // ghosted cell offsets
-  const uint8_t* mask;   // bit 0: Ex, bit 1: Ey
-  real* state;           // [word][comp][cell]
+  const uint8_t* mask;   // bit 0: Ex (TMz: Ez), bit 1: Ey
+  double* state;         // [word][comp][cell]
   int32_t ncells, nstate;
-  AdeCoef<real> co;
+  AdeCoef<double> co;
 };
 
 struct DftDev {
@@ -492,19 +495,19 @@ __global__ __launch_bounds__(256) void k2_ade(const Inst<real>* __restrict__ I, 
   const int n = G.ncells;
   const int c = G.cells[e];
   const int mask = G.mask[e];
-  const AdeCoef<real>& co = G.co;
+  const AdeCoef<double>& co = G.co;
   const int ncp = co.ncp;
   const int i = c % S.px;
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
     if (!(mask & (1 << q))) continue;
-    const real dv = S.tmz ? S.dz[c] : (q == 0 ? S.dx[c] : S.dy[c]);
-    real* Ip = G.state + (int64_t)(0 * 2 + q) * n + e;
-    real* Sp = G.state + (int64_t)(1 * 2 + q) * n + e;
-    const real Iv = *Ip, Sv = *Sp;
-    real xr = (dv - Iv) + co.ha * Sv;
-    real xi = 0;
-    real gr[4], gi[4];
+    const double dv = S.tmz ? S.dz[c] : (q == 0 ? S.dx[c] : S.dy[c]);
+    double* Ip = G.state + (int64_t)(0 * 2 + q) * n + e;
+    double* Sp = G.state + (int64_t)(1 * 2 + q) * n + e;
+    const double Iv = *Ip, Sv = *Sp;
+    double xr = (dv - Iv) + co.ha * Sv;
+    double xi = 0;
+    double gr[4], gi[4];
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
       if (p >= ncp) break;
@@ -513,24 +516,24 @@ __global__ __launch_bounds__(256) void k2_ade(const Inst<real>* __restrict__ I, 
       xr = xr + (co.gbr[p] * gr[p] - co.gbi[p] * gi[p]);
       xi = xi + (co.gbr[p] * gi[p] + co.gbi[p] * gr[p]);
     }
-    real s1 = 0, s2 = 0;
-    real* s1p = nullptr;
-    real* s2p = nullptr;
+    double s1 = 0, s2 = 0;
+    double* s1p = nullptr;
+    double* s2p = nullptr;
     if (co.raman) {
       s1p = G.state + (int64_t)((2 + 2 * ncp) * 2 + q) * n + e;
       s2p = G.state + (int64_t)((3 + 2 * ncp) * 2 + q) * n + e;
       s1 = *s1p;
       s2 = *s2p;
     }
-    const real dr = co.gar + s1, di = co.gai;
-    const real ev = (xr * dr + xi * di) / (dr * dr + di * di);
+    const double dr = co.gar + s1, di = co.gai;
+    const double ev = (xr * dr + xi * di) / (dr * dr + di * di);
     real* E = S.tmz ? S.ez : (q == 0 ? S.ex : S.ey);
-    E[c] = ev;
+    E[c] = (real)ev;
     const bool wrap = S.per && (q == 1 || S.tmz);  // Ey (TEz) and Ez (TMz) wrap in x
-    if (wrap && i == 1) E[c + S.nx] = ev;
-    if (wrap && i == S.nx) E[c - S.nx] = ev;
+    if (wrap && i == 1) E[c + S.nx] = (real)ev;
+    if (wrap && i == S.nx) E[c - S.nx] = (real)ev;
     if (co.raman) {  // graphene_mod.f90:74-77 (Sn_new; shift)
-      const real ns = co.r1 * s1 + co.r2 * s2 + co.r0 * (ev * ev);
+      const double ns = co.r1 * s1 + co.r2 * s2 + co.r0 * (ev * ev);
       *s2p = s1;
       *s1p = ns;
     }
@@ -1056,7 +1059,7 @@ int commit_impl(Sim* s) {
     if (!G.ncells) continue;
     G.d_cells = (int32_t*)upload(s->allocs, &s->dev_bytes, gcells[g].data(), gcells[g].size() * sizeof(int32_t));
     G.d_mask = (uint8_t*)upload(s->allocs, &s->dev_bytes, gmask[g].data(), gmask[g].size());
-    G.d_state = alloc_in(s->allocs, &s->dev_bytes, (size_t)G.nstate * 2 * (size_t)G.ncells * s->wsize);
+    G.d_state = alloc_in(s->allocs, &s->dev_bytes, (size_t)G.nstate * 2 * (size_t)G.ncells * sizeof(double));
     if (!G.d_cells || !G.d_mask || !G.d_state) return set_error(FDTD_ERR_NOMEM, "dispersive state");
   }
   // monitors
@@ -1139,17 +1142,10 @@ Inst<real> make_inst(Sim* s, int32_t* ctl, int tile0) {
     AdeDev<real>& A = d.grp[g];
     A.cells = G.d_cells;
     A.mask = G.d_mask;
-    A.state = (real*)G.d_state;
+    A.state = (double*)G.d_state;
     A.ncells = (int32_t)G.ncells;
     A.nstate = G.nstate;
-    const AdeCoef<double>& c = G.co;
-    A.co.ncp = c.ncp; A.co.raman = c.raman;
-    A.co.ku = (real)c.ku; A.co.ha = (real)c.ha; A.co.hb = (real)c.hb; A.co.gar = (real)c.gar; A.co.gai = (real)c.gai;
-    for (int p = 0; p < 4; ++p) {
-      A.co.gbr[p] = (real)c.gbr[p]; A.co.gbi[p] = (real)c.gbi[p];
-      A.co.gcr[p] = (real)c.gcr[p]; A.co.gci[p] = (real)c.gci[p];
-    }
-    A.co.r0 = (real)c.r0; A.co.r1 = (real)c.r1; A.co.r2 = (real)c.r2;
+    A.co = G.co;
   }
   d.ndft = (int32_t)s->dfts.size();
   for (size_t l = 0; l < s->dfts.size(); ++l) {

@@ -761,7 +761,7 @@ int commit_impl(Sim* s) {
     G.d_cells = (int32_t*)upload_bytes(s, gcells[g].data(), gcells[g].size() * sizeof(int32_t));
     G.d_mask = (uint8_t*)upload_bytes(s, gmask[g].data(), gmask[g].size());
     // Lorentz-Jp pole state is FP64 in both precisions (k_ljp)
-    G.d_state = dev_alloc(s, (size_t)G.nstate * 3 * (size_t)G.ncells * (G.kind == 1 ? 8 : s->wsize));
+    G.d_state = dev_alloc(s, (size_t)G.nstate * 3 * (size_t)G.ncells * 8);  // FP64 state in both precisions
     if (!G.d_cells || !G.d_mask || !G.d_state) return set_error(FDTD_ERR_NOMEM, "dispersive state");
   }
 
@@ -933,7 +933,7 @@ size_t estimate_bytes(const Sim* s) {
     int64_t vol = 1;
     for (int a = 0; a < 3; ++a) vol *= std::max(0, hi[a] - lo[a] + 1);
     const int words = m.type == 3 ? 2 * m.lj.npols + 2 : 1 + 2 + 2 * (m.type == 1 ? 2 : 4) + 2;
-    b += (size_t)vol * (3 * words * (m.type == 3 ? 8 : w) + 5);
+    b += (size_t)vol * (3 * words * 8 + 5);  // dispersive state is FP64 in both precisions
   }
   for (const auto& d : s->dfts) {
     int ns = 0;
@@ -1570,7 +1570,7 @@ int fdtd_algorithmic_bytes(fdtd_sim* s, int kernel, double* bytes) {
       b = w * (9.0 * (N - s->n_pml) + 18.0 * s->n_pml);
     }
     if (kernel == FDTD_K_ADE)
-      for (auto& g : s->groups) b += (w * 6.0 + (g.kind == 1 ? 8.0 : w) * 2.0 * 3.0 * g.nstate) * (double)g.ncells;
+      for (auto& g : s->groups) b += (w * 6.0 + 8.0 * 2.0 * 3.0 * g.nstate) * (double)g.ncells;
   } else {
     const int P = s->pml_mode == FDTD_PML_REF_Y ? s->pml.npml[1] : 0;
     const double Nm = (double)s->nx * (s->ny - 2 * P) * s->nz;
@@ -1579,7 +1579,7 @@ int fdtd_algorithmic_bytes(fdtd_sim* s, int kernel, double* bytes) {
     if (kernel == FDTD_K_MAT) b = w * 6.0 * N;
     if (kernel == FDTD_K_PML_E || kernel == FDTD_K_PML_H) b = w * 24.0 * Ns;
     if (kernel == FDTD_K_ADE)
-      for (auto& g : s->groups) b += (w * 6.0 + (g.kind == 1 ? 8.0 : w) * 2.0 * 3.0 * g.nstate) * (double)g.ncells;
+      for (auto& g : s->groups) b += (w * 6.0 + 8.0 * 2.0 * 3.0 * g.nstate) * (double)g.ncells;
   }
   *bytes = b;
   return 0;

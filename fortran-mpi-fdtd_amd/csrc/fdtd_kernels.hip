@@ -816,9 +816,13 @@ __global__ __launch_bounds__(64 * BY) void k_onepass_short(FieldArgs<real> a, Pm
 // (mod3/gold_mod.f90:61-83, mod3/graphene_mod.f90:70-100).  State SoA:
 // word w of component q for cell e at state[(w*3+q)*n + e];
 // words: [D (FUSED only)], I, S, g_p re, g_p im (p < ncp), sn, snm2 (Raman).
+// The state and the update are FP64 in both precisions: inside a metal E is
+// the small difference of the large D and I, which FP32 would cancel away
+// (an FP32 state gave 6e-5 relative spectra error on the 16-disk case);
+// E is stored in field precision.
 template <typename real, bool FUSED>
-__global__ __launch_bounds__(256) void k_ade(FieldArgs<real> a, AdeCoef<real> co, const int32_t* cells,
-                                             const uint8_t* masks, real* state, int64_t n, SrcArgs<real> src,
+__global__ __launch_bounds__(256) void k_ade(FieldArgs<real> a, AdeCoef<double> co, const int32_t* cells,
+                                             const uint8_t* masks, double* state, int64_t n, SrcArgs<real> src,
                                              int nsrc) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
@@ -828,16 +832,16 @@ __global__ __launch_bounds__(256) void k_ade(FieldArgs<real> a, AdeCoef<real> co
   const int64_t rem = c - (int64_t)k * a.pxy;
   const int j = (int)(rem / a.px);
   const int i = (int)(rem - (int64_t)j * a.px);
-  real cur[3] = {0, 0, 0};
-  real sv[3] = {0, 0, 0};
+  double cur[3] = {0, 0, 0};
+  double sv[3] = {0, 0, 0};
   if (FUSED) {
-    const real hx = a.h[0][c], hy = a.h[1][c], hz = a.h[2][c];
-    cur[0] = (hz - a.h[2][c - a.px]) + (-hy + a.h[1][c - a.pxy]);
-    cur[1] = (hx - a.h[0][c - a.pxy]) + (-hz + a.h[2][c - 1]);
-    cur[2] = (hy - a.h[1][c - 1]) + (-hx + a.h[0][c - a.px]);
+    const double hx = a.h[0][c], hy = a.h[1][c], hz = a.h[2][c];
+    cur[0] = (hz - (double)a.h[2][c - a.px]) + (-hy + (double)a.h[1][c - a.pxy]);
+    cur[1] = (hx - (double)a.h[0][c - a.pxy]) + (-hz + (double)a.h[2][c - 1]);
+    cur[2] = (hy - (double)a.h[1][c - 1]) + (-hx + (double)a.h[0][c - a.px]);
     for (int q = 0; q < nsrc; ++q) {
       const int pos = src.axis[q] == 0 ? i : (src.axis[q] == 1 ? j : k);
-      if (pos == src.index[q]) sv[src.comp[q]] += src_value(src, q);
+      if (pos == src.index[q]) sv[src.comp[q]] += (double)src_value(src, q);
     }
   }
   const int w0 = FUSED ? 1 : 0;  // word offset of I
@@ -845,21 +849,21 @@ __global__ __launch_bounds__(256) void k_ade(FieldArgs<real> a, AdeCoef<real> co
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     if (!(mask & (1 << q))) continue;
-    real dv;
+    double dv;
     if (FUSED) {
-      real* dp = state + (int64_t)(0 * 3 + q) * n + e;
+      double* dp = state + (int64_t)(0 * 3 + q) * n + e;
       dv = *dp + co.ku * cur[q];
       dv = dv + sv[q];
       *dp = dv;
     } else {
       dv = a.d[q][c];
     }
-    real* Ip = state + (int64_t)((w0 + 0) * 3 + q) * n + e;
-    real* Sp = state + (int64_t)((w0 + 1) * 3 + q) * n + e;
-    const real Iv = *Ip, Sv = *Sp;
-    real xr = (dv - Iv) + co.ha * Sv;
-    real xi = 0;
-    real gr[4], gi[4];
+    double* Ip = state + (int64_t)((w0 + 0) * 3 + q) * n + e;
+    double* Sp = state + (int64_t)((w0 + 1) * 3 + q) * n + e;
+    const double Iv = *Ip, Sv = *Sp;
+    double xr = (dv - Iv) + co.ha * Sv;
+    double xi = 0;
+    double gr[4], gi[4];
 #pragma unroll
     for (int pp = 0; pp < 4; ++pp) {
       if (pp >= ncp) break;
@@ -868,9 +872,9 @@ __global__ __launch_bounds__(256) void k_ade(FieldArgs<real> a, AdeCoef<real> co
       xr = xr + (co.gbr[pp] * gr[pp] - co.gbi[pp] * gi[pp]);
       xi = xi + (co.gbr[pp] * gi[pp] + co.gbi[pp] * gr[pp]);
     }
-    real snv = 0, snm2 = 0;
-    real* snp = nullptr;
-    real* snm2p = nullptr;
+    double snv = 0, snm2 = 0;
+    double* snp = nullptr;
+    double* snm2p = nullptr;
     if (co.raman) {
       snp = state + (int64_t)((w0 + 2 + 2 * ncp) * 3 + q) * n + e;
       snm2p = state + (int64_t)((w0 + 3 + 2 * ncp) * 3 + q) * n + e;
@@ -878,12 +882,12 @@ __global__ __launch_bounds__(256) void k_ade(FieldArgs<real> a, AdeCoef<real> co
       snm2 = *snm2p;
     }
     // E = Re[X / (ga + sn)]
-    const real dr = co.gar + snv, di = co.gai;
-    const real ev = (xr * dr + xi * di) / (dr * dr + di * di);
-    a.e[q][c] = ev;
-    put_e_ghost(a.e[q], ev, i, j, k, c, a);
+    const double dr = co.gar + snv, di = co.gai;
+    const double ev = (xr * dr + xi * di) / (dr * dr + di * di);
+    a.e[q][c] = (real)ev;
+    put_e_ghost(a.e[q], (real)ev, i, j, k, c, a);
     if (co.raman) {
-      const real nsn = co.r1 * snv + co.r2 * snm2 + co.r0 * (ev * ev);
+      const double nsn = co.r1 * snv + co.r2 * snm2 + co.r0 * (ev * ev);
       *snm2p = snv;
       *snp = nsn;
     }
@@ -1639,26 +1643,12 @@ int step_impl(Sim* s) {
   if (s->dry) nsrc = 0;
   const dim3 blk((unsigned)s->bx, (unsigned)s->by, 1);
   hipStream_t st = s->stream;
-  auto ade_coef = [&](const AdeGroup& g) {
-    AdeCoef<real> c{};
-    c.ncp = g.coef.ncp;
-    c.raman = g.coef.raman;
-    c.ku = (real)s->ku;
-    c.ha = (real)g.coef.ha;
-    c.hb = (real)g.coef.hb;
-    c.gar = (real)g.coef.gar;
-    c.gai = (real)g.coef.gai;
-    for (int p = 0; p < 4; ++p) {
-      c.gbr[p] = (real)g.coef.gbr[p];
-      c.gbi[p] = (real)g.coef.gbi[p];
-      c.gcr[p] = (real)g.coef.gcr[p];
-      c.gci[p] = (real)g.coef.gci[p];
-    }
-    c.r0 = (real)g.coef.r0;
-    c.r1 = (real)g.coef.r1;
-    c.r2 = (real)g.coef.r2;
+  auto ade_coef = [&](const AdeGroup& g) {  // FP64 in both precisions
+    AdeCoef<double> c = g.coef;
+    c.ku = s->ku;
     return c;
   };
+
   // TFSF lines: E half step after the main E stage (the D terms above read
   // h(lo-1) before this step's line H update); H half step, consistency
   // terms on H and the line probes after the main H stage
@@ -1701,7 +1691,7 @@ int step_impl(Sim* s) {
       FieldArgs<real> ad = a;  // D form from the old H into the new E
       for (int q = 0; q < 3; ++q) ad.e[q] = an.e[q];
       hipLaunchKernelGGL((k_ade<real, true>), dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, ad,
-                         ade_coef(g), g.d_cells, g.d_mask, (real*)g.d_state, g.ncells, src, nsrc);
+                         ade_coef(g), g.d_cells, g.d_mask, (double*)g.d_state, g.ncells, src, nsrc);
     }
     PmlArgs<real> p = pml_args<real>(s);
     OnePassArgs<real> o{};
@@ -1777,7 +1767,7 @@ int step_impl(Sim* s) {
                            g.d_cells, g.d_mask, (double*)g.d_state, g.ncells, src, nsrc);
       else
         hipLaunchKernelGGL((k_ade<real, true>), dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, a,
-                           ade_coef(g), g.d_cells, g.d_mask, (real*)g.d_state, g.ncells, src, nsrc);
+                           ade_coef(g), g.d_cells, g.d_mask, (double*)g.d_state, g.ncells, src, nsrc);
     }
     tfsf_e(st);
     {
@@ -1820,7 +1810,7 @@ int step_impl(Sim* s) {
                            g.d_cells, g.d_mask, (double*)g.d_state, g.ncells, src, nsrc);
       else
         hipLaunchKernelGGL((k_ade<real, false>), dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, a,
-                           ade_coef(g), g.d_cells, g.d_mask, (real*)g.d_state, g.ncells, src, 0);
+                           ade_coef(g), g.d_cells, g.d_mask, (double*)g.d_state, g.ncells, src, 0);
     }
     RefPmlArgs<real> rp{};
     if (refy) {
