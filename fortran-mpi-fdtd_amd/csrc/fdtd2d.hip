@@ -128,6 +128,12 @@ struct Inst {
   real *ez, *hx, *hy, *dz;
   real *ezx, *ezy;
   const real* tprof;
+  // TFSF along y: total field on rows tf_lo..tf_hi (tf_hi 0: one-sided);
+  // the incident line e, h, pe, ph, f1, f2, g1, g2 (ny+2 doubles each)
+  int32_t tf_on, tf_lo, tf_hi, tf_src, tf_npml;
+  double* tf_line;
+  double tf_ck, tf_ku, tf_sg;
+  double tf_t0, tf_st, tf_w0, tf_amp;
 };
 
 // ---- host state ------------------------------------------------------------------
@@ -193,6 +199,10 @@ struct Sim {
   size_t dev_bytes = 0;
   Batch* solo = nullptr;  // batch of one behind fdtd2d_step
   Batch* owner = nullptr;
+  // TFSF along y (one per simulation): the incident line on the device
+  bool has_tf = false;
+  fdtd2d_tfsf tfp{};
+  double* d_tf = nullptr;  // e, h, pe, ph, f1, f2, g1, g2, ny+2 each
   // TMz: F holds Ez, Hx, Hy, Dz; Ez splits and profiles of its PML
   bool tmz = false;
   void* d_ezx = nullptr;
@@ -211,7 +221,8 @@ struct Batch {
   int4* d_ade = nullptr;     // (q, group, first cell, 0)
   int4* d_mon = nullptr;     // (q, line or -1 for probes, first i, 0)
   int2* d_lines = nullptr;   // (q, line)
-  int ntiles = 0, nade = 0, nmon = 0, nlines = 0;
+  int32_t* d_tfl = nullptr;  // simulations with a TFSF
+  int ntiles = 0, nade = 0, nmon = 0, nlines = 0, ntf = 0;
   std::vector<int32_t> ctl;  // host mirror
   std::vector<char> inst;    // host image of the descriptors
   hipGraph_t graph = nullptr;
@@ -248,6 +259,39 @@ __device__ __forceinline__ bool in_pml_tmz(const Inst<real>& S, int i, int j) {
   return (S.Px > 0 && (i <= S.Px || i > S.nx - S.Px)) || (S.P > 0 && (j <= S.P || j > S.ny - S.P));
 }
 
+// TFSF consistency terms of row j (sim1/program.f90:236-238,260-262 with the
+// far plane of the two-sided form), FP64 products rounded once:
+// D(lo) -= Ku*h(lo-1), D(hi+1) += Ku*h(hi); H(lo-1) -= s*Ku*e(lo),
+// H(hi) += s*Ku*e(hi+1).  False: no term on this row.
+template <typename real>
+__device__ __forceinline__ bool tf_d_term(const Inst<real>& S, int j, real& v) {
+  if (!S.tf_on) return false;
+  const double* h = S.tf_line + (S.ny + 2);
+  if (j == S.tf_lo) {
+    v = (real)(-S.tf_ku * h[S.tf_lo - 1]);
+    return true;
+  }
+  if (S.tf_hi > 0 && j == S.tf_hi + 1) {
+    v = (real)(S.tf_ku * h[S.tf_hi]);
+    return true;
+  }
+  return false;
+}
+template <typename real>
+__device__ __forceinline__ bool tf_h_term(const Inst<real>& S, int j, real& v) {
+  if (!S.tf_on) return false;
+  const double* e = S.tf_line;
+  if (j == S.tf_lo - 1) {
+    v = (real)(-(S.tf_sg * S.tf_ku) * e[S.tf_lo]);
+    return true;
+  }
+  if (S.tf_hi > 0 && j == S.tf_hi) {
+    v = (real)((S.tf_sg * S.tf_ku) * e[S.tf_hi + 1]);
+    return true;
+  }
+  return false;
+}
+
 // TMz E half step of cell (i, j); loads first (the pointers may alias as far
 // as the compiler knows)
 template <typename real>
@@ -262,6 +306,8 @@ __device__ __forceinline__ void e_tmz(const Inst<real>& S, int i, int j, real& c
       sv = S.src_val[s];
     }
   const bool pml = in_pml_tmz(S, i, j);
+  real cv = 0;
+  const bool corr = tf_d_term(S, j, cv);
   const real hxc = S.hx[c], hxj = use_carry ? carry : S.hx[c - px], hyc = S.hy[c], hyi = S.hy[c - 1];
   carry = hxc;  // Hx(j) is the next row's Hx(j-1)
   const Mat<real> m = S.tab[S.matid[c]];
@@ -271,6 +317,7 @@ __device__ __forceinline__ void e_tmz(const Inst<real>& S, int i, int j, real& c
     real dz = S.dz[c];
     dz = dz + S.ku * ((hyc - hyi) + (-hxc + hxj));
     if (src) dz = dz + sv;
+    if (corr) dz = dz + cv;
     S.dz[c] = dz;
     if (m.act[0] == ACT_DIEL) {
       ez = dz * m.gax[0];
@@ -284,6 +331,7 @@ __device__ __forceinline__ void e_tmz(const Inst<real>& S, int i, int j, real& c
     const real zx = fx[i] * ox + (fx[(S.nx + 2) + i] * r) * (hyc + -hyi);
     real zy = fy[j] * oy + (fy[(S.ny + 2) + j] * r) * (-hxc + hxj);
     if (src) zy = zy + sv * r;
+    if (corr) zy = zy + cv * r;
     S.ezx[c] = zx;
     S.ezy[c] = zy;
     ez = zx + zy;
@@ -315,6 +363,8 @@ __device__ __forceinline__ void h_tmz(const Inst<real>& S, int i, int j, real& c
     hx = fy[2 * (S.ny + 2) + j] * hxc + fy[3 * (S.ny + 2) + j] * (ezc + -ezj);
     hy = fx[2 * (S.nx + 2) + i] * hyc + fx[3 * (S.nx + 2) + i] * (-ezc + ezi);
   }
+  real cv = 0;
+  if (tf_h_term(S, j, cv)) hx = hx + cv;
   S.hx[c] = hx;
   S.hy[c] = hy;
   if (S.per && i == 1) S.hy[c + S.nx] = hy;
@@ -335,6 +385,8 @@ __device__ __forceinline__ void e_tez(const Inst<real>& S, int i, int j, real& c
       sv = S.src_val[s];
     }
   const bool slab = S.has_pml && (j <= S.ja || j >= S.jb);
+  real cv = 0;
+  const bool corr = tf_d_term(S, j, cv);
   // loads first: the field pointers may alias as far as the compiler knows
   const real hzc = S.hz[c];
   real dxv = S.dx[c], dyv = S.dy[c];
@@ -347,8 +399,9 @@ __device__ __forceinline__ void e_tez(const Inst<real>& S, int i, int j, real& c
     dyv = dyv - S.ku * (hzc - hzi);  // dy - Ku*curlhx
   }
   if (src) dxv = dxv + sv;
+  if (corr) dxv = dxv + cv;
   if (main) S.dy[c] = dyv;
-  if (main || src) S.dx[c] = dxv;
+  if (main || src || corr) S.dx[c] = dxv;
   // dielectric writers (graphene's Ey = Dy is one with gax 1)
   if (m.act[1] == ACT_DIEL) {
     const real ey = dyv * m.gax[1];
@@ -397,6 +450,8 @@ __device__ __forceinline__ void h_tez(const Inst<real>& S, int i, int j, real& c
     const real curlE = S.ey[c + 1] - S.ey[c] - exj + exc;
     hz = S.hz[c] - S.ku * curlE;
     wrote = true;
+    real cv = 0;
+    if (tf_h_term(S, j, cv)) hz = hz + cv;
   }
   carry = exj;  // Ex(j+1) is the next row's Ex(j)
   if (slab) {
@@ -604,6 +659,63 @@ __global__ void k2_adv(const Inst<real>* __restrict__ I, int count, int inc) {
   for (int s = 0; s < S.nsrc; ++s) {
     const double a = (t - S.src_t0[s]) / S.src_st[s];
     S.src_val[s] = (real)(S.src_amp[s] * (exp(-0.5 * (a * a)) * sin(S.src_w0[s] * t)));
+  }
+}
+
+// TFSF incident line, one block per simulation with a TFSF (list[b]): E half
+// step (ex_inc update, the pulse at t = (n-1)*dt, mypmly_inc%updateE with
+// the hand-over of its end slabs; sim1/program.f90:222-231) and H half step
+// (hz_inc update, mypmly_inc%updateH; :252-255).
+template <typename real>
+__global__ __launch_bounds__(256) void k2_tfsf_line(const Inst<real>* __restrict__ I, const int32_t* __restrict__ list,
+                                                    int n, int h_half) {
+  const int w = blockIdx.x;
+  if (w >= n) return;
+  const Inst<real>& S = I[list[w]];
+  if (S.ctl[0] >= S.ctl[1]) return;
+  const int N = S.ny, P = S.tf_npml, hi = N - P + 1;
+  const int64_t m = N + 2;
+  double* e = S.tf_line;
+  double* h = e + m;
+  double* pe = e + 2 * m;
+  double* ph = e + 3 * m;
+  const double* f1 = e + 4 * m;
+  const double* f2 = e + 5 * m;
+  const double* g1 = e + 6 * m;
+  const double* g2 = e + 7 * m;
+  if (!h_half) {
+    const double t = (double)S.ctl[0] * S.dt;  // step ctl[0]+1
+    const double a = (t - S.tf_t0) / S.tf_st;
+    const double g = S.tf_amp * (exp(-0.5 * (a * a)) * sin(S.tf_w0 * t));
+    for (int j = P + 1 + (int)threadIdx.x; j <= hi - 1; j += blockDim.x) {
+      double v = e[j] + S.tf_ck * (h[j] - h[j - 1]);
+      if (j == S.tf_src) v = v + g;
+      e[j] = v;
+    }
+    if (P > 0) {
+      for (int j = 1 + (int)threadIdx.x; j <= P; j += blockDim.x) pe[j] = f1[j] * pe[j] + f2[j] * (ph[j] - ph[j - 1]);
+      for (int j = hi + (int)threadIdx.x; j <= N; j += blockDim.x) pe[j] = f1[j] * pe[j] + f2[j] * (ph[j] - ph[j - 1]);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        pe[P + 1] = e[P + 1];
+        e[P] = pe[P];
+        pe[hi - 1] = e[hi - 1];
+        e[hi] = pe[hi];
+      }
+    }
+  } else {
+    for (int j = P + 1 + (int)threadIdx.x; j <= hi - 1; j += blockDim.x) h[j] = h[j] + S.tf_ku * (e[j + 1] - e[j]);
+    if (P > 0) {
+      for (int j = 1 + (int)threadIdx.x; j <= P; j += blockDim.x) ph[j] = g1[j] * ph[j] + g2[j] * (pe[j + 1] - pe[j]);
+      for (int j = hi + (int)threadIdx.x; j <= N; j += blockDim.x) ph[j] = g1[j] * ph[j] + g2[j] * (pe[j + 1] - pe[j]);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        ph[P + 1] = h[P + 1];
+        h[P] = ph[P];
+        ph[hi - 1] = h[hi - 1];
+        h[hi] = ph[hi];
+      }
+    }
   }
 }
 
@@ -1062,6 +1174,18 @@ int commit_impl(Sim* s) {
     G.d_state = alloc_in(s->allocs, &s->dev_bytes, (size_t)G.nstate * 2 * (size_t)G.ncells * sizeof(double));
     if (!G.d_cells || !G.d_mask || !G.d_state) return set_error(FDTD_ERR_NOMEM, "dispersive state");
   }
+  // TFSF incident line
+  if (s->has_tf) {
+    const int top = s->tfp.hi > 0 ? s->tfp.hi + 1 : s->tfp.lo;
+    const int P = s->has_pml ? s->pml.npml[1] : 0;
+    if (s->has_pml && (s->tfp.lo - 1 <= P || top >= ny - P + (s->tmz ? 1 : 0)))
+      return set_error(FDTD_ERR_ARG, "TFSF rows inside the PML");
+    const size_t m = (size_t)ny + 2;
+    std::vector<double> line(8 * m, 0.0);
+    tfsf_line_profiles(s->dt, s->ds, ny, s->tfp.npml, s->tfp.eps, s->tfp.order, s->tfp.R0, line.data() + 4 * m);
+    s->d_tf = (double*)upload(s->allocs, &s->dev_bytes, line.data(), line.size() * sizeof(double));
+    if (!s->d_tf) return set_error(FDTD_ERR_NOMEM, "TFSF line");
+  }
   // monitors
   for (auto& d : s->dfts) {
     d.nslots = s->tmz ? ((d.mask & FDTD2D_DFT_EZ) ? 1 : 0) + ((d.mask & FDTD2D_DFT_HX) ? 1 : 0)
@@ -1160,6 +1284,21 @@ Inst<real> make_inst(Sim* s, int32_t* ctl, int tile0) {
   d.probe_rec = s->d_probe;
   d.probe_cap = s->probe_cap;
   d.ctl = ctl;
+  if (s->has_tf) {
+    d.tf_on = 1;
+    d.tf_lo = s->tfp.lo;
+    d.tf_hi = s->tfp.hi;
+    d.tf_src = s->tfp.src_index;
+    d.tf_npml = s->tfp.npml;
+    d.tf_line = s->d_tf;
+    d.tf_ck = s->ku * (1.0 / s->tfp.eps);
+    d.tf_ku = s->ku;
+    d.tf_sg = s->tmz ? -1.0 : 1.0;  // (Ez, Hx) along y: -1; (Ex, Hz): +1
+    d.tf_t0 = s->tfp.pulse.t0;
+    d.tf_st = s->tfp.pulse.st;
+    d.tf_w0 = s->tfp.pulse.w0;
+    d.tf_amp = s->tfp.pulse.amplitude;
+  }
   return d;
 }
 
@@ -1189,7 +1328,7 @@ int batch_build(Batch* b) {
   b->d_inst = alloc_in(b->allocs, nullptr, isz * count);
   b->d_ctl = (int32_t*)alloc_in(b->allocs, nullptr, sizeof(int32_t) * 2 * count);
   if (!b->d_inst || !b->d_ctl) return set_error(FDTD_ERR_NOMEM, "batch descriptors");
-  std::vector<int32_t> tile_inst;
+  std::vector<int32_t> tile_inst, tfl;
   std::vector<int4> ade, mon;
   std::vector<int2> lines;
   for (int q = 0; q < count; ++q) {
@@ -1203,6 +1342,7 @@ int batch_build(Batch* b) {
       lines.push_back(make_int2(q, (int)l));
     }
     if (!s->probes.empty()) mon.push_back(make_int4(q, -1, 0, 0));
+    if (s->has_tf) tfl.push_back(q);
   }
   b->ntiles = (int)tile_inst.size();
   b->nade = (int)ade.size();
@@ -1212,7 +1352,10 @@ int batch_build(Batch* b) {
   b->d_ade = (int4*)upload(b->allocs, nullptr, ade.data(), ade.size() * sizeof(int4));
   b->d_mon = (int4*)upload(b->allocs, nullptr, mon.data(), mon.size() * sizeof(int4));
   b->d_lines = (int2*)upload(b->allocs, nullptr, lines.data(), lines.size() * sizeof(int2));
-  if (!b->d_tile_inst || !b->d_ade || !b->d_mon || !b->d_lines) return set_error(FDTD_ERR_NOMEM, "batch work lists");
+  b->ntf = (int)tfl.size();
+  b->d_tfl = (int32_t*)upload(b->allocs, nullptr, tfl.data(), tfl.size() * sizeof(int32_t));
+  if (!b->d_tile_inst || !b->d_ade || !b->d_mon || !b->d_lines || !b->d_tfl)
+    return set_error(FDTD_ERR_NOMEM, "batch work lists");
   b->ctl.assign(2 * (size_t)count, 0);
   for (int q = 0; q < count; ++q) b->ctl[2 * q] = b->ctl[2 * q + 1] = b->sims[q]->n_done;
   FDTD_HIP(hipMemcpy(b->d_ctl, b->ctl.data(), sizeof(int32_t) * 2 * count, hipMemcpyHostToDevice));
@@ -1232,9 +1375,13 @@ void enqueue_block(Batch* b) {
   const unsigned grid = (unsigned)std::min(b->ntiles, std::max(8192, (b->ntiles + tpb - 1) / tpb));
   const int count = (int)b->sims.size();
   for (int s = 0; s < kBlockSteps; ++s) {
+    // the line's E half step may precede k2_e, which reads only the line's h
+    if (b->ntf) hipLaunchKernelGGL(k2_tfsf_line<real>, dim3((unsigned)b->ntf), dim3(256), 0, st, I, b->d_tfl, b->ntf, 0);
     hipLaunchKernelGGL(k2_e<real>, dim3(grid), dim3(kTX * kWaves), 0, st, I, b->d_tile_inst, b->ntiles);
     if (b->nade) hipLaunchKernelGGL(k2_ade<real>, dim3((unsigned)b->nade), dim3(kChunk), 0, st, I, b->d_ade, b->nade);
     hipLaunchKernelGGL(k2_h<real>, dim3(grid), dim3(kTX * kWaves), 0, st, I, b->d_tile_inst, b->ntiles);
+    // k2_h read the line's new e; its H half step follows
+    if (b->ntf) hipLaunchKernelGGL(k2_tfsf_line<real>, dim3((unsigned)b->ntf), dim3(256), 0, st, I, b->d_tfl, b->ntf, 1);
     if (b->nmon) hipLaunchKernelGGL(k2_mon<real>, dim3((unsigned)b->nmon), dim3(kChunk), 0, st, I, b->d_mon, b->nmon, s);
     hipLaunchKernelGGL(k2_adv<real>, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, st, I, count, 1);
   }
@@ -1506,6 +1653,36 @@ int fdtd2d_add_probe(fdtd2d_sim* s, int i, int j, int* id) {
   if ((int)s->probes.size() >= kMaxProbe) return set_error(FDTD_ERR_UNSUPPORTED, "too many probes");
   s->probes.push_back({i, j});
   *id = (int)s->probes.size() - 1;
+  return 0;
+}
+
+int fdtd2d_add_tfsf(fdtd2d_sim* s, const fdtd2d_tfsf* p, int* id) {
+  GUARD2(s);
+  SETUP2(s);
+  if (!p || !id) return set_error(FDTD_ERR_ARG, "null argument");
+  if (s->has_tf) return set_error(FDTD_ERR_UNSUPPORTED, "one TFSF per simulation");
+  const int N = s->ny;
+  if (p->lo < 2 || p->lo > N || (p->hi != 0 && (p->hi < p->lo || p->hi >= N)) || p->src_index < 1 ||
+      p->src_index > N || p->npml < 0 || 2 * p->npml + 2 >= N || !(p->eps > 0) || !(p->pulse.st > 0) ||
+      p->src_index <= p->npml || p->src_index > N - p->npml ||
+      (p->npml > 0 && (!(p->order > 0) || !(p->R0 > 0 && p->R0 < 1))))
+    return set_error(FDTD_ERR_ARG, "bad TFSF rows");
+  s->has_tf = true;
+  s->tfp = *p;
+  *id = 0;
+  return 0;
+}
+
+int fdtd2d_get_tfsf_line(fdtd2d_sim* s, int id, double* e, double* h) {
+  GUARD2(s);
+  if (!e || !h || id != 0 || !s->has_tf) return set_error(FDTD_ERR_ARG, "bad argument");
+  int rc = ensure_committed(s);
+  if (rc) return rc;
+  rc = sync_sim(s);
+  if (rc) return rc;
+  const size_t m = (size_t)s->ny + 2;
+  FDTD_HIP(hipMemcpy(e, s->d_tf, m * sizeof(double), hipMemcpyDeviceToHost));
+  FDTD_HIP(hipMemcpy(h, s->d_tf + m, m * sizeof(double), hipMemcpyDeviceToHost));
   return 0;
 }
 

@@ -248,14 +248,11 @@ void ljp_coef(const Sim* s, const fdtd_lorentz_jp& p, LjpCoef<double>& c) {
 // cells 1..npml and the high slab of rank Ymax-1 on n-npml+1..n, both in
 // the line's medium (eplow = ephigh = eps*ep0, sim1/program.f90:134).
 // out: f1, f2, g1, g2, each n+2 doubles.
-void tfsf_profiles(const Sim* s, const Tfsf& t, double* out) {
+void tfsf_profiles_dt(double dt, double ds, int N, int np, double eps, double order, double R0, double* out) {
   const Consts& K = consts();
-  const int N = t.n, np = t.p.npml;
-  const double dt = s->dt, ds = s->ds;
   const double z0 = std::sqrt(K.mu0 / K.ep0);
   const double Ku = dt * K.c0 / ds;
-  const double ep = t.p.eps * K.ep0, epr = ep / K.ep0;
-  const double order = t.p.order;
+  const double ep = eps * K.ep0, epr = ep / K.ep0;
   double* f1 = out;
   double* f2 = out + (N + 2);
   double* g1 = out + 2 * (N + 2);
@@ -267,7 +264,7 @@ void tfsf_profiles(const Sim* s, const Tfsf& t, double* out) {
     g2[j] = Ku;
   }
   if (np <= 0) return;
-  const double sigmam = (order + 1.0) * K.c0 * std::log(1.0 / t.p.R0) / (2.0 * np * ds);  // sim1/program.f90:132
+  const double sigmam = (order + 1.0) * K.c0 * std::log(1.0 / R0) / (2.0 * np * ds);  // sim1/program.f90:132
   for (int j = 1; j <= np; ++j) {
     const double sE = ep * sigmam * std::pow(((double)(np - j) + 0.5) / (double)np, order);
     f1[j] = std::exp(-(sE * dt / ep));
@@ -792,7 +789,7 @@ int commit_impl(Sim* s) {
         if (pl[q] >= lo[a] && pl[q] <= hi[a]) return set_error(FDTD_ERR_UNSUPPORTED, "TFSF plane through Lorentz-Jp cells");
     }
     std::vector<double> line((size_t)8 * (t.n + 2), 0.0);
-    tfsf_profiles(s, t, line.data() + (size_t)4 * (t.n + 2));
+    tfsf_profiles_dt(s->dt, s->ds, t.n, t.p.npml, t.p.eps, t.p.order, t.p.R0, line.data() + (size_t)4 * (t.n + 2));
     t.d_line = (double*)upload_bytes(s, line.data(), line.size() * sizeof(double));
     if (!t.d_line) return set_error(FDTD_ERR_NOMEM, "TFSF line");
     std::vector<int32_t> lp;
@@ -893,6 +890,10 @@ void* field_ptr(Sim* s, int comp) {
 }
 
 }  // namespace
+
+void tfsf_line_profiles(double dt, double ds, int n, int npml, double eps, double order, double R0, double* out) {
+  tfsf_profiles_dt(dt, ds, n, npml, eps, order, R0, out);
+}
 
 int graphene3d_coef(double dt, double ds, const fdtd_graphene& p, AdeCoef<double>& c, double* eps_inf) {
   return graphene_coef_dt(dt, ds, p, c, eps_inf);

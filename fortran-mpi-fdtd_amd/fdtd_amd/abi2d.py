@@ -25,6 +25,11 @@ class Region2D(C.Structure):
                 ("center", C.c_double * 2), ("radius", C.c_double)]
 
 
+class Tfsf2D(C.Structure):
+    _fields_ = [("lo", C.c_int32), ("hi", C.c_int32), ("src_index", C.c_int32), ("npml", C.c_int32),
+                ("eps", C.c_double), ("order", C.c_double), ("R0", C.c_double), ("pulse", Pulse)]
+
+
 _P = C.c_void_p
 _I = C.c_int
 _D = C.c_double
@@ -40,6 +45,8 @@ PROTOTYPES_2D = {
     "add_line_source": (_I, [_P, _I, C.POINTER(Pulse)]),
     "add_dft_line": (_I, [_P, _I, _I, _PD, _I, _I, _I, C.POINTER(_I)]),
     "add_probe": (_I, [_P, _I, _I, C.POINTER(_I)]),
+    "add_tfsf": (_I, [_P, C.POINTER(Tfsf2D), C.POINTER(_I)]),
+    "get_tfsf_line": (_I, [_P, _I, _PD, _PD]),
     "commit": (_I, [_P]),
     "step": (_I, [_P, _I]),
     "sync": (_I, [_P]),

@@ -117,6 +117,21 @@ class Simulation2D:
         self.b.check(self.b.add_probe(self.h, int(i), int(j), C.byref(r)))
         return r.value
 
+    def add_tfsf(self, lo, src_index, t0, st, w0, amplitude=1.0, hi=0, npml=10, eps=1.0, order=3.0,
+                 R0=math.exp(-16.0)):
+        """Plane wave along y by total-field / scattered-field (fdtd2d_add_tfsf)."""
+        t = abi2d.Tfsf2D(int(lo), int(hi), int(src_index), int(npml), float(eps), float(order), float(R0),
+                         abi.Pulse(t0, st, w0, amplitude))
+        i = C.c_int()
+        self.b.check(self.b.add_tfsf(self.h, C.byref(t), C.byref(i)))
+        return i.value
+
+    def tfsf_line(self, tid=0):
+        e = np.empty(self.shape[1] + 2)
+        h = np.empty(self.shape[1] + 2)
+        self.b.check(self.b.get_tfsf_line(self.h, int(tid), e.ctypes.data_as(abi._PD), h.ctypes.data_as(abi._PD)))
+        return e, h
+
     def commit(self):
         self.b.check(self.b.commit(self.h))
 
@@ -217,6 +232,9 @@ def apply_spec2d(sim, spec):
             raise ValueError(m["type"])
     for s in spec.get("sources", []):
         sim.add_line_source(s["j"], s["t0"], s["st"], s["w0"], s.get("amplitude", 1.0))
+    for t in spec.get("tfsf", []):
+        sim.add_tfsf(t["lo"], t["src_index"], t["t0"], t["st"], t["w0"], t.get("amplitude", 1.0), t.get("hi", 0),
+                     t.get("npml", 10), t.get("eps", 1.0), t.get("order", 3.0), t.get("R0", math.exp(-16.0)))
     ids = [sim.add_dft_line(d["j"], np.array(d["omega"]), d.get("mask"), d.get("n_begin", 1),
                             d.get("n_end", 0)) for d in spec.get("dft", [])]
     pids = [sim.add_probe(*pr) for pr in spec.get("probes", [])]

@@ -23,6 +23,12 @@ module fdtd2d_c
         real(c_double) :: order, R0, sigmam, eps_low, eps_high
     end type fdtd2d_pml
 
+    type, bind(c) :: fdtd2d_tfsf
+        integer(c_int32_t) :: lo, hi, src_index, npml
+        real(c_double) :: eps, order, R0
+        type(fdtd_pulse) :: pulse
+    end type fdtd2d_tfsf
+
     type, bind(c) :: fdtd2d_region
         integer(c_int32_t) :: kind, reserved, lo(2), hi(2)
         real(c_double) :: center(2), radius
@@ -80,6 +86,18 @@ module fdtd2d_c
             type(c_ptr), value :: sim
             integer(c_int), value :: i, j
             integer(c_int), intent(out) :: id
+        end function
+        integer(c_int) function fdtd2d_add_tfsf(sim, t, id) bind(c, name="fdtd2d_add_tfsf")
+            import :: c_int, c_ptr, fdtd2d_tfsf
+            type(c_ptr), value :: sim
+            type(fdtd2d_tfsf), intent(in) :: t
+            integer(c_int), intent(out) :: id
+        end function
+        integer(c_int) function fdtd2d_get_tfsf_line(sim, tid, e, h) bind(c, name="fdtd2d_get_tfsf_line")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: sim
+            integer(c_int), value :: tid
+            real(c_double), intent(out) :: e(*), h(*)
         end function
         integer(c_int) function fdtd2d_commit(sim) bind(c, name="fdtd2d_commit")
             import :: c_int, c_ptr

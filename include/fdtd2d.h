@@ -129,6 +129,29 @@ int fdtd2d_add_line_source(fdtd2d_sim* sim, int j, const fdtd_pulse* pulse);
  * n_end <= 0: no upper gate.                                               */
 int fdtd2d_add_dft_line(fdtd2d_sim* sim, int j, int nfreq, const double* omega, int comp_mask, int n_begin,
                         int n_end, int* id);
+/* Plane wave by total-field / scattered-field along y (no counterpart in
+ * the reference's 2D driver): the 1D driver's scheme (sim1/program.f90:86-91,
+ * 217-262; the same terms as fdtd_add_tfsf with axis 1) with an incident
+ * line of ny cells along y carrying Ex/Hz (TEz) or Ez/Hx (TMz), its own
+ * end-slab PML (mod1/BERpmly_ExHz_mod.f90) and the pulse on line cell
+ * src_index.  Total field on rows lo..hi (hi = 0: lo..ny, one-sided); per
+ * step D(lo) -= Ku*h(lo-1), D(hi+1) += Ku*h(hi) and H(lo-1) -= s*Ku*e(lo),
+ * H(hi) += s*Ku*e(hi+1) with s = +1 (TEz) or -1 (TMz), on every column.
+ * The four correction rows must lie outside the PML.  The dormant 2D
+ * include sim2/include/myTFSF_2Dfunc.f90 is not followed: its Hz term sits
+ * one row off its Dx term, which leaks the incident wave into the
+ * scattered region.  One per simulation.                                  */
+typedef struct fdtd2d_tfsf {
+  int32_t lo, hi;        /* total-field rows                                 */
+  int32_t src_index;     /* line cell of the pulse                           */
+  int32_t npml;          /* line PML cells at each end (sim1: npml)          */
+  double eps;            /* line medium, relative permittivity               */
+  double order, R0;      /* line PML grading                                 */
+  fdtd_pulse pulse;
+} fdtd2d_tfsf;
+int fdtd2d_add_tfsf(fdtd2d_sim* sim, const fdtd2d_tfsf* tfsf, int* id);
+/* The incident line: e and h, ny+2 doubles each (cells 0..ny+1).          */
+int fdtd2d_get_tfsf_line(fdtd2d_sim* sim, int id, double* e, double* h);
 /* Time trace of (Ex, Hz) (TMz: (Ez, Hx)) at (i,j) after each step
  * (sim2/program.f90:376,397,417).                                          */
 int fdtd2d_add_probe(fdtd2d_sim* sim, int i, int j, int* id);

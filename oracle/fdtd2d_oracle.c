@@ -35,6 +35,11 @@
  * gold on Ez (mod3/gold_mod.f90:50-88) and graphene on Ez with the 3D
  * module's coefficients (mod3/graphene_mod.f90:202-283).  It is pinned
  * against the 3D oracle run with nz = 1 and periodic z.
+ *
+ * TFSF along y (either polarisation) restates the 3D oracle's TFSF
+ * (the 1D driver's incident line, sim1/program.f90:217-262, with its
+ * mod1/BERpmly_ExHz_mod.f90 end slabs) with axis y: the same line update,
+ * profiles and consistency terms, pinned against the 3D oracle at nz = 1.
  */
 #include <complex.h>
 #include <math.h>
@@ -121,6 +126,12 @@ struct o2sim {
   double *ezx, *ezy, *rz;         /* PML splits of Ez, PML medium 1/eps */
   int tp[2];                      /* PML depth along x, y */
   double *tf1[2], *tf2[2], *tg1[2], *tg2[2]; /* profiles per axis, 0..n+1 */
+  /* TFSF along y: the incident line (cells 0..ny+1) and its PML */
+  int has_tf;
+  fdtd2d_tfsf tfp;
+  int tf_lo, tf_hi; /* line slab bounds: npml and ny-npml+1 */
+  double tf_sg;
+  double *le, *lh, *lpe, *lph, *lf1, *lf2, *lg1, *lg2;
 };
 typedef struct o2sim o2sim;
 
@@ -157,6 +168,7 @@ int ofdtd2d_destroy(o2sim* s) {
   free(s->f1x); free(s->f2x); free(s->f1y); free(s->f2y);
   free(s->g1x); free(s->g2x); free(s->g1y); free(s->g2y);
   free(s->ez); free(s->hx); free(s->hy); free(s->dz); free(s->ezx); free(s->ezy); free(s->rz);
+  free(s->le); free(s->lh); free(s->lpe); free(s->lph); free(s->lf1); free(s->lf2); free(s->lg1); free(s->lg2);
   for (int a = 0; a < 2; ++a) { free(s->tf1[a]); free(s->tf2[a]); free(s->tg1[a]); free(s->tg2[a]); }
   for (int q = 0; q < s->nmats; ++q) {
     o2mat* m = &s->mats[q];
@@ -406,6 +418,134 @@ int ofdtd2d_add_dft_line(o2sim* s, int j, int nfreq, const double* omega, int ma
   return 0;
 }
 
+/* the incident line's end-slab profiles: tfsf_line_profiles of the 3D
+ * oracle (mod1/BERpmly_ExHz_mod.f90:156-255, sim1/program.f90:132-134) */
+static void tf_line_profiles(o2sim* s) {
+  const int N = s->ny, np = s->tfp.npml;
+  const double dt = s->dt, ds = s->ds;
+  const double z0 = sqrt(O2_MU0 / O2_EP0);
+  const double Ku = dt * O2_C0 / ds;
+  const double ep = s->tfp.eps * O2_EP0, epr = ep / O2_EP0;
+  const double order = s->tfp.order;
+  const double sigmam = (order + 1.0) * O2_C0 * log(1.0 / s->tfp.R0) / (2.0 * np * ds);
+  for (int j = 0; j <= N + 1; ++j) {
+    s->lf1[j] = 1.0; s->lf2[j] = Ku / epr; s->lg1[j] = 1.0; s->lg2[j] = Ku;
+  }
+  if (np <= 0) return;
+  for (int j = 1; j <= s->tf_lo; ++j) {
+    const double sE = ep * sigmam * pow(((double)(np - j) + 0.5) / (double)np, order);
+    const double aE = sE * dt / ep;
+    s->lf1[j] = exp(-aE);
+    s->lf2[j] = (1.0 - s->lf1[j]) / (z0 * sE * ds);
+    if (j < np) {
+      const double sH = O2_MU0 * sigmam * pow((double)(np - j) / (double)np, order);
+      const double aH = sH * dt / O2_MU0;
+      s->lg1[j] = exp(-aH);
+      s->lg2[j] = z0 * (1.0 - s->lg1[j]) / (sH * ds);
+    }
+  }
+  int jj = np;
+  for (int j = s->tf_hi; j <= N; ++j) {
+    if (jj < np) {
+      const double sE = ep * sigmam * pow((double)(np - jj) / (double)np, order);
+      const double aE = sE * dt / ep;
+      s->lf1[j] = exp(-aE);
+      s->lf2[j] = (1.0 - s->lf1[j]) / (z0 * sE * ds);
+    }
+    const double sH = O2_MU0 * sigmam * pow(((double)(np - jj) + 0.5) / (double)np, order);
+    const double aH = sH * dt / O2_MU0;
+    s->lg1[j] = exp(-aH);
+    s->lg2[j] = z0 * (1.0 - s->lg1[j]) / (sH * ds);
+    jj = jj - 1;
+  }
+}
+
+int ofdtd2d_add_tfsf(o2sim* s, const fdtd2d_tfsf* p, int* id) {
+  if (!s || !p || !id) return o2_fail(FDTD_ERR_ARG, "null argument");
+  if (s->committed) return o2_fail(FDTD_ERR_STATE, "set-up after commit");
+  if (s->has_tf) return o2_fail(FDTD_ERR_UNSUPPORTED, "one TFSF per simulation");
+  const int N = s->ny;
+  if (p->lo < 2 || p->lo > N || (p->hi != 0 && (p->hi < p->lo || p->hi >= N)) || p->src_index < 1 ||
+      p->src_index > N || p->npml < 0 || 2 * p->npml + 2 >= N || !(p->eps > 0) || p->src_index <= p->npml ||
+      p->src_index > N - p->npml)
+    return o2_fail(FDTD_ERR_ARG, "bad TFSF rows");
+  s->has_tf = 1;
+  s->tfp = *p;
+  s->tf_lo = p->npml;
+  s->tf_hi = N - p->npml + 1;
+  s->tf_sg = s->tmz ? -1.0 : 1.0; /* (Ez, Hx) along y: s = -1; (Ex, Hz): +1 */
+  s->le = z2(N + 2); s->lh = z2(N + 2); s->lpe = z2(N + 2); s->lph = z2(N + 2);
+  s->lf1 = z2(N + 2); s->lf2 = z2(N + 2); s->lg1 = z2(N + 2); s->lg2 = z2(N + 2);
+  tf_line_profiles(s);
+  *id = 0;
+  return 0;
+}
+
+int ofdtd2d_get_tfsf_line(o2sim* s, int id, double* e, double* h) {
+  if (!s || !e || !h || id != 0 || !s->has_tf) return o2_fail(FDTD_ERR_ARG, "bad argument");
+  memcpy(e, s->le, sizeof(double) * (size_t)(s->ny + 2));
+  memcpy(h, s->lh, sizeof(double) * (size_t)(s->ny + 2));
+  return 0;
+}
+
+/* line E then H half step: tfsf_line_E / tfsf_line_H of the 3D oracle */
+static void tf_line_E(o2sim* s, double tnow) {
+  const int N = s->ny, lo = s->tf_lo, hi = s->tf_hi;
+  const double ck = s->ku * (1.0 / s->tfp.eps);
+  for (int j = lo + 1; j <= hi - 1; ++j) s->le[j] = s->le[j] + ck * (s->lh[j] - s->lh[j - 1]);
+  const double a = (tnow - s->tfp.pulse.t0) / s->tfp.pulse.st;
+  s->le[s->tfp.src_index] = s->le[s->tfp.src_index] +
+                            s->tfp.pulse.amplitude * (exp(-0.5 * (a * a)) * sin(s->tfp.pulse.w0 * tnow));
+  if (s->tfp.npml <= 0) return;
+  for (int j = 1; j <= lo; ++j) s->lpe[j] = s->lf1[j] * s->lpe[j] + s->lf2[j] * (s->lph[j] - s->lph[j - 1]);
+  s->lpe[lo + 1] = s->le[lo + 1];
+  s->le[lo] = s->lpe[lo];
+  for (int j = hi; j <= N; ++j) s->lpe[j] = s->lf1[j] * s->lpe[j] + s->lf2[j] * (s->lph[j] - s->lph[j - 1]);
+  s->lpe[hi - 1] = s->le[hi - 1];
+  s->le[hi] = s->lpe[hi];
+}
+
+static void tf_line_H(o2sim* s) {
+  const int N = s->ny, lo = s->tf_lo, hi = s->tf_hi;
+  for (int j = lo + 1; j <= hi - 1; ++j) s->lh[j] = s->lh[j] + s->ku * (s->le[j + 1] - s->le[j]);
+  if (s->tfp.npml <= 0) return;
+  for (int j = 1; j <= lo; ++j) s->lph[j] = s->lg1[j] * s->lph[j] + s->lg2[j] * (s->lpe[j + 1] - s->lpe[j]);
+  s->lph[lo + 1] = s->lh[lo + 1];
+  s->lh[lo] = s->lph[lo];
+  for (int j = hi; j <= N; ++j) s->lph[j] = s->lg1[j] * s->lph[j] + s->lg2[j] * (s->lpe[j + 1] - s->lpe[j]);
+  s->lph[hi - 1] = s->lh[hi - 1];
+  s->lh[hi] = s->lph[hi];
+}
+
+/* add v to D of row j (field 0) or to H (field 1: Hz for TEz, Hx for TMz)
+ * on every column; TMz PML cells take the E term on the split along y
+ * scaled by their PML medium, as the 3D oracle's plane_add does */
+static int tmz_in_pml(const o2sim* s, int i, int j);
+static void row_add(o2sim* s, int field, int j, double v) {
+  for (int i = 1; i <= s->nx; ++i) {
+    const long c = F2(s, i, j);
+    if (field == 1) {
+      double* h = s->tmz ? s->hx : s->hz;
+      h[c] = h[c] + v;
+    } else if (s->tmz && tmz_in_pml(s, i, j)) {
+      s->ezy[c] = s->ezy[c] + v * s->rz[c];
+      s->ez[c] = s->ezx[c] + s->ezy[c];
+    } else {
+      double* d = s->tmz ? s->dz : s->dx;
+      d[c] = d[c] + v;
+    }
+  }
+}
+
+static void tf_corr_D(o2sim* s) {
+  row_add(s, 0, s->tfp.lo, -s->ku * s->lh[s->tfp.lo - 1]);
+  if (s->tfp.hi > 0) row_add(s, 0, s->tfp.hi + 1, s->ku * s->lh[s->tfp.hi]);
+}
+static void tf_corr_H(o2sim* s) {
+  row_add(s, 1, s->tfp.lo - 1, -(s->tf_sg * s->ku) * s->le[s->tfp.lo]);
+  if (s->tfp.hi > 0) row_add(s, 1, s->tfp.hi, (s->tf_sg * s->ku) * s->le[s->tfp.hi + 1]);
+}
+
 int ofdtd2d_add_probe(o2sim* s, int i, int j, int* id) {
   if (!s || !id) return o2_fail(FDTD_ERR_ARG, "null argument");
   if (s->committed) return o2_fail(FDTD_ERR_STATE, "set-up after commit");
@@ -582,6 +722,12 @@ static int tmz_init(o2sim* s) {
 int ofdtd2d_commit(o2sim* s) {
   if (!s) return o2_fail(FDTD_ERR_ARG, "null argument");
   if (s->committed) return 0;
+  if (s->has_tf && s->has_pml) {
+    const int P = s->pml.npml[1];
+    const int top = s->tfp.hi > 0 ? s->tfp.hi + 1 : s->tfp.lo;
+    if (s->tfp.lo - 1 <= P || top >= s->ny - P + (s->tmz ? 1 : 0))
+      return o2_fail(FDTD_ERR_ARG, "TFSF rows inside the PML");
+  }
   if (s->tmz) {
     int rc = tmz_init(s);
     if (rc) return rc;
@@ -836,6 +982,10 @@ static void tmz_step(o2sim* s, int n) {
       }
     }
   }
+  if (s->has_tf) {
+    tf_line_E(s, t);
+    tf_corr_D(s);
+  }
   tmz_materials(s);
   if (s->g.periodic_x) wrap_x(s, ez);
   for (int j = 1; j <= ny; ++j)
@@ -849,6 +999,10 @@ static void tmz_step(o2sim* s, int n) {
         hy[c] = g1x[i] * hy[c] + g2x[i] * (-ez[c] + ez[c + 1]);
       }
     }
+  if (s->has_tf) {
+    tf_line_H(s);
+    tf_corr_H(s);
+  }
   if (s->g.periodic_x) wrap_x(s, hy);
   monitors(s, n);
 }
@@ -887,6 +1041,10 @@ int ofdtd2d_step(o2sim* s, int nsteps) {
       const double g = pulse_at(&s->srcs[q].p, t);
       for (int i = 1; i <= nx; ++i) s->dx[F2(s, i, s->srcs[q].j)] = s->dx[F2(s, i, s->srcs[q].j)] + g;
     }
+    if (s->has_tf) {
+      tf_line_E(s, t);
+      tf_corr_D(s);
+    }
     update_materials(s);
     if (s->has_pml) pml_E(s);
     if (s->g.periodic_x) wrap_x(s, s->ey);
@@ -898,6 +1056,10 @@ int ofdtd2d_step(o2sim* s, int nsteps) {
         s->hz[c] = s->hz[c] - ku * curlE;
       }
     if (s->has_pml) pml_H(s);
+    if (s->has_tf) {
+      tf_line_H(s);
+      tf_corr_H(s);
+    }
     if (s->g.periodic_x) wrap_x(s, s->hz);
     monitors(s, n);
     s->n_done = n;

@@ -85,10 +85,10 @@ def test_struct_layouts_match_the_header():
 
     from fdtd_amd import abi2d
 
-    code = ('#include "fdtd2d.h"\n#include <stdio.h>\nint main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n",'
+    code = ('#include "fdtd2d.h"\n#include <stdio.h>\nint main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n",'
             "sizeof(fdtd_grid),sizeof(fdtd_pml),sizeof(fdtd_region),sizeof(fdtd_drude2cp),"
             "sizeof(fdtd_graphene),sizeof(fdtd_pulse),sizeof(fdtd2d_grid),sizeof(fdtd2d_pml),"
-            "sizeof(fdtd2d_region),sizeof(fdtd_lorentz_jp),sizeof(fdtd_tfsf));return 0;}\n")
+            "sizeof(fdtd2d_region),sizeof(fdtd_lorentz_jp),sizeof(fdtd_tfsf),sizeof(fdtd2d_tfsf));return 0;}\n")
     with tempfile.TemporaryDirectory() as t:
         src = os.path.join(t, "p.c")
         open(src, "w").write(code)
@@ -96,5 +96,5 @@ def test_struct_layouts_match_the_header():
         subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), src, "-o", exe], check=True)
         sizes = [int(v) for v in subprocess.run([exe], capture_output=True, text=True).stdout.split()]
     ours = [C.sizeof(t) for t in (abi.Grid, abi.Pml, abi.Region, abi.Drude2CP, abi.Graphene, abi.Pulse,
-                                  abi2d.Grid2D, abi2d.Pml2D, abi2d.Region2D, abi.LorentzJp, abi.Tfsf)]
+                                  abi2d.Grid2D, abi2d.Pml2D, abi2d.Region2D, abi.LorentzJp, abi.Tfsf, abi2d.Tfsf2D)]
     assert sizes == ours
