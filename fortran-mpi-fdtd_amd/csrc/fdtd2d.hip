@@ -86,6 +86,17 @@ struct AdeDev {
   AdeCoef<double> co;
 };
 
+// Lorentz-Jp cells of one coefficient group: local component 0 = x (TEz) or
+// z (TMz), 1 = y (TEz); FP64 pole state [word][comp][cell], words P_l
+// (l < npols), Jp_l, [Q, G].
+struct LjpDev {
+  const int32_t* cells;
+  const uint8_t* mask;
+  double* state;
+  int32_t ncells;
+  LjpCoef<double> co;
+};
+
 struct DftDev {
   int32_t j, nfreq, mask, n_begin, n_end, nslots;  // slots: Ex first, then Hz
   const double* omega;
@@ -115,6 +126,8 @@ struct Inst {
   real* src_val;
   int32_t ngroup;
   AdeDev<real> grp[kMaxGroup];
+  int32_t nljp;
+  LjpDev ljp[kMaxGroup];
   int32_t ndft;
   DftDev dft[kMaxDft];
   int32_t nprobe;
@@ -138,13 +151,30 @@ struct Inst {
 
 // ---- host state ------------------------------------------------------------------
 struct HostMat {
-  int type;  // 0 dielectric, 1 drude2cp, 2 graphene
+  int type;  // 0 dielectric, 1 drude2cp, 2 graphene, 3 Lorentz-Jp
   fdtd2d_region reg;
   double eps_r;
   fdtd_drude2cp d2;
   fdtd_graphene gr;
+  fdtd_lorentz_jp lj;
   int writes[2];
   int group;
+};
+
+// whether material m updates local component q (TEz 0 = x, 1 = y; TMz 0 =
+// z) through its Lorentz-Jp poles
+bool ljp_comp(const HostMat& m, bool tmz, int q) {
+  if (m.type != 3) return false;
+  return tmz ? (q == 0 && ((m.lj.jp_mask >> 2) & 1)) : ((m.lj.jp_mask >> q) & 1) != 0;
+}
+
+struct HostLjp {
+  LjpCoef<double> co;
+  int nstate = 0;
+  int64_t ncells = 0;
+  int32_t* d_cells = nullptr;
+  uint8_t* d_mask = nullptr;
+  double* d_state = nullptr;
 };
 
 struct HostGroup {
@@ -192,6 +222,7 @@ struct Sim {
   void* d_pml[2][4] = {{nullptr}};
   void* d_prof[2] = {nullptr, nullptr};
   std::vector<HostGroup> groups;
+  std::vector<HostLjp> ljps;
   void* d_srcval = nullptr;
   double* d_probe = nullptr;
   int probe_cap = 0;
@@ -219,6 +250,8 @@ struct Batch {
   int32_t* d_ctl = nullptr;  // [count][2]
   int32_t* d_tile_inst = nullptr;
   int4* d_ade = nullptr;     // (q, group, first cell, 0)
+  int4* d_ljp = nullptr;     // (q, Lorentz-Jp group, first cell, 0)
+  int nljp = 0;
   int4* d_mon = nullptr;     // (q, line or -1 for probes, first i, 0)
   int2* d_lines = nullptr;   // (q, line)
   int32_t* d_tfl = nullptr;  // simulations with a TFSF
@@ -599,6 +632,89 @@ __global__ __launch_bounds__(256) void k2_ade(const Inst<real>* __restrict__ I, 
       if (p >= ncp) break;
       G.state[(int64_t)((2 + p) * 2 + q) * n + e] = (co.gbr[p] * gr[p] - co.gbi[p] * gi[p]) + co.gcr[p] * ev;
       G.state[(int64_t)((2 + ncp + p) * 2 + q) * n + e] = (co.gbr[p] * gi[p] + co.gbi[p] * gr[p]) + co.gci[p] * ev;
+    }
+  }
+}
+
+// Lorentz-Jp cells, E form (mod2/filmGoldLorentzJp_mod.f90:35-75,
+// siLorentzJp_mod.f90, graphene3LorentzJp1nm_mod.f90: E_q += Ku*ga*(curl_q -
+// ds*JpT_q), then the poles from the new E).  E is read as the previous
+// step left it (k2_e does not write these components), the update runs in
+// FP64, the line source of the row adds ga*g.
+template <typename real>
+__global__ __launch_bounds__(256) void k2_ljp(const Inst<real>* __restrict__ I, const int4* __restrict__ work,
+                                              int nwork) {
+  const int w = blockIdx.x;
+  if (w >= nwork) return;
+  const int4 wk = work[w];
+  const Inst<real>& S = I[wk.x];
+  if (S.ctl[0] >= S.ctl[1]) return;
+  const LjpDev& G = S.ljp[wk.y];
+  const int e = wk.z + (int)threadIdx.x;
+  if (e >= G.ncells) return;
+  const int n = G.ncells;
+  const int c = G.cells[e];
+  const int mask = G.mask[e];
+  const LjpCoef<double>& co = G.co;
+  const int px = S.px;
+  const int i = c % px, j = c / px;
+  real cur[2];
+  real* E[2];
+  if (S.tmz) {
+    cur[0] = (S.hy[c] - S.hy[c - 1]) + (-S.hx[c] + S.hx[c - px]);  // curlhyx + curlhxy
+    cur[1] = 0;
+    E[0] = S.ez;
+    E[1] = nullptr;
+  } else {
+    cur[0] = S.hz[c] - S.hz[c - px];   // curlhzy
+    cur[1] = -S.hz[c] + S.hz[c - 1];   // curlhzx
+    E[0] = S.ex;
+    E[1] = S.ey;
+  }
+  real sv = 0;
+  bool hit = false;  // the line source drives local component 0 (Dx or Dz)
+  for (int s = 0; s < S.nsrc; ++s)
+    if (j == S.src_j[s]) {
+      sv += S.src_val[s];
+      hit = true;
+    }
+  const int np = co.npols;
+  const int wq = 2 * np;  // word of Q
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    if (!(mask & (1 << q))) continue;
+    double jt = 0;
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      if (l >= np) break;
+      jt = jt + G.state[(int64_t)((np + l) * 2 + q) * n + e];
+    }
+    double ev = (double)E[q][c] + co.kuga * ((double)cur[q] - co.ds * jt);
+    if (q == 0 && hit) ev = ev + co.ga * (double)sv;
+    const real er = (real)ev;
+    E[q][c] = er;
+    const bool wrap = S.per && (S.tmz || q == 1);  // Ey (TEz) and Ez (TMz) wrap in x
+    if (wrap && i == 1) E[q][c + S.nx] = er;
+    if (wrap && i == S.nx) E[q][c - S.nx] = er;
+    double Qv = 0;
+    if (co.nl) Qv = G.state[(int64_t)(wq * 2 + q) * n + e];
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      if (l >= np) break;
+      double* Pp = G.state + (int64_t)(l * 2 + q) * n + e;
+      double* Jp = G.state + (int64_t)((np + l) * 2 + q) * n + e;
+      double p0 = (co.xi[l] * ev) / co.c0;
+      if (l == 0 && co.nl) p0 = p0 + ((co.xi3 * Qv) * ev) / co.c0;
+      const double jn = co.g1[l] * *Jp + co.g2[l] * (p0 - *Pp);
+      *Jp = jn;
+      *Pp = *Pp + co.dt * jn;
+    }
+    if (co.nl) {
+      double* Gp = G.state + (int64_t)((wq + 1) * 2 + q) * n + e;
+      const double ze = co.z0 * ev;
+      const double gn = co.ng1 * *Gp + co.ng2 * (ze * ze - Qv);
+      *Gp = gn;
+      G.state[(int64_t)(wq * 2 + q) * n + e] = Qv + co.dt * gn;
     }
   }
 }
@@ -1034,8 +1150,32 @@ int commit_impl(Sim* s) {
   }
   // dispersive groups (identical coefficients share one)
   s->groups.clear();
+  s->ljps.clear();
   for (auto& m : s->mats) {
-    if (m.type == 0) continue;
+    if (m.type != 3) continue;
+    LjpCoef<double> c;
+    lorentz_jp_coef(s->dt, s->ds, m.lj, c);
+    if (s->tmz) {  // local component 0 is z; Ez's JpT_x (ez_uses_jx) stays 0 here
+      c.jmask = (m.lj.jp_mask >> 2) & 1;
+    } else {
+      c.jmask = m.lj.jp_mask & 3;
+      c.ezjx = 0;
+    }
+    int gi = -1;
+    for (size_t q = 0; q < s->ljps.size(); ++q)
+      if (std::memcmp(&s->ljps[q].co, &c, sizeof c) == 0) gi = (int)q;
+    if (gi < 0) {
+      if ((int)s->ljps.size() >= kMaxGroup) return set_error(FDTD_ERR_UNSUPPORTED, "too many Lorentz-Jp materials");
+      HostLjp g;
+      g.co = c;
+      g.nstate = 2 * c.npols + (c.nl ? 2 : 0);
+      s->ljps.push_back(g);
+      gi = (int)s->ljps.size() - 1;
+    }
+    m.group = gi;
+  }
+  for (auto& m : s->mats) {
+    if (m.type == 0 || m.type == 3) continue;
     AdeCoef<double> c;
     if (m.type == 1) {
       drude2cp_coef(s, m.d2, c);
@@ -1102,6 +1242,7 @@ int commit_impl(Sim* s) {
         const HostMat& m = s->mats[w];
         tab[q].act[0] = m.type == 0 ? ACT_DIEL : ACT_ADE;
         if (m.type == 0) tab[q].gax[0] = m.eps_r == 0.0 ? 0.0 : 1.0 / m.eps_r;
+        if (m.type == 3 && !ljp_comp(m, true, 0)) tab[q].act[0] = ACT_DIEL;  // E = D
       }
       if (d >= 0) tab[q].gax[1] = s->mats[d].eps_r == 0.0 ? 0.0 : 1.0 / s->mats[d].eps_r;
     }
@@ -1116,6 +1257,9 @@ int commit_impl(Sim* s) {
       if (m.type == 0) {
         tab[q].act[c] = ACT_DIEL;
         tab[q].gax[c] = m.eps_r == 0.0 ? 0.0 : 1.0 / m.eps_r;  // simpledielectric_mod.f90:66-75
+      } else if (m.type == 3 && !ljp_comp(m, false, c)) {
+        tab[q].act[c] = ACT_DIEL;  // a component outside jp_mask: E = D
+        tab[q].gax[c] = 1.0;
       } else if (m.type == 2 && c == 1) {
         tab[q].act[c] = ACT_DIEL;  // graphene: Ey = Dy (graphene_mod.f90:68)
         tab[q].gax[c] = 1.0;
@@ -1136,19 +1280,39 @@ int commit_impl(Sim* s) {
   // dispersive cell lists: cells must lie strictly between the slabs (the
   // slabs overwrite Ex there after the materials, which a separate launch
   // for dispersive cells would undo)
-  std::vector<std::vector<int32_t>> gcells(s->groups.size());
-  std::vector<std::vector<uint8_t>> gmask(s->groups.size());
+  std::vector<std::vector<int32_t>> gcells(s->groups.size()), lcells(s->ljps.size());
+  std::vector<std::vector<uint8_t>> gmask(s->groups.size()), lmask(s->ljps.size());
   for (int j = 1; j <= ny; ++j)
     for (int i = 1; i <= nx; ++i) {
       const int64_t c = i + s->px * j;
       const Key& key = combos[cid[(size_t)c]];
-      int gm[2] = {-1, -1};
+      int gm[2] = {-1, -1}, lm[2] = {-1, -1};
       if (s->tmz) {
-        if (key[0] >= 0 && s->mats[key[0]].type != 0) gm[0] = s->mats[key[0]].group;
+        if (key[0] >= 0 && (s->mats[key[0]].type == 1 || s->mats[key[0]].type == 2)) gm[0] = s->mats[key[0]].group;
+        if (key[0] >= 0 && ljp_comp(s->mats[key[0]], true, 0)) lm[0] = s->mats[key[0]].group;
       } else {
-        for (int q = 0; q < 2; ++q)
-          if (key[q] >= 0 && s->mats[key[q]].type != 0 && !(s->mats[key[q]].type == 2 && q == 1))
-            gm[q] = s->mats[key[q]].group;
+        for (int q = 0; q < 2; ++q) {
+          if (key[q] < 0) continue;
+          const HostMat& m = s->mats[key[q]];
+          if ((m.type == 1 || m.type == 2) && !(m.type == 2 && q == 1)) gm[q] = m.group;
+          if (ljp_comp(m, false, q)) lm[q] = m.group;
+        }
+      }
+      if (lm[0] >= 0 || lm[1] >= 0) {
+        if (s->tmz ? ((s->Px > 0 && (i <= s->Px || i > nx - s->Px)) || (s->P > 0 && (j <= s->P || j > ny - s->P)))
+                   : (s->has_pml && (j <= s->ja || j >= s->jb)))
+          return set_error(FDTD_ERR_UNSUPPORTED, "Lorentz-Jp cells inside the PML");
+        if (s->has_tf && (j == s->tfp.lo || (s->tfp.hi > 0 && j == s->tfp.hi + 1)))
+          return set_error(FDTD_ERR_UNSUPPORTED, "a TFSF row through Lorentz-Jp cells");
+        for (size_t g = 0; g < s->ljps.size(); ++g) {
+          uint8_t mk = 0;
+          for (int q = 0; q < 2; ++q)
+            if (lm[q] == (int)g) mk |= (uint8_t)(1 << q);
+          if (mk) {
+            lcells[g].push_back((int32_t)c);
+            lmask[g].push_back(mk);
+          }
+        }
       }
       if (gm[0] < 0 && gm[1] < 0) continue;
       if (s->tmz && ((s->Px > 0 && (i <= s->Px || i > nx - s->Px)) || (s->P > 0 && (j <= s->P || j > ny - s->P))))
@@ -1165,6 +1329,15 @@ int commit_impl(Sim* s) {
         }
       }
     }
+  for (size_t g = 0; g < s->ljps.size(); ++g) {
+    HostLjp& G = s->ljps[g];
+    G.ncells = (int64_t)lcells[g].size();
+    if (!G.ncells) continue;
+    G.d_cells = (int32_t*)upload(s->allocs, &s->dev_bytes, lcells[g].data(), lcells[g].size() * sizeof(int32_t));
+    G.d_mask = (uint8_t*)upload(s->allocs, &s->dev_bytes, lmask[g].data(), lmask[g].size());
+    G.d_state = (double*)alloc_in(s->allocs, &s->dev_bytes, (size_t)G.nstate * 2 * (size_t)G.ncells * sizeof(double));
+    if (!G.d_cells || !G.d_mask || !G.d_state) return set_error(FDTD_ERR_NOMEM, "Lorentz-Jp state");
+  }
   for (size_t g = 0; g < s->groups.size(); ++g) {
     HostGroup& G = s->groups[g];
     G.ncells = (int64_t)gcells[g].size();
@@ -1271,6 +1444,15 @@ Inst<real> make_inst(Sim* s, int32_t* ctl, int tile0) {
     A.nstate = G.nstate;
     A.co = G.co;
   }
+  d.nljp = (int32_t)s->ljps.size();
+  for (size_t g = 0; g < s->ljps.size(); ++g) {
+    const HostLjp& G = s->ljps[g];
+    d.ljp[g].cells = G.d_cells;
+    d.ljp[g].mask = G.d_mask;
+    d.ljp[g].state = G.d_state;
+    d.ljp[g].ncells = (int32_t)G.ncells;
+    d.ljp[g].co = G.co;
+  }
   d.ndft = (int32_t)s->dfts.size();
   for (size_t l = 0; l < s->dfts.size(); ++l) {
     const HostDft& h = s->dfts[l];
@@ -1329,7 +1511,7 @@ int batch_build(Batch* b) {
   b->d_ctl = (int32_t*)alloc_in(b->allocs, nullptr, sizeof(int32_t) * 2 * count);
   if (!b->d_inst || !b->d_ctl) return set_error(FDTD_ERR_NOMEM, "batch descriptors");
   std::vector<int32_t> tile_inst, tfl;
-  std::vector<int4> ade, mon;
+  std::vector<int4> ade, mon, ljp;
   std::vector<int2> lines;
   for (int q = 0; q < count; ++q) {
     Sim* s = b->sims[q];
@@ -1337,6 +1519,8 @@ int batch_build(Batch* b) {
     tile_inst.insert(tile_inst.end(), (size_t)nt, q);
     for (size_t g = 0; g < s->groups.size(); ++g)
       for (int64_t c0 = 0; c0 < s->groups[g].ncells; c0 += kChunk) ade.push_back(make_int4(q, (int)g, (int)c0, 0));
+    for (size_t g = 0; g < s->ljps.size(); ++g)
+      for (int64_t c0 = 0; c0 < s->ljps[g].ncells; c0 += kChunk) ljp.push_back(make_int4(q, (int)g, (int)c0, 0));
     for (size_t l = 0; l < s->dfts.size(); ++l) {
       for (int i0 = 0; i0 < s->nx; i0 += kChunk) mon.push_back(make_int4(q, (int)l, i0, 0));
       lines.push_back(make_int2(q, (int)l));
@@ -1353,8 +1537,10 @@ int batch_build(Batch* b) {
   b->d_mon = (int4*)upload(b->allocs, nullptr, mon.data(), mon.size() * sizeof(int4));
   b->d_lines = (int2*)upload(b->allocs, nullptr, lines.data(), lines.size() * sizeof(int2));
   b->ntf = (int)tfl.size();
+  b->nljp = (int)ljp.size();
+  b->d_ljp = (int4*)upload(b->allocs, nullptr, ljp.data(), ljp.size() * sizeof(int4));
   b->d_tfl = (int32_t*)upload(b->allocs, nullptr, tfl.data(), tfl.size() * sizeof(int32_t));
-  if (!b->d_tile_inst || !b->d_ade || !b->d_mon || !b->d_lines || !b->d_tfl)
+  if (!b->d_tile_inst || !b->d_ade || !b->d_mon || !b->d_lines || !b->d_tfl || !b->d_ljp)
     return set_error(FDTD_ERR_NOMEM, "batch work lists");
   b->ctl.assign(2 * (size_t)count, 0);
   for (int q = 0; q < count; ++q) b->ctl[2 * q] = b->ctl[2 * q + 1] = b->sims[q]->n_done;
@@ -1379,6 +1565,7 @@ void enqueue_block(Batch* b) {
     if (b->ntf) hipLaunchKernelGGL(k2_tfsf_line<real>, dim3((unsigned)b->ntf), dim3(256), 0, st, I, b->d_tfl, b->ntf, 0);
     hipLaunchKernelGGL(k2_e<real>, dim3(grid), dim3(kTX * kWaves), 0, st, I, b->d_tile_inst, b->ntiles);
     if (b->nade) hipLaunchKernelGGL(k2_ade<real>, dim3((unsigned)b->nade), dim3(kChunk), 0, st, I, b->d_ade, b->nade);
+    if (b->nljp) hipLaunchKernelGGL(k2_ljp<real>, dim3((unsigned)b->nljp), dim3(kChunk), 0, st, I, b->d_ljp, b->nljp);
     hipLaunchKernelGGL(k2_h<real>, dim3(grid), dim3(kTX * kWaves), 0, st, I, b->d_tile_inst, b->ntiles);
     // k2_h read the line's new e; its H half step follows
     if (b->ntf) hipLaunchKernelGGL(k2_tfsf_line<real>, dim3((unsigned)b->ntf), dim3(256), 0, st, I, b->d_tfl, b->ntf, 1);
@@ -1609,6 +1796,18 @@ int fdtd2d_add_graphene(fdtd2d_sim* s, const fdtd2d_region* r, const fdtd_graphe
   m.type = 2;
   m.gr = *p;
   m.writes[0] = m.writes[1] = 1;  // Ex through the ADE, Ey = Dy
+  return add_mat(s, r, m);
+}
+
+int fdtd2d_add_lorentz_jp(fdtd2d_sim* s, const fdtd2d_region* r, const fdtd_lorentz_jp* p) {
+  GUARD2(s);
+  SETUP2(s);
+  if (!p || p->npols < 1 || p->npols > 4 || p->jp_mask < 1 || p->jp_mask > 7 || !(p->eps_inf > 0))
+    return set_error(FDTD_ERR_ARG, "bad Lorentz-Jp parameters");
+  HostMat m{};
+  m.type = 3;
+  m.lj = *p;
+  m.writes[0] = m.writes[1] = 1;
   return add_mat(s, r, m);
 }
 

@@ -217,17 +217,16 @@ bool is_ade_comp(const HostMaterial& m, int q) {
   return true;
 }
 
-void ljp_coef(const Sim* s, const fdtd_lorentz_jp& p, LjpCoef<double>& c) {
+void ljp_coef_dt(double dt, double ds, const fdtd_lorentz_jp& p, LjpCoef<double>& c) {
   const Consts& K = consts();
-  const double dt = s->dt;
   std::memset(&c, 0, sizeof c);
   c.npols = p.npols;
   c.jmask = p.jp_mask;
   c.nl = p.nonlinear ? 1 : 0;
   c.ezjx = p.ez_uses_jx ? 1 : 0;
   c.ga = 1.0 / p.eps_inf;
-  c.kuga = (dt * K.c0 / s->ds) * c.ga;
-  c.ds = s->ds;
+  c.kuga = (dt * K.c0 / ds) * c.ga;
+  c.ds = ds;
   c.dt = dt;
   c.c0 = K.c0;
   c.z0 = std::sqrt(K.mu0 / K.ep0);
@@ -582,7 +581,7 @@ int commit_impl(Sim* s) {
     if (m.type == 0) continue;
     if (m.type == 3) {
       LjpCoef<double> c;
-      ljp_coef(s, m.lj, c);
+      ljp_coef_dt(s->dt, s->ds, m.lj, c);
       int gi = -1;
       for (size_t q = 0; q < s->groups.size(); ++q)
         if (s->groups[q].kind == 1 && std::memcmp(&s->groups[q].lj, &c, sizeof c) == 0) gi = (int)q;
@@ -890,6 +889,8 @@ void* field_ptr(Sim* s, int comp) {
 }
 
 }  // namespace
+
+void lorentz_jp_coef(double dt, double ds, const fdtd_lorentz_jp& p, LjpCoef<double>& c) { ljp_coef_dt(dt, ds, p, c); }
 
 void tfsf_line_profiles(double dt, double ds, int n, int npml, double eps, double order, double R0, double* out) {
   tfsf_profiles_dt(dt, ds, n, npml, eps, order, R0, out);

@@ -266,6 +266,10 @@ int graphene3d_coef(double dt, double ds, const fdtd_graphene& p, AdeCoef<double
 void pml_grade(double dt, double ds, double order, double sigmam, int P, double ep, int n, double* f1, double* f2,
                double* g1, double* g2);
 
+// Lorentz-Jp coefficients (mod3/goldLorentzJp_mod.f90:37-91,141-163 and the
+// family) for time step dt and cell size ds
+void lorentz_jp_coef(double dt, double ds, const fdtd_lorentz_jp& p, LjpCoef<double>& c);
+
 // end-slab profiles of a TFSF incident line of n cells (mod1/
 // BERpmly_ExHz_mod.f90:156-255): f1, f2, g1, g2, n+2 doubles each, into out
 void tfsf_line_profiles(double dt, double ds, int n, int npml, double eps, double order, double R0, double* out);

@@ -1,7 +1,7 @@
 """ctypes mirror of include/fdtd2d.h (2D TEz / TMz stepper and batches)."""
 import ctypes as C
 
-from .abi import Drude2CP, FdtdError, Graphene, Pulse  # noqa: F401
+from .abi import Drude2CP, FdtdError, Graphene, LorentzJp, Pulse  # noqa: F401
 
 TEZ, TMZ = 0, 1
 EX2, EY2, HZ2, DX2, DY2, EZ2, HX2, HY2, DZ2 = range(9)
@@ -42,6 +42,7 @@ PROTOTYPES_2D = {
     "add_dielectric": (_I, [_P, C.POINTER(Region2D), _D]),
     "add_drude2cp": (_I, [_P, C.POINTER(Region2D), C.POINTER(Drude2CP)]),
     "add_graphene": (_I, [_P, C.POINTER(Region2D), C.POINTER(Graphene)]),
+    "add_lorentz_jp": (_I, [_P, C.POINTER(Region2D), C.POINTER(LorentzJp)]),
     "add_line_source": (_I, [_P, _I, C.POINTER(Pulse)]),
     "add_dft_line": (_I, [_P, _I, _I, _PD, _I, _I, _I, C.POINTER(_I)]),
     "add_probe": (_I, [_P, _I, _I, C.POINTER(_I)]),

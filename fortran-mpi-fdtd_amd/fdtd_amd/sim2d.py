@@ -26,6 +26,7 @@ def library2d():
         from . import library
         b = library()
         _binding2d = abi2d.Binding2D(b.dll, "fdtd2d_", b.last_error)
+        _binding2d.preset_binding = b  # fdtd_lorentz_jp_preset
     return _binding2d
 
 
@@ -96,6 +97,13 @@ class Simulation2D:
     def add_graphene(self, region, params):
         p = _graphene(params)
         self.b.check(self.b.add_graphene(self.h, C.byref(_region2d(region)), C.byref(p)))
+
+    def add_lorentz_jp(self, region, params):
+        """A Lorentz-Jp material: params as fdtd_amd.lorentz_jp takes them
+        (a preset name plus wNL, dnuNL, xi3 for the graphene kinds)."""
+        from . import lorentz_jp
+        p = lorentz_jp(params, getattr(self.b, "preset_binding", None))
+        self.b.check(self.b.add_lorentz_jp(self.h, C.byref(_region2d(region)), C.byref(p)))
 
     def add_line_source(self, j, t0, st, w0, amplitude=1.0):
         p = abi.Pulse(t0, st, w0, amplitude)
@@ -228,6 +236,8 @@ def apply_spec2d(sim, spec):
             sim.add_drude2cp(m["region"], m.get("params") or gold_params_dict())
         elif m["type"] == "graphene":
             sim.add_graphene(m["region"], m["params"])
+        elif m["type"] == "lorentz_jp":
+            sim.add_lorentz_jp(m["region"], m["params"])
         else:
             raise ValueError(m["type"])
     for s in spec.get("sources", []):

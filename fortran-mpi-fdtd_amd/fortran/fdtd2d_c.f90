@@ -87,6 +87,12 @@ module fdtd2d_c
             integer(c_int), value :: i, j
             integer(c_int), intent(out) :: id
         end function
+        integer(c_int) function fdtd2d_add_lorentz_jp(sim, region, p) bind(c, name="fdtd2d_add_lorentz_jp")
+            import :: c_int, c_ptr, fdtd2d_region, fdtd_lorentz_jp
+            type(c_ptr), value :: sim
+            type(fdtd2d_region), intent(in) :: region
+            type(fdtd_lorentz_jp), intent(in) :: p
+        end function
         integer(c_int) function fdtd2d_add_tfsf(sim, t, id) bind(c, name="fdtd2d_add_tfsf")
             import :: c_int, c_ptr, fdtd2d_tfsf
             type(c_ptr), value :: sim

@@ -119,6 +119,14 @@ int fdtd2d_add_drude2cp(fdtd2d_sim* sim, const fdtd2d_region* region, const fdtd
  * sets its exponentials before use).  TMz: Ez through the ADE with the 3D
  * module's coefficients.  Dispersive cells must lie outside the PML.      */
 int fdtd2d_add_graphene(fdtd2d_sim* sim, const fdtd2d_region* region, const fdtd_graphene* params);
+/* <material>_LJp%initialize_material + update_EJp of the 2D Lorentz-Jp
+ * modules (mod2/siLorentzJp_mod.f90, filmGoldLorentzJp_mod.f90,
+ * graphene3LorentzJp1nm_mod.f90; the pole tables of fdtd_lorentz_jp_preset,
+ * which the 2D and 3D modules share): E form from curl H, no D --
+ * E_q += Ku*ga*(curl_q - ds*JpT_q) for q in jp_mask (TEz: x, y; TMz: z),
+ * then the poles; the other components take E = D.  A line source through
+ * these cells adds ga*g.  Cells must lie outside the PML.                  */
+int fdtd2d_add_lorentz_jp(fdtd2d_sim* sim, const fdtd2d_region* region, const fdtd_lorentz_jp* params);
 /* Soft line source on Dx along row j, all x (sim2/program.f90:336-340):
  * Dx(i,j) += amplitude*exp(-((t-t0)/st)^2/2)*sin(w0*t), t = (n-1)*dt.
  * TMz: on Dz; in PML cells on the split Ezy, scaled by the PML medium.    */

@@ -47,6 +47,7 @@ def binding2d():
         err.restype = C.c_char_p
         err.argtypes = []
         _b2 = abi2d.Binding2D(dll, "ofdtd2d_", err, batches=False)
+        _b2.preset_binding = binding()  # ofdtd_lorentz_jp_preset
     return _b2
 
 

@@ -64,7 +64,7 @@ static int o2_fail(int code, const char* msg) {
 }
 const char* ofdtd2d_last_error(void) { return o2_err; }
 
-enum { M2_DIEL = 0, M2_DRUDE2CP = 1, M2_GRAPHENE = 2 };
+enum { M2_DIEL = 0, M2_DRUDE2CP = 1, M2_GRAPHENE = 2, M2_LJP = 3 };
 
 typedef struct {
   int type;
@@ -80,6 +80,10 @@ typedef struct {
   double *Ix, *Sx, *Iy, *Sy;
   double complex *gx[4], *gy[4];
   double *sn, *snm1, *snm2;
+  /* Lorentz-Jp (the mod2 LorentzJp modules): per component x, y, z */
+  fdtd_lorentz_jp lj;
+  double lj_ga;
+  double *P[4][3], *Jp[4][3], *JpT[3], *Q[3], *G[3];
 } o2mat;
 
 typedef struct {
@@ -132,6 +136,8 @@ struct o2sim {
   int tf_lo, tf_hi; /* line slab bounds: npml and ny-npml+1 */
   double tf_sg;
   double *le, *lh, *lpe, *lph, *lf1, *lf2, *lg1, *lg2;
+  /* E before the material pass, for the E-form Lorentz-Jp update */
+  double *eprev[3];
 };
 typedef struct o2sim o2sim;
 
@@ -175,7 +181,12 @@ int ofdtd2d_destroy(o2sim* s) {
     free(m->inside); free(m->Ix); free(m->Sx); free(m->Iy); free(m->Sy);
     for (int p = 0; p < 4; ++p) { free(m->gx[p]); free(m->gy[p]); }
     free(m->sn); free(m->snm1); free(m->snm2);
+    for (int c = 0; c < 3; ++c) {
+      for (int l = 0; l < 4; ++l) { free(m->P[l][c]); free(m->Jp[l][c]); }
+      free(m->JpT[c]); free(m->Q[c]); free(m->G[c]);
+    }
   }
+  for (int c = 0; c < 3; ++c) free(s->eprev[c]);
   free(s->mats);
   free(s->srcs);
   for (int q = 0; q < s->ndfts; ++q) {
@@ -388,6 +399,90 @@ int ofdtd2d_add_graphene(o2sim* s, const fdtd2d_region* r, const fdtd_graphene* 
   const long nb = (long)m->bx * m->by;
   m->sn = z2(nb); m->snm1 = z2(nb); m->snm2 = z2(nb);
   return 0;
+}
+
+/* <material>_LJp%initialize_material (mod2/filmGoldLorentzJp_mod.f90:79-122
+ * and siblings): ga = 1/epinf over the region, state zero */
+int ofdtd2d_add_lorentz_jp(o2sim* s, const fdtd2d_region* r, const fdtd_lorentz_jp* p) {
+  if (!s || !r || !p) return o2_fail(FDTD_ERR_ARG, "null argument");
+  if (s->committed) return o2_fail(FDTD_ERR_STATE, "set-up after commit");
+  if (p->npols < 1 || p->npols > 4 || p->jp_mask < 1 || p->jp_mask > 7 || !(p->eps_inf > 0))
+    return o2_fail(FDTD_ERR_ARG, "bad Lorentz-Jp parameters");
+  o2mat* m = new_mat(s, r, M2_LJP);
+  if (!m) return o2_fail(FDTD_ERR_ARG, "bad region");
+  m->lj = *p;
+  m->lj_ga = 1.0 / p->eps_inf;
+  const long nb = (long)m->bx * m->by;
+  for (int c = 0; c < 3; ++c) {
+    for (int l = 0; l < p->npols; ++l) { m->P[l][c] = z2(nb); m->Jp[l][c] = z2(nb); }
+    m->JpT[c] = z2(nb); m->Q[c] = z2(nb); m->G[c] = z2(nb);
+  }
+  return 0;
+}
+
+static double pulse_at(const fdtd_pulse* p, double t);
+
+/* update_EJp of one cell, mod2/filmGoldLorentzJp_mod.f90:35-75 (and the
+ * nonlinear pole of graphene3LorentzJp1nm_mod.f90:52-75) in the 3D oracle's
+ * form: for each component q in jp_mask, E_q = Eprev_q + Ku*ga*(curl_q -
+ * ds*JpT_q) (+ ga*g on a source row), then the poles from the new E; the
+ * other components take E = D.  cur, D, E: per component x, y, z (TMz uses
+ * z only, TEz x and y). */
+static void ljp_cell2(o2sim* s, o2mat* m, long b, long c, int j, const int* comps, int nc, const double* cur,
+                      double** D, double** E, double tnow) {
+  const fdtd_lorentz_jp* p = &m->lj;
+  const double dt = s->dt;
+  const double z0 = sqrt(O2_MU0 / O2_EP0);
+  const double Ku = s->dt * O2_C0 / s->ds;
+  double e[3] = {0, 0, 0};
+  for (int t = 0; t < nc; ++t) {
+    const int q = comps[t];
+    if (!((p->jp_mask >> q) & 1)) {
+      e[q] = D[q][c] * 1.0;
+      E[q][c] = e[q];
+      continue;
+    }
+    const double jt = (q == 2 && p->ez_uses_jx) ? m->JpT[0][b] : m->JpT[q][b];
+    e[q] = s->eprev[q][c] + Ku * m->lj_ga * (cur[q] - s->ds * jt);
+    /* the line source drives Dx (TEz) or Dz (TMz): here ga*g */
+    const int sq = s->tmz ? 2 : 0;
+    double sv = 0.0;
+    int hit = 0;
+    for (int r = 0; r < s->nsrcs; ++r)
+      if (q == sq && s->srcs[r].j == j) { sv += pulse_at(&s->srcs[r].p, tnow); hit = 1; }
+    if (hit) e[q] = e[q] + m->lj_ga * sv;
+    E[q][c] = e[q];
+  }
+  for (int t = 0; t < nc; ++t) {
+    const int q = comps[t];
+    if (!((p->jp_mask >> q) & 1)) continue;
+    m->JpT[q][b] = 0.0;
+    for (int l = 0; l < p->npols; ++l) {
+      double P0 = p->xi[l] * e[q] / O2_C0;
+      if (l == 0 && p->nonlinear) P0 = P0 + p->xi3 * m->Q[q][b] * e[q] / O2_C0;
+      m->Jp[l][q][b] = ((1.0 - p->dnu[l] * dt) / (1.0 + p->dnu[l] * dt)) * m->Jp[l][q][b] +
+                       ((p->w0[l] * p->w0[l]) * dt / (1.0 + p->dnu[l] * dt)) * (P0 - m->P[l][q][b]);
+      m->P[l][q][b] = m->P[l][q][b] + dt * m->Jp[l][q][b];
+      m->JpT[q][b] = m->JpT[q][b] + m->Jp[l][q][b];
+    }
+    if (p->nonlinear) {
+      m->G[q][b] = ((1.0 - p->dnuNL * dt) / (1.0 + p->dnuNL * dt)) * m->G[q][b] +
+                   ((p->wNL * p->wNL) * dt / (1.0 + p->dnuNL * dt)) * ((z0 * e[q]) * (z0 * e[q]) - m->Q[q][b]);
+      m->Q[q][b] = m->Q[q][b] + dt * m->G[q][b];
+    }
+  }
+}
+
+static void save_eprev(o2sim* s) {
+  int any = 0;
+  for (int q = 0; q < s->nmats; ++q) any |= s->mats[q].type == M2_LJP;
+  if (!any) return;
+  double* src[3] = {s->ex, s->ey, s->ez};
+  for (int c = 0; c < 3; ++c) {
+    if (!src[c]) continue;
+    if (!s->eprev[c]) s->eprev[c] = z2(s->n2);
+    memcpy(s->eprev[c], src[c], sizeof(double) * (size_t)s->n2);
+  }
 }
 
 int ofdtd2d_add_line_source(o2sim* s, int j, const fdtd_pulse* p) {
@@ -749,8 +844,10 @@ static double pulse_at(const fdtd_pulse* p, double t) {
   return p->amplitude * (exp(-0.5 * (a * a)) * sin(p->w0 * t)); /* sim2/program.f90:337 */
 }
 
-/* materials in call order: simpledielectric :29-36, gold :54-73, graphene :63-92 */
-static void update_materials(o2sim* s) {
+/* materials in call order: simpledielectric :29-36, gold :54-73, graphene :63-92,
+ * Lorentz-Jp update_EJp (E form) */
+static void update_materials(o2sim* s, double tnow) {
+  save_eprev(s);
   for (int q = 0; q < s->nmats; ++q) {
     o2mat* m = &s->mats[q];
     for (int j = m->y0; j <= m->y1; ++j)
@@ -758,7 +855,13 @@ static void update_materials(o2sim* s) {
         const long b = (i - m->x0) + (long)m->bx * (j - m->y0);
         if (!m->inside[b]) continue;
         const long c = F2(s, i, j);
-        if (m->type == M2_DIEL) {
+        if (m->type == M2_LJP) {
+          static const int xy[2] = {0, 1};
+          const double cur[3] = {s->hz[c] - s->hz[c - s->sx], -s->hz[c] + s->hz[c - 1], 0.0};
+          double* D[3] = {s->dx, s->dy, NULL};
+          double* E[3] = {s->ex, s->ey, NULL};
+          ljp_cell2(s, m, b, c, j, xy, 2, cur, D, E, tnow);
+        } else if (m->type == M2_DIEL) {
           s->ex[c] = s->dx[c] * m->gax;
           s->ey[c] = s->dy[c] * m->gax;
         } else if (m->type == M2_DRUDE2CP) {
@@ -905,7 +1008,8 @@ static int tmz_in_pml(const o2sim* s, int i, int j) {
          (s->tp[1] > 0 && (j <= s->tp[1] || j > s->ny - s->tp[1]));
 }
 
-static void tmz_materials(o2sim* s) {
+static void tmz_materials(o2sim* s, double tnow) {
+  save_eprev(s);
   for (int q = 0; q < s->nmats; ++q) {
     o2mat* m = &s->mats[q];
     for (int j = m->y0; j <= m->y1; ++j)
@@ -913,7 +1017,13 @@ static void tmz_materials(o2sim* s) {
         const long b = (i - m->x0) + (long)m->bx * (j - m->y0);
         if (!m->inside[b] || tmz_in_pml(s, i, j)) continue;
         const long c = F2(s, i, j);
-        if (m->type == M2_DIEL) {
+        if (m->type == M2_LJP) {
+          static const int z[1] = {2};
+          const double cur[3] = {0.0, 0.0, (s->hy[c] - s->hy[c - 1]) + (-s->hx[c] + s->hx[c - s->sx])};
+          double* D[3] = {NULL, NULL, s->dz};
+          double* E[3] = {NULL, NULL, s->ez};
+          ljp_cell2(s, m, b, c, j, z, 1, cur, D, E, tnow);
+        } else if (m->type == M2_DIEL) {
           s->ez[c] = s->dz[c] * m->gax;
         } else if (m->type == M2_DRUDE2CP) {
           const double complex num =
@@ -986,7 +1096,7 @@ static void tmz_step(o2sim* s, int n) {
     tf_line_E(s, t);
     tf_corr_D(s);
   }
-  tmz_materials(s);
+  tmz_materials(s, t);
   if (s->g.periodic_x) wrap_x(s, ez);
   for (int j = 1; j <= ny; ++j)
     for (int i = 1; i <= nx; ++i) {
@@ -1045,7 +1155,7 @@ int ofdtd2d_step(o2sim* s, int nsteps) {
       tf_line_E(s, t);
       tf_corr_D(s);
     }
-    update_materials(s);
+    update_materials(s, t);
     if (s->has_pml) pml_E(s);
     if (s->g.periodic_x) wrap_x(s, s->ey);
     /* updateH, myfdtd_2Dfunc.f90:34-44 */
