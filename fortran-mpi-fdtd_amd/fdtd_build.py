@@ -63,6 +63,7 @@ def build_oracles(with_reference=True):
     _run(["make", "-C", odir, "all"])
     if with_reference and os.path.isdir("/root/reference") and shutil.which(FLANG):
         _run(["make", "-C", odir, "ref"])
+        _run(["make", "-C", odir, "ref2d"])
         if os.path.isfile("/opt/conda/include/mpif.h"):  # the 1D TFSF driver links MPICH
             _run(["make", "-C", odir, "ref1d"])
 

@@ -1,0 +1,29 @@
+"""The 2D GPU path against the reference's own 2D modules (golden vectors
+from tests/golden/gen_golden2d_mod.py): graphene CP4 + Raman with gold, and
+the si, film gold and graphene3 1 nm Lorentz-Jp modules.  FP64, fields and
+traces within 1e-11 of their maxima."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", ["ref2d_graphene_gold", "ref2d_ljp_si", "ref2d_ljp_film_gold",
+                                  "ref2d_ljp_graphene3_1nm"])
+def test_gpu_matches_the_reference_2d_modules(name):
+    from fdtd_amd import sim2d
+
+    g = np.load(os.path.join(ROOT, "tests", "golden", name + ".npz"))
+    spec = json.loads(bytes(g["spec"]).decode())
+    sim, _, pids = sim2d.make_simulation2d(spec, precision="fp64")
+    sim.step(spec["nsteps"])
+    for comp, key in ((sim2d.EX2, "Ex"), (sim2d.EY2, "Ey"), (sim2d.HZ2, "Hz")):
+        assert relerr(sim.field(comp), g[key]) <= 1e-11, (name, key)
+    for p, pid in enumerate(pids):
+        assert relerr(sim.probe(pid), g["probes"][p]) <= 1e-11, (name, p)
+    sim.close()
