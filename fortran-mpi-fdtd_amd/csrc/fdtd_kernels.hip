@@ -148,16 +148,36 @@ __device__ __forceinline__ void fused_e_cell(const FieldArgs<real>& a, const Pml
   const unsigned id = UNI ? (unsigned)__builtin_amdgcn_readfirstlane(rm) : (unsigned)a.matid[c];
   const uint32_t fl = a.flags[id];
   const Vec4<real> cf = a.coef[id];
-  if (in_pml(p.P, i, j, k, a.nx, a.ny, a.nz)) {
+  // FP32: the loads of both forms are issued before either computes, so a
+  // wavefront with PML and interior lanes (the x slabs' edges) waits for
+  // memory once rather than once per form (+1.6 % on the E half step).  In
+  // FP64 the extra live registers cost more than that (-3 %): the interior
+  // loads stay after the PML branch.
+  constexpr bool kHoist = sizeof(real) == 4;
+  const bool pml = in_pml(p.P, i, j, k, a.nx, a.ny, a.nz);
+  const bool any_src = sa[0] >= 0 || sa[1] >= 0 || sa[2] >= 0;
+  int s = 0;
+  real sp[6], fx1 = 0, fx2 = 0, e0[3] = {0, 0, 0};
+  Vec4<real> gx;
+  if (pml) {
+    s = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
+    fx1 = p.f1[0][i];
+    fx2 = p.f2[0][i];
+    sp[0] = p.s[S_EYX][s]; sp[1] = p.s[S_EZX][s]; sp[2] = p.s[S_EXY][s];
+    sp[3] = p.s[S_EZY][s]; sp[4] = p.s[S_EXZ][s]; sp[5] = p.s[S_EYZ][s];
+  } else if (kHoist) {
+    e0[0] = E0[c]; e0[1] = E1[c]; e0[2] = E2[c];
+    if (any_src) gx = a.gaxe[id];
+  }
+  if (pml) {
     // mod3/BERpmly_mod.f90:150-155 with the splits driven by total H
-    const int s = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
     const real r = cf.v[3];
-    real eyx = p.f1[0][i] * p.s[S_EYX][s] + (p.f2[0][i] * r) * (-hz + hz_im);
-    real ezx = p.f1[0][i] * p.s[S_EZX][s] + (p.f2[0][i] * r) * (hy + (-hy_im));
-    real exy = p.f1[1][j] * p.s[S_EXY][s] + (p.f2[1][j] * r) * (hz + (-hz_jm));
-    real ezy = p.f1[1][j] * p.s[S_EZY][s] + (p.f2[1][j] * r) * (-hx + hx_jm);
-    real exz = p.f1[2][k] * p.s[S_EXZ][s] + (p.f2[2][k] * r) * (-hy + hy_km);
-    real eyz = p.f1[2][k] * p.s[S_EYZ][s] + (p.f2[2][k] * r) * (hx + (-hx_km));
+    real eyx = fx1 * sp[0] + (fx2 * r) * (-hz + hz_im);
+    real ezx = fx1 * sp[1] + (fx2 * r) * (hy + (-hy_im));
+    real exy = p.f1[1][j] * sp[2] + (p.f2[1][j] * r) * (hz + (-hz_jm));
+    real ezy = p.f1[1][j] * sp[3] + (p.f2[1][j] * r) * (-hx + hx_jm);
+    real exz = p.f1[2][k] * sp[4] + (p.f2[2][k] * r) * (-hy + hy_km);
+    real eyz = p.f1[2][k] * sp[5] + (p.f2[2][k] * r) * (hx + (-hx_km));
     // a source plane through the PML feeds the split along its normal
     if (sa[0] == 1) exy = exy + sv[0] * r;
     if (sa[0] == 2) exz = exz + sv[0] * r;
@@ -178,12 +198,12 @@ __device__ __forceinline__ void fused_e_cell(const FieldArgs<real>& a, const Pml
     put_e_ghost(E2, ez, i, j, k, c, a);
     return;
   }
+  if (!kHoist) {
+    e0[0] = E0[c]; e0[1] = E1[c]; e0[2] = E2[c];
+    if (any_src) gx = a.gaxe[id];
+  }
   // E_q += Ku*gax_q*curl_q (+ gax_q * source) for every component a
   // dielectric writes
-  const real e0[3] = {E0[c], E1[c], E2[c]};
-  const bool any_src = sa[0] >= 0 || sa[1] >= 0 || sa[2] >= 0;
-  Vec4<real> gx;
-  if (any_src) gx = a.gaxe[id];
   const real cur[3] = {(hz - hz_jm) + (-hy + hy_km),   // curlhzy + curlhyz
                        (hx - hx_km) + (-hz + hz_im),   // curlhxz + curlhzx
                        (hy - hy_im) + (-hx + hx_jm)};  // curlhyx + curlhxy
@@ -295,15 +315,28 @@ __device__ __forceinline__ void fused_h_cell(const FieldArgs<real>& a, const Pml
   const real ez_ip = E2[c + 1], ey_ip = E1[c + 1];
   cex = ex_kp;
   cey = ey_kp;
-  if (in_pml(p.P, i, j, k, a.nx, a.ny, a.nz)) {
+  // FP32: loads of both forms before either computes (as in fused_e_cell)
+  constexpr bool kHoist = sizeof(real) == 4;
+  const bool pml = in_pml(p.P, i, j, k, a.nx, a.ny, a.nz);
+  int s = 0;
+  real sp[6], gx1 = 0, gx2 = 0, h0[3] = {0, 0, 0};
+  if (pml) {
+    s = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
+    gx1 = p.g1[0][i];
+    gx2 = p.g2[0][i];
+    sp[0] = p.s[S_HYX][s]; sp[1] = p.s[S_HZX][s]; sp[2] = p.s[S_HXY][s];
+    sp[3] = p.s[S_HZY][s]; sp[4] = p.s[S_HXZ][s]; sp[5] = p.s[S_HYZ][s];
+  } else if (kHoist) {
+    h0[0] = H0[c]; h0[1] = H1[c]; h0[2] = H2[c];
+  }
+  if (pml) {
     // mod3/BERpmly_mod.f90:186-191 with the splits driven by total E
-    const int s = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
-    real hyx = p.g1[0][i] * p.s[S_HYX][s] + p.g2[0][i] * (-ez + ez_ip);
-    real hzx = p.g1[0][i] * p.s[S_HZX][s] + p.g2[0][i] * (ey + (-ey_ip));
-    real hxy = p.g1[1][j] * p.s[S_HXY][s] + p.g2[1][j] * (ez + (-ez_jp));
-    real hzy = p.g1[1][j] * p.s[S_HZY][s] + p.g2[1][j] * (-ex + ex_jp);
-    real hxz = p.g1[2][k] * p.s[S_HXZ][s] + p.g2[2][k] * (-ey + ey_kp);
-    real hyz = p.g1[2][k] * p.s[S_HYZ][s] + p.g2[2][k] * (ex + (-ex_kp));
+    real hyx = gx1 * sp[0] + gx2 * (-ez + ez_ip);
+    real hzx = gx1 * sp[1] + gx2 * (ey + (-ey_ip));
+    real hxy = p.g1[1][j] * sp[2] + p.g2[1][j] * (ez + (-ez_jp));
+    real hzy = p.g1[1][j] * sp[3] + p.g2[1][j] * (-ex + ex_jp);
+    real hxz = p.g1[2][k] * sp[4] + p.g2[2][k] * (-ey + ey_kp);
+    real hyz = p.g1[2][k] * sp[5] + p.g2[2][k] * (ex + (-ex_kp));
     p.s[S_HYX][s] = hyx; p.s[S_HZX][s] = hzx; p.s[S_HXY][s] = hxy;
     p.s[S_HZY][s] = hzy; p.s[S_HXZ][s] = hxz; p.s[S_HYZ][s] = hyz;
     const real hx = hxy + hxz, hy = hyz + hyx, hz = hzx + hzy;
@@ -318,9 +351,12 @@ __device__ __forceinline__ void fused_h_cell(const FieldArgs<real>& a, const Pml
   const real curleyz = ey_kp - ey, curlezy = -ez_jp + ez;
   const real curlezx = ez_ip - ez, curlexz = -ex_kp + ex;
   const real curlexy = ex_jp - ex, curleyx = -ey_ip + ey;
-  const real hx = H0[c] + a.ku * (curleyz + curlezy);
-  const real hy = H1[c] + a.ku * (curlezx + curlexz);
-  const real hz = H2[c] + a.ku * (curlexy + curleyx);
+  if (!kHoist) {
+    h0[0] = H0[c]; h0[1] = H1[c]; h0[2] = H2[c];
+  }
+  const real hx = h0[0] + a.ku * (curleyz + curlezy);
+  const real hy = h0[1] + a.ku * (curlezx + curlexz);
+  const real hz = h0[2] + a.ku * (curlexy + curleyx);
   H0[c] = hx;
   H1[c] = hy;
   H2[c] = hz;
