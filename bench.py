@@ -178,10 +178,11 @@ def served(args):
     return out
 
 
-def _batch_vs_separate(specs, steps):
+def _batch_vs_separate(specs, steps, separate=True):
     """Time `steps` steps of `specs` stepped as one batch (fdtd2d_batch_run),
-    then the same simulations as separate handles (one fdtd2d_step call per
-    handle, each on its own stream, all enqueued before the first sync)."""
+    then (unless `separate` is false) the same simulations as separate
+    handles (one fdtd2d_step call per handle, each on its own stream, all
+    enqueued before the first sync)."""
     import numpy as np
     from fdtd_amd import sim2d
 
@@ -206,6 +207,16 @@ def _batch_vs_separate(specs, steps):
     batch.close()
     for s, _, _ in sims:
         s.close()
+    # algorithmic words per main cell-update: TEz E 7 (Hz, Dx, Dy rw, Ex, Ey),
+    # H 4; TMz E 5 (Hx, Hy, Dz rw, Ez), H 5 (Ez, Hx rw, Hy rw): 11 and 10
+    words = sum((10 if s["grid"].get("polarization") == "tmz" else 11) * s["grid"]["nx"] * s["grid"]["ny"]
+                for s in specs) / cells
+    n_tmz = sum(1 for s in specs if s["grid"].get("polarization") == "tmz")
+    out = {"instances": len(specs), "tmz_instances": n_tmz, "cells": cells, "steps": steps, "dtype": "f32",
+           "batched_mcells_per_s": work / tb / 1e6, "achieved_GBs_all_in": work * words * 4 / tb / 1e9,
+           "separate_handles_mcells_per_s": None, "speedup": None, "results_identical_batched_and_alone": None}
+    if not separate:
+        return out
     sims = build()
     for s, _, _ in sims:
         s.step(8)
@@ -219,15 +230,9 @@ def _batch_vs_separate(specs, steps):
     same = all(np.array_equal(a, s.flux(ids[1])) for a, (s, ids, _) in zip(fb, sims))
     for s, _, _ in sims:
         s.close()
-    # algorithmic words per main cell-update: TEz E 7 (Hz, Dx, Dy rw, Ex, Ey),
-    # H 4; TMz E 5 (Hx, Hy, Dz rw, Ez), H 5 (Ez, Hx rw, Hy rw): 11 and 10
-    words = sum((10 if s["grid"].get("polarization") == "tmz" else 11) * s["grid"]["nx"] * s["grid"]["ny"]
-                for s in specs) / cells
-    n_tmz = sum(1 for s in specs if s["grid"].get("polarization") == "tmz")
-    return {"instances": len(specs), "tmz_instances": n_tmz, "cells": cells, "steps": steps, "dtype": "f32",
-            "batched_mcells_per_s": work / tb / 1e6, "separate_handles_mcells_per_s": work / ts / 1e6,
-            "speedup": ts / tb, "achieved_GBs_all_in": work * words * 4 / tb / 1e9,
-            "results_identical_batched_and_alone": bool(same)}
+    out.update({"separate_handles_mcells_per_s": work / ts / 1e6, "speedup": ts / tb,
+                "results_identical_batched_and_alone": bool(same)})
+    return out
 
 
 def batched_2d(args):
@@ -238,7 +243,7 @@ def batched_2d(args):
     from fdtd_amd import sim2d
 
     specs = sim2d.sweep_stream(seed=args.seed, count=args.b2_count, nsteps=args.b2_steps)
-    return _batch_vs_separate(specs, args.b2_steps)
+    return _batch_vs_separate(specs, args.b2_steps, not args.b2_no_separate)
 
 
 def batched_2d_small(args):
@@ -247,7 +252,7 @@ def batched_2d_small(args):
     from fdtd_amd import sim2d
 
     specs = sim2d.sweep_stream(seed=args.seed, count=args.b2_count, nsteps=args.b2s_steps, side=args.b2s_side)
-    out = _batch_vs_separate(specs, args.b2s_steps)
+    out = _batch_vs_separate(specs, args.b2s_steps, not args.b2_no_separate)
     out["side"] = args.b2s_side
     return out
 
@@ -333,6 +338,8 @@ def main():
     ap.add_argument("--b2-steps", type=int, default=500, help="batched 2D sweep: timed steps")
     ap.add_argument("--b2s-side", type=int, default=128, help="small batched 2D sweep: grid side")
     ap.add_argument("--b2s-steps", type=int, default=1000, help="small batched 2D sweep: timed steps")
+    ap.add_argument("--b2-no-separate", action="store_true",
+                    help="batched 2D sweeps: time the batch only, not the separate handles")
     ap.add_argument("--jobs", type=int, default=24)
     ap.add_argument("--rate", type=float, default=20.0, help="served-stream arrival rate [jobs/s]")
     ap.add_argument("--max-active", type=int, default=16)
@@ -382,11 +389,14 @@ def main():
     b2 = b2s = None
     if not args.no_2d:
         b2 = batched_2d(args)
-        print(f"[bench] batched 2d {b2['batched_mcells_per_s']:.0f} Mcell/s ({b2['speedup']:.2f}x separate handles)",
+        def vs(r):
+            return "not timed" if r["speedup"] is None else f"{r['speedup']:.2f}x"
+
+        print(f"[bench] batched 2d {b2['batched_mcells_per_s']:.0f} Mcell/s (separate handles: {vs(b2)})",
               file=sys.stderr, flush=True)
         b2s = batched_2d_small(args)
-        print(f"[bench] batched 2d small {b2s['batched_mcells_per_s']:.0f} Mcell/s "
-              f"({b2s['speedup']:.2f}x separate handles)", file=sys.stderr, flush=True)
+        print(f"[bench] batched 2d small {b2s['batched_mcells_per_s']:.0f} Mcell/s (separate handles: {vs(b2s)})",
+              file=sys.stderr, flush=True)
     spec_err = None if args.no_spectra else fp32_spectra_error(args)
     if spec_err:
         print("[bench] fp32 spectra error " + ", ".join(f"{k} {v['max_rel_err']:.1e}" for k, v in spec_err.items()),

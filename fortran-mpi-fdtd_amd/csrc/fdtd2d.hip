@@ -515,6 +515,18 @@ __device__ __forceinline__ void h_tez(const Inst<real>& S, int i, int j, real& c
   }
 }
 
+// Logical block of this block.  Blocks are dealt to the 8 XCDs (one L2
+// each) round robin; renumbered, each XCD takes runs of kXcdRun consecutive
+// logical blocks, so the x neighbours of a tile (whose edge lines it reads)
+// mostly run on the same XCD and come from its L2, while the runs stay
+// short enough that every XCD gets a like mix of tiles (TEz/TMz, PML rows).
+// The grid is a multiple of 8 * kXcdRun (set at launch).
+constexpr int kXcdRun = 32;
+__device__ __forceinline__ int xcd_block() {
+  const int b = blockIdx.x, k = b >> 3;
+  return (k / kXcdRun) * (8 * kXcdRun) + (b & 7) * kXcdRun + k % kXcdRun;
+}
+
 // E and H over the packed tile list: block = one tile at a time, wavefront
 // w marches rows 4w..4w+3 of the tile (j uniform per wavefront).
 template <typename real>
@@ -522,7 +534,7 @@ __global__ __launch_bounds__(kTX * kWaves) void k2_e(const Inst<real>* __restric
                                                      const int32_t* __restrict__ tile_inst, int ntiles) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  for (int t = xcd_block(); t < ntiles; t += gridDim.x) {
     const int q = tile_inst[t];
     const Inst<real>& S = I[q];
     if (S.ctl[0] >= S.ctl[1]) continue;  // finished: nothing to do this step
@@ -547,7 +559,7 @@ __global__ __launch_bounds__(kTX * kWaves) void k2_h(const Inst<real>* __restric
                                                      const int32_t* __restrict__ tile_inst, int ntiles) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  for (int t = xcd_block(); t < ntiles; t += gridDim.x) {
     const int q = tile_inst[t];
     const Inst<real>& S = I[q];
     if (S.ctl[0] >= S.ctl[1]) continue;
@@ -1558,7 +1570,8 @@ void enqueue_block(Batch* b) {
     const int v = std::atoi(e);
     if (v >= 1 && v <= 4096) tpb = v;
   }
-  const unsigned grid = (unsigned)std::min(b->ntiles, std::max(8192, (b->ntiles + tpb - 1) / tpb));
+  unsigned grid = (unsigned)std::min(b->ntiles, std::max(8192, (b->ntiles + tpb - 1) / tpb));
+  grid = (grid + 8 * kXcdRun - 1) / (8 * kXcdRun) * (8 * kXcdRun);  // whole XCD runs (xcd_block)
   const int count = (int)b->sims.size();
   for (int s = 0; s < kBlockSteps; ++s) {
     // the line's E half step may precede k2_e, which reads only the line's h
@@ -1625,9 +1638,24 @@ int batch_run(Batch* b, const int32_t* nsteps) {
   else
     hipLaunchKernelGGL(k2_adv<double>, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, b->stream,
                        (const Inst<double>*)b->d_inst, count, 0);
-  int rc = capture(b);
-  if (rc) return rc;
-  for (int r = 0; r < (maxn + kBlockSteps - 1) / kBlockSteps; ++r) FDTD_HIP(hipGraphLaunch(b->exec, b->stream));
+  const int nblocks = (maxn + kBlockSteps - 1) / kBlockSteps;
+  // FDTD2D_GRAPH=0: the same launches of each 32-step block are enqueued
+  // directly instead of replaying the captured graph (for tools that cannot
+  // follow graph launches); results are the same
+  const char* ge = std::getenv("FDTD2D_GRAPH");
+  if (ge && std::atoi(ge) == 0) {
+    for (int r = 0; r < nblocks; ++r) {
+      if (b->prec == FDTD_FP32)
+        enqueue_block<float>(b);
+      else
+        enqueue_block<double>(b);
+      FDTD_HIP(hipGetLastError());
+    }
+  } else {
+    int rc = capture(b);
+    if (rc) return rc;
+    for (int r = 0; r < nblocks; ++r) FDTD_HIP(hipGraphLaunch(b->exec, b->stream));
+  }
   for (int q = 0; q < count; ++q) b->sims[q]->n_done += nsteps[q];
   return 0;
 }

@@ -147,3 +147,28 @@ def test_batch_equals_each_simulation_alone():
     batch.close()
     for s, _, _ in built:
         s.close()
+
+
+def test_direct_launches_equal_the_graph(monkeypatch):
+    """FDTD2D_GRAPH=0 enqueues each 32-step block's launches directly instead
+    of replaying the captured graph: bitwise the same results (FP32)."""
+    from fdtd_amd import sim2d
+
+    specs = [_sweep(96, 300, 21), _sweep(130, 250, 22)]
+    out = []
+    for direct in (False, True):
+        if direct:
+            monkeypatch.setenv("FDTD2D_GRAPH", "0")
+        built = [sim2d.make_simulation2d(sp, precision="fp32") for sp in specs]
+        for s, _, _ in built:
+            s.commit()
+        batch = sim2d.Batch2D([s for s, _, _ in built])
+        batch.run([sp["nsteps"] for sp in specs])
+        batch.sync()
+        out.append([([s.flux(i) for i in ids], s.field(sim2d.HZ2), s.probe(pids[0])) for s, ids, pids in built])
+        batch.close()
+        for s, _, _ in built:
+            s.close()
+    for (fa, ha, pa), (fb, hb, pb) in zip(*out):
+        assert all(np.array_equal(a, b) for a, b in zip(fa, fb))
+        assert np.array_equal(ha, hb) and np.array_equal(pa, pb)
