@@ -527,6 +527,166 @@ __device__ __forceinline__ int xcd_block() {
   return (k / kXcdRun) * (8 * kXcdRun) + (b & 7) * kXcdRun + k % kXcdRun;
 }
 
+// ---- plain row strips ------------------------------------------------------------
+// A wavefront's kRowsPerWave rows are "plain" when none is a PML, source or
+// TFSF row and (TMz) none of its lanes lies in an x slab: the cell update is
+// then the interior one of e_tez / h_tez / e_tmz / h_tmz, and the loads of
+// all rows are issued before the first row's stores (one memory round trip
+// for the strip instead of one per row).  The arithmetic is the same,
+// operation for operation.
+template <typename real>
+__device__ __forceinline__ bool src_row(const Inst<real>& S, int j) {
+  bool src = false;
+  for (int s = 0; s < S.nsrc; ++s) src |= j == S.src_j[s];
+  return src;
+}
+template <typename real>
+__device__ __forceinline__ bool strip_plain(const Inst<real>& S, int i0, int j0, bool e_half) {
+  const int j1 = j0 + kRowsPerWave - 1;
+  if (j1 > S.ny) return false;
+  if (S.tmz) {
+    if (S.P > 0 && (j0 <= S.P || j1 > S.ny - S.P)) return false;
+    if (S.Px > 0 && (i0 + 1 <= S.Px || std::min(i0 + kTX, S.nx) > S.nx - S.Px)) return false;
+  } else {
+    if (j0 < S.j0 || j1 > S.j1) return false;
+    if (S.has_pml && (j0 <= S.ja || j1 >= S.jb)) return false;
+  }
+  if (S.tf_on) {
+    const int a = e_half ? S.tf_lo : S.tf_lo - 1;
+    const int b = e_half ? S.tf_hi + 1 : S.tf_hi;
+    if (a >= j0 && a <= j1) return false;
+    if (S.tf_hi > 0 && b >= j0 && b <= j1) return false;
+  }
+  if (e_half)
+    for (int j = j0; j <= j1; ++j)
+      if (src_row(S, j)) return false;
+  return true;
+}
+
+template <typename real>
+__device__ __forceinline__ void e_tez_strip(const Inst<real>& S, int i, int j0) {
+  constexpr int R = kRowsPerWave;
+  const int px = S.px;
+  const int c0 = i + px * j0;
+  real hz[R + 1], hzi[R], dx[R], dy[R];
+  uint8_t id[R];
+  hz[0] = S.hz[c0 - px];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int c = c0 + px * r;
+    hz[r + 1] = S.hz[c];
+    hzi[r] = S.hz[c - 1];
+    dx[r] = S.dx[c];
+    dy[r] = S.dy[c];
+    id[r] = S.matid[c];
+  }
+  Mat<real> m[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) m[r] = S.tab[id[r]];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int c = c0 + px * r;
+    const real dxv = dx[r] + S.ku * (hz[r + 1] - hz[r]);
+    const real dyv = dy[r] - S.ku * (hz[r + 1] - hzi[r]);
+    S.dy[c] = dyv;
+    S.dx[c] = dxv;
+    if (m[r].act[1] == ACT_DIEL) {
+      const real ey = dyv * m[r].gax[1];
+      S.ey[c] = ey;
+      if (S.per && i == 1) S.ey[c + S.nx] = ey;
+      if (S.per && i == S.nx) S.ey[c - S.nx] = ey;
+    }
+    if (m[r].act[0] == ACT_DIEL) S.ex[c] = dxv * m[r].gax[0];
+  }
+}
+
+template <typename real>
+__device__ __forceinline__ void h_tez_strip(const Inst<real>& S, int i, int j0) {
+  constexpr int R = kRowsPerWave;
+  const int px = S.px;
+  const int c0 = i + px * j0;
+  real ex[R + 1], ey[R], eyi[R], hz[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int c = c0 + px * r;
+    ex[r] = S.ex[c];
+    ey[r] = S.ey[c];
+    eyi[r] = S.ey[c + 1];
+    hz[r] = S.hz[c];
+  }
+  ex[R] = S.ex[c0 + px * R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int c = c0 + px * r;
+    const real curlE = eyi[r] - ey[r] - ex[r + 1] + ex[r];
+    const real h = hz[r] - S.ku * curlE;
+    S.hz[c] = h;
+    if (S.per && i == 1) S.hz[c + S.nx] = h;
+    if (S.per && i == S.nx) S.hz[c - S.nx] = h;
+  }
+}
+
+template <typename real>
+__device__ __forceinline__ void e_tmz_strip(const Inst<real>& S, int i, int j0) {
+  constexpr int R = kRowsPerWave;
+  const int px = S.px;
+  const int c0 = i + px * j0;
+  real hx[R + 1], hy[R], hyi[R], dz[R];
+  uint8_t id[R];
+  hx[0] = S.hx[c0 - px];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int c = c0 + px * r;
+    hx[r + 1] = S.hx[c];
+    hy[r] = S.hy[c];
+    hyi[r] = S.hy[c - 1];
+    dz[r] = S.dz[c];
+    id[r] = S.matid[c];
+  }
+  Mat<real> m[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) m[r] = S.tab[id[r]];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int c = c0 + px * r;
+    const real d = dz[r] + S.ku * ((hy[r] - hyi[r]) + (-hx[r + 1] + hx[r]));
+    S.dz[c] = d;
+    if (m[r].act[0] == ACT_DIEL) {
+      const real ez = d * m[r].gax[0];
+      S.ez[c] = ez;
+      if (S.per && i == 1) S.ez[c + S.nx] = ez;
+      if (S.per && i == S.nx) S.ez[c - S.nx] = ez;
+    }
+  }
+}
+
+template <typename real>
+__device__ __forceinline__ void h_tmz_strip(const Inst<real>& S, int i, int j0) {
+  constexpr int R = kRowsPerWave;
+  const int px = S.px;
+  const int c0 = i + px * j0;
+  real ez[R + 1], ezi[R], hx[R], hy[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int c = c0 + px * r;
+    ez[r] = S.ez[c];
+    ezi[r] = S.ez[c + 1];
+    hx[r] = S.hx[c];
+    hy[r] = S.hy[c];
+  }
+  ez[R] = S.ez[c0 + px * R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int c = c0 + px * r;
+    const real x = hx[r] + S.ku * (-ez[r + 1] + ez[r]);
+    const real y = hy[r] + S.ku * (ezi[r] - ez[r]);
+    S.hx[c] = x;
+    S.hy[c] = y;
+    if (S.per && i == 1) S.hy[c + S.nx] = y;
+    if (S.per && i == S.nx) S.hy[c - S.nx] = y;
+  }
+}
+
 // E and H over the packed tile list: block = one tile at a time, wavefront
 // w marches rows 4w..4w+3 of the tile (j uniform per wavefront).
 template <typename real>
@@ -542,6 +702,13 @@ __global__ __launch_bounds__(kTX * kWaves) void k2_e(const Inst<real>* __restric
     const int i = (lt % S.ntx) * kTX + lane + 1;
     const int j0 = (lt / S.ntx) * kTY + wave * kRowsPerWave + 1;
     if (i > S.nx) continue;
+    if (strip_plain(S, (lt % S.ntx) * kTX, j0, true)) {
+      if (S.tmz)
+        e_tmz_strip(S, i, j0);
+      else
+        e_tez_strip(S, i, j0);
+      continue;
+    }
     real carry = 0;
     for (int r = 0; r < kRowsPerWave; ++r) {
       const int j = j0 + r;
@@ -567,6 +734,13 @@ __global__ __launch_bounds__(kTX * kWaves) void k2_h(const Inst<real>* __restric
     const int i = (lt % S.ntx) * kTX + lane + 1;
     const int j0 = (lt / S.ntx) * kTY + wave * kRowsPerWave + 1;
     if (i > S.nx) continue;
+    if (strip_plain(S, (lt % S.ntx) * kTX, j0, false)) {
+      if (S.tmz)
+        h_tmz_strip(S, i, j0);
+      else
+        h_tez_strip(S, i, j0);
+      continue;
+    }
     real carry = 0;
     for (int r = 0; r < kRowsPerWave; ++r) {
       const int j = j0 + r;
