@@ -1739,7 +1739,7 @@ void enqueue_block(Batch* b) {
   hipStream_t st = b->stream;
   const Inst<real>* I = (const Inst<real>*)b->d_inst;
   // blocks of the E and H launches; each strides over the packed tile list
-  int tpb = 2;
+  int tpb = 1;
   if (const char* e = std::getenv("FDTD2D_TILES_PER_BLOCK")) {
     const int v = std::atoi(e);
     if (v >= 1 && v <= 4096) tpb = v;
