@@ -35,26 +35,30 @@ def _ragged(n, P, periodic, nsteps=200):
     }
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("n,P,periodic", [((37, 29, 23), (4, 3, 5), (0, 0, 0)),
                                           ((65, 17, 19), (2, 4, 3), (0, 0, 0)),
                                           ((130, 12, 21), (5, 0, 3), (0, 1, 0)),
                                           ((9, 70, 14), (0, 6, 2), (1, 0, 0))])
-def test_ragged_grids_vs_oracle(n, P, periodic):
+def test_ragged_grids_vs_oracle(n, P, periodic, precision):
+    """FP64 at the parity bar; FP32 fields 2e-4, spectra 1e-4, traces 1e-3
+    (FP32 fields, FP64 dispersive state and DFT accumulators)."""
     import fdtd_amd
     import oracle
 
     spec = _ragged(n, P, periodic)
-    g, gids, gp = fdtd_amd.make_simulation(spec, precision="fp64")
+    g, gids, gp = fdtd_amd.make_simulation(spec, precision=precision)
     o, oids, op = oracle.make_simulation(spec)
     g.step(spec["nsteps"])
     o.step(spec["nsteps"])
+    ftol, stol, ptol = (1e-10, 1e-12, 1e-12) if precision == "fp64" else (2e-4, 1e-4, 1e-3)
     for q in range(6):
-        assert relerr(g.field(q), o.field(q)) < 1e-10, (n, q)
-    assert relerr(g.flux(gids[0]), o.flux(oids[0])) < 1e-12
+        assert relerr(g.field(q), o.field(q)) < ftol, (n, q)
+    assert relerr(g.flux(gids[0]), o.flux(oids[0])) < stol
     for a, b in zip(gp, op):
         ref = o.probe(b)
         if np.abs(ref).max() > 0:
-            assert relerr(g.probe(a), ref) < 1e-12, (n, a)
+            assert relerr(g.probe(a), ref) < ptol, (n, a)
         else:
             assert np.abs(g.probe(a)).max() == 0
     g.close()
