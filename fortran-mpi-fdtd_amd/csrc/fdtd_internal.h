@@ -119,6 +119,7 @@ struct DftPlane {
   int n_first = 0;           // step of the first buffered sample
   int slot[6];
   int nslots = 0;
+  bool fused_now = false;  // this step's sample is written by the H kernel
 };
 
 struct Probe {

@@ -300,14 +300,49 @@ __global__ __launch_bounds__(256) void k_fused_e_mixed(FieldArgs<real> a, PmlArg
                             a.e[2]);
 }
 
+// Monitor planes sampled by the FUSED H kernel: a cell on such a plane
+// writes its E and new H components to the plane's sample buffer (the same
+// values and layout as k_dft_sample, which would read them again
+// afterwards; an x-normal plane has one sample per 128-byte line).
+// FP32 only: in FP64 the extra registers cut the H kernel's occupancy
+// (8 -> 5 waves per SIMD) and cost more than the separate sample launch.
+constexpr int kFuseDft = 4;
+template <typename real>
+constexpr bool kFuseDftReal = sizeof(real) == 4;
+template <typename real>
+struct DftFuse {
+  int32_t n;
+  int32_t axis[kFuseDft], index[kFuseDft], n1[kFuseDft], ns[kFuseDft];
+  int64_t np[kFuseDft];
+  int32_t comp[kFuseDft][6];
+  real* dst[kFuseDft];
+};
+
+template <typename real>
+__device__ __forceinline__ void dft_fuse_write(const DftFuse<real>& df, int i, int j, int k, real ex, real ey,
+                                               real ez, real hx, real hy, real hz) {
+  for (int q = 0; q < df.n; ++q) {
+    const int ax = df.axis[q];
+    if ((ax == 0 ? i : (ax == 1 ? j : k)) != df.index[q]) continue;
+    const int u = ax == 0 ? j : i, v = ax == 2 ? j : k;
+    const int64_t o = (int64_t)(u - 1) + (int64_t)(v - 1) * df.n1[q];
+    for (int sl = 0; sl < df.ns[q]; ++sl) {
+      const int c = df.comp[q][sl];
+      const real x = c == 0 ? ex : c == 1 ? ey : c == 2 ? ez : c == 3 ? hx : c == 4 ? hy : hz;
+      df.dst[q][sl * df.np[q] + o] = x;
+    }
+  }
+}
+
 // FUSED engine: H half step (myfdtd_func.f90:45-64 + split-field H) of one
 // cell.  z march: with use_carry, Ex and Ey of plane k come from cex, cey
 // (loaded as plane k+1 by the previous plane of the same thread); the cell
 // leaves Ex, Ey of plane k+1 there.
 template <typename real>
-__device__ __forceinline__ void fused_h_cell(const FieldArgs<real>& a, const PmlArgs<real>& p, int i, int j, int k,
-                                             real& cex, real& cey, bool use_carry, const real* E0,
-                                             const real* E1, const real* E2, real* H0, real* H1, real* H2) {
+__device__ __forceinline__ void fused_h_cell(const FieldArgs<real>& a, const PmlArgs<real>& p,
+                                             const DftFuse<real>& df, int i, int j, int k, real& cex, real& cey,
+                                             bool use_carry, const real* E0, const real* E1, const real* E2,
+                                             real* H0, real* H1, real* H2) {
   const int c = i + a.px * j + a.pxy * k;
   const real ex = use_carry ? cex : E0[c], ey = use_carry ? cey : E1[c], ez = E2[c];
   const real ey_kp = E1[c + a.pxy], ex_kp = E0[c + a.pxy];
@@ -346,6 +381,7 @@ __device__ __forceinline__ void fused_h_cell(const FieldArgs<real>& a, const Pml
     put_h_ghost(H0, hx, i, j, k, c, a);
     put_h_ghost(H1, hy, i, j, k, c, a);
     put_h_ghost(H2, hz, i, j, k, c, a);
+    if constexpr (kFuseDftReal<real>) dft_fuse_write(df, i, j, k, ex, ey, ez, hx, hy, hz);
     return;
   }
   const real curleyz = ey_kp - ey, curlezy = -ez_jp + ez;
@@ -363,12 +399,14 @@ __device__ __forceinline__ void fused_h_cell(const FieldArgs<real>& a, const Pml
   put_h_ghost(H0, hx, i, j, k, c, a);
   put_h_ghost(H1, hy, i, j, k, c, a);
   put_h_ghost(H2, hz, i, j, k, c, a);
+  if constexpr (kFuseDftReal<real>) dft_fuse_write(df, i, j, k, ex, ey, ez, hx, hy, hz);
 }
 
 // H over tiles of KZ planes, each thread marching its (i,j) column; restrict
 // pointers as in e_column (no E array is written here).
 template <typename real, int KZ>
-__device__ __forceinline__ void h_column(const FieldArgs<real>& a, const PmlArgs<real>& p, int i, int j, int tz,
+__device__ __forceinline__ void h_column(const FieldArgs<real>& a, const PmlArgs<real>& p, const DftFuse<real>& df,
+                                         int i, int j, int tz,
                                          const real* __restrict__ E0, const real* __restrict__ E1,
                                          const real* __restrict__ E2, real* __restrict__ H0, real* __restrict__ H1,
                                          real* __restrict__ H2) {
@@ -377,18 +415,18 @@ __device__ __forceinline__ void h_column(const FieldArgs<real>& a, const PmlArgs
   for (int r = 0; r < KZ; ++r) {
     const int k = tz * KZ + r + 1;
     if (k > a.nz) break;
-    fused_h_cell(a, p, i, j, k, cex, cey, r > 0, E0, E1, E2, H0, H1, H2);
+    fused_h_cell(a, p, df, i, j, k, cex, cey, r > 0, E0, E1, E2, H0, H1, H2);
   }
 }
 
 template <typename real, int KZ>
-__global__ __launch_bounds__(512) void k_fused_h(FieldArgs<real> a, PmlArgs<real> p) {
+__global__ __launch_bounds__(512) void k_fused_h(FieldArgs<real> a, PmlArgs<real> p, DftFuse<real> df) {
   int tx, ty, tz;
   if (!tile_of(a, tx, ty, tz)) return;
   const int i = tx * blockDim.x + threadIdx.x + 1;
   const int j = ty * blockDim.y + __builtin_amdgcn_readfirstlane((int)threadIdx.y) + 1;  // wave-uniform row
   if (i > a.nx || j > a.ny) return;
-  h_column<real, KZ>(a, p, i, j, tz, a.e[0], a.e[1], a.e[2], a.h[0], a.h[1], a.h[2]);
+  h_column<real, KZ>(a, p, df, i, j, tz, a.e[0], a.e[1], a.e[2], a.h[0], a.h[1], a.h[2]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1806,16 +1844,36 @@ int step_impl(Sim* s) {
                            ade_coef(g), g.d_cells, g.d_mask, (double*)g.d_state, g.ncells, src, nsrc);
     }
     tfsf_e(st);
+    // this step's monitor samples are written by the H kernel (not with a
+    // TFSF wave: its H correction follows the H kernel)
+    DftFuse<real> df{};
+    if (kFuseDftReal<real> && !s->dry && s->tfsfs.empty()) {
+      for (auto& d : s->dfts) {
+        if (df.n == kFuseDft) break;
+        if (n < d.n_begin || (d.n_end > 0 && n > d.n_end)) continue;
+        const int q = df.n++;
+        int ns = 0;
+        for (int c = 0; c < 6; ++c)
+          if (d.mask & (1 << c)) df.comp[q][ns++] = c;
+        df.axis[q] = d.axis;
+        df.index[q] = d.index;
+        df.n1[q] = d.n1;
+        df.ns[q] = ns;
+        df.np[q] = (int64_t)d.n1 * d.n2;
+        df.dst[q] = (real*)d.d_samp + (int64_t)d.nbuf * ns * df.np[q];
+        d.fused_now = true;
+      }
+    }
     {
       Timer tm(s, FDTD_K_H);
       if (s->zm == 8)
-        hipLaunchKernelGGL((k_fused_h<real, 8>), gz, blk, 0, st, az, p);
+        hipLaunchKernelGGL((k_fused_h<real, 8>), gz, blk, 0, st, az, p, df);
       else if (s->zm == 4)
-        hipLaunchKernelGGL((k_fused_h<real, 4>), gz, blk, 0, st, az, p);
+        hipLaunchKernelGGL((k_fused_h<real, 4>), gz, blk, 0, st, az, p, df);
       else if (s->zm == 2)
-        hipLaunchKernelGGL((k_fused_h<real, 2>), gz, blk, 0, st, az, p);
+        hipLaunchKernelGGL((k_fused_h<real, 2>), gz, blk, 0, st, az, p, df);
       else
-        hipLaunchKernelGGL((k_fused_h<real, 1>), gz, blk, 0, st, az, p);
+        hipLaunchKernelGGL((k_fused_h<real, 1>), gz, blk, 0, st, az, p, df);
     }
     tfsf_h(st, p, 1);
   } else {
@@ -1878,11 +1936,12 @@ int step_impl(Sim* s) {
     const int64_t np = (int64_t)d.n1 * d.n2;
     if (d.nbuf == 0) d.n_first = n;
     real* dst = (real*)d.d_samp + (int64_t)d.nbuf * ns * np;
-    {
+    if (!d.fused_now) {
       Timer tm(s, FDTD_K_MON);
       hipLaunchKernelGGL(k_dft_sample<real>, dim3((unsigned)((np + 255) / 256), (unsigned)ns), dim3(256), 0, st, a,
                          d.axis, d.index, d.n1, d.n2, ns, cl, dst);
     }
+    d.fused_now = false;
     d.nbuf += 1;
     if (d.nbuf == kDftBlock) dft_flush_plane<real>(s, d);
   }
