@@ -26,6 +26,11 @@
  * Extensions: disks next to boxes; graphene KUBO (Kubo intraband Drude
  * weight, the same form as the 3D oracle's).
  *
+ * Pins: the shipped 2D run's output files (whole program, PML included),
+ * and golden vectors of the reference's own compiled 2D modules (graphene
+ * CP4 + Raman, gold, dielectrics, the si / film gold / graphene3 1 nm
+ * Lorentz-Jp modules) run by oracle/ref2d_driver.f90, to 1e-12.
+ *
  * TMz (Ez, Hx, Hy; the reference's 2D driver has none) restates the 3D
  * oracle's FUSED stepping (fdtd_oracle.c, ALL_FACES PML) for fields that do
  * not vary along z: the 3D updates of sim3/include/myfdtd_func.f90:19-64
