@@ -106,6 +106,18 @@ def profiled_traffic(kernel, prec):
     return None, None
 
 
+def cpu_model():
+    """The host CPU's model name from /proc/cpuinfo (None where absent)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(spec, steps):
     """FP64 CPU oracle (OpenMP) on `steps` steps of the same workload."""
     import oracle
@@ -120,40 +132,9 @@ def cpu_baseline(spec, steps):
     nx, ny, nz = sim.shape
     sim.close()
     return {"value": nx * ny * nz * steps / el / 1e6, "unit": "Mcell-updates/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
             "sample": f"FP64 oracle (fdtd_oracle.c, OpenMP) on the same {nx}x{ny}x{nz} workload, "
                       f"{steps} timed steps after 1 warm-up step ({el:.1f} s)"}
-
-
-def cpu_reference(steps=60, n=96):
-    """The reference's own 3D modules and Yee kernels on one core: the binary
-    the oracle recipe builds from the reference sources (oracle/_ref/ref3d, one
-    rank), on the reference-envelope twin of the gold-sphere case (y-slab PML,
-    gold cube, n^3 cells).  Loop time = run(steps) - run(0), which removes its
-    set-up and output.  None where that binary was not built."""
-    import subprocess
-    import tempfile
-    import oracle
-    from fdtd_amd import workloads
-
-    if not os.path.exists(oracle.REF_EXE):
-        return None
-    wall = {}
-    for k in (0, steps):
-        spec = workloads.ref_c1_twin(nsteps=k, n=n)
-        with tempfile.TemporaryDirectory() as tmp:
-            cfg = os.path.join(tmp, "cfg.nml")
-            with open(cfg, "w") as f:
-                f.write(oracle.ref3d_namelist(spec))
-            t0 = time.perf_counter()
-            res = subprocess.run([oracle.REF_EXE, cfg, tmp], capture_output=True, text=True, timeout=600)
-            wall[k] = time.perf_counter() - t0
-            if res.returncode != 0:
-                return None
-    loop = wall[steps] - wall[0]
-    return {"value": n ** 3 * steps / loop / 1e6, "unit": "Mcell-updates/s", "cores": 1, "kind": "reference",
-            "sample": f"reference 3D modules + myfdtd_func.f90 (oracle/_ref/ref3d, one rank) on the {n}^3 "
-                      f"y-PML gold-cube twin, {steps} steps: {loop:.2f} s of loop"}
 
 
 def served(args):
@@ -373,14 +354,10 @@ def main():
     if not args.no_fp64:
         r64 = run_gpu(spec, "fp64", args.steps, args.warmup)
         print(f"[bench] fp64 {r64['value']:.0f} Mcell/s", file=sys.stderr, flush=True)
-    cpu = cpu_ref = None
+    cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(spec, args.cpu_steps)
         print(f"[bench] cpu {cpu['value']:.1f} Mcell/s", file=sys.stderr, flush=True)
-        cpu_ref = cpu_reference()
-        if cpu_ref:
-            print(f"[bench] cpu reference path {cpu_ref['value']:.1f} Mcell/s on one core", file=sys.stderr,
-                  flush=True)
     srv = None
     if not args.no_serve:
         srv = served(args)
@@ -434,7 +411,6 @@ def main():
         "fp32_spectra_error": spec_err,
         "graphene_disks_mixed_precision": c3,
         "cpu_baseline": cpu,
-        "cpu_reference_path": cpu_ref,
         "p99_job_latency_ms": None if srv is None else srv["p99_ms"],
         "served_stream": srv,
         "batched_2d": b2,

@@ -541,6 +541,7 @@ int commit_impl(Sim* s) {
       const int v = std::atoi(e);
       if (v >= 2 && v <= 4096) s->op_kc = v;
     }
+    if (const char* e = std::getenv("FDTD_DFT_FUSE")) s->dft_fuse = std::atoi(e) != 0;
     if (const char* e = std::getenv("FDTD_ZMARCH")) {
       const int v = std::atoi(e);
       if (v == 1 || v == 2 || v == 4 || v == 8) s->zm = v;

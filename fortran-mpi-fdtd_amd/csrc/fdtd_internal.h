@@ -216,6 +216,9 @@ struct Sim {
   // two-kernel step: E and H tiles span zm planes (1, 2, 4 or 8), each
   // thread marching its column and carrying the neighbour plane in registers
   int zm = 1;
+  // FP32 two-kernel step: the H kernel writes the monitor-plane samples
+  // (FDTD_DFT_FUSE=0: the separate sample kernel does, bitwise the same)
+  int dft_fuse = 1;
   // dry steps (autotuner): no sources, TFSF or monitors, no step count; run
   // only while every field is still zero, so they leave the state as it was
   bool dry = false;

@@ -1847,7 +1847,7 @@ int step_impl(Sim* s) {
     // this step's monitor samples are written by the H kernel (not with a
     // TFSF wave: its H correction follows the H kernel)
     DftFuse<real> df{};
-    if (kFuseDftReal<real> && !s->dry && s->tfsfs.empty()) {
+    if (kFuseDftReal<real> && s->dft_fuse && !s->dry && s->tfsfs.empty()) {
       for (auto& d : s->dfts) {
         if (df.n == kFuseDft) break;
         if (n < d.n_begin || (d.n_end > 0 && n > d.n_end)) continue;

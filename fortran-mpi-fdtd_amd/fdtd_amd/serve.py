@@ -64,14 +64,18 @@ class Pool:
             pass
 
 
-def job_spec(rng, pml=10):
+def job_spec(rng, pml=10, side=None, geom=None, nsteps=None):
     """One served job (draws in this order: side, ds, geometry, feature size,
-    steps, nfreq)."""
-    side = int(round(math.exp(rng.uniform(math.log(64), math.log(320)))))
+    steps, nfreq).  `side`, `geom` and `nsteps` replace their draws (which
+    are still taken, so the other draws stay those of the stream)."""
+    side_d = int(round(math.exp(rng.uniform(math.log(64), math.log(320)))))
+    side = side_d if side is None else int(side)
     ds = rng.uniform(1.0, 2.0) * NANO
-    geom = ("sphere", "rod", "dimer")[int(rng.integers(0, 3))]
+    geom_d = ("sphere", "rod", "dimer")[int(rng.integers(0, 3))]
+    geom = geom_d if geom is None else geom
     feat = rng.uniform(10.0, 60.0) * NANO
-    nsteps = int(rng.integers(1500, 5001))
+    nsteps_d = int(rng.integers(1500, 5001))
+    nsteps = nsteps_d if nsteps is None else int(nsteps)
     nfreq = int(rng.integers(16, 65))
     n = side
     c = (n + 1) / 2.0

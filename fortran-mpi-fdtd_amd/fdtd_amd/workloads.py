@@ -362,16 +362,17 @@ def c1_gold_sphere(nsteps=2000, n=96):
     }
 
 
-def c2_gold_dimer(seed=0, nsteps=4000, n=512, nfreq=64):
+def c2_gold_dimer(seed=0, nsteps=4000, n=512, nfreq=64, radius=40.0):
     """512^3, ds = 1 nm, 12-cell PML, a gold sphere dimer (r = 40 nm, gap
     ~ U(2, 10) nm from the seed) along x, x-polarised Gaussian plane source,
     DFT at 64 wavelengths (400-900 nm) on 3 orthogonal planes through the
-    dimer centre, 8 point probes."""
+    dimer centre, 8 point probes.  `n` and `radius` (nm) shrink the case
+    (same PML, planes, probes and draws) to sizes the CPU oracle runs."""
     rng = np.random.default_rng(seed)
     gap_nm = float(rng.uniform(2.0, 10.0))
     ds = 1.0 * NANO
     P = 12
-    r = 40.0
+    r = float(radius)
     c = (n + 1) / 2.0
     off = r + gap_nm / 2.0
     pul = pulse_for_band(400e-9, 900e-9, 600e-9)
@@ -396,12 +397,13 @@ def c2_gold_dimer(seed=0, nsteps=4000, n=512, nfreq=64):
     }
 
 
-def c3_graphene_disks(seed=0, nsteps=6000, nx=384, ny=384, nz=256, nfreq=128):
+def c3_graphene_disks(seed=0, nsteps=6000, nx=384, ny=384, nz=256, nfreq=128, disk_radius=30.0,
+                      disk_height=20.0):
     """384x384x256, ds = 1 nm, 12-cell PML; a one-cell graphene sheet at
     z = 128 (Kubo intraband Drude, E_F = 0.4 eV, tau = 0.1 ps, T = 300 K,
     plus the Raman term); beneath it a 4x4 array of gold disks (r = 30 nm,
     h = 20 nm) with centres jittered by U(-3, 3) nm from the seed; DFT at
-    128 wavelengths."""
+    128 wavelengths.  Grid and disk sizes shrink the case (same draws)."""
     rng = np.random.default_rng(seed)
     ds = 1.0 * NANO
     P = 12
@@ -414,7 +416,8 @@ def c3_graphene_disks(seed=0, nsteps=6000, nx=384, ny=384, nz=256, nfreq=128):
             jx, jy = rng.uniform(-3.0, 3.0, size=2)
             cx = P + pitch_x * (a + 0.5) + 0.5 + jx
             cy = P + pitch_y * (b + 0.5) + 0.5 + jy
-            mats.append({"type": "drude2cp", "region": cylinder((cx, cy, zs - 10.5), 30.0, 10.0, 2),
+            mats.append({"type": "drude2cp",
+                         "region": cylinder((cx, cy, zs - disk_height / 2.0 - 0.5), disk_radius, disk_height / 2.0, 2),
                          "params": gold_params_dict()})
     gr = {"model": "kubo", "normal_axis": 2, "eps_inf": 1.0, "fermi_ev": 0.4, "tau": 1e-13,
           "temperature": 300.0, "wNL": 2.976e14, "dnuNL": 3.0e12, "xi3": 0.05}

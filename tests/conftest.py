@@ -24,6 +24,13 @@ def relerr(a, b):
     return float(np.abs(a - b).max() / (scale if scale > 0 else 1.0))
 
 
+def check(label, err, tol):
+    """Print a measured error beside its tolerance (visible with -s), then
+    assert it."""
+    print(f"[parity] {label}: {err:.3e} (tol {tol:.0e})", flush=True)
+    assert err < tol, (label, err, tol)
+
+
 @pytest.fixture(scope="session")
 def golden():
     import json

@@ -2,12 +2,12 @@
 are no multiple of the 64-lane rows or of the tiles), unequal and absent PML
 per axis, a periodic axis beside PML slabs, probes in the corner cells;
 stepping in pieces against stepping at once; step(0); the argument checks
-of the set-up calls.  Tolerances as in test_gpu_parity.py (FP64 fields
-1e-10, spectra and traces 1e-12)."""
+of the set-up calls.  Tolerances as in test_gpu_parity.py (FP64 fields,
+spectra and traces 1e-12)."""
 import numpy as np
 import pytest
 
-from conftest import relerr
+from conftest import check, relerr
 
 pytestmark = pytest.mark.gpu
 
@@ -41,7 +41,7 @@ def _ragged(n, P, periodic, nsteps=200):
                                           ((130, 12, 21), (5, 0, 3), (0, 1, 0)),
                                           ((9, 70, 14), (0, 6, 2), (1, 0, 0))])
 def test_ragged_grids_vs_oracle(n, P, periodic, precision):
-    """FP64 at the parity bar; FP32 fields 2e-4, spectra 1e-4, traces 1e-3
+    """FP64 at the parity bar; FP32 fields 2e-4, spectra 1e-5, traces 1e-3
     (FP32 fields, FP64 dispersive state and DFT accumulators)."""
     import fdtd_amd
     import oracle
@@ -51,10 +51,10 @@ def test_ragged_grids_vs_oracle(n, P, periodic, precision):
     o, oids, op = oracle.make_simulation(spec)
     g.step(spec["nsteps"])
     o.step(spec["nsteps"])
-    ftol, stol, ptol = (1e-10, 1e-12, 1e-12) if precision == "fp64" else (2e-4, 1e-4, 1e-3)
+    ftol, stol, ptol = (1e-12, 1e-12, 1e-12) if precision == "fp64" else (2e-4, 1e-5, 1e-3)
     for q in range(6):
-        assert relerr(g.field(q), o.field(q)) < ftol, (n, q)
-    assert relerr(g.flux(gids[0]), o.flux(oids[0])) < stol
+        check(f"ragged {n} {precision} field {q}", relerr(g.field(q), o.field(q)), ftol)
+    check(f"ragged {n} {precision} flux", relerr(g.flux(gids[0]), o.flux(oids[0])), stol)
     for a, b in zip(gp, op):
         ref = o.probe(b)
         if np.abs(ref).max() > 0:

@@ -4,17 +4,19 @@ Fortran modules (golden vectors) and against the CPU oracle.
 Tolerances, all normwise (max |a-b| / max |b|):
   FP64, spectra (flux) and probe traces ........ 1e-12
   FP64, D fields (staged engine) ............... 1e-12 (well-conditioned)
-  FP64, E/H fields ............................. 1e-10: the reference's y-slab
-      PML amplifies last-bit differences; two CPU builds of the same FP64
-      algorithm (with and without FMA contraction) already differ by up to
-      3e-11 normwise on these cases after ~1000 steps.
+  FP64, E/H fields ............................. 1e-12: the library is built
+      without multiply-add contraction, as the CPU builds it is compared
+      with (x86-64 without FMA); the reference's y-slab PML amplifies
+      last-bit differences (one build with contraction and one without
+      differ by 3e-11 on these cases after ~1000 steps), so the operation
+      order has to match.
   FP32 fields / spectra ........................ 2e-4 / 1e-4 (FP32 fields and
       ADE state, FP64 DFT accumulation)
 """
 import numpy as np
 import pytest
 
-from conftest import relerr
+from conftest import check, relerr
 
 pytestmark = pytest.mark.gpu
 
@@ -54,7 +56,9 @@ def test_staged_fp64_vs_reference_fortran(golden, case):
     e = vec_err(sim.field, ref, (0, 1, 2))
     h = vec_err(sim.field, ref, (3, 4, 5))
     d = vec_err(sim.field, ref, (6, 7, 8))
-    assert e < 1e-10 and h < 1e-10 and d < 1e-12, (e, h, d)
+    check(f"{case} E", e, 1e-12)
+    check(f"{case} H", h, 1e-12)
+    check(f"{case} D", d, 1e-12)
     for p in ids:
         assert relerr(sim.flux(p), z["flux"][p]) < 1e-12
     for p in pids:
@@ -68,7 +72,8 @@ def test_staged_fp64_c1_twin_vs_reference_fortran(golden):
     plane = lambda q: sim.field(q)[:, g["ny"] // 2, :].T  # noqa: E731
     ref = lambda q: z["fy_" + FIELDS[q]]  # noqa: E731
     e, h = vec_err(plane, ref, (0, 1, 2)), vec_err(plane, ref, (3, 4, 5))
-    assert e < 1e-10 and h < 1e-10, (e, h)
+    check("ref_c1_twin E", e, 1e-12)
+    check("ref_c1_twin H", h, 1e-12)
     assert relerr(sim.flux(ids[0]), z["flux"][0]) < 1e-12
     for p in pids:
         assert relerr(sim.probe(p), z["probes"][p]) < 1e-12
@@ -93,11 +98,12 @@ def test_fused_all_faces_vs_oracle(precision):
     spec = _small_sphere()
     gsim, gids, gp = _run_gpu(spec, precision=precision)
     osim, oids, op = _run_oracle(spec)
-    ftol, stol = (1e-10, 1e-12) if precision == "fp64" else (2e-4, 1e-4)
+    ftol, stol = (1e-12, 1e-12) if precision == "fp64" else (2e-4, 1e-5)
     e = vec_err(gsim.field, osim.field, (0, 1, 2))
     h = vec_err(gsim.field, osim.field, (3, 4, 5))
-    assert e < ftol and h < ftol, (e, h)
-    assert relerr(gsim.flux(gids[0]), osim.flux(oids[0])) < stol
+    check(f"small sphere {precision} E", e, ftol)
+    check(f"small sphere {precision} H", h, ftol)
+    check(f"small sphere {precision} flux", relerr(gsim.flux(gids[0]), osim.flux(oids[0])), stol)
     for a, b in zip(gp, op):
         assert relerr(gsim.probe(a), osim.probe(b)) < max(stol, 10 * stol if precision == "fp32" else stol)
 
@@ -124,10 +130,9 @@ def test_periodic_vacuum_box_fused_vs_staged_vs_oracle():
     a, _, _ = _run_gpu(spec, engine="fused")
     b, _, _ = _run_gpu(spec, engine="staged")
     o, _, _ = _run_oracle(spec)
-    for sim in (a, b):
-        e = vec_err(sim.field, o.field, (0, 1, 2))
-        h = vec_err(sim.field, o.field, (3, 4, 5))
-        assert e < 1e-11 and h < 1e-11, (e, h)
+    for name, sim in (("fused", a), ("staged", b)):
+        check(f"periodic box {name} E", vec_err(sim.field, o.field, (0, 1, 2)), 1e-12)
+        check(f"periodic box {name} H", vec_err(sim.field, o.field, (3, 4, 5)), 1e-12)
 
 
 def test_kubo_graphene_over_disks_fused_vs_oracle():
@@ -143,10 +148,9 @@ def test_kubo_graphene_over_disks_fused_vs_oracle():
     spec["dft"] = [{"axis": 2, "index": 32, "omega": spec["dft"][0]["omega"], "comps": [0, 1, 3, 4]}]
     gsim, gids, _ = _run_gpu(spec)
     osim, oids, _ = _run_oracle(spec)
-    e = vec_err(gsim.field, osim.field, (0, 1, 2))
-    h = vec_err(gsim.field, osim.field, (3, 4, 5))
-    assert e < 1e-10 and h < 1e-10, (e, h)
-    assert relerr(gsim.flux(gids[0]), osim.flux(oids[0])) < 1e-12
+    check("graphene disks E", vec_err(gsim.field, osim.field, (0, 1, 2)), 1e-12)
+    check("graphene disks H", vec_err(gsim.field, osim.field, (3, 4, 5)), 1e-12)
+    check("graphene disks flux", relerr(gsim.flux(gids[0]), osim.flux(oids[0])), 1e-12)
 
 
 def test_energy_matches_oracle():
@@ -155,7 +159,8 @@ def test_energy_matches_oracle():
     o, _, _ = _run_oracle(spec)
     ge, gh = g.energy()
     oe, oh = o.energy()
-    assert abs(ge - oe) / oe < 1e-10 and abs(gh - oh) / oh < 1e-10
+    check("energy E", abs(ge - oe) / oe, 1e-12)
+    check("energy H", abs(gh - oh) / oh, 1e-12)
 
 
 def _ljp_all_faces(nsteps):
@@ -176,12 +181,11 @@ def test_fused_lorentz_jp_vs_oracle(precision):
     spec = _ljp_all_faces(600)
     gsim, gids, gp = _run_gpu(spec, precision=precision, engine="fused")
     osim, oids, op = _run_oracle(spec)
-    ftol, stol = (1e-10, 1e-12) if precision == "fp64" else (2e-4, 1e-4)
-    e = vec_err(gsim.field, osim.field, (0, 1, 2))
-    h = vec_err(gsim.field, osim.field, (3, 4, 5))
-    assert e < ftol and h < ftol, (e, h)
+    ftol, stol = (1e-12, 1e-12) if precision == "fp64" else (2e-4, 1e-5)
+    check(f"lorentz-jp {precision} E", vec_err(gsim.field, osim.field, (0, 1, 2)), ftol)
+    check(f"lorentz-jp {precision} H", vec_err(gsim.field, osim.field, (3, 4, 5)), ftol)
     for a, b in zip(gids, oids):
-        assert relerr(gsim.flux(a), osim.flux(b)) < stol
+        check(f"lorentz-jp {precision} flux {a}", relerr(gsim.flux(a), osim.flux(b)), stol)
     for a, b in zip(gp, op):
         assert relerr(gsim.probe(a), osim.probe(b)) < (stol if precision == "fp64" else 1e-3)
 

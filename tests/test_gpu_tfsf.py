@@ -3,14 +3,13 @@ run (golden vectors, the full 54,747 steps) and against the oracle on the
 orientation, engine, PML and precision cases.
 
 Tolerances, normwise: FP64 traces and spectra against the reference 1e-12;
-FP64 fields against the oracle 1e-10 (FMA contraction on the GPU); FP32
-fields 2e-4, FP32 traces 1e-3."""
+FP64 fields against the oracle 1e-12; FP32 fields 2e-4, FP32 traces 1e-3."""
 import math
 
 import numpy as np
 import pytest
 
-from conftest import relerr
+from conftest import check, relerr
 from test_oracle_tfsf import ORIENT, tfsf_box
 
 pytestmark = pytest.mark.gpu
@@ -60,7 +59,7 @@ def test_gpu_tfsf_orientations_vs_oracle(axis, comp, engine):
     spec = tfsf_box(axis, comp, True, nsteps=200)
     spec["tfsf_probes"] = [[0, 30]]
     g, o, gp, op = _both(spec, engine=engine)
-    assert _field_err(g, o) < 1e-10
+    check(f"tfsf axis {axis} comp {comp} {engine} fields", _field_err(g, o), 1e-12)
     e_g, h_g = g.tfsf_line(0)
     e_o, h_o = o.tfsf_line(0)
     assert relerr(e_g, e_o) < 1e-12 and relerr(h_g, h_o) < 1e-12
@@ -86,8 +85,8 @@ def _tfsf_through_pml(nsteps=240):
 def test_gpu_tfsf_through_all_faces_pml_vs_oracle(precision):
     spec = _tfsf_through_pml()
     g, o, gp, op = _both(spec, precision=precision, engine="fused")
-    tol = 1e-10 if precision == "fp64" else 2e-4
-    assert _field_err(g, o) < tol
+    tol = 1e-12 if precision == "fp64" else 2e-4
+    check(f"tfsf through pml {precision} fields", _field_err(g, o), tol)
     for a, b in zip(gp, op):
         assert relerr(g.probe(a), o.probe(b)) < (1e-12 if precision == "fp64" else 1e-3)
     # the block scatters back into the scattered-field region
@@ -102,4 +101,4 @@ def test_gpu_tfsf_with_reference_y_pml_vs_oracle():
     spec["pml"] = {"mode": "ref_y", "npml": [0, 10, 0], "order": 3.0, "R0": math.exp(-16.0),
                    "eps_low": 1.0, "eps_high": 1.0}
     g, o, _, _ = _both(spec)
-    assert _field_err(g, o) < 1e-10
+    check("tfsf ref-y pml fields", _field_err(g, o), 1e-12)
