@@ -34,6 +34,26 @@ def binding():
     return _b
 
 
+class complex_division:
+    """Context manager selecting how the oracle takes the real part of the
+    dispersive update's complex quotient: "modules" ((a*c + b*d) / (c*c +
+    d*d), as the reference's modules compiled here evaluate it; the default)
+    or "shipped" (C's complex division, as the build behind the reference's
+    shipped 2D output evaluates it)."""
+
+    def __init__(self, form):
+        self.mode = {"modules": 0, "shipped": 1}[form]
+
+    def __enter__(self):
+        dll = binding().dll
+        self.prev = C.c_int.in_dll(dll, "ofdtd_cdiv_mode").value
+        dll.ofdtd_set_cdiv_mode(C.c_int(self.mode))
+        return self
+
+    def __exit__(self, *exc):
+        binding().dll.ofdtd_set_cdiv_mode(C.c_int(self.prev))
+
+
 _b2 = None
 
 

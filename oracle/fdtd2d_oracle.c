@@ -52,6 +52,10 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* the complex quotient's real part in the form fdtd_oracle.c selects */
+double ofdtd_cdiv_re(double complex num, double complex den);
+#define cdiv_re ofdtd_cdiv_re
+
 #include "../include/fdtd2d.h"
 
 #define O2_PI (4.0 * atan(1.0))
@@ -871,10 +875,10 @@ static void update_materials(o2sim* s, double tnow) {
           s->ey[c] = s->dy[c] * m->gax;
         } else if (m->type == M2_DRUDE2CP) {
           double complex num = s->dx[c] - m->Ix[b] + m->ha * m->Sx[b] + m->gb[0] * m->gx[0][b] + m->gb[1] * m->gx[1][b];
-          const double ex = creal(num / m->ga);
+          const double ex = cdiv_re(num, m->ga);
           s->ex[c] = ex;
           num = s->dy[c] - m->Iy[b] + m->ha * m->Sy[b] + m->gb[0] * m->gy[0][b] + m->gb[1] * m->gy[1][b];
-          const double ey = creal(num / m->ga);
+          const double ey = cdiv_re(num, m->ga);
           s->ey[c] = ey;
           m->Ix[b] = m->Ix[b] + m->hb * ex;
           m->Sx[b] = m->ha * m->Sx[b] + m->hb * ex;
@@ -888,7 +892,7 @@ static void update_materials(o2sim* s, double tnow) {
           s->ey[c] = s->dy[c];
           const double complex num = s->dx[c] - m->Ix[b] + m->ha * m->Sx[b] + m->gb[0] * m->gx[0][b] +
                                      m->gb[1] * m->gx[1][b] + m->gb[2] * m->gx[2][b] + m->gb[3] * m->gx[3][b];
-          const double ex = creal(num / (m->ga + m->sn[b]));
+          const double ex = cdiv_re(num, m->ga + m->sn[b]);
           s->ex[c] = ex;
           m->sn[b] = m->raman1 * m->snm1[b] + m->raman2 * m->snm2[b] + m->raman0 * (ex * ex);
           m->snm2[b] = m->snm1[b];
@@ -1033,7 +1037,7 @@ static void tmz_materials(o2sim* s, double tnow) {
         } else if (m->type == M2_DRUDE2CP) {
           const double complex num =
               (s->dz[c] - m->Ix[b] + m->ha * m->Sx[b]) + m->gb[0] * m->gx[0][b] + m->gb[1] * m->gx[1][b];
-          const double e = creal(num / m->ga);
+          const double e = cdiv_re(num, m->ga);
           s->ez[c] = e;
           m->Ix[b] = m->Ix[b] + m->hb * e;
           m->Sx[b] = m->ha * m->Sx[b] + m->hb * e;
@@ -1042,7 +1046,7 @@ static void tmz_materials(o2sim* s, double tnow) {
         } else {
           const double complex num = (s->dz[c] - m->Ix[b] + m->ha * m->Sx[b]) + m->gb[0] * m->gx[0][b] +
                                      m->gb[1] * m->gx[1][b] + m->gb[2] * m->gx[2][b] + m->gb[3] * m->gx[3][b];
-          const double e = creal(num / (m->ga + m->sn[b]));
+          const double e = cdiv_re(num, m->ga + m->sn[b]);
           s->ez[c] = e;
           m->sn[b] = m->raman1 * m->snm1[b] + m->raman2 * m->snm2[b] + m->raman0 * (e * e);
           m->snm2[b] = m->snm1[b];

@@ -7,7 +7,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import ROOT, relerr
+from conftest import check, ROOT, relerr
 
 pytestmark = pytest.mark.gpu
 
@@ -84,8 +84,8 @@ def test_sweep_instance_fp64_matches_the_oracle():
     for a, b in zip(gi, oi):
         assert _max_rel(gs.flux(a), os_.flux(b)) <= 1e-12
     for comp in (sim2d.EX2, sim2d.EY2, sim2d.HZ2, sim2d.DX2, sim2d.DY2):
-        assert relerr(gs.field(comp), os_.field(comp)) <= 1e-11
-    assert _max_rel(gs.probe(gp[0])[:, 1:], os_.probe(op[0])[:, 1:]) <= 1e-11
+        check("test_gpu2d:87", relerr(gs.field(comp), os_.field(comp)), 1e-12)
+    check("test_gpu2d:88", _max_rel(gs.probe(gp[0])[:, 1:], os_.probe(op[0])[:, 1:]), 1e-12)
     gs.close()
     os_.close()
 
@@ -111,7 +111,7 @@ def test_non_periodic_cp4_graphene_fp64_matches_the_oracle():
     for a, b in zip(gi, oi):
         assert _max_rel(gs.flux(a), os_.flux(b)) <= 1e-12
     for comp in (sim2d.EX2, sim2d.EY2, sim2d.HZ2):
-        assert relerr(gs.field(comp), os_.field(comp)) <= 1e-11
+        check("test_gpu2d:114", relerr(gs.field(comp), os_.field(comp)), 1e-12)
     gs.close()
     os_.close()
 

@@ -4,7 +4,7 @@ nz = 1), FP32 spectra, and a Lorentz-Jp member of a batch."""
 import numpy as np
 import pytest
 
-from conftest import relerr
+from conftest import check, relerr
 from test_oracle2d_ljp import ljp_case
 
 pytestmark = pytest.mark.gpu
@@ -29,11 +29,11 @@ def test_lorentz_jp_fp64_matches_the_oracle(polarization, preset):
         os_.step(n)
     comps = (sim2d.EZ2, sim2d.HX2, sim2d.HY2) if polarization == "tmz" else (sim2d.EX2, sim2d.EY2, sim2d.HZ2)
     for c in comps:
-        assert relerr(gs.field(c), os_.field(c)) <= 1e-11, c
+        check(f"test_gpu2d_ljp:32 {c}", relerr(gs.field(c), os_.field(c)), 1e-12)
     for a, b in zip(gi, oi):
         assert _max_rel(gs.flux(a), os_.flux(b)) <= 1e-12
     for a, b in zip(gp, op):
-        assert _max_rel(gs.probe(a)[:, 1:], os_.probe(b)[:, 1:]) <= 1e-11
+        check("test_gpu2d_ljp:36", _max_rel(gs.probe(a)[:, 1:], os_.probe(b)[:, 1:]), 1e-12)
     gs.close()
     os_.close()
 

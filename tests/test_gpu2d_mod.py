@@ -1,14 +1,14 @@
 """The 2D GPU path against the reference's own 2D modules (golden vectors
 from tests/golden/gen_golden2d_mod.py): graphene CP4 + Raman with gold, and
 the si, film gold and graphene3 1 nm Lorentz-Jp modules.  FP64, fields and
-traces within 1e-11 of their maxima."""
+traces within 1e-12 of their maxima."""
 import json
 import os
 
 import numpy as np
 import pytest
 
-from conftest import ROOT, relerr
+from conftest import check, ROOT, relerr
 
 pytestmark = pytest.mark.gpu
 
@@ -23,7 +23,7 @@ def test_gpu_matches_the_reference_2d_modules(name):
     sim, _, pids = sim2d.make_simulation2d(spec, precision="fp64")
     sim.step(spec["nsteps"])
     for comp, key in ((sim2d.EX2, "Ex"), (sim2d.EY2, "Ey"), (sim2d.HZ2, "Hz")):
-        assert relerr(sim.field(comp), g[key]) <= 1e-11, (name, key)
+        check(f"test_gpu2d_mod:26 {(name, key)}", relerr(sim.field(comp), g[key]), 1e-12)
     for p, pid in enumerate(pids):
-        assert relerr(sim.probe(pid), g["probes"][p]) <= 1e-11, (name, p)
+        check(f"test_gpu2d_mod:28 {(name, p)}", relerr(sim.probe(pid), g["probes"][p]), 1e-12)
     sim.close()

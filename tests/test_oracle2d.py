@@ -25,12 +25,14 @@ def shipped_run():
     fe = spec["frame_every"]
     frames = {}
     n = 0
-    for f in range(1, 21):
-        sim.step(fe)
-        n += fe
-        if f in (5, 10, 20):
-            frames[f] = (sim.field(sim2d.EX2), sim.field(sim2d.HZ2))
-    sim.step(spec["nsteps"] - n)
+    # the shipped output comes from a build whose complex division is C's
+    with oracle.complex_division("shipped"):
+        for f in range(1, 21):
+            sim.step(fe)
+            n += fe
+            if f in (5, 10, 20):
+                frames[f] = (sim.field(sim2d.EX2), sim.field(sim2d.HZ2))
+        sim.step(spec["nsteps"] - n)
     out = {"spec": spec, "flux": [sim.flux(i) for i in ids], "probes": [sim.probe(p) for p in pids],
            "frames": frames}
     sim.close()

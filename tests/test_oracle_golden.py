@@ -1,10 +1,13 @@
 """The CPU oracle against the golden vectors written by the reference's own
 Fortran modules (tests/golden/gen_golden.py, oracle/_ref/ref3d).
 
-Tolerances (normwise): spectra and probe traces 1e-12; D fields 1e-12; E
-and H field vectors 1e-10 -- the reference's y-slab PML amplifies last-bit
-differences (two builds of the oracle, with and without FMA contraction,
-differ by up to 3e-11 on these cases, the same as oracle vs Fortran).
+Fields (E, H, D), spectra and probe traces are compared bit for bit: the
+oracle is built without multiply-add contraction, as the modules are (x86-64
+without FMA), follows their operation order, and takes the real part of the
+dispersive update's complex quotient as (a*c + b*d) / (c*c + d*d), as the
+compiled modules evaluate it (C's own complex division rounds differently).
+The running-DFT accumulators agree to 1e-13 of their scale (the cosine and
+sine of the two libraries' builds).
 """
 import numpy as np
 import pytest
@@ -30,18 +33,16 @@ def test_oracle_matches_reference_fortran(golden, case):
     spec, z = golden(case)
     sim, ids, pids = oracle.make_simulation(spec)
     sim.step(spec["nsteps"])
-    ref = lambda q: z["f_" + FIELDS[q]].transpose(2, 1, 0)  # noqa: E731
-    assert relerr(_vec(sim.field, (0, 1, 2)), _vec(ref, (0, 1, 2))) < 1e-10
-    assert relerr(_vec(sim.field, (3, 4, 5)), _vec(ref, (3, 4, 5))) < 1e-10
-    assert relerr(_vec(sim.field, (6, 7, 8)), _vec(ref, (6, 7, 8))) < 1e-12
+    for q in range(9):
+        assert np.array_equal(sim.field(q), z["f_" + FIELDS[q]].transpose(2, 1, 0)), FIELDS[q]
     for p in ids:
-        assert relerr(sim.flux(p), z["flux"][p]) < 1e-12
+        assert np.array_equal(sim.flux(p), z["flux"][p])
         for c, nm in ((0, "Ex"), (2, "Ez"), (3, "Hx"), (5, "Hz")):
             re, im = sim.dft(p, c)
             scale = np.abs(z[f"dft{p}_{nm}_re"]).max() + np.abs(z[f"dft{p}_{nm}_im"]).max()
-            assert np.abs(re.transpose(2, 1, 0) - z[f"dft{p}_{nm}_re"]).max() / scale < 1e-10
+            assert np.abs(re.transpose(2, 1, 0) - z[f"dft{p}_{nm}_re"]).max() / scale < 1e-13
     for p in pids:
-        assert relerr(sim.probe(p), z["probes"][p]) < 1e-12
+        assert np.array_equal(sim.probe(p), z["probes"][p])
 
 
 def test_oracle_matches_reference_fortran_96(golden):
@@ -51,14 +52,12 @@ def test_oracle_matches_reference_fortran_96(golden):
     spec, z = golden("ref_c1_twin")
     sim, ids, pids = oracle.make_simulation(spec)
     sim.step(spec["nsteps"])
-    assert relerr(sim.flux(ids[0]), z["flux"][0]) < 1e-12
+    assert np.array_equal(sim.flux(ids[0]), z["flux"][0])
     for p in pids:
-        assert relerr(sim.probe(p), z["probes"][p]) < 1e-12
+        assert np.array_equal(sim.probe(p), z["probes"][p])
     ny = spec["grid"]["ny"]
-    plane = lambda q: sim.field(q)[:, ny // 2, :].T  # noqa: E731
-    ref = lambda q: z["fy_" + FIELDS[q]]  # noqa: E731
-    assert relerr(_vec(plane, (0, 1, 2)), _vec(ref, (0, 1, 2))) < 1e-10
-    assert relerr(_vec(plane, (3, 4, 5)), _vec(ref, (3, 4, 5))) < 1e-10
+    for q in range(6):
+        assert np.array_equal(sim.field(q)[:, ny // 2, :].T, z["fy_" + FIELDS[q]]), FIELDS[q]
 
 
 def test_graphene_reference_init_order_gives_unit_gax(golden):
