@@ -535,7 +535,7 @@ int commit_impl(Sim* s) {
     s->onepass = !ljp && env && std::atoi(env) == 1;
     if (const char* e = std::getenv("FDTD_ONEPASS_BY")) {
       const int v = std::atoi(e);
-      if (v == 4 || v == 8 || v == 16) s->op_by = v;
+      if (v == 2 || v == 4 || (v == 8 && s->prec == FDTD_FP32)) s->op_by = v;
     }
     if (const char* e = std::getenv("FDTD_ONEPASS_KC")) {
       const int v = std::atoi(e);
@@ -709,6 +709,7 @@ int commit_impl(Sim* s) {
       }
     s->d_rowmat = (int32_t*)upload_bytes(s, rowmat.data(), rowmat.size() * sizeof(int32_t));
     if (!s->d_rowmat) return set_error(FDTD_ERR_NOMEM, "row material table");
+    s->h_rowmat = rowmat;
     std::vector<int32_t> mix;
     for (int k = 1; k <= s->nz; ++k)
       for (int j = 1; j <= s->ny; ++j)
@@ -890,6 +891,172 @@ void* field_ptr(Sim* s, int comp) {
 }
 
 }  // namespace
+
+// Tile lists of the one-pass step for tiles of 64 x by cells (x from 1 in
+// steps of 64, y from 1 in steps of by) and z ranges of at most kc planes;
+// 8 words per tile: i0, j0, k0, k1, xlo, xhi, then (plain tiles) the
+// material id and 0.
+//
+// Plain tiles cover x from the first 128-byte line after the low x PML
+// slab to the last line before the cell next to the high slab, in steps of
+// 64; the x edges outside that range are general.  A tile is plain when
+// every cell whose new E it computes -- the cells of its owned x range
+// [xlo, xhi] over its rows and planes, the cell after xhi, the row after
+// its rows, the plane after its z range, each wrapped to cell 1 on a
+// periodic axis -- is a non-PML cell in rows of one material whose three
+// components are dielectric, and no x- or y-normal source plane crosses
+// them (z-normal ones are added per plane); the z ranges end one plane
+// before the high z slab, so no such cell is a PML cell there.  General
+// work goes to the list as workgroups of one plane: 8 words each, x0, x1,
+// j0, j1, k and the workgroup's width (64, 32 or 16).
+int build_march_tiles(Sim* s, int by, int kc) {
+  if (s->tl_by == by && s->tl_kc == kc) return 0;
+  const int nx = s->nx, ny = s->ny, nz = s->nz;
+  const bool af = s->pml_mode == FDTD_PML_ALL_FACES;
+  const int Px = af ? s->P[0] : 0, Py = af ? s->P[1] : 0, Pz = af ? s->P[2] : 0;
+  const int pers[3] = {s->g.periodic[0] ? 1 : 0, s->g.periodic[1] ? 1 : 0, s->g.periodic[2] ? 1 : 0};
+  std::vector<std::pair<int, int>> zr;
+  auto segment = [&](int a, int b) {
+    for (int k = a; k <= b; k += kc) zr.push_back({k, std::min(b, k + kc - 1)});
+  };
+  if (Pz > 0) {
+    segment(1, Pz);
+    segment(Pz + 1, nz - Pz - 1);
+    segment(std::max(Pz + 1, nz - Pz), nz - Pz);
+    segment(nz - Pz + 1, nz);
+  } else {
+    segment(1, nz);
+  }
+  // x- and y-normal source planes, TFSF terms on D included
+  std::vector<std::pair<int, int>> planes;  // (axis, index)
+  for (const auto& src : s->srcs)
+    if (src.axis != 2) planes.push_back({src.axis, src.index});
+  for (const auto& t : s->tfsfs)
+    if (t.p.axis != 2) {
+      planes.push_back({t.p.axis, t.p.lo});
+      if (t.p.hi > 0) planes.push_back({t.p.axis, t.p.hi + 1});
+    }
+  auto plain_id = [&](int id) {
+    const MatEntry<double>& m = s->tab[(size_t)id];
+    return m.act[0] == ACT_DIEL && m.act[1] == ACT_DIEL && m.act[2] == ACT_DIEL;
+  };
+  auto pml_y = [&](int j) { return j <= Py || j > ny - Py; };
+  auto pml_z = [&](int k) { return k <= Pz || k > nz - Pz; };
+  // cell v of an axis of n as a tile sees it (0: a dead ghost)
+  auto wrap = [](int v, int n, int per) { return v <= n ? v : (v == n + 1 && per ? 1 : 0); };
+  // plain tiles start at the first 128-byte line after the low x slab and
+  // end with the last line before the cell next to the high slab (i = 1
+  // starts a line), so the general x edges are whole lines
+  const int le = (int)(128 / s->wsize);
+  const int xs = Px > 0 ? 1 + le * ((Px + le - 1) / le) : 1;
+  const int xe = Px > 0 ? le * ((nx - Px - 1) / le) : nx;
+  const int ntx = xe >= xs ? (xe - xs + 64) / 64 : 0, nty = (ny + by - 1) / by;
+  std::vector<int32_t> plain, gen;
+  struct GBox {
+    int x0, x1, j0, j1, k0, k1;
+  };
+  std::vector<GBox> gbox;
+  int64_t nplain = 0;
+  std::vector<int> ys, zs;
+  for (const auto& z : zr)
+    for (int ty = 0; ty < nty; ++ty)
+      for (int tx = -1; tx <= ntx; ++tx) {
+        const int j0 = 1 + by * ty, k0 = z.first, k1 = z.second;
+        const int jend = std::min(j0 + by - 1, ny);
+        if (tx < 0 || tx == ntx) {  // the x edges: general
+          const int a = tx < 0 ? 1 : std::max(xe + 1, xs), b = tx < 0 ? std::min(xs - 1, nx) : nx;
+          if (a <= b) gbox.push_back({a, b, j0, jend, k0, k1});
+          continue;
+        }
+        const int i0 = xs + 64 * tx;
+        const int iend = std::min(i0 + 63, xe);
+        const int xlo = i0, xhi = iend;
+        bool ok = true;
+        // rows and planes E is computed on: own ones, the row and the plane
+        // after (wrapped on a periodic axis); none may be a PML row or plane
+        ys.clear();
+        zs.clear();
+        for (int j = j0; j <= std::min(j0 + by - 1, ny); ++j) ys.push_back(j);
+        if (j0 + by - 1 <= ny) {
+          if (const int jh = wrap(j0 + by, ny, pers[1])) ys.push_back(jh);
+        } else if (pers[1]) {
+          ys.push_back(1);  // the wrapped row after ny inside the tile
+        }
+        for (int k = k0; k <= k1; ++k) zs.push_back(k);
+        if (const int kh = wrap(k1 + 1, nz, pers[2])) zs.push_back(kh);
+        for (int j : ys) ok = ok && !pml_y(j);
+        for (int k : zs) ok = ok && !pml_z(k);
+        // one plain material over every row computed
+        int mid = -1;
+        for (size_t a = 0; a < zs.size() && ok; ++a)
+          for (size_t b = 0; b < ys.size() && ok; ++b) {
+            const int rm = s->h_rowmat[(size_t)ys[b] + (size_t)(ny + 2) * zs[a]];
+            if (rm < 0 || !plain_id(rm) || (mid >= 0 && rm != mid)) ok = false;
+            mid = rm;
+          }
+        // x- and y-normal source planes through the cells computed (x: up
+        // to the cell after xhi, wrapped to 1 past nx)
+        for (const auto& pl : planes) {
+          if (!ok) break;
+          if (pl.first == 0) {
+            const int xn = xhi + 1 <= nx ? xhi + 1 : (pers[0] ? 1 : 0);  // the cell after xhi
+            ok = !(pl.second >= xlo && pl.second <= xhi) && pl.second != xn;
+          } else {
+            for (int j : ys) ok = ok && j != pl.second;
+          }
+        }
+        if (ok) {
+          const int32_t e[8] = {i0, j0, k0, k1, xlo, xhi, mid, 0};
+          plain.insert(plain.end(), e, e + 8);
+          nplain += (int64_t)(xhi - xlo + 1) * (jend - j0 + 1) * (k1 - k0 + 1);
+        } else {
+          gbox.push_back({i0, iend, j0, jend, k0, k1});
+        }
+      }
+  // general boxes: rows of the same x range and z range that follow each
+  // other merge; each plane of a box is cut into workgroups of 256 threads,
+  // 64 x 4 (wide boxes) or 16 x 16 cells (the x slabs' narrow boxes)
+  std::sort(gbox.begin(), gbox.end(), [](const GBox& u, const GBox& v) {
+    if (u.k0 != v.k0) return u.k0 < v.k0;
+    if (u.x0 != v.x0) return u.x0 < v.x0;
+    if (u.x1 != v.x1) return u.x1 < v.x1;
+    return u.j0 < v.j0;
+  });
+  std::vector<GBox> merged;
+  for (const GBox& g : gbox) {
+    if (!merged.empty()) {
+      GBox& m = merged.back();
+      if (m.k0 == g.k0 && m.k1 == g.k1 && m.x0 == g.x0 && m.x1 == g.x1 && m.j1 + 1 == g.j0) {
+        m.j1 = g.j1;
+        continue;
+      }
+    }
+    merged.push_back(g);
+  }
+  for (const GBox& g : merged) {
+    const int w = g.x1 - g.x0 + 1;
+    const int bw = w <= 16 ? 16 : (w <= 32 ? 32 : 64), bh = 256 / bw;
+    for (int k = g.k0; k <= g.k1; ++k)
+      for (int j = g.j0; j <= g.j1; j += bh)
+        for (int x = g.x0; x <= g.x1; x += bw) {
+          const int32_t e[8] = {x, std::min(x + bw - 1, g.x1), j, std::min(j + bh - 1, g.j1), k, bw, 0, 0};
+          gen.insert(gen.end(), e, e + 8);
+        }
+  }
+  if (s->d_tl_plain) dev_free(s, s->d_tl_plain);
+  if (s->d_tl_gen) dev_free(s, s->d_tl_gen);
+  s->d_tl_plain = s->d_tl_gen = nullptr;
+  s->n_tl_plain = (int)(plain.size() / 8);
+  s->n_tl_gen = (int)(gen.size() / 8);
+  if (s->n_tl_plain) s->d_tl_plain = (int32_t*)upload_bytes(s, plain.data(), plain.size() * sizeof(int32_t));
+  if (s->n_tl_gen) s->d_tl_gen = (int32_t*)upload_bytes(s, gen.data(), gen.size() * sizeof(int32_t));
+  if ((s->n_tl_plain && !s->d_tl_plain) || (s->n_tl_gen && !s->d_tl_gen))
+    return set_error(FDTD_ERR_NOMEM, "one-pass tile lists");
+  s->n_plain_cells = nplain;
+  s->tl_by = by;
+  s->tl_kc = kc;
+  return 0;
+}
 
 void lorentz_jp_coef(double dt, double ds, const fdtd_lorentz_jp& p, LjpCoef<double>& c) { ljp_coef_dt(dt, ds, p, c); }
 
@@ -1311,7 +1478,7 @@ int fdtd_autotune(fdtd_sim* s, int steps, int* out_by, int* out_kc) {
     return set_error(FDTD_ERR_STATE, "autotune runs before the first step, on untouched fields");
   FDTD_HIP(hipSetDevice(s->device));
   // candidates: the two-kernel step (by = 0) and one-pass tiles of 64 x by
-  // threads marching kc planes
+  // cells marching kc planes
   std::vector<std::pair<int, int>> cand;
   bool ljp = false;
   for (const auto& m : s->mats) ljp |= m.type == 3;
@@ -1323,8 +1490,9 @@ int fdtd_autotune(fdtd_sim* s, int steps, int* out_by, int* out_kc) {
   }
   if (s->engine == FDTD_ENGINE_FUSED && s->E2[0]) {
     for (int zm : {1, 2, 4, 8}) cand.push_back({0, zm});  // two-kernel step, zm planes per tile
-    for (int by : {4, 8, 16})
-      for (int kc : {2, 4, 16, 32, 64, 128}) {
+    for (int by : {2, 4, 8})
+      for (int kc : {16, 32, 64, 128}) {
+        if (by == 8 && s->prec != FDTD_FP32) continue;  // FP64 tiles stop at 4 rows
         if (kc > s->nz && kc > 16) continue;
         cand.push_back({by, kc});
       }
@@ -1566,9 +1734,13 @@ int fdtd_algorithmic_bytes(fdtd_sim* s, int kernel, double* bytes) {
     // per non-PML cell: read H(3) + read/write E(6); per PML cell: read the
     // other field (3), read/write 6 splits (12), write the main field (3).
     if (s->onepass) {
-      // one pass per step: read E, H and write E, H (12 words); a PML cell
-      // reads H, reads and writes 12 splits, writes E and H (33 words)
-      if (kernel == FDTD_K_E) b = w * (12.0 * (N - s->n_pml) + 33.0 * s->n_pml);
+      // plain tiles: a whole step of their (non-PML) cells per launch, 18
+      // words per cell (the two half steps' 9 each); general tiles: each
+      // half step as above.  The tile lists are those of the last step.
+      const double Np = (double)s->n_plain_cells;
+      if (kernel == FDTD_K_E) b = w * 18.0 * Np;
+      if (kernel == FDTD_K_GEN_E || kernel == FDTD_K_GEN_H)
+        b = w * (9.0 * (N - Np - s->n_pml) + 18.0 * s->n_pml);
     } else if (kernel == FDTD_K_E || kernel == FDTD_K_H) {
       b = w * (9.0 * (N - s->n_pml) + 18.0 * s->n_pml);
     }

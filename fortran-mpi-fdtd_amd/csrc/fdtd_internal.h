@@ -207,12 +207,22 @@ struct Sim {
   // whether fused kernels deal tiles to XCDs in contiguous runs
   int bx = 64, by = 4;
   int xcdmap = 1;
-  // one-pass step (FUSED engine): E and H of a step in one z-marching
-  // kernel over ping-pong buffers; E2/H2/d_split2 hold the other copy.
-  // Tiles are 64 x op_by threads (63 x (op_by-1) cells owned), op_kc planes
-  // deep.
+  // one-pass step (FUSED engine): E and H of a step read from one copy of
+  // the fields and written to the other (E2/H2/d_split2 hold the second
+  // copy).  The grid is cut into tiles of 64 x op_by cells (op_by in
+  // {2, 4, 8}) and z ranges of at most op_kc planes (broken at the z PML
+  // slabs).  Plain tiles (one dielectric material, no PML, no x- or
+  // y-normal source plane) are marched through their planes by one
+  // wavefront each, E and H in one pass; the other tiles run the fused
+  // cell updates, E then H, over the same two copies.
   int onepass = 0;
-  int op_by = 16, op_kc = 64;
+  int op_by = 4, op_kc = 64;
+  int tl_by = -1, tl_kc = -1;        // tile lists below built for these
+  int32_t* d_tl_plain = nullptr;     // i0, j0, k0, k1, xlo, xhi, material id, 0
+  int32_t* d_tl_gen = nullptr;       // workgroups: x0, x1, j0, j1, k, width (64 or 16), 0, 0
+  int n_tl_plain = 0, n_tl_gen = 0;
+  int64_t n_plain_cells = 0;         // cells owned by plain tiles
+  std::vector<int32_t> h_rowmat;     // host copy of d_rowmat
   // two-kernel step: E and H tiles span zm planes (1, 2, 4 or 8), each
   // thread marching its column and carrying the neighbour plane in registers
   int zm = 1;
@@ -258,6 +268,7 @@ int launch_dft_flush(Sim* s);      // fold every buffered DFT sample
 int launch_energy(Sim* s, double* we, double* wh);
 int launch_flux(Sim* s, int id, double* S);
 int launch_set_ghosts(Sim* s, int comp);
+int build_march_tiles(Sim* s, int by, int kc);  // one-pass tile lists (fdtd_api.hip)
 
 size_t estimate_bytes(const Sim* s);  // device bytes commit would allocate
 

@@ -133,11 +133,17 @@ __device__ __forceinline__ int pml_index(const int32_t* P, int i, int j, int k, 
 // z march: with use_carry, Hy and Hx of plane k-1 come from cy, cx (the
 // previous plane of the same thread) instead of memory; the cell leaves its
 // own Hy, Hx there for plane k+1.
+//
+// In place (the two-kernel step) EO* are E* and pw is p.  Over two copies
+// (the general tiles of the one-pass step, pp): the old E comes from EO*,
+// the old splits from p, the new E goes to E* and the new splits to pw, and
+// components no material writes are copied over.
 template <typename real, bool UNI>
-__device__ __forceinline__ void fused_e_cell(const FieldArgs<real>& a, const PmlArgs<real>& p, const real* sv,
-                                             const int* sa, int rm, int i, int j, int k, real& cy, real& cx,
-                                             bool use_carry, const real* H0, const real* H1, const real* H2,
-                                             real* E0, real* E1, real* E2) {
+__device__ __forceinline__ void fused_e_cell(const FieldArgs<real>& a, const PmlArgs<real>& p,
+                                             const PmlArgs<real>& pw, bool pp, const real* sv, const int* sa, int rm,
+                                             int i, int j, int k, real& cy, real& cx, bool use_carry,
+                                             const real* H0, const real* H1, const real* H2, const real* EO0,
+                                             const real* EO1, const real* EO2, real* E0, real* E1, real* E2) {
   const int c = i + a.px * j + a.pxy * k;
   const real hx = H0[c], hy = H1[c], hz = H2[c];
   const real hz_jm = H2[c - a.px], hx_jm = H0[c - a.px];
@@ -166,7 +172,7 @@ __device__ __forceinline__ void fused_e_cell(const FieldArgs<real>& a, const Pml
     sp[0] = p.s[S_EYX][s]; sp[1] = p.s[S_EZX][s]; sp[2] = p.s[S_EXY][s];
     sp[3] = p.s[S_EZY][s]; sp[4] = p.s[S_EXZ][s]; sp[5] = p.s[S_EYZ][s];
   } else if (kHoist) {
-    e0[0] = E0[c]; e0[1] = E1[c]; e0[2] = E2[c];
+    e0[0] = EO0[c]; e0[1] = EO1[c]; e0[2] = EO2[c];
     if (any_src) gx = a.gaxe[id];
   }
   if (pml) {
@@ -185,8 +191,8 @@ __device__ __forceinline__ void fused_e_cell(const FieldArgs<real>& a, const Pml
     if (sa[1] == 2) eyz = eyz + sv[1] * r;
     if (sa[2] == 0) ezx = ezx + sv[2] * r;
     if (sa[2] == 1) ezy = ezy + sv[2] * r;
-    p.s[S_EYX][s] = eyx; p.s[S_EZX][s] = ezx; p.s[S_EXY][s] = exy;
-    p.s[S_EZY][s] = ezy; p.s[S_EXZ][s] = exz; p.s[S_EYZ][s] = eyz;
+    pw.s[S_EYX][s] = eyx; pw.s[S_EZX][s] = ezx; pw.s[S_EXY][s] = exy;
+    pw.s[S_EZY][s] = ezy; pw.s[S_EXZ][s] = exz; pw.s[S_EYZ][s] = eyz;
     const real ex = exy + exz, ey = eyz + eyx, ez = ezx + ezy;
     E0[c] = ex;
     E1[c] = ey;
@@ -199,7 +205,7 @@ __device__ __forceinline__ void fused_e_cell(const FieldArgs<real>& a, const Pml
     return;
   }
   if (!kHoist) {
-    e0[0] = E0[c]; e0[1] = E1[c]; e0[2] = E2[c];
+    e0[0] = EO0[c]; e0[1] = EO1[c]; e0[2] = EO2[c];
     if (any_src) gx = a.gaxe[id];
   }
   // E_q += Ku*gax_q*curl_q (+ gax_q * source) for every component a
@@ -209,7 +215,14 @@ __device__ __forceinline__ void fused_e_cell(const FieldArgs<real>& a, const Pml
                        (hy - hy_im) + (-hx + hx_jm)};  // curlhyx + curlhxy
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    if (!(fl & (1u << q))) continue;
+    if (!(fl & (1u << q))) {
+      if (pp && !(fl & (16u << q))) {  // no writer: the value carries over
+        real* Eq = q == 0 ? E0 : (q == 1 ? E1 : E2);
+        Eq[c] = e0[q];
+        put_e_ghost(Eq, e0[q], i, j, k, c, a);
+      }
+      continue;
+    }
     real ev = e0[q] + cf.v[q] * cur[q];
     if (sa[q] >= 0) ev = ev + gx.v[q] * sv[q];
     real* Eq = q == 0 ? E0 : (q == 1 ? E1 : E2);
@@ -261,7 +274,7 @@ __device__ __forceinline__ void e_column(const FieldArgs<real>& a, const PmlArgs
     real sv[3] = {0, 0, 0};
     int sa[3] = {-1, -1, -1};
     plane_sources(src, nsrc, i, j, k, sv, sa);
-    fused_e_cell<real, true>(a, p, sv, sa, rm, i, j, k, cy, cx, have, H0, H1, H2, E0, E1, E2);
+    fused_e_cell<real, true>(a, p, p, false, sv, sa, rm, i, j, k, cy, cx, have, H0, H1, H2, E0, E1, E2, E0, E1, E2);
     have = true;
   }
 }
@@ -296,8 +309,8 @@ __global__ __launch_bounds__(256) void k_fused_e_mixed(FieldArgs<real> a, PmlArg
   int sa[3] = {-1, -1, -1};
   plane_sources(src, nsrc, i, j, k, sv, sa);
   real cy = 0, cx = 0;
-  fused_e_cell<real, false>(a, p, sv, sa, -1, i, j, k, cy, cx, false, a.h[0], a.h[1], a.h[2], a.e[0], a.e[1],
-                            a.e[2]);
+  fused_e_cell<real, false>(a, p, p, false, sv, sa, -1, i, j, k, cy, cx, false, a.h[0], a.h[1], a.h[2], a.e[0],
+                            a.e[1], a.e[2], a.e[0], a.e[1], a.e[2]);
 }
 
 // Monitor planes sampled by the FUSED H kernel: a cell on such a plane
@@ -338,10 +351,13 @@ __device__ __forceinline__ void dft_fuse_write(const DftFuse<real>& df, int i, i
 // cell.  z march: with use_carry, Ex and Ey of plane k come from cex, cey
 // (loaded as plane k+1 by the previous plane of the same thread); the cell
 // leaves Ex, Ey of plane k+1 there.
+// In place HO* are H* and pw is p; over two copies the old H and splits
+// come from HO* and p, the new ones go to H* and pw.
 template <typename real>
 __device__ __forceinline__ void fused_h_cell(const FieldArgs<real>& a, const PmlArgs<real>& p,
-                                             const DftFuse<real>& df, int i, int j, int k, real& cex, real& cey,
-                                             bool use_carry, const real* E0, const real* E1, const real* E2,
+                                             const PmlArgs<real>& pw, const DftFuse<real>& df, int i, int j, int k,
+                                             real& cex, real& cey, bool use_carry, const real* E0, const real* E1,
+                                             const real* E2, const real* HO0, const real* HO1, const real* HO2,
                                              real* H0, real* H1, real* H2) {
   const int c = i + a.px * j + a.pxy * k;
   const real ex = use_carry ? cex : E0[c], ey = use_carry ? cey : E1[c], ez = E2[c];
@@ -362,7 +378,7 @@ __device__ __forceinline__ void fused_h_cell(const FieldArgs<real>& a, const Pml
     sp[0] = p.s[S_HYX][s]; sp[1] = p.s[S_HZX][s]; sp[2] = p.s[S_HXY][s];
     sp[3] = p.s[S_HZY][s]; sp[4] = p.s[S_HXZ][s]; sp[5] = p.s[S_HYZ][s];
   } else if (kHoist) {
-    h0[0] = H0[c]; h0[1] = H1[c]; h0[2] = H2[c];
+    h0[0] = HO0[c]; h0[1] = HO1[c]; h0[2] = HO2[c];
   }
   if (pml) {
     // mod3/BERpmly_mod.f90:186-191 with the splits driven by total E
@@ -372,8 +388,8 @@ __device__ __forceinline__ void fused_h_cell(const FieldArgs<real>& a, const Pml
     real hzy = p.g1[1][j] * sp[3] + p.g2[1][j] * (-ex + ex_jp);
     real hxz = p.g1[2][k] * sp[4] + p.g2[2][k] * (-ey + ey_kp);
     real hyz = p.g1[2][k] * sp[5] + p.g2[2][k] * (ex + (-ex_kp));
-    p.s[S_HYX][s] = hyx; p.s[S_HZX][s] = hzx; p.s[S_HXY][s] = hxy;
-    p.s[S_HZY][s] = hzy; p.s[S_HXZ][s] = hxz; p.s[S_HYZ][s] = hyz;
+    pw.s[S_HYX][s] = hyx; pw.s[S_HZX][s] = hzx; pw.s[S_HXY][s] = hxy;
+    pw.s[S_HZY][s] = hzy; pw.s[S_HXZ][s] = hxz; pw.s[S_HYZ][s] = hyz;
     const real hx = hxy + hxz, hy = hyz + hyx, hz = hzx + hzy;
     H0[c] = hx;
     H1[c] = hy;
@@ -388,7 +404,7 @@ __device__ __forceinline__ void fused_h_cell(const FieldArgs<real>& a, const Pml
   const real curlezx = ez_ip - ez, curlexz = -ex_kp + ex;
   const real curlexy = ex_jp - ex, curleyx = -ey_ip + ey;
   if (!kHoist) {
-    h0[0] = H0[c]; h0[1] = H1[c]; h0[2] = H2[c];
+    h0[0] = HO0[c]; h0[1] = HO1[c]; h0[2] = HO2[c];
   }
   const real hx = h0[0] + a.ku * (curleyz + curlezy);
   const real hy = h0[1] + a.ku * (curlezx + curlexz);
@@ -415,7 +431,7 @@ __device__ __forceinline__ void h_column(const FieldArgs<real>& a, const PmlArgs
   for (int r = 0; r < KZ; ++r) {
     const int k = tz * KZ + r + 1;
     if (k > a.nz) break;
-    fused_h_cell(a, p, df, i, j, k, cex, cey, r > 0, E0, E1, E2, H0, H1, H2);
+    fused_h_cell(a, p, p, df, i, j, k, cex, cey, r > 0, E0, E1, E2, H0, H1, H2, H0, H1, H2);
   }
 }
 
@@ -430,459 +446,449 @@ __global__ __launch_bounds__(512) void k_fused_h(FieldArgs<real> a, PmlArgs<real
 }
 
 // ---------------------------------------------------------------------------
-// One-pass step (FUSED engine): the E half step and the H half step of one
-// time step in a single kernel, so each field crosses HBM once per step
-// (read E, H, write E, H: 12 words per interior cell instead of 18).
+// One-pass step (FUSED engine): the E and the H half step of a time step
+// read from one copy of the fields and written to the other (old: m.eo,
+// m.ho and the splits p.s; new: a.e, a.h and pw.s), so no cell reads a value
+// another workgroup has already replaced, whatever order they run in.  The
+// grid is cut into tiles of 64 x R cells over z ranges of at most kc planes
+// (lists built on the host, fdtd_api.hip build_march_tiles):
 //
-// Each block owns a tile of 63 x (BY-1) cells of an xy plane range and
-// marches a chunk of kc planes in z.  Its 64 x BY threads compute the new E
-// on the tile plus one halo column (i+1) and one halo row (j+1), and on the
-// plane after the chunk, because the new H of a cell needs the new E at i+1,
-// j+1 and k+1.  Halo values are computed, never stored: the fields, the PML
-// splits and the dispersive state live in two copies (old: read, new:
-// written), so a cell's neighbours read old values however the blocks are
-// ordered.  Per plane k the block computes E(k), stores it, puts Ex and Ez
-// in LDS (three rotating plane buffers, one barrier per plane) and then the
-// H of plane k-1: E(k-1) at i+1 by a lane shuffle, at j+1 from LDS, E(k)
-// and the old H(k-1) from registers (the register queue of the z march).
+//  plain tiles (one dielectric material, no PML cell among the cells they
+//  own, no x- or y-normal source plane; z-normal sources are added per
+//  plane) go to k_march_plain: one wavefront marches the tile through its
+//  planes and does E and H in one pass, so each field crosses HBM once per
+//  step (12 words per cell where the two-kernel step moves 18).  A plain
+//  tile owns the cells of its x range [xlo, xhi]; every cell whose new E
+//  it computes (its own, the row, plane and column after them) is a
+//  non-PML cell of its material.
 //
-// The arithmetic of every cell is that of k_fused_e / k_fused_h term for
-// term.  Dispersive components (flag bit 4+q) were written into the new E
-// by k_ade before this kernel; components nobody writes keep their value.
+//  general tiles (PML, dispersive or mixed materials, other sources, and
+//  the x-slab parts of plain tiles) go to k_gen_e and then k_gen_h, the
+//  fused cell updates of k_fused_e / k_fused_h over the two copies.
+//
+// The arithmetic of every cell is that of the two-kernel step term for
+// term, so both forms give the same bits.  Dispersive components were
+// written into the new E by k_ade before these kernels.
 template <typename real>
-struct OnePassArgs {
-  const real* eo[3];  // E after the previous step
-  const real* ho[3];  // H after the previous step
-  real* sn[S_N];      // split fields written by this step (p.s: read)
-  int32_t kc, ntx, nty, nch, tper, ntab;
+struct MarchArgs {
+  const real* eo[3];      // E after the previous step
+  const real* ho[3];      // H after the previous step
+  // 8 words per entry; plain tiles: i0, j0, k0, k1, xlo, xhi (owned x
+  // range), material id; general workgroups: x0, x1, j0, j1, k, width
+  const int32_t* tiles;
+  int32_t ntiles, tper;
 };
 
-// Workgroup barrier for LDS only: this kernel never reads global data that
-// another thread of the launch writes (old and new copies are distinct), so
-// the loads of the next plane stay in flight across it.
-__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-// Old fields around one cell of one plane, loaded a plane ahead of use.
+// lane l receives lane l-1 (lane 0: 0) / lane l+1 (lane 63: 0); every lane
+// of the wavefront must be active
 template <typename real>
-struct PlaneLoad {
-  real hx, hy, hz, hz_jm, hx_jm, hz_im, hy_im, ex, ey, ez;
-  uint32_t id;
-};
-
+__device__ __forceinline__ real lane_prev(real v) {
+  if constexpr (sizeof(real) == 4) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+  } else {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  }
+}
 template <typename real>
-__device__ __forceinline__ void load_plane(const OnePassArgs<real>& o, const FieldArgs<real>& a, int c,
-                                           PlaneLoad<real>& L) {
-  L.hx = o.ho[0][c];
-  L.hy = o.ho[1][c];
-  L.hz = o.ho[2][c];
-  L.hz_jm = o.ho[2][c - a.px];
-  L.hx_jm = o.ho[0][c - a.px];
-  L.hz_im = o.ho[2][c - 1];
-  L.hy_im = o.ho[1][c - 1];
-  L.ex = o.eo[0][c];
-  L.ey = o.eo[1][c];
-  L.ez = o.eo[2][c];
-  L.id = a.matid[c];
+__device__ __forceinline__ real lane_next(real v) {
+  if constexpr (sizeof(real) == 4) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+  } else {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), 0x130, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  }
+}
+// value of lane l (wave-uniform l)
+template <typename real>
+__device__ __forceinline__ real lane_at(real v, int l) {
+  if constexpr (sizeof(real) == 4) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+  } else {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  }
 }
 
-template <typename real, int BY>
-__global__ __launch_bounds__(64 * BY) void k_onepass(FieldArgs<real> a, PmlArgs<real> p, OnePassArgs<real> o,
-                                                     SrcArgs<real> src, int nsrc) {
-  constexpr int kTab = 64;  // material-table entries kept in LDS
-  __shared__ real lds[3][2][BY][64];  // [plane k % 3][Ex, Ez][row][lane]
-  __shared__ Vec4<real> tcoef[kTab], tgaxe[kTab];
-  __shared__ uint32_t tflags[kTab];
-  const int lane = threadIdx.x, row = threadIdx.y;
-  const int b = blockIdx.x;
-  const int t = a.xcdmap ? (b & 7) * o.tper + (b >> 3) : b;
-  if (t >= o.ntx * o.nty * o.nch) return;  // whole (padding) block
-  const bool lds_tab = o.ntab <= kTab;
-  if (lds_tab)
-    for (int q = lane + 64 * row; q < o.ntab; q += 64 * BY) {
-      tcoef[q] = a.coef[q];
-      tgaxe[q] = a.gaxe[q];
-      tflags[q] = a.flags[q];
-    }
-  __syncthreads();
-  const int tx = t % o.ntx;
-  const int r1 = t / o.ntx;
-  const int ty = r1 % o.nty;
-  const int tz = r1 / o.nty;
-  const int i0 = tx * 63 + lane + 1;
-  const int j0 = ty * (BY - 1) + __builtin_amdgcn_readfirstlane(row) + 1;  // a wavefront is one row
-  const int k0 = tz * o.kc + 1;
-  const int k1 = min(k0 + o.kc - 1, a.nz);
-  const bool own = lane < 63 && row < BY - 1 && i0 <= a.nx && j0 <= a.ny;
-  // physical cell of this lane: the halo column/row past the last cell wraps
-  // on a periodic axis and is a zero ghost otherwise
-  int i = i0, j = j0;
-  bool live = true;
-  if (i0 > a.nx) {
-    if (i0 == a.nx + 1 && a.per[0]) i = 1;
-    else live = false;
-  }
-  if (j0 > a.ny) {
-    if (j0 == a.ny + 1 && a.per[1]) j = 1;
-    else live = false;
-  }
-  const bool pmlxy = i <= p.P[0] || i > a.nx - p.P[0] || j <= p.P[1] || j > a.ny - p.P[1];
-  // plane index of march step k (wrapped past nz on a periodic z axis) and
-  // whether this lane computes a cell there
-  auto plane = [&](int k, int& kk) {
-    kk = k;
-    if (k > a.nz) {
-      if (!a.per[2]) return false;
-      kk = 1;
-    }
-    return live;
-  };
-  real hxp = 0, hyp = 0, hzp = 0;  // old H of plane k-1 at this cell
-  real exq = 0, eyq = 0, ezq = 0;  // new E of plane k-1 at this cell
-  if (live) {
-    const int c = i + a.px * j + a.pxy * (k0 - 1);
-    hxp = o.ho[0][c];
-    hyp = o.ho[1][c];
-    hzp = o.ho[2][c];
-  }
-  // one march step: E of plane k from `cur` (loaded a step earlier), the
-  // loads of plane k+1 into `nxt`, H of plane k-1.  Two register sets swap
-  // roles from step to step (no copies, which would wait for the loads).
-  auto march = [&](int k, const PlaneLoad<real>& cur, PlaneLoad<real>& nxt) __attribute__((always_inline)) {
-    int kk;
-    const bool lk = plane(k, kk);
-    int kn;
-    if (k + 1 <= k1 + 1 && plane(k + 1, kn)) load_plane(o, a, i + a.px * j + a.pxy * kn, nxt);  // z register queue
-    real ex = 0, ey = 0, ez = 0;
-    const bool store_e = own && k <= k1;
-    if (lk) {
-      const int c = i + a.px * j + a.pxy * kk;
-      real sv[3] = {0, 0, 0};
-      int sa[3] = {-1, -1, -1};
-      plane_sources(src, nsrc, i, j, kk, sv, sa);
-      const real hx = cur.hx, hy = cur.hy, hz = cur.hz;
-      const real hz_jm = cur.hz_jm, hx_jm = cur.hx_jm, hz_im = cur.hz_im, hy_im = cur.hy_im;
-      const real hy_km = hyp, hx_km = hxp;
-      const unsigned id = cur.id;
-      const uint32_t fl = lds_tab ? tflags[id] : a.flags[id];
-      const Vec4<real> cf = lds_tab ? tcoef[id] : a.coef[id];
-      if (pmlxy || kk <= p.P[2] || kk > a.nz - p.P[2]) {
-        // mod3/BERpmly_mod.f90:150-155 with the splits driven by total H
-        const int s = pml_index(p.P, i, j, kk, a.nx, a.ny, a.nz);
-        const real r = cf.v[3];
-        real eyx = p.f1[0][i] * p.s[S_EYX][s] + (p.f2[0][i] * r) * (-hz + hz_im);
-        real ezx = p.f1[0][i] * p.s[S_EZX][s] + (p.f2[0][i] * r) * (hy + (-hy_im));
-        real exy = p.f1[1][j] * p.s[S_EXY][s] + (p.f2[1][j] * r) * (hz + (-hz_jm));
-        real ezy = p.f1[1][j] * p.s[S_EZY][s] + (p.f2[1][j] * r) * (-hx + hx_jm);
-        real exz = p.f1[2][kk] * p.s[S_EXZ][s] + (p.f2[2][kk] * r) * (-hy + hy_km);
-        real eyz = p.f1[2][kk] * p.s[S_EYZ][s] + (p.f2[2][kk] * r) * (hx + (-hx_km));
-        if (sa[0] == 1) exy = exy + sv[0] * r;
-        if (sa[0] == 2) exz = exz + sv[0] * r;
-        if (sa[1] == 0) eyx = eyx + sv[1] * r;
-        if (sa[1] == 2) eyz = eyz + sv[1] * r;
-        if (sa[2] == 0) ezx = ezx + sv[2] * r;
-        if (sa[2] == 1) ezy = ezy + sv[2] * r;
-        if (store_e) {
-          o.sn[S_EYX][s] = eyx; o.sn[S_EZX][s] = ezx; o.sn[S_EXY][s] = exy;
-          o.sn[S_EZY][s] = ezy; o.sn[S_EXZ][s] = exz; o.sn[S_EYZ][s] = eyz;
-        }
-        ex = exy + exz;
-        ey = eyz + eyx;
-        ez = ezx + ezy;
-        if (store_e) {
-          a.e[0][c] = ex; a.e[1][c] = ey; a.e[2][c] = ez;
-          put_e_ghost(a.e[0], ex, i, j, kk, c, a);
-          put_e_ghost(a.e[1], ey, i, j, kk, c, a);
-          put_e_ghost(a.e[2], ez, i, j, kk, c, a);
-        }
-      } else {
-        const real e0[3] = {cur.ex, cur.ey, cur.ez};
-        const bool any_src = sa[0] >= 0 || sa[1] >= 0 || sa[2] >= 0;
-        Vec4<real> gx;
-        if (any_src) gx = lds_tab ? tgaxe[id] : a.gaxe[id];
-        const real cu[3] = {(hz - hz_jm) + (-hy + hy_km),   // curlhzy + curlhyz
-                            (hx - hx_km) + (-hz + hz_im),   // curlhxz + curlhzx
-                            (hy - hy_im) + (-hx + hx_jm)};  // curlhyx + curlhxy
-        real ev[3];
+// cell index v of an axis of n cells as a tile sees it: v <= n is the cell
+// itself; v = n+1 is cell 1 on a periodic axis and a dead (zero) ghost
+// otherwise
+__device__ __forceinline__ int march_map(int v, int n, int per, bool& live) {
+  live = v <= n || (v == n + 1 && per);
+  return v <= n ? v : 1;
+}
+
+// What a wavefront marches: its columns and rows (physical cells after the
+// periodic wrap) and which of them are live / owned.
+template <int R>
+struct MarchTile {
+  int ip, i0, ixh, j0, jyh, jrow[R], jxg, jxl, k0, k1;
+  bool lane_live, lane_own, xh_need, xg_live, yh_live, yh_need;
+  bool row_live[R], row_own[R];
+};
+
+// Old fields of one plane of a plain tile, loaded a plane ahead of use.
+template <typename real, int R>
+struct PlainPlane {
+  real ex[R], ey[R], ez[R], hx[R], hy[R], hz[R];
+  real yl_hz, yl_hx;                       // row j0-1
+  real yh_ex, yh_ez, yh_hx, yh_hy, yh_hz;  // row j0+R
+  real xh_ey, xh_ez, xh_hx, xh_hy, xh_hz;  // column i0+64, lane q: row j0-1+q
+  real xl_hz, xl_hy;                       // column i0-1, lane q: tile row q (q = R: row j0+R)
+};
+
+// Every address below lies inside the ghosted box (rows, columns and planes
+// are mapped to 0..n+1 before use), so the loads are unconditional.  A value
+// loaded for a dead lane, row or plane is read only by that position's own
+// new E, which is zeroed; nothing consumes the loaded registers early, so
+// the loads stay in flight until the plane that uses them.
+template <typename real, int R>
+__device__ __forceinline__ void plain_load(const FieldArgs<real>& a, const MarchArgs<real>& m, const MarchTile<R>& w,
+                                           int k, PlainPlane<real, R>& P) {
+  const int px = a.px;
+  const int kb = a.pxy * k;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          if (fl & (1u << q)) {
-            ev[q] = e0[q] + cf.v[q] * cu[q];
-            if (sa[q] >= 0) ev[q] = ev[q] + gx.v[q] * sv[q];
-          } else if (fl & (16u << q)) {
-            ev[q] = a.e[q][c];  // written by k_ade for this step
-            // wait here, in the rare dispersive branch, rather than where
-            // the branches merge (which would drain the plane queue)
-            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-          } else {
-            ev[q] = e0[q];
-          }
-          if (store_e && !(fl & (16u << q))) {
-            a.e[q][c] = ev[q];
-            put_e_ghost(a.e[q], ev[q], i, j, kk, c, a);
-          }
-        }
-        ex = ev[0];
-        ey = ev[1];
-        ez = ev[2];
-      }
-    }
-    lds[k % 3][0][row][lane] = ex;
-    lds[k % 3][1][row][lane] = ez;
-    // new E of plane k-1 at i+1 (the next lane; lane 63 owns no H)
-    const real ey_ip = __shfl_down(eyq, 1);
-    const real ez_ip = __shfl_down(ezq, 1);
-    lds_barrier();
-    if (k > k0 && own) {
-      const int kh = k - 1;
-      const int c = i + a.px * j + a.pxy * kh;
-      const real ex_jp = lds[(k - 1) % 3][0][row + 1][lane];
-      const real ez_jp = lds[(k - 1) % 3][1][row + 1][lane];
-      const real ey_kp = ey, ex_kp = ex;
-      real nhx, nhy, nhz;
-      if (pmlxy || kh <= p.P[2] || kh > a.nz - p.P[2]) {
-        // mod3/BERpmly_mod.f90:186-191 with the splits driven by total E
-        const int s = pml_index(p.P, i, j, kh, a.nx, a.ny, a.nz);
-        const real hyx = p.g1[0][i] * p.s[S_HYX][s] + p.g2[0][i] * (-ezq + ez_ip);
-        const real hzx = p.g1[0][i] * p.s[S_HZX][s] + p.g2[0][i] * (eyq + (-ey_ip));
-        const real hxy = p.g1[1][j] * p.s[S_HXY][s] + p.g2[1][j] * (ezq + (-ez_jp));
-        const real hzy = p.g1[1][j] * p.s[S_HZY][s] + p.g2[1][j] * (-exq + ex_jp);
-        const real hxz = p.g1[2][kh] * p.s[S_HXZ][s] + p.g2[2][kh] * (-eyq + ey_kp);
-        const real hyz = p.g1[2][kh] * p.s[S_HYZ][s] + p.g2[2][kh] * (exq + (-ex_kp));
-        o.sn[S_HYX][s] = hyx; o.sn[S_HZX][s] = hzx; o.sn[S_HXY][s] = hxy;
-        o.sn[S_HZY][s] = hzy; o.sn[S_HXZ][s] = hxz; o.sn[S_HYZ][s] = hyz;
-        nhx = hxy + hxz;
-        nhy = hyz + hyx;
-        nhz = hzx + hzy;
-      } else {
-        nhx = hxp + a.ku * ((ey_kp - eyq) + (-ez_jp + ezq));
-        nhy = hyp + a.ku * ((ez_ip - ezq) + (-ex_kp + exq));
-        nhz = hzp + a.ku * ((ex_jp - exq) + (-ey_ip + eyq));
-      }
-      a.h[0][c] = nhx;
-      a.h[1][c] = nhy;
-      a.h[2][c] = nhz;
-      put_h_ghost(a.h[0], nhx, i, j, kh, c, a);
-      put_h_ghost(a.h[1], nhy, i, j, kh, c, a);
-      put_h_ghost(a.h[2], nhz, i, j, kh, c, a);
-    }
-    exq = ex; eyq = ey; ezq = ez;
-    hxp = cur.hx; hyp = cur.hy; hzp = cur.hz;
-  };
-  PlaneLoad<real> A{}, B{};
-  {
-    int kk;
-    if (plane(k0, kk)) load_plane(o, a, i + a.px * j + a.pxy * kk, A);
+  for (int r = 0; r < R; ++r) {
+    const int c = kb + w.ip + px * w.jrow[r];
+    P.ex[r] = m.eo[0][c];
+    P.ey[r] = m.eo[1][c];
+    P.ez[r] = m.eo[2][c];
+    P.hx[r] = m.ho[0][c];
+    P.hy[r] = m.ho[1][c];
+    P.hz[r] = m.ho[2][c];
   }
-  for (int k = k0; k <= k1 + 1; k += 2) {
-    march(k, A, B);
-    if (k + 1 <= k1 + 1) march(k + 1, B, A);
+  {  // row j0-1 (a real row or the ghost row 0)
+    const int c = kb + w.ip + px * (w.j0 - 1);
+    P.yl_hz = m.ho[2][c];
+    P.yl_hx = m.ho[0][c];
+  }
+  if (w.yh_need) {  // row j0+R
+    const int c = kb + w.ip + px * w.jyh;
+    P.yh_ex = m.eo[0][c];
+    P.yh_ez = m.eo[2][c];
+    P.yh_hx = m.ho[0][c];
+    P.yh_hy = m.ho[1][c];
+    P.yh_hz = m.ho[2][c];
+  }
+  if (w.xh_need) {  // column i0+64, one row per lane 0..R (lane 0: row j0-1, Hx only)
+    const int c = kb + w.ixh + px * w.jxg;
+    P.xh_hx = m.ho[0][c];
+    P.xh_ey = m.eo[1][c];
+    P.xh_ez = m.eo[2][c];
+    P.xh_hy = m.ho[1][c];
+    P.xh_hz = m.ho[2][c];
+  }
+  {  // column i0-1 (a real column or the ghost column 0), one row per lane
+    const int c = kb + (w.i0 - 1) + px * w.jxl;
+    P.xl_hz = m.ho[2][c];
+    P.xl_hy = m.ho[1][c];
   }
 }
 
-// Short-chunk form of the one-pass step: each thread owns a column of KC
-// planes, issues every load of its KC+1 planes up front, computes their new
-// E (the last plane only for the H below it), meets the block once at an
-// LDS barrier and computes the KC new H.  The extra plane costs 1/KC more E
-// and H reads; the loads of a column are all in flight together, as in the
-// two-kernel step.  Same per-cell arithmetic as k_onepass.
+// registers carried from plane k-1 to plane k
+template <typename real, int R>
+struct PlainCarry {
+  real hx[R], hy[R], hz[R];  // old H of plane k-1
+  real ex[R], ey[R], ez[R];  // new E of plane k-1
+  real yh_hy, yh_ex, yh_ez;  // row j0+R: old Hy, new Ex, Ez of plane k-1
+  real xh_hx, xh_ey, xh_ez;  // column i0+64 (lane q: row j0-1+q): old Hx, new Ey, Ez of plane k-1
+};
+
+// material coefficients of a plain tile (wave-uniform) and this plane's
+// z-normal source terms, as fused_e_cell applies them to a dielectric cell
 template <typename real>
-struct NewE {
-  real x, y, z;
+struct PlainCoef {
+  real c[3];   // Ku*gax
+  real g[3];   // gax (source scaling)
+  real sv[3];  // sources of this plane per component
+  bool sa[3];
 };
 
 template <typename real>
-__device__ __forceinline__ NewE<real> onepass_e(const FieldArgs<real>& a, const PmlArgs<real>& p,
-                                                const OnePassArgs<real>& o, const SrcArgs<real>& src, int nsrc,
-                                                const PlaneLoad<real>& L, real hx_km, real hy_km, int i, int j,
-                                                int kk, bool store_e, bool pmlxy, const Vec4<real>* tcoef,
-                                                const Vec4<real>* tgaxe, const uint32_t* tflags, bool lds_tab) {
-  const int c = i + a.px * j + a.pxy * kk;
+__device__ __forceinline__ real plain_e(const PlainCoef<real>& q, int c, real e0, real cur) {
+  real ev = e0 + q.c[c] * cur;
+  if (q.sa[c]) ev = ev + q.g[c] * q.sv[c];
+  return ev;
+}
+
+template <typename real, int R>
+__device__ __forceinline__ void plain_plane(const FieldArgs<real>& a, const MarchArgs<real>& m,
+                                            const SrcArgs<real>& src, int nsrc, PlainCoef<real> q,
+                                            const MarchTile<R>& w, int lane, int k,
+                                            const PlainPlane<real, R>& cur, PlainPlane<real, R>& nxt,
+                                            PlainCarry<real, R>& c) {
+  if (k + 1 <= w.k1 + 1) {
+    bool lv;
+    plain_load(a, m, w, march_map(k + 1, a.nz, a.per[2], lv), nxt);
+  }
+  bool live_k;
+  const int kk = march_map(k, a.nz, a.per[2], live_k);
+  const bool store_k = k <= w.k1;
+  // this plane's z-normal sources (wave-uniform), in source order
+  q.sv[0] = q.sv[1] = q.sv[2] = 0;
+  q.sa[0] = q.sa[1] = q.sa[2] = false;
+  for (int s = 0; s < nsrc; ++s)
+    if (src.axis[s] == 2 && src.index[s] == kk) {
+      const int cq = src.comp[s];
+      const real v = src_value(src, s);
+      if (cq == 0) { q.sv[0] += v; q.sa[0] = true; }
+      else if (cq == 1) { q.sv[1] += v; q.sa[1] = true; }
+      else { q.sv[2] += v; q.sa[2] = true; }
+    }
+  // new E of plane k on the tile's rows
+  real exn[R], eyn[R], ezn[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const real hz_jm = r ? cur.hz[r - 1] : cur.yl_hz;
+    const real hx_jm = r ? cur.hx[r - 1] : cur.yl_hx;
+    real hz_im = lane_prev(cur.hz[r]);
+    real hy_im = lane_prev(cur.hy[r]);
+    const real lz = lane_at(cur.xl_hz, r), ly = lane_at(cur.xl_hy, r);
+    if (lane == 0) {
+      hz_im = lz;
+      hy_im = ly;
+    }
+    const bool live = live_k && w.lane_live && w.row_live[r];
+    const real hx = cur.hx[r], hy = cur.hy[r], hz = cur.hz[r];
+    const real ex = plain_e(q, 0, cur.ex[r], (hz - hz_jm) + (-hy + c.hy[r]));  // curlhzy + curlhyz
+    const real ey = plain_e(q, 1, cur.ey[r], (hx - c.hx[r]) + (-hz + hz_im));  // curlhxz + curlhzx
+    const real ez = plain_e(q, 2, cur.ez[r], (hy - hy_im) + (-hx + hx_jm));    // curlhyx + curlhxy
+    exn[r] = live ? ex : (real)0;
+    eyn[r] = live ? ey : (real)0;
+    ezn[r] = live ? ez : (real)0;
+  }
+  // new Ex, Ez of row j0+R
+  real yex = 0, yez = 0;
+  if (w.yh_need) {
+    real hy_im = lane_prev(cur.yh_hy);
+    const real ly = lane_at(cur.xl_hy, R);
+    if (lane == 0) hy_im = ly;
+    const bool live = live_k && w.lane_live && w.yh_live;
+    const real ex = plain_e(q, 0, cur.yh_ex, (cur.yh_hz - cur.hz[R - 1]) + (-cur.yh_hy + c.yh_hy));
+    const real ez = plain_e(q, 2, cur.yh_ez, (cur.yh_hy - hy_im) + (-cur.yh_hx + cur.hx[R - 1]));
+    yex = live ? ex : (real)0;
+    yez = live ? ez : (real)0;
+  }
+  // new Ey, Ez of column i0+64 (lanes 1..R: row j0-1+lane)
+  real xey = 0, xez = 0;
+  if (w.xh_need) {
+    real hzl = 0, hyl = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const real vz = lane_at(cur.hz[r], 63), vy = lane_at(cur.hy[r], 63);
+      if (lane == r + 1) {
+        hzl = vz;
+        hyl = vy;
+      }
+    }
+    const real hx_jm = lane_prev(cur.xh_hx);
+    const bool live = live_k && w.xg_live && lane >= 1 && lane <= R;
+    const real ey = plain_e(q, 1, cur.xh_ey, (cur.xh_hx - c.xh_hx) + (-cur.xh_hz + hzl));
+    const real ez = plain_e(q, 2, cur.xh_ez, (cur.xh_hy - hyl) + (-cur.xh_hx + hx_jm));
+    xey = live ? ey : (real)0;
+    xez = live ? ez : (real)0;
+  }
+  // stores of the new E of plane k
+  if (store_k) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (!(w.lane_own && w.row_own[r])) continue;
+      const int cc = w.ip + a.px * w.jrow[r] + a.pxy * kk;
+      a.e[0][cc] = exn[r];
+      a.e[1][cc] = eyn[r];
+      a.e[2][cc] = ezn[r];
+      put_e_ghost(a.e[0], exn[r], w.ip, w.jrow[r], kk, cc, a);
+      put_e_ghost(a.e[1], eyn[r], w.ip, w.jrow[r], kk, cc, a);
+      put_e_ghost(a.e[2], ezn[r], w.ip, w.jrow[r], kk, cc, a);
+    }
+  }
+  // new H of plane k-1 (myfdtd_func.f90:45-64)
+  if (k > w.k0) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const real ez_jp = r < R - 1 ? c.ez[r + 1] : c.yh_ez;
+      const real ex_jp = r < R - 1 ? c.ex[r + 1] : c.yh_ex;
+      real ez_ip = lane_next(c.ez[r]);
+      real ey_ip = lane_next(c.ey[r]);
+      const real xz = lane_at(c.xh_ez, r + 1), xy = lane_at(c.xh_ey, r + 1);
+      if (lane == 63) {
+        ez_ip = xz;
+        ey_ip = xy;
+      }
+      if (!(w.lane_own && w.row_own[r])) continue;
+      const real hx = c.hx[r] + a.ku * ((eyn[r] - c.ey[r]) + (-ez_jp + c.ez[r]));
+      const real hy = c.hy[r] + a.ku * ((ez_ip - c.ez[r]) + (-exn[r] + c.ex[r]));
+      const real hz = c.hz[r] + a.ku * ((ex_jp - c.ex[r]) + (-ey_ip + c.ey[r]));
+      const int kh = k - 1;
+      const int cc = w.ip + a.px * w.jrow[r] + a.pxy * kh;
+      a.h[0][cc] = hx;
+      a.h[1][cc] = hy;
+      a.h[2][cc] = hz;
+      put_h_ghost(a.h[0], hx, w.ip, w.jrow[r], kh, cc, a);
+      put_h_ghost(a.h[1], hy, w.ip, w.jrow[r], kh, cc, a);
+      put_h_ghost(a.h[2], hz, w.ip, w.jrow[r], kh, cc, a);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    c.hx[r] = cur.hx[r];
+    c.hy[r] = cur.hy[r];
+    c.hz[r] = cur.hz[r];
+    c.ex[r] = exn[r];
+    c.ey[r] = eyn[r];
+    c.ez[r] = ezn[r];
+  }
+  c.yh_hy = cur.yh_hy;
+  c.yh_ex = yex;
+  c.yh_ez = yez;
+  c.xh_hx = cur.xh_hx;
+  c.xh_ey = xey;
+  c.xh_ez = xez;
+}
+
+// Plain tiles, four independent wavefronts per workgroup (one tile each):
+// the wavefront owns the 64 x R cells of its tile over planes k0..k1; per
+// plane k it computes the new E of plane k and the new H of plane k-1,
+// carrying the old H and the new E of the plane below in registers while
+// the loads of plane k+1 are in flight.  Neighbours come from the row
+// before / after (registers), from lanes l-1 / l+1 (DPP wave shifts), and
+// from halos it loads and recomputes itself: row j0-1 (old Hz, Hx), row
+// j0+R (its new Ex, Ez), column i0-1 (old Hz, Hy: lanes 0..R gather one
+// row each), column i0+64 (its new Ey, Ez: lanes 1..R), plane k1+1 (its new
+// E).  No LDS and no barrier.
+template <typename real, int R>
+__global__ __launch_bounds__(256) void k_march_plain(FieldArgs<real> a, MarchArgs<real> m, SrcArgs<real> src,
+                                                     int nsrc) {
+  const int lane = threadIdx.x & 63;
+  const int b = blockIdx.x;
+  const int lb = a.xcdmap ? (b & 7) * m.tper + (b >> 3) : b;
+  const int tile = lb * 4 + (int)(threadIdx.x >> 6);
+  if (tile >= m.ntiles) return;  // whole wavefront
+  const int32_t* T = m.tiles + 8 * tile;
+  MarchTile<R> w;
+  w.i0 = T[0];
+  w.j0 = T[1];
+  w.k0 = T[2];
+  w.k1 = T[3];
+  const int xlo = T[4], xhi = T[5], mid = T[6];
+  PlainCoef<real> q;
+  {
+    const Vec4<real> cf = a.coef[mid], gx = a.gaxe[mid];
+    for (int d = 0; d < 3; ++d) {
+      q.c[d] = cf.v[d];
+      q.g[d] = gx.v[d];
+    }
+  }
+  // lanes xlo..xhi are owned; lane xhi+1 supplies the new E after the
+  // range (the next cell, or the wrapped cell 1)
+  w.ip = march_map(w.i0 + lane, a.nx, a.per[0], w.lane_live);
+  w.lane_live = w.lane_live && w.i0 + lane >= xlo && w.i0 + lane <= xhi + 1;
+  w.lane_own = w.i0 + lane >= xlo && w.i0 + lane <= xhi;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    w.jrow[r] = march_map(w.j0 + r, a.ny, a.per[1], w.row_live[r]);
+    w.row_own[r] = w.j0 + r <= a.ny;
+  }
+  // halo row j0+R: needed when the last tile row is owned
+  w.yh_need = w.j0 + R - 1 <= a.ny;
+  w.jyh = march_map(w.j0 + R, a.ny, a.per[1], w.yh_live);
+  // halo column i0+64: needed when lane 63 is owned
+  w.xh_need = xhi == w.i0 + 63;
+  bool xh_live;
+  w.ixh = march_map(w.i0 + 64, a.nx, a.per[0], xh_live);
+  // gather rows: column i0+64 lane q -> row j0-1+q (q = 0..R; lane 0 only
+  // supplies Hx), column i0-1 lane q -> tile row q (q < R) or the halo row
+  {
+    const int g = min(lane, R);
+    bool lv;
+    w.jxg = g == 0 ? w.j0 - 1 : march_map(w.j0 + g - 1, a.ny, a.per[1], lv);
+    w.xg_live = xh_live && (g == 0 || lv);
+    w.jxl = march_map(w.j0 + g, a.ny, a.per[1], lv);
+  }
+  PlainCarry<real, R> c;
+  {
+    // old Hx, Hy of plane k0-1 (a real plane or the ghost plane 0) for the
+    // k-1 terms of plane k0
+    const int kb = a.pxy * (w.k0 - 1);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int cc = kb + w.ip + a.px * w.jrow[r];
+      c.hx[r] = m.ho[0][cc];
+      c.hy[r] = m.ho[1][cc];
+      c.hz[r] = 0;
+      c.ex[r] = c.ey[r] = c.ez[r] = 0;
+    }
+    c.yh_hy = m.ho[1][kb + w.ip + a.px * w.jyh];
+    c.yh_ex = c.yh_ez = 0;
+    c.xh_hx = m.ho[0][kb + w.ixh + a.px * w.jxg];
+    c.xh_ey = c.xh_ez = 0;
+  }
+  PlainPlane<real, R> A, B;
+  plain_load(a, m, w, w.k0, A);
+  for (int k = w.k0; k <= w.k1 + 1; k += 2) {
+    plain_plane(a, m, src, nsrc, q, w, lane, k, A, B, c);
+    if (k + 1 <= w.k1 + 1) plain_plane(a, m, src, nsrc, q, w, lane, k + 1, B, A, c);
+  }
+}
+
+// General work: workgroup = 256 threads over cells x0..x1 x j0..j1 of
+// plane k (list entry: x0, x1, j0, j1, k, width), 64 x 4 cells (a
+// wavefront is one row), 32 x 8 or 16 x 16 (narrow boxes: the x edges),
+// one cell per thread, the fused cell updates over the two copies.  k_gen_e reads the
+// old H and E and writes the new E; k_gen_h then reads the new E and the
+// old H and writes the new H.
+template <typename real>
+__device__ __forceinline__ bool gen_cell(const FieldArgs<real>& a, const MarchArgs<real>& m, int& i, int& j, int& k,
+                                         bool& wide) {
+  const int b = blockIdx.x;
+  const int lb = a.xcdmap ? (b & 7) * m.tper + (b >> 3) : b;
+  if (lb >= m.ntiles) return false;
+  const int32_t* T = m.tiles + 8 * lb;
+  const int bw = T[5];
+  wide = bw == 64;
+  const int t = (int)threadIdx.x;
+  i = T[0] + (t & (bw - 1));
+  j = T[2] + (wide ? __builtin_amdgcn_readfirstlane(t >> 6) : t / bw);
+  k = T[4];
+  return i <= T[1] && j <= T[3];
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_gen_e(FieldArgs<real> a, PmlArgs<real> p, PmlArgs<real> pw,
+                                               MarchArgs<real> m, SrcArgs<real> src, int nsrc) {
+  int i, j, k;
+  bool wide;
+  if (!gen_cell(a, m, i, j, k, wide)) return;
   real sv[3] = {0, 0, 0};
   int sa[3] = {-1, -1, -1};
-  plane_sources(src, nsrc, i, j, kk, sv, sa);
-  const real hx = L.hx, hy = L.hy, hz = L.hz;
-  const unsigned id = L.id;
-  const uint32_t fl = lds_tab ? tflags[id] : a.flags[id];
-  const Vec4<real> cf = lds_tab ? tcoef[id] : a.coef[id];
-  NewE<real> r;
-  if (pmlxy || kk <= p.P[2] || kk > a.nz - p.P[2]) {
-    // mod3/BERpmly_mod.f90:150-155 with the splits driven by total H
-    const int s = pml_index(p.P, i, j, kk, a.nx, a.ny, a.nz);
-    const real m = cf.v[3];
-    real eyx = p.f1[0][i] * p.s[S_EYX][s] + (p.f2[0][i] * m) * (-hz + L.hz_im);
-    real ezx = p.f1[0][i] * p.s[S_EZX][s] + (p.f2[0][i] * m) * (hy + (-L.hy_im));
-    real exy = p.f1[1][j] * p.s[S_EXY][s] + (p.f2[1][j] * m) * (hz + (-L.hz_jm));
-    real ezy = p.f1[1][j] * p.s[S_EZY][s] + (p.f2[1][j] * m) * (-hx + L.hx_jm);
-    real exz = p.f1[2][kk] * p.s[S_EXZ][s] + (p.f2[2][kk] * m) * (-hy + hy_km);
-    real eyz = p.f1[2][kk] * p.s[S_EYZ][s] + (p.f2[2][kk] * m) * (hx + (-hx_km));
-    if (sa[0] == 1) exy = exy + sv[0] * m;
-    if (sa[0] == 2) exz = exz + sv[0] * m;
-    if (sa[1] == 0) eyx = eyx + sv[1] * m;
-    if (sa[1] == 2) eyz = eyz + sv[1] * m;
-    if (sa[2] == 0) ezx = ezx + sv[2] * m;
-    if (sa[2] == 1) ezy = ezy + sv[2] * m;
-    r.x = exy + exz;
-    r.y = eyz + eyx;
-    r.z = ezx + ezy;
-    if (store_e) {
-      o.sn[S_EYX][s] = eyx; o.sn[S_EZX][s] = ezx; o.sn[S_EXY][s] = exy;
-      o.sn[S_EZY][s] = ezy; o.sn[S_EXZ][s] = exz; o.sn[S_EYZ][s] = eyz;
-      a.e[0][c] = r.x; a.e[1][c] = r.y; a.e[2][c] = r.z;
-      put_e_ghost(a.e[0], r.x, i, j, kk, c, a);
-      put_e_ghost(a.e[1], r.y, i, j, kk, c, a);
-      put_e_ghost(a.e[2], r.z, i, j, kk, c, a);
-    }
-    return r;
-  }
-  const real e0[3] = {L.ex, L.ey, L.ez};
-  const bool any_src = sa[0] >= 0 || sa[1] >= 0 || sa[2] >= 0;
-  Vec4<real> gx;
-  if (any_src) gx = lds_tab ? tgaxe[id] : a.gaxe[id];
-  const real cu[3] = {(hz - L.hz_jm) + (-hy + hy_km),   // curlhzy + curlhyz
-                      (hx - hx_km) + (-hz + L.hz_im),   // curlhxz + curlhzx
-                      (hy - L.hy_im) + (-hx + L.hx_jm)};  // curlhyx + curlhxy
-  real ev[3];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    if (fl & (1u << q)) {
-      ev[q] = e0[q] + cf.v[q] * cu[q];
-      if (sa[q] >= 0) ev[q] = ev[q] + gx.v[q] * sv[q];
-    } else if (fl & (16u << q)) {
-      ev[q] = a.e[q][c];  // written by k_ade for this step
-      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): wait in this rare branch only
-    } else {
-      ev[q] = e0[q];
-    }
-    if (store_e && !(fl & (16u << q))) {
-      a.e[q][c] = ev[q];
-      put_e_ghost(a.e[q], ev[q], i, j, kk, c, a);
-    }
-  }
-  r.x = ev[0];
-  r.y = ev[1];
-  r.z = ev[2];
-  return r;
+  plane_sources(src, nsrc, i, j, k, sv, sa);
+  real cy = 0, cx = 0;
+  const int rm = wide ? a.rowmat[__builtin_amdgcn_readfirstlane(j + (a.ny + 2) * k)] : -1;
+  if (rm >= 0)
+    fused_e_cell<real, true>(a, p, pw, true, sv, sa, rm, i, j, k, cy, cx, false, a.h[0], a.h[1], a.h[2], m.eo[0],
+                             m.eo[1], m.eo[2], a.e[0], a.e[1], a.e[2]);
+  else
+    fused_e_cell<real, false>(a, p, pw, true, sv, sa, -1, i, j, k, cy, cx, false, a.h[0], a.h[1], a.h[2], m.eo[0],
+                              m.eo[1], m.eo[2], a.e[0], a.e[1], a.e[2]);
 }
 
-template <typename real, int BY, int KC>
-__global__ __launch_bounds__(64 * BY) void k_onepass_short(FieldArgs<real> a, PmlArgs<real> p, OnePassArgs<real> o,
-                                                           SrcArgs<real> src, int nsrc) {
-  constexpr int kTab = 64;
-  __shared__ real lds[KC][2][BY][64];  // new Ex, Ez of the KC owned planes
-  __shared__ Vec4<real> tcoef[kTab], tgaxe[kTab];
-  __shared__ uint32_t tflags[kTab];
-  const int lane = threadIdx.x, row = threadIdx.y;
-  const int b = blockIdx.x;
-  const int t = a.xcdmap ? (b & 7) * o.tper + (b >> 3) : b;
-  if (t >= o.ntx * o.nty * o.nch) return;  // whole (padding) block
-  const bool lds_tab = o.ntab <= kTab;
-  if (lds_tab)
-    for (int q = lane + 64 * row; q < o.ntab; q += 64 * BY) {
-      tcoef[q] = a.coef[q];
-      tgaxe[q] = a.gaxe[q];
-      tflags[q] = a.flags[q];
-    }
-  const int tx = t % o.ntx;
-  const int r1 = t / o.ntx;
-  const int ty = r1 % o.nty;
-  const int tz = r1 / o.nty;
-  const int i0 = tx * 63 + lane + 1;
-  const int j0 = ty * (BY - 1) + __builtin_amdgcn_readfirstlane(row) + 1;
-  const int k0 = tz * KC + 1;
-  const int np = min(KC, a.nz - k0 + 1);  // owned planes k0..k0+np-1
-  const bool own = lane < 63 && row < BY - 1 && i0 <= a.nx && j0 <= a.ny;
-  int i = i0, j = j0;
-  bool live = true;
-  if (i0 > a.nx) {
-    if (i0 == a.nx + 1 && a.per[0]) i = 1;
-    else live = false;
-  }
-  if (j0 > a.ny) {
-    if (j0 == a.ny + 1 && a.per[1]) j = 1;
-    else live = false;
-  }
-  const bool pmlxy = i <= p.P[0] || i > a.nx - p.P[0] || j <= p.P[1] || j > a.ny - p.P[1];
-  auto plane = [&](int k, int& kk) {
-    kk = k;
-    if (k > a.nz) {
-      if (!a.per[2]) return false;
-      kk = 1;
-    }
-    return live;
-  };
-  // every load of the column first
-  PlaneLoad<real> L[KC + 1];
-  bool lk[KC + 1];
-  int kq[KC + 1];
-  real hxb = 0, hyb = 0;  // old Hx, Hy of plane k0-1
-  if (live) {
-    const int c = i + a.px * j + a.pxy * (k0 - 1);
-    hxb = o.ho[0][c];
-    hyb = o.ho[1][c];
-  }
-#pragma unroll
-  for (int q = 0; q <= KC; ++q) {
-    lk[q] = q <= np && plane(k0 + q, kq[q]);
-    if (lk[q]) load_plane(o, a, i + a.px * j + a.pxy * kq[q], L[q]);
-  }
-  __syncthreads();  // the material table
-  // new E of planes k0..k0+np (the last one only for the H below it)
-  NewE<real> E[KC + 1];
-#pragma unroll
-  for (int q = 0; q <= KC; ++q) {
-    E[q] = NewE<real>{0, 0, 0};
-    if (lk[q])
-      E[q] = onepass_e(a, p, o, src, nsrc, L[q], q == 0 ? hxb : L[q - 1].hx, q == 0 ? hyb : L[q - 1].hy, i, j, kq[q],
-                       own && q < np, pmlxy, tcoef, tgaxe, tflags, lds_tab);
-    if (q < KC) {
-      lds[q][0][row][lane] = E[q].x;
-      lds[q][1][row][lane] = E[q].z;
-    }
-  }
-  real ey_ip[KC], ez_ip[KC];
-#pragma unroll
-  for (int q = 0; q < KC; ++q) {
-    ey_ip[q] = __shfl_down(E[q].y, 1);
-    ez_ip[q] = __shfl_down(E[q].z, 1);
-  }
-  lds_barrier();
-  if (!own) return;
-#pragma unroll
-  for (int q = 0; q < KC; ++q) {
-    if (q >= np) break;
-    const int kh = k0 + q;
-    const int c = i + a.px * j + a.pxy * kh;
-    const real ex_jp = lds[q][0][row + 1][lane];
-    const real ez_jp = lds[q][1][row + 1][lane];
-    const real exq = E[q].x, eyq = E[q].y, ezq = E[q].z;
-    const real ey_kp = E[q + 1].y, ex_kp = E[q + 1].x;
-    real nhx, nhy, nhz;
-    if (pmlxy || kh <= p.P[2] || kh > a.nz - p.P[2]) {
-      // mod3/BERpmly_mod.f90:186-191 with the splits driven by total E
-      const int s = pml_index(p.P, i, j, kh, a.nx, a.ny, a.nz);
-      const real hyx = p.g1[0][i] * p.s[S_HYX][s] + p.g2[0][i] * (-ezq + ez_ip[q]);
-      const real hzx = p.g1[0][i] * p.s[S_HZX][s] + p.g2[0][i] * (eyq + (-ey_ip[q]));
-      const real hxy = p.g1[1][j] * p.s[S_HXY][s] + p.g2[1][j] * (ezq + (-ez_jp));
-      const real hzy = p.g1[1][j] * p.s[S_HZY][s] + p.g2[1][j] * (-exq + ex_jp);
-      const real hxz = p.g1[2][kh] * p.s[S_HXZ][s] + p.g2[2][kh] * (-eyq + ey_kp);
-      const real hyz = p.g1[2][kh] * p.s[S_HYZ][s] + p.g2[2][kh] * (exq + (-ex_kp));
-      o.sn[S_HYX][s] = hyx; o.sn[S_HZX][s] = hzx; o.sn[S_HXY][s] = hxy;
-      o.sn[S_HZY][s] = hzy; o.sn[S_HXZ][s] = hxz; o.sn[S_HYZ][s] = hyz;
-      nhx = hxy + hxz;
-      nhy = hyz + hyx;
-      nhz = hzx + hzy;
-    } else {
-      nhx = L[q].hx + a.ku * ((ey_kp - eyq) + (-ez_jp + ezq));
-      nhy = L[q].hy + a.ku * ((ez_ip[q] - ezq) + (-ex_kp + exq));
-      nhz = L[q].hz + a.ku * ((ex_jp - exq) + (-ey_ip[q] + eyq));
-    }
-    a.h[0][c] = nhx;
-    a.h[1][c] = nhy;
-    a.h[2][c] = nhz;
-    put_h_ghost(a.h[0], nhx, i, j, kh, c, a);
-    put_h_ghost(a.h[1], nhy, i, j, kh, c, a);
-    put_h_ghost(a.h[2], nhz, i, j, kh, c, a);
-  }
+template <typename real>
+__global__ __launch_bounds__(256) void k_gen_h(FieldArgs<real> a, PmlArgs<real> p, PmlArgs<real> pw,
+                                               MarchArgs<real> m) {
+  int i, j, k;
+  bool wide;
+  if (!gen_cell(a, m, i, j, k, wide)) return;
+  const DftFuse<real> df{};
+  real cex = 0, cey = 0;
+  fused_h_cell(a, p, pw, df, i, j, k, cex, cey, false, a.e[0], a.e[1], a.e[2], m.ho[0], m.ho[1], m.ho[2], a.h[0],
+               a.h[1], a.h[2]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1767,39 +1773,51 @@ int step_impl(Sim* s) {
       hipLaunchKernelGGL((k_ade<real, true>), dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, ad,
                          ade_coef(g), g.d_cells, g.d_mask, (double*)g.d_state, g.ncells, src, nsrc);
     }
-    PmlArgs<real> p = pml_args<real>(s);
-    OnePassArgs<real> o{};
-    for (int q = 0; q < 3; ++q) {
-      o.eo[q] = a.e[q];
-      o.ho[q] = a.h[q];
-    }
-    for (int q = 0; q < S_N; ++q)
-      o.sn[q] = s->d_split2 ? (real*)s->d_split2 + (int64_t)q * s->npml_cells : nullptr;
-    const int BY = s->op_by;
-    o.kc = s->op_kc;
-    o.ntx = (s->nx + 62) / 63;
-    o.nty = (s->ny + BY - 2) / (BY - 1);
-    o.nch = (s->nz + o.kc - 1) / o.kc;
-    o.ntab = s->ntab;
-    const int tiles = o.ntx * o.nty * o.nch;
-    o.tper = (tiles + 7) / 8;
-    const dim3 grid((unsigned)(an.xcdmap ? 8 * o.tper : tiles));
     {
+      int rc = build_march_tiles(s, s->op_by, s->op_kc);
+      if (rc) return rc;
+    }
+    PmlArgs<real> p = pml_args<real>(s);  // old splits
+    PmlArgs<real> pw = p;                 // new splits
+    for (int q = 0; q < S_N; ++q)
+      pw.s[q] = s->d_split2 ? (real*)s->d_split2 + (int64_t)q * s->npml_cells : nullptr;
+    MarchArgs<real> mg{};
+    for (int q = 0; q < 3; ++q) {
+      mg.eo[q] = a.e[q];
+      mg.ho[q] = a.h[q];
+    }
+    const int R = s->op_by;
+    if (s->n_tl_plain) {
+      mg.tiles = s->d_tl_plain;
+      mg.ntiles = s->n_tl_plain;
+      const int blocks = (mg.ntiles + 3) / 4;  // four wavefronts per workgroup
+      mg.tper = (blocks + 7) / 8;
+      const dim3 grid((unsigned)(an.xcdmap ? 8 * mg.tper : blocks));
       Timer tm(s, FDTD_K_E);
-      if (o.kc == 2 || o.kc == 4) {
-        auto go = [&](auto kern, int by) { hipLaunchKernelGGL(kern, grid, dim3(64, by), 0, st, an, p, o, src, nsrc); };
-        if (o.kc == 2)
-          BY == 16 ? go(k_onepass_short<real, 16, 2>, 16) : BY == 8 ? go(k_onepass_short<real, 8, 2>, 8)
-                                                           : go(k_onepass_short<real, 4, 2>, 4);
-        else
-          BY == 16 ? go(k_onepass_short<real, 16, 4>, 16) : BY == 8 ? go(k_onepass_short<real, 8, 4>, 8)
-                                                           : go(k_onepass_short<real, 4, 4>, 4);
-      } else if (BY == 16)
-        hipLaunchKernelGGL((k_onepass<real, 16>), grid, dim3(64, 16), 0, st, an, p, o, src, nsrc);
-      else if (BY == 8)
-        hipLaunchKernelGGL((k_onepass<real, 8>), grid, dim3(64, 8), 0, st, an, p, o, src, nsrc);
-      else
-        hipLaunchKernelGGL((k_onepass<real, 4>), grid, dim3(64, 4), 0, st, an, p, o, src, nsrc);
+      if (R == 2) {
+        hipLaunchKernelGGL((k_march_plain<real, 2>), grid, dim3(256), 0, st, an, mg, src, nsrc);
+      } else if (R == 4) {
+        hipLaunchKernelGGL((k_march_plain<real, 4>), grid, dim3(256), 0, st, an, mg, src, nsrc);
+      } else {
+        if constexpr (sizeof(real) == 4)  // FP64 tiles stop at 4 rows (8 would spill)
+          hipLaunchKernelGGL((k_march_plain<real, 8>), grid, dim3(256), 0, st, an, mg, src, nsrc);
+      }
+    }
+    if (s->n_tl_gen) {
+      mg.tiles = s->d_tl_gen;
+      mg.ntiles = s->n_tl_gen;  // one workgroup each
+      mg.tper = (mg.ntiles + 7) / 8;
+      const dim3 grid((unsigned)(an.xcdmap ? 8 * mg.tper : mg.ntiles));
+      FieldArgs<real> ae = an;  // E: old H in, new E out
+      for (int q = 0; q < 3; ++q) ae.h[q] = a.h[q];
+      {
+        Timer tm(s, FDTD_K_GEN_E);
+        hipLaunchKernelGGL(k_gen_e<real>, grid, dim3(256), 0, st, ae, p, pw, mg, src, nsrc);
+      }
+      {
+        Timer tm(s, FDTD_K_GEN_H);
+        hipLaunchKernelGGL(k_gen_h<real>, grid, dim3(256), 0, st, an, p, pw, mg);
+      }
     }
     for (int q = 0; q < 3; ++q) {
       std::swap(s->E[q], s->E2[q]);

@@ -88,14 +88,17 @@ enum {
 
 /* Kernel ids for fdtd_kernel_stats. */
 enum {
-  FDTD_K_E = 0,       /* E half step (fused) or updateD (staged)   */
+  FDTD_K_E = 0,       /* E half step (fused), updateD (staged), or the
+                         one-pass step over its plain tiles            */
   FDTD_K_H = 1,       /* H half step (fused) or updateH (staged)   */
   FDTD_K_ADE = 2,     /* dispersive cells (gold, graphene)         */
   FDTD_K_PML_E = 3,   /* staged split-field E                       */
   FDTD_K_PML_H = 4,   /* staged split-field H                       */
   FDTD_K_MAT = 5,     /* staged dielectric E = D*gax + source       */
   FDTD_K_MON = 6,     /* DFT monitors and probes                    */
-  FDTD_K_COUNT = 7
+  FDTD_K_GEN_E = 7,   /* one-pass step: E of its general tiles      */
+  FDTD_K_GEN_H = 8,   /* one-pass step: H of its general tiles      */
+  FDTD_K_COUNT = 9
 };
 
 typedef struct fdtd_sim fdtd_sim;
@@ -318,9 +321,8 @@ int fdtd_commit(fdtd_sim* sim);
 /* Optional, no reference counterpart: pick the FUSED engine's step form by
  * timing `steps` steps of each candidate on this handle -- the two-kernel
  * step with tiles of kc = 1, 2, 4 or 8 planes (each thread marching its
- * column) and the one-pass step with tiles of 64 x by threads marching kc
- * planes (by in {4, 8, 16}, kc in {2, 4, 16, 32, 64, 128}; kc 2 and 4 use a
- * variant that issues a chunk's loads before its first store).  The timed
+ * column) and the one-pass step, each wavefront marching a tile of 64 x by
+ * cells through kc planes (by in {2, 4, 8}, kc in {16, 32, 64, 128}).  The timed
  * steps run without sources or monitors on the untouched (zero) fields and
  * leave them zero; call it after set-up and before the first fdtd_step (and
  * before any fdtd_set_field).  *by = 0 reports the two-kernel step (*kc its

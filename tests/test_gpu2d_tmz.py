@@ -16,8 +16,8 @@ def _max_rel(a, b):
 @pytest.mark.parametrize("periodic_x,graphene", [(False, "cp4"), (True, "kubo"), (True, "cp4")])
 def test_tmz_fp64_matches_the_oracle(periodic_x, graphene):
     """FP64: spectra within 1e-12, fields and traces within 1e-12 (relative
-    to each quantity's maximum; the GPU contracts a*b+c into FMAs, the oracle
-    does not), stepped in uneven calls across the 32-step graph blocks."""
+    to each quantity's maximum; both are built without multiply-add
+    contraction), stepped in uneven calls across the 32-step graph blocks."""
     import oracle
     from fdtd_amd import sim2d
 
