@@ -1863,9 +1863,10 @@ int step_impl(Sim* s) {
     }
     tfsf_e(st);
     // this step's monitor samples are written by the H kernel (not with a
-    // TFSF wave: its H correction follows the H kernel)
+    // TFSF wave: its H correction follows the H kernel); dry steps write
+    // them too, into the ring slot the next real step overwrites
     DftFuse<real> df{};
-    if (kFuseDftReal<real> && s->dft_fuse && !s->dry && s->tfsfs.empty()) {
+    if (kFuseDftReal<real> && s->dft_fuse && s->tfsfs.empty()) {
       for (auto& d : s->dfts) {
         if (df.n == kFuseDft) break;
         if (n < d.n_begin || (d.n_end > 0 && n > d.n_end)) continue;
@@ -1941,9 +1942,26 @@ int step_impl(Sim* s) {
     }
     tfsf_h(st, PmlArgs<real>{}, 0);
   }
-  // monitors: buffer this step's plane samples, fold a full block
+  // monitors: buffer this step's plane samples, fold a full block.  Dry
+  // steps (the autotuner's) sample into the ring slot the next real step
+  // overwrites, so each step form is timed with its sampling cost, and
+  // neither advance the ring, fold nor record probes.
   (void)t;
-  if (s->dry) return 0;
+  if (s->dry) {
+    for (auto& d : s->dfts) {
+      if (!d.fused_now && !(n < d.n_begin || (d.n_end > 0 && n > d.n_end))) {
+        CompList cl{};
+        int ns = 0;
+        for (int c = 0; c < 6; ++c)
+          if (d.mask & (1 << c)) cl.c[ns++] = c;
+        real* dst = (real*)d.d_samp + (int64_t)d.nbuf * ns * d.n1 * d.n2;
+        hipLaunchKernelGGL(k_dft_sample<real>, dim3((unsigned)(((int64_t)d.n1 * d.n2 + 255) / 256), (unsigned)ns),
+                           dim3(256), 0, st, a, d.axis, d.index, d.n1, d.n2, ns, cl, dst);
+      }
+      d.fused_now = false;
+    }
+    return 0;
+  }
   for (auto& d : s->dfts) {
     if (n < d.n_begin) continue;
     if (d.n_end > 0 && n > d.n_end) continue;

@@ -322,12 +322,13 @@ int fdtd_commit(fdtd_sim* sim);
  * timing `steps` steps of each candidate on this handle -- the two-kernel
  * step with tiles of kc = 1, 2, 4 or 8 planes (each thread marching its
  * column) and the one-pass step, each wavefront marching a tile of 64 x by
- * cells through kc planes (by in {2, 4, 8}, kc in {16, 32, 64, 128}).  The timed
- * steps run without sources or monitors on the untouched (zero) fields and
- * leave them zero; call it after set-up and before the first fdtd_step (and
- * before any fdtd_set_field).  *by = 0 reports the two-kernel step (*kc its
- * planes per tile).  Other engines:
- * returns 0 with *by = *kc = 0.                                             */
+ * cells through kc planes (by in {2, 4, 8}, FP64 {2, 4}; kc in {16, 32,
+ * 64, 128}).  The timed steps run without sources on the untouched (zero)
+ * fields and leave them zero; they sample the DFT planes as real steps do
+ * (into a buffer slot the first real step overwrites) but fold nothing and
+ * record no probes.  Call it after set-up and before the first fdtd_step
+ * (and before any fdtd_set_field).  *by = 0 reports the two-kernel step
+ * (*kc its planes per tile).  Other engines: returns 0 with *by = *kc = 0. */
 int fdtd_autotune(fdtd_sim* sim, int steps, int* by, int* kc);
 
 /* ---- time loop ----------------------------------------------------------- */
