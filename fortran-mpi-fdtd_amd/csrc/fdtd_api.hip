@@ -68,7 +68,7 @@ void dev_free(Sim* s, void* p) {
     }
 }
 
-double* scratch(Sim* s, size_t bytes) {
+double* reduction_buffer(Sim* s, size_t bytes) {
   if ((size_t)s->red_cap < bytes) {
     s->d_red = (double*)dev_alloc(s, bytes);
     s->red_cap = s->d_red ? (int)bytes : 0;

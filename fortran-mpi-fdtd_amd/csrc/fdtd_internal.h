@@ -260,13 +260,15 @@ int hip_check(hipError_t e, const char* what);
   } while (0)
 
 void* dev_alloc(Sim* s, size_t bytes);  // zero-filled; nullptr on failure
-double* scratch(Sim* s, size_t bytes); // reusable reduction buffer
+double* reduction_buffer(Sim* s, size_t bytes);  // reusable, grown with dev_alloc
 
 // kernels (fdtd_kernels.hip)
 int launch_step(Sim* s);           // one full time step at s->n_done + 1
 int launch_dft_flush(Sim* s);      // fold every buffered DFT sample
 int launch_energy(Sim* s, double* we, double* wh);
 int launch_flux(Sim* s, int id, double* S);
+int flux_check(Sim* s, int id);                            // the plane carries the four components
+int launch_flux_into(Sim* s, int id, double* dS, double* S);  // dS: nfreq doubles of device memory
 int launch_set_ghosts(Sim* s, int comp);
 int build_march_tiles(Sim* s, int by, int kc);  // one-pass tile lists (fdtd_api.hip)
 

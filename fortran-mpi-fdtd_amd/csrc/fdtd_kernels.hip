@@ -2017,17 +2017,34 @@ int launch_dft_flush(Sim* s) {
   return 0;
 }
 
+int flux_check(Sim* s, int id) {
+  const DftPlane& d = s->dfts[id];
+  const int axis = d.axis;
+  const int need[4] = {axis == 1 ? FDTD_HX : (axis == 2 ? FDTD_HY : FDTD_HZ),
+                       axis == 1 ? FDTD_EZ : (axis == 2 ? FDTD_EX : FDTD_EY),
+                       axis == 1 ? FDTD_HZ : (axis == 2 ? FDTD_HX : FDTD_HY),
+                       axis == 1 ? FDTD_EX : (axis == 2 ? FDTD_EY : FDTD_EZ)};
+  for (int q = 0; q < 4; ++q)
+    if (!(d.mask & (1 << need[q]))) return set_error(FDTD_ERR_ARG, "flux needs the four tangential components");
+  return 0;
+}
+
 int launch_flux(Sim* s, int id, double* S) {
+  int rc = flux_check(s, id);
+  if (rc) return rc;
+  double* dS = reduction_buffer(s, sizeof(double) * (size_t)s->dfts[id].nfreq);
+  if (!dS) return set_error(FDTD_ERR_NOMEM, "flux buffer");
+  return launch_flux_into(s, id, dS, S);
+}
+
+// The flux reduction of plane `id` into the device buffer dS (nfreq
+// doubles, each written whole), then copied to the host array S.
+int launch_flux_into(Sim* s, int id, double* dS, double* S) {
   DftPlane& d = s->dfts[id];
   int eA, eB, hA, hB;
   if (d.axis == 1) { hA = FDTD_HX; eA = FDTD_EZ; hB = FDTD_HZ; eB = FDTD_EX; }
   else if (d.axis == 2) { hA = FDTD_HY; eA = FDTD_EX; hB = FDTD_HX; eB = FDTD_EY; }
   else { hA = FDTD_HZ; eA = FDTD_EY; hB = FDTD_HY; eB = FDTD_EZ; }
-  int need[4] = {hA, eA, hB, eB};
-  for (int q = 0; q < 4; ++q)
-    if (!(d.mask & (1 << need[q]))) return set_error(FDTD_ERR_ARG, "flux needs the four tangential components");
-  double* dS = scratch(s, sizeof(double) * (size_t)d.nfreq);
-  if (!dS) return set_error(FDTD_ERR_NOMEM, "flux buffer");
   const int64_t np = (int64_t)d.n1 * d.n2;
   hipLaunchKernelGGL(k_flux, dim3((unsigned)d.nfreq), dim3(256), 0, s->stream, d.d_acc, np, d.nfreq, d.slot[hA],
                      d.slot[eA], d.slot[hB], d.slot[eB], s->ds, dS);
@@ -2039,7 +2056,7 @@ int launch_flux(Sim* s, int id, double* S) {
 
 int launch_energy(Sim* s, double* we, double* wh) {
   const int nb = 1024;
-  double* part = scratch(s, sizeof(double) * 2 * nb);
+  double* part = reduction_buffer(s, sizeof(double) * 2 * nb);
   if (!part) return set_error(FDTD_ERR_NOMEM, "energy buffer");
   if (s->prec == FDTD_FP32)
     hipLaunchKernelGGL(k_energy<float>, dim3(nb), dim3(256), 0, s->stream, field_args<float>(s), part);

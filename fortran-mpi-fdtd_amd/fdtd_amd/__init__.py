@@ -324,6 +324,14 @@ class Simulation:
         self.b.check(self.b.get_flux(self.h, int(plane), S.ctypes.data_as(abi._PD)))
         return S
 
+    def flux_noalloc(self, plane):
+        """fdtd_get_flux_noalloc: the flux through the library's fixed device
+        array, with no device allocation (the handle must be committed)."""
+        nf = self._dfts[plane][2]
+        S = np.empty(nf, dtype=np.float64)
+        self.b.check(self.b.get_flux_noalloc(self.h, int(plane), S.ctypes.data_as(abi._PD)))
+        return S
+
     def probe(self, pid):
         n = C.c_int()
         self.b.check(self.b.get_probe_count(self.h, int(pid), C.byref(n)))

@@ -174,6 +174,7 @@ PROTOTYPES = {
 
 # entry points only the HIP library has
 GPU_PROTOTYPES = {
+    "get_flux_noalloc": (_I, [_P, _I, _PD]),
     "memory_bytes": (_I, [_P, C.POINTER(C.c_size_t)]),
     "set_profiling": (_I, [_P, _I]),
     "kernel_stats": (_I, [_P, _I, _PD, C.POINTER(C.c_int64)]),

@@ -15,7 +15,7 @@ LIB = os.path.join(HERE, "libfdtd_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
 SOURCES = [os.path.join(HERE, "csrc", f) for f in ("fdtd_api.hip", "fdtd_kernels.hip", "fdtd_pool.hip",
-                                                   "fdtd2d.hip")]
+                                                   "fdtd2d.hip", "fdtd_noalloc.hip")]
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 instructions stay off on this target

@@ -357,6 +357,19 @@ int fdtd_get_dft(fdtd_sim* sim, int id, int comp, double* re, double* im);
  * (real*real + imag*imag).  Needs the four tangential components.        */
 int fdtd_get_flux(fdtd_sim* sim, int id, double* S);
 
+/* No counterpart in the reference: fdtd_get_flux for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The per-frequency sums go to a
+ * fixed device array of the library (FDTD_FLUX_NOALLOC_MAX_FREQ doubles,
+ * 8 KiB, one per GPU, held until the process ends) instead of a buffer the
+ * handle allocates; planes with more frequencies are refused
+ * (FDTD_ERR_ARG), and the handle must be committed (FDTD_ERR_STATE
+ * otherwise: committing allocates).  Every call on one GPU shares that
+ * array, so calls that overlap on one GPU race: serialize them.  The
+ * result equals fdtd_get_flux's bit for bit.                              */
+#define FDTD_FLUX_NOALLOC_MAX_FREQ 1024
+int fdtd_get_flux_noalloc(fdtd_sim* sim, int id, double* S);
+
 int fdtd_get_probe_count(fdtd_sim* sim, int id, int* nrec);
 int fdtd_get_probe(fdtd_sim* sim, int id, double* out /* nrec*7 */);
 
