@@ -69,12 +69,12 @@ def run_gpu(spec, precision, steps, warmup):
         "setup_s": setup_s,
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_onepass" if onepass else "k_fused_e",
+                     "kernel": "k_march_plain" if onepass else "k_fused_e",
                      "bytes_per_launch": e["bytes_per_launch"], "mean_launch_ms": e["mean_ms"]},
         "h_update": h_upd,
         "kernels": kern,
         "step_form": None if tuned is None else ({"two_kernel_planes_per_tile": tuned[1]} if tuned[0] == 0 else
-                                                 {"onepass_tile": [63, tuned[0] - 1], "z_chunk": tuned[1]}),
+                                                 {"onepass_tile": [64, tuned[0]], "z_chunk": tuned[1]}),
         "device_bytes": sim.memory_bytes(),
         "cell_counts": dict(zip(("pml", "ade"), sim.cell_counts())),
     }

@@ -270,6 +270,7 @@ int launch_flux(Sim* s, int id, double* S);
 int flux_check(Sim* s, int id);                            // the plane carries the four components
 int launch_flux_into(Sim* s, int id, double* dS, double* S);  // dS: nfreq doubles of device memory
 int launch_set_ghosts(Sim* s, int comp);
+int launch_nonfinite(Sim* s, int* d_flag);  // d_flag = 1 if E or H holds NaN/Inf (sticky)
 int build_march_tiles(Sim* s, int by, int kc);  // one-pass tile lists (fdtd_api.hip)
 
 size_t estimate_bytes(const Sim* s);  // device bytes commit would allocate

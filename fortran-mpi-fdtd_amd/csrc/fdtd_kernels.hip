@@ -1501,6 +1501,20 @@ __global__ __launch_bounds__(256) void k_flux(const double* acc, int64_t np, int
   if (threadIdx.x == 0) S[m] = red[0];
 }
 
+// Served-job sentinel: flag = 1 when any of the six field arrays holds a
+// NaN or Inf (ghosts and padding are zero).  Lanes that find one store 1
+// (the same value, so the race is harmless); the flag is never cleared.
+template <typename real>
+__global__ __launch_bounds__(256) void k_nonfinite(const real* e0, const real* e1, const real* e2, const real* h0,
+                                                   const real* h1, const real* h2, int64_t n, int* flag) {
+  const real* f[6] = {e0, e1, e2, h0, h1, h2};
+  bool bad = false;
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x)
+#pragma unroll
+    for (int q = 0; q < 6; ++q) bad |= !isfinite(f[q][c]);
+  if (bad) flag[0] = 1;
+}
+
 // Energy sums over interior cells; one partial pair per block.
 template <typename real>
 __global__ __launch_bounds__(256) void k_energy(FieldArgs<real> a, double* part) {
@@ -2073,6 +2087,21 @@ int launch_energy(Sim* s, double* we, double* wh) {
   }
   *we = a;
   *wh = b;
+  return 0;
+}
+
+int launch_nonfinite(Sim* s, int* d_flag) {
+  const int64_t n = s->ncell_alloc;
+  const dim3 grid((unsigned)std::min<int64_t>((n + 255) / 256, 4096));
+  if (s->prec == FDTD_FP32)
+    hipLaunchKernelGGL(k_nonfinite<float>, grid, dim3(256), 0, s->stream, (const float*)s->E[0],
+                       (const float*)s->E[1], (const float*)s->E[2], (const float*)s->H[0], (const float*)s->H[1],
+                       (const float*)s->H[2], n, d_flag);
+  else
+    hipLaunchKernelGGL(k_nonfinite<double>, grid, dim3(256), 0, s->stream, (const double*)s->E[0],
+                       (const double*)s->E[1], (const double*)s->E[2], (const double*)s->H[0],
+                       (const double*)s->H[1], (const double*)s->H[2], n, d_flag);
+  FDTD_HIP(hipGetLastError());
   return 0;
 }
 

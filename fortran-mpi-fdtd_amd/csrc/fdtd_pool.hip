@@ -4,8 +4,12 @@
 // footprint fits the HBM budget and a slot is free; admitted jobs each run on
 // their own HIP stream and advance in chunks, up to two queued per job, so
 // the chunks of co-resident jobs overlap on the device.
-// A finished job keeps its memory (and its budget share) until the caller
-// releases it after reading its results.  No counterpart in the reference.
+// After every chunk a check on the job's stream flags NaN or Inf in its E
+// and H; the flag comes back with the chunk's event, and a job whose fields
+// are no longer finite ends FAILED (FDTD_ERR_NONFINITE) without touching
+// the others.  A finished job keeps its memory (and its budget share) until
+// the caller releases it after reading its results.  No counterpart in the
+// reference.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,6 +40,10 @@ struct Job {
   int head = 0, inflight = 0;
   bool cancel = false;
   int err = 0;
+  // finite-field sentinel: device flag (on the handle, set by the check
+  // after each chunk) and its pinned host copies, one per queued chunk
+  int* d_flag = nullptr;
+  int* h_flag = nullptr;
 };
 
 // chunks kept queued per running job: the next one is on the device while
@@ -70,6 +78,12 @@ int advance(fdtd_pool* p, Job& j) {
       j.t_done = now_ms();
       return 0;
     }
+    if (j.h_flag[j.head]) {  // this chunk left NaN or Inf in the fields
+      j.err = set_error(FDTD_ERR_NONFINITE, "served job: fields are no longer finite");
+      j.state = FDTD_JOB_FAILED;
+      j.t_done = now_ms();
+      return 0;
+    }
     j.head ^= 1;
     --j.inflight;
   }
@@ -91,6 +105,18 @@ int advance(fdtd_pool* p, Job& j) {
     }
     j.left -= k;
     const int slot = (j.head + j.inflight) & 1;
+    // the finite-field check of this chunk, its flag copied back before the
+    // chunk's event
+    rc = launch_nonfinite(s, j.d_flag);
+    if (rc == 0 && hipMemcpyAsync(&j.h_flag[slot], j.d_flag, sizeof(int), hipMemcpyDeviceToHost, s->stream) !=
+                       hipSuccess)
+      rc = set_error(FDTD_ERR_HIP, "sentinel copy");
+    if (rc) {
+      j.err = rc;
+      j.state = FDTD_JOB_FAILED;
+      j.t_done = now_ms();
+      return 0;
+    }
     if (!j.ev[slot] && hipEventCreateWithFlags(&j.ev[slot], hipEventDisableTiming) != hipSuccess) {
       j.err = set_error(FDTD_ERR_HIP, "event create");
       j.state = FDTD_JOB_FAILED;
@@ -130,12 +156,14 @@ int fdtd_pool_create(fdtd_pool** out, size_t hbm_budget, int max_active, int chu
 int fdtd_pool_destroy(fdtd_pool* p) {
   if (!p) return 0;
   // events of jobs never released (their handles must still be alive)
-  for (auto& j : p->jobs)
+  for (auto& j : p->jobs) {
     for (int e = 0; e < 2; ++e)
       if (j.ev[e]) {
         (void)hipEventSynchronize(j.ev[e]);
         (void)hipEventDestroy(j.ev[e]);
       }
+    if (j.h_flag) (void)hipHostFree(j.h_flag);
+  }
   delete p;
   return 0;
 }
@@ -178,7 +206,17 @@ int fdtd_pool_progress(fdtd_pool* p, int* n_running, int* n_waiting) {
       j.t_done = j.t_start;
       continue;
     }
-    j.bytes = ((Sim*)j.sim)->dev_bytes;
+    Sim* s = (Sim*)j.sim;
+    j.d_flag = (int*)dev_alloc(s, sizeof(int));  // zero-filled, on the handle
+    if (!j.d_flag || hipHostMalloc((void**)&j.h_flag, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+      j.h_flag = nullptr;
+      j.err = set_error(FDTD_ERR_NOMEM, "finite-field sentinel");
+      j.state = FDTD_JOB_FAILED;
+      j.t_done = j.t_start;
+      continue;
+    }
+    j.h_flag[0] = j.h_flag[1] = 0;
+    j.bytes = s->dev_bytes;
     p->used += j.bytes;
     j.counted = true;
     j.state = FDTD_JOB_RUNNING;
@@ -237,6 +275,10 @@ int fdtd_pool_release(fdtd_pool* p, int ticket) {
       (void)hipEventDestroy(j.ev[e]);
       j.ev[e] = nullptr;
     }
+  if (j.h_flag) {  // its copies have landed: the events above are complete
+    (void)hipHostFree(j.h_flag);
+    j.h_flag = nullptr;
+  }
   j.inflight = 0;
   return 0;
 }

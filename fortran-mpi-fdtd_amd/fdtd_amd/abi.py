@@ -12,6 +12,7 @@ FDTD_ERR_STATE = -2
 FDTD_ERR_NOMEM = -3
 FDTD_ERR_HIP = -4
 FDTD_ERR_UNSUPPORTED = -5
+FDTD_ERR_NONFINITE = -6
 
 FP64, FP32 = 0, 1
 PML_NONE, PML_REF_Y, PML_ALL_FACES = 0, 1, 2

@@ -47,7 +47,8 @@ enum {
   FDTD_ERR_STATE = -2,       /* call not valid in the handle's state       */
   FDTD_ERR_NOMEM = -3,       /* device or host allocation failed           */
   FDTD_ERR_HIP = -4,         /* HIP runtime error                          */
-  FDTD_ERR_UNSUPPORTED = -5  /* combination not supported by this path     */
+  FDTD_ERR_UNSUPPORTED = -5, /* combination not supported by this path     */
+  FDTD_ERR_NONFINITE = -6    /* a served job's fields left the finite range */
 };
 
 /* ---- enumerations ---------------------------------------------------- */
@@ -398,8 +399,13 @@ int fdtd_estimate_bytes(fdtd_sim* sim, size_t* bytes);
  * the HBM budget and fewer than max_active run; every admitted handle runs
  * on its own stream and advances chunk_steps at a time, one chunk queued per
  * handle per fdtd_pool_progress call, so co-resident jobs overlap on the
- * device.  A finished job keeps its memory until fdtd_pool_release; the
- * caller owns the handles (read results, then release, then destroy).      */
+ * device.  After every chunk the pool checks the job's E and H for NaN or
+ * Inf (one read of the fields per chunk, on the job's stream): a job whose
+ * fields are no longer finite (an unstable time step, a diverging
+ * material) ends FDTD_JOB_FAILED and fdtd_pool_status returns
+ * FDTD_ERR_NONFINITE for it; the other jobs are not affected.  A finished
+ * job keeps its memory until fdtd_pool_release; the caller owns the handles
+ * (read results, then release, then destroy).                              */
 typedef struct fdtd_pool fdtd_pool;
 enum {
   FDTD_JOB_WAITING = 0, FDTD_JOB_RUNNING = 1, FDTD_JOB_DONE = 2,

@@ -124,3 +124,38 @@ def _estimate(sim):
     n = C.c_size_t()
     sim.b.check(sim.b.estimate_bytes(sim.h, C.byref(n)))
     return n.value
+
+
+@pytest.mark.gpu
+def test_a_diverging_job_fails_alone():
+    """A job stepped above the 3D Courant limit (S = 0.9 > 1/sqrt(3)) among
+    stable ones: the finite-field check after its chunks marks it FAILED
+    (FDTD_ERR_NONFINITE) while the others finish with the same spectra as
+    alone, bit for bit."""
+    import fdtd_amd
+    from fdtd_amd import abi, serve
+
+    jobs = _small_jobs(n=3)
+    bad = dict(jobs[1][1])
+    bad["grid"] = dict(bad["grid"], courant=0.9)
+    bad["nsteps"] = 600
+    specs = [jobs[0][1], bad, jobs[2][1]]
+    alone = serve.sequential_run([(0.0, jobs[0][1]), (0.0, jobs[2][1])])
+    sims = [fdtd_amd.make_simulation(s, precision="fp32") for s in specs]
+    pool = serve.Pool(1 << 40, max_active=3, chunk_steps=32)
+    tickets = [pool.submit(sim, spec["nsteps"]) for (sim, _, _), spec in zip(sims, specs)]
+    import time
+
+    t_end = time.time() + 120
+    while time.time() < t_end:
+        pool.progress()
+        if all(pool.status(t)[0] in (abi.JOB_DONE, abi.JOB_FAILED) for t in tickets):
+            break
+    st = [pool.status(t) for t in tickets]
+    assert st[1][0] == abi.JOB_FAILED and st[1][3] == abi.FDTD_ERR_NONFINITE
+    assert st[0][0] == abi.JOB_DONE and st[2][0] == abi.JOB_DONE
+    for (sim, ids, _), ref in zip((sims[0], sims[2]), alone["flux"]):
+        assert np.array_equal(sim.flux(ids[0]), ref)
+    for t in tickets:
+        pool.release(t)
+    pool.close()
