@@ -258,9 +258,11 @@ struct Batch {
   int ntiles = 0, nade = 0, nmon = 0, nlines = 0, ntf = 0;
   std::vector<int32_t> ctl;  // host mirror
   std::vector<char> inst;    // host image of the descriptors
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  hipGraphExec_t exec = nullptr;  // one captured 32-step block
   std::vector<DevBuf> allocs;
+  // batch_free deletes both solo batches and the public fdtd2d_batch
+  // (derived) through this type
+  virtual ~Batch() = default;
 };
 
 int sync_sim(Sim* s);
@@ -1765,6 +1767,9 @@ void enqueue_block(Batch* b) {
   }
 }
 
+// Captures one 32-step block and instantiates it.  The exec holds its own
+// copy of the nodes, so the captured graph is destroyed at once; on any
+// failure nothing is kept and the next call captures afresh.
 int capture(Batch* b) {
   if (b->exec) return 0;
   FDTD_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -1772,8 +1777,17 @@ int capture(Batch* b) {
     enqueue_block<float>(b);
   else
     enqueue_block<double>(b);
-  FDTD_HIP(hipStreamEndCapture(b->stream, &b->graph));
-  FDTD_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return hip_check(e, "hipStreamEndCapture");
+  }
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return hip_check(e, "hipGraphInstantiate");
+  b->exec = x;
   return 0;
 }
 
@@ -1839,7 +1853,6 @@ int batch_free(Batch* b) {
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
   for (auto& a : b->allocs) (void)hipFree(a.p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   for (Sim* s : b->sims)
