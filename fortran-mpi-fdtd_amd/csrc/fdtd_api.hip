@@ -709,7 +709,23 @@ int commit_impl(Sim* s) {
       }
     s->d_rowmat = (int32_t*)upload_bytes(s, rowmat.data(), rowmat.size() * sizeof(int32_t));
     if (!s->d_rowmat) return set_error(FDTD_ERR_NOMEM, "row material table");
-    s->h_rowmat = rowmat;
+    // runs of equal ids along each row (the one-pass tile classifier asks
+    // whether a 64-cell stretch of a row holds one material)
+    s->h_runoff.assign(rowmat.size() + 1, 0);
+    s->h_runx.clear();
+    s->h_runid.clear();
+    for (size_t r = 0; r < rowmat.size(); ++r) {
+      s->h_runoff[r] = (int32_t)s->h_runx.size();
+      const int j = (int)(r % (size_t)(s->ny + 2)), k = (int)(r / (size_t)(s->ny + 2));
+      if (j < 1 || j > s->ny || k < 1 || k > s->nz) continue;
+      const uint8_t* row = cid.data() + (size_t)(s->px * j + s->pxy * k);
+      for (int i = 1; i <= s->nx; ++i)
+        if (i == 1 || row[i] != row[i - 1]) {
+          s->h_runx.push_back(i);
+          s->h_runid.push_back(row[i]);
+        }
+    }
+    s->h_runoff[rowmat.size()] = (int32_t)s->h_runx.size();
     std::vector<int32_t> mix;
     for (int k = 1; k <= s->nz; ++k)
       for (int j = 1; j <= s->ny; ++j)
@@ -899,12 +915,18 @@ void* field_ptr(Sim* s, int comp) {
 //
 // Plain tiles cover x from the first 128-byte line after the low x PML
 // slab to the last line before the cell next to the high slab, in steps of
-// 64; the x edges outside that range are general.  A tile is plain when
+// 64, so that no 128-byte line is written partly by a plain and partly by a
+// general tile (partial-line stores cost far more than their bytes); the x
+// edges outside that range are general.  Plain tiles are listed row by row
+// (x fastest): the lines a tile gathers for its halo columns are its x
+// neighbours' own lines, which those load at about the same time (listed
+// column by column instead, the halo reads from HBM grow by 40 %).  A tile
+// is plain when
 // every cell whose new E it computes -- the cells of its owned x range
 // [xlo, xhi] over its rows and planes, the cell after xhi, the row after
 // its rows, the plane after its z range, each wrapped to cell 1 on a
-// periodic axis -- is a non-PML cell in rows of one material whose three
-// components are dielectric, and no x- or y-normal source plane crosses
+// periodic axis -- is a non-PML cell of one material whose three components
+// are dielectric, and no x- or y-normal source plane crosses
 // them (z-normal ones are added per plane); the z ranges end one plane
 // before the high z slab, so no such cell is a PML cell there.  General
 // work goes to the list as workgroups of one plane: 8 words each, x0, x1,
@@ -940,6 +962,16 @@ int build_march_tiles(Sim* s, int by, int kc) {
     const MatEntry<double>& m = s->tab[(size_t)id];
     return m.act[0] == ACT_DIEL && m.act[1] == ACT_DIEL && m.act[2] == ACT_DIEL;
   };
+  // the one material id of cells [a, b] of row (j, k), or -1
+  auto seg_id = [&](int j, int k, int a, int b) {
+    const size_t r = (size_t)j + (size_t)(ny + 2) * k;
+    const int lo = s->h_runoff[r], hi = s->h_runoff[r + 1];
+    if (lo == hi) return -1;
+    const int q = (int)(std::upper_bound(s->h_runx.begin() + lo, s->h_runx.begin() + hi, a) - s->h_runx.begin()) - 1;
+    if (q < lo) return -1;
+    const int end = q + 1 < hi ? s->h_runx[(size_t)q + 1] - 1 : nx;
+    return end >= b ? (int)s->h_runid[(size_t)q] : -1;
+  };
   auto pml_y = [&](int j) { return j <= Py || j > ny - Py; };
   auto pml_z = [&](int k) { return k <= Pz || k > nz - Pz; };
   // cell v of an axis of n as a tile sees it (0: a dead ghost)
@@ -969,8 +1001,8 @@ int build_march_tiles(Sim* s, int by, int kc) {
           continue;
         }
         const int i0 = xs + 64 * tx;
-        const int iend = std::min(i0 + 63, xe);
-        const int xlo = i0, xhi = iend;
+        const int xlo = i0, xhi = std::min(i0 + 63, xe);
+        if (xlo > xhi) continue;
         bool ok = true;
         // rows and planes E is computed on: own ones, the row and the plane
         // after (wrapped on a periodic axis); none may be a PML row or plane
@@ -986,12 +1018,16 @@ int build_march_tiles(Sim* s, int by, int kc) {
         if (const int kh = wrap(k1 + 1, nz, pers[2])) zs.push_back(kh);
         for (int j : ys) ok = ok && !pml_y(j);
         for (int k : zs) ok = ok && !pml_z(k);
-        // one plain material over every row computed
+        // one plain material over the cells computed: [xlo, xhi] and the
+        // cell after xhi (cell 1 past nx on a periodic x axis) of every row
         int mid = -1;
+        const int xa = std::min(xhi + 1, nx);
+        const bool wrap1 = xhi + 1 > nx && pers[0];
         for (size_t a = 0; a < zs.size() && ok; ++a)
           for (size_t b = 0; b < ys.size() && ok; ++b) {
-            const int rm = s->h_rowmat[(size_t)ys[b] + (size_t)(ny + 2) * zs[a]];
+            const int rm = seg_id(ys[b], zs[a], xlo, xa);
             if (rm < 0 || !plain_id(rm) || (mid >= 0 && rm != mid)) ok = false;
+            if (ok && wrap1 && seg_id(ys[b], zs[a], 1, 1) != rm) ok = false;
             mid = rm;
           }
         // x- and y-normal source planes through the cells computed (x: up
@@ -1010,7 +1046,7 @@ int build_march_tiles(Sim* s, int by, int kc) {
           plain.insert(plain.end(), e, e + 8);
           nplain += (int64_t)(xhi - xlo + 1) * (jend - j0 + 1) * (k1 - k0 + 1);
         } else {
-          gbox.push_back({i0, iend, j0, jend, k0, k1});
+          gbox.push_back({xlo, xhi, j0, jend, k0, k1});
         }
       }
   // general boxes: rows of the same x range and z range that follow each

@@ -222,7 +222,10 @@ struct Sim {
   int32_t* d_tl_gen = nullptr;       // workgroups: x0, x1, j0, j1, k, width (64 or 16), 0, 0
   int n_tl_plain = 0, n_tl_gen = 0;
   int64_t n_plain_cells = 0;         // cells owned by plain tiles
-  std::vector<int32_t> h_rowmat;     // host copy of d_rowmat
+  // material runs of every (j,k) row r = j + (ny+2)*k: runs h_runoff[r] ..
+  // h_runoff[r+1]-1, run q starting at cell h_runx[q] with id h_runid[q]
+  std::vector<int32_t> h_runoff, h_runx;
+  std::vector<uint8_t> h_runid;
   // two-kernel step: E and H tiles span zm planes (1, 2, 4 or 8), each
   // thread marching its column and carrying the neighbour plane in registers
   int zm = 1;

@@ -99,8 +99,11 @@ __device__ __forceinline__ bool in_pml(const int32_t* P, int i, int j, int k, in
 }
 
 // Compact index of an ALL_FACES PML cell: z slabs (full xy planes), then y
-// slabs of the middle planes (full x rows), then x slabs of the rest.
-// The count of PML cells is below the ghosted cell count, which fits 31 bits.
+// slabs of the middle planes (full x rows), then the low and then the high
+// x slab of the rest, each of those row after row (Px cells a row), so the
+// rows a workgroup of one x slab updates are contiguous in every split
+// array.  The count of PML cells is below the ghosted cell count, which
+// fits 31 bits.
 __device__ __forceinline__ int pml_index(const int32_t* P, int i, int j, int k, int nx, int ny, int nz) {
   const int Px = P[0], Py = P[1], Pz = P[2];
   if (k <= Pz || k > nz - Pz) {
@@ -113,10 +116,12 @@ __device__ __forceinline__ int pml_index(const int32_t* P, int i, int j, int k, 
     const int jj = j <= Py ? j - 1 : Py + (j - (ny - Py + 1));
     return baseY + (km * 2 * Py + jj) * nx + (i - 1);
   }
+  const int nym = ny - 2 * Py;
   const int baseX = baseY + (nz - 2 * Pz) * 2 * Py * nx;
   const int jm = j - Py - 1;
-  const int ii = i <= Px ? i - 1 : Px + (i - (nx - Px + 1));
-  return baseX + (km * (ny - 2 * Py) + jm) * 2 * Px + ii;
+  const bool hi = i > Px;
+  const int ii = hi ? i - (nx - Px + 1) : i - 1;
+  return baseX + (hi ? (nz - 2 * Pz) * nym * Px : 0) + (km * nym + jm) * Px + ii;
 }
 
 // ---------------------------------------------------------------------------
@@ -1772,6 +1777,30 @@ int step_impl(Sim* s) {
                          (double)n * s->dt, n1, n2);
     }
   };
+  // FP32 two-kernel steps: this step's monitor samples are written by the
+  // H kernel (not with a TFSF wave: its H correction follows it); dry steps
+  // write them too, into the ring slot the next real step overwrites
+  auto fused_monitors = [&]() {
+    DftFuse<real> df{};
+    if (kFuseDftReal<real> && s->dft_fuse && s->tfsfs.empty()) {
+      for (auto& d : s->dfts) {
+        if (df.n == kFuseDft) break;
+        if (n < d.n_begin || (d.n_end > 0 && n > d.n_end)) continue;
+        const int q = df.n++;
+        int ns = 0;
+        for (int c = 0; c < 6; ++c)
+          if (d.mask & (1 << c)) df.comp[q][ns++] = c;
+        df.axis[q] = d.axis;
+        df.index[q] = d.index;
+        df.n1[q] = d.n1;
+        df.ns[q] = ns;
+        df.np[q] = (int64_t)d.n1 * d.n2;
+        df.dst[q] = (real*)d.d_samp + (int64_t)d.nbuf * ns * df.np[q];
+        d.fused_now = true;
+      }
+    }
+    return df;
+  };
   if (s->engine == FDTD_ENGINE_FUSED && s->onepass) {
     // new fields go to the second copies, then the copies swap roles
     FieldArgs<real> an = a;
@@ -1801,6 +1830,8 @@ int step_impl(Sim* s) {
       mg.ho[q] = a.h[q];
     }
     const int R = s->op_by;
+    // monitor samples: the separate sample kernel (written from the march
+    // instead, they cost the plain tiles more than the launches they save)
     if (s->n_tl_plain) {
       mg.tiles = s->d_tl_plain;
       mg.ntiles = s->n_tl_plain;
@@ -1876,27 +1907,7 @@ int step_impl(Sim* s) {
                            ade_coef(g), g.d_cells, g.d_mask, (double*)g.d_state, g.ncells, src, nsrc);
     }
     tfsf_e(st);
-    // this step's monitor samples are written by the H kernel (not with a
-    // TFSF wave: its H correction follows the H kernel); dry steps write
-    // them too, into the ring slot the next real step overwrites
-    DftFuse<real> df{};
-    if (kFuseDftReal<real> && s->dft_fuse && s->tfsfs.empty()) {
-      for (auto& d : s->dfts) {
-        if (df.n == kFuseDft) break;
-        if (n < d.n_begin || (d.n_end > 0 && n > d.n_end)) continue;
-        const int q = df.n++;
-        int ns = 0;
-        for (int c = 0; c < 6; ++c)
-          if (d.mask & (1 << c)) df.comp[q][ns++] = c;
-        df.axis[q] = d.axis;
-        df.index[q] = d.index;
-        df.n1[q] = d.n1;
-        df.ns[q] = ns;
-        df.np[q] = (int64_t)d.n1 * d.n2;
-        df.dst[q] = (real*)d.d_samp + (int64_t)d.nbuf * ns * df.np[q];
-        d.fused_now = true;
-      }
-    }
+    const DftFuse<real> df = fused_monitors();
     {
       Timer tm(s, FDTD_K_H);
       if (s->zm == 8)

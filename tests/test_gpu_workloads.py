@@ -118,15 +118,18 @@ def test_c2_planes_are_all_exercised():
     assert all(np.abs(f).max() > 0 for f in o["flux"])
 
 
+@pytest.mark.parametrize("onepass", ["0", "1"])
 @pytest.mark.parametrize("name", ["c2", "c2_5planes"])
-def test_fp32_fused_monitor_samples_equal_the_sample_kernel(name, monkeypatch):
-    """FP32: the H kernel's monitor samples and the separate sample kernel
-    give bit-identical accumulators and spectra (three planes: all fused;
-    five: four fused, one not), and the spectra hold 1e-5 against the FP64
-    oracle."""
+def test_fp32_fused_monitor_samples_equal_the_sample_kernel(name, onepass, monkeypatch):
+    """FP32: the monitor samples written by the two-kernel step's H kernel
+    and by the separate sample kernel give bit-identical accumulators and
+    spectra (three planes: all fused; five: four fused, one not), and the
+    spectra hold 1e-5 against the FP64 oracle, in both step forms (the
+    one-pass step samples with the sample kernel either way)."""
     spec = SPECS[name]()
-    a = gpu_run(spec, "fp32", dft=True)
-    b = gpu_run(spec, "fp32", env={"FDTD_DFT_FUSE": "0"}, monkeypatch=monkeypatch, dft=True)
+    a = gpu_run(spec, "fp32", env={"FDTD_ONEPASS": onepass}, monkeypatch=monkeypatch, dft=True)
+    b = gpu_run(spec, "fp32", env={"FDTD_ONEPASS": onepass, "FDTD_DFT_FUSE": "0"}, monkeypatch=monkeypatch,
+                dft=True)
     for pa, pb in zip(a["dft"], b["dft"]):
         for x, y in zip(pa, pb):
             assert np.array_equal(x, y)

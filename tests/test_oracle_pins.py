@@ -146,10 +146,10 @@ def _eps_from_impulse(dt, e, w):
 
 
 def _gold_ep(w, p):
-    """Drude + 2 critical points (Vial & Laroche, Appl Phys A 103:849),
-    exp(-i w t) convention."""
+    """Drude + critical points (Vial & Laroche, Appl Phys A 103:849),
+    exp(-i w t) convention; as many critical points as p["A"] lists."""
     crit = 0
-    for q in range(2):
+    for q in range(len(p["A"])):
         crit = crit + p["A"][q] * p["Omega"][q] * (
             np.exp(1j * p["phi"][q]) / (p["Omega"][q] - w - 1j * p["Gamma"][q])
             + np.exp(-1j * p["phi"][q]) / (p["Omega"][q] + w + 1j * p["Gamma"][q]))
@@ -177,6 +177,42 @@ def test_gold_ade_converges_to_the_analytic_drude_2cp():
     assert (errs[1] < 0.035).all()
     ratio = errs[0] / errs[1]
     assert ((ratio > 1.7) & (ratio < 2.3)).all()  # first order in dt
+
+
+def test_graphene_cp4_ade_converges_to_the_analytic_cp4():
+    """3D graphene CP4 with its critical-point terms active
+    (ref_init_order 0: the coefficients set before they are read, as the 2D
+    module does, mod2/graphene_mod.f90:170-186): the discrete permittivity of
+    a 0D sheet cell (in-plane Ex) converges at first order in dt to the
+    closed-form Drude + 4 critical points of mod3/graphene_mod.f90:33-41
+    with the sheet scaling of scaleParams (:264-269): A_p and wD^2 times
+    thickness/ds.  Catches a dropped CP term, a wrong sign of phi, a wrong
+    scale factor or a swapped gb/gc."""
+    ap = [13.1068, 1.8417, 0.4191, 0.7737]
+    phip = [-1.8391, 0.3505, 3.0565, -0.1199]
+    omp = [6.1264e14, 6.6843e15, 5.6289e15, 1.118e16]
+    rp = [1e14, 7.905e14, 6.7526e14, 1.0005e14]
+    gr = {"model": "cp4", "normal_axis": 1, "ref_init_order": 0, "thickness": 3.4e-10, "eps_inf": 1.0,
+          "gamma": 9e15, "omega_d": 5.2796e15, "A": ap, "phi": phip, "Omega": omp, "Gamma": rp}
+    w = 2 * PI * C0 / np.array([500e-9, 800e-9, 1500e-9])
+    # the sheet scaling depends on ds, so dt is refined at fixed ds through
+    # the Courant number (a 0D cell is stable at any)
+    ds = 2e-9
+    sc = 3.4e-10 / ds
+    analytic = _gold_ep(w, {"eps_inf": 1.0, "omega_d": math.sqrt(sc) * 5.2796e15, "gamma": 9e15,
+                            "A": [sc * a for a in ap], "phi": phip, "Omega": omp, "Gamma": rp})
+    drude = 1.0 - sc * 5.2796e15 ** 2 / (w * w + 1j * 9e15 * w)
+    errs = []
+    for courant in (0.5, 0.25):
+        _, dt, e = _impulse("graphene", ds, int(50.0 / 1e14 / (courant * ds / C0)), gr, courant=courant)
+        assert abs(e[-1]) < 1e-12 * np.abs(e).max()
+        eps = _eps_from_impulse(dt, e, w)
+        errs.append(np.abs(eps - analytic) / np.abs(analytic))
+        # the CP terms matter: without them the response is another one
+        assert (np.abs(eps - drude) > 5 * np.abs(eps - analytic)).all()
+    assert (errs[1] < 0.02).all(), errs
+    ratio = errs[0] / errs[1]
+    assert ((ratio > 1.7) & (ratio < 2.3)).all(), ratio  # first order in dt
 
 
 def _lorentz_eps(w, p):
