@@ -107,13 +107,17 @@ def ref_graphene_small(nsteps=900):
     }
 
 
-def ref_ljp_metals(nsteps=1000, titanium=False):
+def ref_ljp_metals(nsteps=1000, titanium=False, eform=False):
     """Lorentz-Jp metals (gold antenna, Si and film-gold slabs) between air
     and SiO2 along y, y-PML 10 with x/z grading depth 2, periodic x/z.
     Every cell has exactly one writer (no reference driver calls
     update_EJp, so the case keeps E-form and D-form regions apart).
     titanium=True puts the titanium table (mod3/titaniumJp_mod.f90, negative
-    damping on pole 1: the recursion grows without bound) in row 50."""
+    damping on pole 1: the recursion grows without bound) in row 50.
+    eform=True makes every dielectric but the rows around the source (which
+    adds to D) the E-form one written for these simulations
+    (mod3/simpledielectricLorentzJp_mod.f90, "form": "E"): the same material
+    to the library, E += Ku*ga*curl H in the reference."""
     nx, ny, nz = 8, 72, 6
     ds = 3.999 * NANO
     P = 10
@@ -134,8 +138,16 @@ def ref_ljp_metals(nsteps=1000, titanium=False):
         {"type": "lorentz_jp", "region": box((1, 51, 1), (nx, 52, nz)), "params": ljp("film_gold")},
         {"type": "dielectric", "region": box((1, 53, 1), (nx, ny - P, nz)), "eps": 1.46 ** 2},
     ]
+    if eform:
+        # the source adds to D (sim3/program.f90:396-406), which E-form cells
+        # never read: the rows around it stay D-form
+        mats[0] = {"type": "dielectric", "region": box((1, P + 4, 1), (nx, 40, nz)), "eps": 1.0}
+        for m in mats:
+            if m["type"] == "dielectric":
+                m["form"] = "E"
+        mats.insert(0, {"type": "dielectric", "region": box((1, P + 1, 1), (nx, P + 3, nz)), "eps": 1.0})
     return {
-        "name": "ref_ljp_titanium" if titanium else "ref_ljp_metals",
+        "name": "ref_ljp_titanium" if titanium else ("ref_ljp_eform" if eform else "ref_ljp_metals"),
         "grid": {"nx": nx, "ny": ny, "nz": nz, "ds": ds, "courant": 0.5, "periodic": [1, 0, 1]},
         "pml": {"mode": "ref_y", "npml": [2, P, 2], "order": 3.0, "R0": math.exp(-16.0),
                 "eps_low": 1.0, "eps_high": 1.46 ** 2},

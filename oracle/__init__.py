@@ -11,7 +11,8 @@ import sys
 
 ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
 REPO_DIR = os.path.dirname(ORACLE_DIR)
-LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+# OFDTD_ORACLE_LIB: another build of the same sources (the sanitizer build)
+LIB = os.environ.get("OFDTD_ORACLE_LIB") or os.path.join(ORACLE_DIR, "liboracle.so")
 REF_EXE = os.path.join(ORACLE_DIR, "_ref", "ref3d")
 sys.path.insert(0, os.path.join(REPO_DIR, "fortran-mpi-fdtd_amd"))
 
@@ -128,6 +129,8 @@ def ref3d_namelist(spec):
             mt = LJP_MTYPE[lp["preset"]]
             if mt >= 8:
                 lines.append(f" gwNL({q})={lp['wNL']!r}, gdnuNL({q})={lp['dnuNL']!r}, gxi3({q})={lp['xi3']!r},")
+        elif m["type"] == "dielectric" and m.get("form") == "E":
+            mt = 11  # mod3/simpledielectricLorentzJp_mod.f90
         else:
             mt = MTYPE[m["type"]]
         lines.append(f" mtype({q})={mt}, mbox(1:6,{q})={','.join(str(v) for v in b)},")

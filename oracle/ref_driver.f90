@@ -18,6 +18,8 @@
 !     NodispersiveGraphene.f90), which no reference driver calls, is applied
 !     through its update_EJp in the same material pass: E form from curl H,
 !     so those cells ignore D (test cases keep them apart from the others);
+!     so is the E-form dielectric written for those simulations
+!     (mod3/simpledielectricLorentzJp_mod.f90), mtype 11;
 !   - source, DFT gating, probes and flux follow sim3/program.f90:396-685.
 ! Usage: ref3d <config.nml> <outdir>
 program ref3d
@@ -35,6 +37,7 @@ program ref3d
     use grapheneLorentzJp_mod
     use graphene3LorentzJp1nm_mod
     use nondispersivegrapheneJp_mod
+    use simpledielectricLorentzJp_mod
     implicit none
     integer :: nx, ny, nz, nsteps
     real*8 :: ds, Ku, dt
@@ -70,6 +73,9 @@ program ref3d
     type(SLG_LJp) :: lgr(MAXM)
     type(SLG_3LJp1nm) :: lgr3(MAXM)
     type(SLG_ndJp) :: lgrnd(MAXM)
+    ! mtype 11: the E-form dielectric of the Lorentz-Jp simulations
+    ! (mod3/simpledielectricLorentzJp_mod.f90: E += Ku*ga*curl H)
+    type(ConstantDielectricLJp) :: dlj(MAXM)
     type(pmly) :: lowp, highp
     type(Irange) :: absIx, absIy, absIz
     real*8, allocatable :: dre(:,:,:,:,:), dim_(:,:,:,:,:), arg(:), prb(:,:,:), Sy(:,:)
@@ -165,6 +171,11 @@ program ref3d
             lgrnd(q)%dt = dt; lgrnd(q)%ds = ds
             lgrnd(q)%wNL = gwNL(q); lgrnd(q)%dnuNL = gdnuNL(q); lgrnd(q)%xi3 = gxi3(q)
             call lgrnd(q)%initialize_material(absIx, absIy, absIz, (/ nx, ny, nz /))
+        case (11)
+            call setbox(dlj(q)%material, mbox(:, q))
+            dlj(q)%dt = dt; dlj(q)%ds = ds
+            dlj(q)%constant_ep = meps(q)
+            call dlj(q)%initialize_material(absIx, absIy, absIz, (/ nx, ny, nz /))
         end select
     end do
 
@@ -223,6 +234,8 @@ program ref3d
                 call lgr3(q)%update_EJp(PEx, PEy, PEz, PHx, PHy, PHz)
             case (10)
                 call lgrnd(q)%update_EJp(PEx, PEy, PEz, PHx, PHy, PHz)
+            case (11)
+                call dlj(q)%update_EJp(PEx, PEy, PEz, PHx, PHy, PHz)
             end select
         end do
         call lowp%updateE(PEx, PEy, PEz, PHx, PHy, PHz)

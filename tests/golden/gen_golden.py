@@ -88,6 +88,7 @@ CASES = {
     "ref_ljp_metals": (workloads.ref_ljp_metals, True),
     "ref_ljp_graphene": (workloads.ref_ljp_graphene, True),
     "ref_ljp_titanium": (lambda: workloads.ref_ljp_metals(nsteps=300, titanium=True), True),
+    "ref_ljp_eform": (lambda: workloads.ref_ljp_metals(eform=True), True),
 }
 
 if __name__ == "__main__":

@@ -45,6 +45,43 @@ def test_oracle_matches_reference_fortran(golden, case):
         assert np.array_equal(sim.probe(p), z["probes"][p])
 
 
+def test_oracle_matches_the_e_form_dielectric(golden):
+    """ref_ljp_eform: ref_ljp_metals with its dielectrics (all but the rows
+    around the D-side source) as the E-form dielectric written for the
+    Lorentz-Jp simulations (mod3/simpledielectricLorentzJp_mod.f90:20-45,
+    E += Ku*ga*curl H), linked into ref3d.  To the library both forms are
+    one dielectric.  Away from the y PML the oracle's E equals the E-form
+    run to 1e-12 of its scale; everywhere, every field is within the
+    reference's own D-form-to-E-form spread (ref_ljp_metals against this
+    run: rounding that the PML interface rows and the unused D of the
+    Lorentz-Jp cells amplify to at most 2e-10); spectra and probes hold
+    1e-14.  A wrong ga (eps for 1/eps), Ku or curl sign is off by O(1)."""
+    import oracle
+
+    spec, z = golden("ref_ljp_eform")
+    _, zd = golden("ref_ljp_metals")
+    assert sum(1 for m in spec["materials"] if m.get("form") == "E") >= 6
+    sim, ids, pids = oracle.make_simulation(spec)
+    sim.step(spec["nsteps"])
+    P = spec["pml"]["npml"][1]
+    ny = spec["grid"]["ny"]
+    inner = slice(P + 1, ny - P - 1)  # 0-based rows two or more from the PML
+    for q in range(9):
+        got = sim.field(q).transpose(2, 1, 0)  # [k][j][i]
+        want = z["f_" + FIELDS[q]]
+        scale = np.abs(want).max()
+        own = np.abs(zd["f_" + FIELDS[q]] - want).max() / scale  # the reference's D form vs E form
+        assert own < 2e-10, (FIELDS[q], own)
+        assert np.abs(got - want).max() / scale <= own + 1e-14, FIELDS[q]
+        if q < 3:
+            err = np.abs(got[:, inner, :] - want[:, inner, :]).max() / scale
+            assert err < 1e-12, (FIELDS[q], err)
+    for p in ids:
+        assert relerr(sim.flux(p), z["flux"][p]) < 1e-14
+    for p in pids:
+        assert relerr(sim.probe(p), z["probes"][p]) < 1e-14
+
+
 def test_oracle_matches_reference_fortran_96(golden):
     """96^3 reference-envelope twin, 2000 steps: flux, probes, a y plane."""
     import oracle
