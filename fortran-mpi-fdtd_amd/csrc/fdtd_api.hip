@@ -217,6 +217,20 @@ bool is_ade_comp(const HostMaterial& m, int q) {
   return true;
 }
 
+// si_mod's recursion coefficients, mod3/si_mod.f90:249-257, in its
+// operation order
+void lord_coef_dt(double dt, const fdtd_lorentz_jp& p, LordCoef& c) {
+  std::memset(&c, 0, sizeof c);
+  c.npols = p.npols;
+  for (int l = 0; l < p.npols; ++l) {
+    const double den = 1.0 + dt * p.dnu[l];
+    const double wd = dt * p.w0[l], dw = p.w0[l] * dt;
+    c.ga[l] = (2.0 - wd * wd) / den;
+    c.gb[l] = (dt * p.dnu[l] - 1.0) / den;
+    c.gc[l] = (p.xi[l] * (dw * dw)) / den;
+  }
+}
+
 void ljp_coef_dt(double dt, double ds, const fdtd_lorentz_jp& p, LjpCoef<double>& c) {
   const Consts& K = consts();
   std::memset(&c, 0, sizeof c);
@@ -580,6 +594,25 @@ int commit_impl(Sim* s) {
   s->groups.clear();
   for (auto& m : s->mats) {
     if (m.type == 0) continue;
+    if (m.type == 4) {
+      LordCoef c;
+      lord_coef_dt(s->dt, m.lj, c);
+      int gi = -1;
+      for (size_t q = 0; q < s->groups.size(); ++q)
+        if (s->groups[q].kind == 2 && std::memcmp(&s->groups[q].ld, &c, sizeof c) == 0) gi = (int)q;
+      if (gi < 0) {
+        AdeGroup g;
+        g.kind = 2;
+        std::memset(&g.coef, 0, sizeof g.coef);
+        g.ld = c;
+        g.eps_inf = 1.0;
+        g.nstate = (s->engine == FDTD_ENGINE_FUSED ? 1 : 0) + 2 * c.npols;  // [D], S_l, S_l older
+        s->groups.push_back(g);
+        gi = (int)s->groups.size() - 1;
+      }
+      m.group = gi;
+      continue;
+    }
     if (m.type == 3) {
       LjpCoef<double> c;
       ljp_coef_dt(s->dt, s->ds, m.lj, c);
@@ -1138,7 +1171,9 @@ size_t estimate_bytes(const Sim* s) {
     if (!region_bounds(s, m.reg, lo, hi)) continue;
     int64_t vol = 1;
     for (int a = 0; a < 3; ++a) vol *= std::max(0, hi[a] - lo[a] + 1);
-    const int words = m.type == 3 ? 2 * m.lj.npols + 2 : 1 + 2 + 2 * (m.type == 1 ? 2 : 4) + 2;
+    const int words = m.type == 3   ? 2 * m.lj.npols + 2
+                      : m.type == 4 ? 1 + 2 * m.lj.npols
+                                    : 1 + 2 + 2 * (m.type == 1 ? 2 : 4) + 2;
     b += (size_t)vol * (3 * words * 8 + 5);  // dispersive state is FP64 in both precisions
   }
   for (const auto& d : s->dfts) {
@@ -1318,6 +1353,21 @@ int fdtd_add_lorentz_jp(fdtd_sim* s, const fdtd_region* r, const fdtd_lorentz_jp
     return set_error(FDTD_ERR_ARG, "ez_uses_jx needs Ex and Ez in jp_mask");
   HostMaterial m{};
   m.type = 3;
+  m.lj = *p;
+  m.writes[0] = m.writes[1] = m.writes[2] = 1;
+  return add_material(s, r, m);
+}
+
+int fdtd_add_lorentz_d(fdtd_sim* s, const fdtd_region* r, const fdtd_lorentz_jp* p) {
+  GUARD(s);
+  SETUP_ONLY(s);
+  if (!p || p->npols < 1 || p->npols > 4 || p->jp_mask != 7 || p->nonlinear || p->ez_uses_jx ||
+      p->eps_inf != 1.0)
+    return set_error(FDTD_ERR_ARG, "bad D-form Lorentz parameters (eps_inf 1, all components, linear)");
+  for (int l = 0; l < p->npols; ++l)
+    if (1.0 + p->dnu[l] * s->dt == 0.0) return set_error(FDTD_ERR_ARG, "pole with dnu*dt = -1");
+  HostMaterial m{};
+  m.type = 4;
   m.lj = *p;
   m.writes[0] = m.writes[1] = m.writes[2] = 1;
   return add_material(s, r, m);

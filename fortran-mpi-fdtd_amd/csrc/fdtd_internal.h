@@ -61,6 +61,14 @@ struct LjpCoef {
   real xi3, ng1, ng2;
 };
 
+// Lorentz poles in D form (mod3/si_mod.f90:234-259): per pole l,
+// S_l <- ga_l*S_l + gb_l*S_l(one step older) + gc_l*E.
+struct LordCoef {
+  int32_t npols, pad;
+  double ku;
+  double ga[4], gb[4], gc[4];
+};
+
 // Up to this many plane sources per handle.
 constexpr int kMaxSrc = 8;
 
@@ -89,7 +97,7 @@ struct DevBuf {
 };
 
 struct HostMaterial {
-  int type;  // 0 dielectric, 1 drude2cp, 2 graphene, 3 Lorentz-Jp
+  int type;  // 0 dielectric, 1 drude2cp, 2 graphene, 3 Lorentz-Jp, 4 Lorentz D form (lj's table)
   fdtd_region reg;
   double eps_r;
   fdtd_drude2cp d2;
@@ -139,9 +147,10 @@ struct Tfsf {
 };
 
 struct AdeGroup {
-  int kind = 0;          // 0 Drude/CP/Raman (coef), 1 Lorentz-Jp (lj)
+  int kind = 0;          // 0 Drude/CP/Raman (coef), 1 Lorentz-Jp (lj), 2 Lorentz D form (ld)
   AdeCoef<double> coef;  // FP64 master copy
   LjpCoef<double> lj{};
+  LordCoef ld{};
   double eps_inf;
   int nstate;                 // state words per component
   int64_t ncells = 0;

@@ -989,6 +989,72 @@ __global__ __launch_bounds__(256) void k_ade(FieldArgs<real> a, AdeCoef<double> 
 }
 
 // ---------------------------------------------------------------------------
+// Lorentz poles in D form (mod3/si_mod.f90:46-82): E = D - S_1 - ... - S_np
+// with the S of the previous step, then per pole S_l <- ga_l*S_l +
+// gb_l*S_l(older) + gc_l*E, in the module's operation order.  FUSED: D is
+// kept in state word 0 and advanced here from curl H (E-form engine), as
+// k_ade does.  State SoA, word w of component q for cell e at
+// state[(w*3+q)*n + e]; words: [D], S_l (l < np), S_l one step older.  FP64
+// in both precisions; E is stored in field precision.
+template <typename real, bool FUSED>
+__global__ __launch_bounds__(256) void k_lord(FieldArgs<real> a, LordCoef co, const int32_t* cells,
+                                              const uint8_t* masks, double* state, int64_t n, SrcArgs<real> src,
+                                              int nsrc) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int64_t c = cells[e];
+  const int mask = masks[e];
+  const int k = (int)(c / a.pxy);
+  const int64_t rem = c - (int64_t)k * a.pxy;
+  const int j = (int)(rem / a.px);
+  const int i = (int)(rem - (int64_t)j * a.px);
+  double cur[3] = {0, 0, 0};
+  double sv[3] = {0, 0, 0};
+  if (FUSED) {
+    const double hx = a.h[0][c], hy = a.h[1][c], hz = a.h[2][c];
+    cur[0] = (hz - (double)a.h[2][c - a.px]) + (-hy + (double)a.h[1][c - a.pxy]);
+    cur[1] = (hx - (double)a.h[0][c - a.pxy]) + (-hz + (double)a.h[2][c - 1]);
+    cur[2] = (hy - (double)a.h[1][c - 1]) + (-hx + (double)a.h[0][c - a.px]);
+    for (int q = 0; q < nsrc; ++q) {
+      const int pos = src.axis[q] == 0 ? i : (src.axis[q] == 1 ? j : k);
+      if (pos == src.index[q]) sv[src.comp[q]] += (double)src_value(src, q);
+    }
+  }
+  const int w0 = FUSED ? 1 : 0;
+  const int np = co.npols;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    if (!(mask & (1 << q))) continue;
+    double dv;
+    if (FUSED) {
+      double* dp = state + (int64_t)q * n + e;
+      dv = *dp + co.ku * cur[q];
+      dv = dv + sv[q];
+      *dp = dv;
+    } else {
+      dv = a.d[q][c];
+    }
+    double s1[4], s2[4];
+    double ev = dv;
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      if (l >= np) break;
+      s1[l] = state[(int64_t)((w0 + l) * 3 + q) * n + e];
+      s2[l] = state[(int64_t)((w0 + np + l) * 3 + q) * n + e];
+      ev = ev - s1[l];
+    }
+    a.e[q][c] = (real)ev;
+    put_e_ghost(a.e[q], (real)ev, i, j, k, c, a);
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      if (l >= np) break;
+      state[(int64_t)((w0 + l) * 3 + q) * n + e] = (co.ga[l] * s1[l] + co.gb[l] * s2[l]) + co.gc[l] * ev;
+      state[(int64_t)((w0 + np + l) * 3 + q) * n + e] = s1[l];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Lorentz-Jp cells, E form (mod3/goldLorentzJp_mod.f90:37-91 and the family:
 // grapheneLorentzJp_mod.f90:42-101 adds the third-order oscillator and
 // drives Ez with JpT_x).  The pole state is FP64 in both precisions (an FP32
@@ -1747,6 +1813,11 @@ int step_impl(Sim* s) {
     c.ku = s->ku;
     return c;
   };
+  auto lord_coef = [&](const AdeGroup& g) {
+    LordCoef c = g.ld;
+    c.ku = s->ku;
+    return c;
+  };
 
   // TFSF lines: E half step after the main E stage (the D terms above read
   // h(lo-1) before this step's line H update); H half step, consistency
@@ -1813,8 +1884,12 @@ int step_impl(Sim* s) {
       Timer tm(s, FDTD_K_ADE);
       FieldArgs<real> ad = a;  // D form from the old H into the new E
       for (int q = 0; q < 3; ++q) ad.e[q] = an.e[q];
-      hipLaunchKernelGGL((k_ade<real, true>), dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, ad,
-                         ade_coef(g), g.d_cells, g.d_mask, (double*)g.d_state, g.ncells, src, nsrc);
+      if (g.kind == 2)
+        hipLaunchKernelGGL((k_lord<real, true>), dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, ad,
+                           lord_coef(g), g.d_cells, g.d_mask, (double*)g.d_state, g.ncells, src, nsrc);
+      else
+        hipLaunchKernelGGL((k_ade<real, true>), dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, ad,
+                           ade_coef(g), g.d_cells, g.d_mask, (double*)g.d_state, g.ncells, src, nsrc);
     }
     {
       int rc = build_march_tiles(s, s->op_by, s->op_kc);
@@ -1902,6 +1977,9 @@ int step_impl(Sim* s) {
       if (g.kind == 1)
         hipLaunchKernelGGL(k_ljp<real>, dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, a, g.lj,
                            g.d_cells, g.d_mask, (double*)g.d_state, g.ncells, src, nsrc);
+      else if (g.kind == 2)
+        hipLaunchKernelGGL((k_lord<real, true>), dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, a,
+                           lord_coef(g), g.d_cells, g.d_mask, (double*)g.d_state, g.ncells, src, nsrc);
       else
         hipLaunchKernelGGL((k_ade<real, true>), dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, a,
                            ade_coef(g), g.d_cells, g.d_mask, (double*)g.d_state, g.ncells, src, nsrc);
@@ -1946,6 +2024,9 @@ int step_impl(Sim* s) {
       if (g.kind == 1)  // E form from curl H: the plane sources enter here, not through D
         hipLaunchKernelGGL(k_ljp<real>, dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, a, g.lj,
                            g.d_cells, g.d_mask, (double*)g.d_state, g.ncells, src, nsrc);
+      else if (g.kind == 2)
+        hipLaunchKernelGGL((k_lord<real, false>), dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, a,
+                           lord_coef(g), g.d_cells, g.d_mask, (double*)g.d_state, g.ncells, src, 0);
       else
         hipLaunchKernelGGL((k_ade<real, false>), dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, a,
                            ade_coef(g), g.d_cells, g.d_mask, (double*)g.d_state, g.ncells, src, 0);

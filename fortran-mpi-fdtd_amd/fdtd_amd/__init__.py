@@ -218,6 +218,12 @@ class Simulation:
         p = lorentz_jp(params, self.b)
         self.b.check(self.b.add_lorentz_jp(self.h, C.byref(_region(region)), C.byref(p)))
 
+    def add_lorentz_d(self, region, params):
+        """fdtd_add_lorentz_d: the D-form Lorentz poles of mod3/si_mod.f90
+        over a pole table ({"preset": "si"} is the module's own)."""
+        p = lorentz_jp(params, self.b)
+        self.b.check(self.b.add_lorentz_d(self.h, C.byref(_region(region)), C.byref(p)))
+
     def add_plane_source(self, axis, index, comp, t0, st, w0, amplitude=1.0):
         p = abi.Pulse(t0, st, w0, amplitude)
         self.b.check(self.b.add_plane_source(self.h, int(axis), int(index), int(comp), C.byref(p)))
@@ -405,6 +411,8 @@ def apply_spec(sim, spec):
             sim.add_graphene(m["region"], m["params"])
         elif m["type"] == "lorentz_jp":
             sim.add_lorentz_jp(m["region"], m["params"])
+        elif m["type"] == "lorentz_d":
+            sim.add_lorentz_d(m["region"], m["params"])
         else:
             raise ValueError(m["type"])
     for s in spec.get("sources", []):

@@ -152,6 +152,7 @@ PROTOTYPES = {
     "add_drude2cp": (_I, [_P, C.POINTER(Region), C.POINTER(Drude2CP)]),
     "add_graphene": (_I, [_P, C.POINTER(Region), C.POINTER(Graphene)]),
     "add_lorentz_jp": (_I, [_P, C.POINTER(Region), C.POINTER(LorentzJp)]),
+    "add_lorentz_d": (_I, [_P, C.POINTER(Region), C.POINTER(LorentzJp)]),
     "lorentz_jp_preset": (_I, [C.c_int32, C.POINTER(LorentzJp)]),
     "add_plane_source": (_I, [_P, _I, _I, _I, C.POINTER(Pulse)]),
     "add_dft_plane": (_I, [_P, _I, _I, _I, _PD, _I, _I, _I, C.POINTER(_I)]),

@@ -70,6 +70,20 @@ def ref_box_small(nsteps=1200):
     }
 
 
+def ref_si_small(nsteps=1200):
+    """ref_box_small with its Si substrate, and a block in the air above the
+    gold, as the D-form Lorentz silicon of mod3/si_mod.f90 (three poles, the
+    first undamped) instead of the constant eps 3.88^2."""
+    spec = ref_box_small(nsteps=nsteps)
+    spec["name"] = "ref_si_small"
+    nx, ny, nz = spec["grid"]["nx"], spec["grid"]["ny"], spec["grid"]["nz"]
+    P = spec["pml"]["npml"][1]
+    mats = spec["materials"]
+    mats[3] = {"type": "lorentz_d", "region": box((1, 57, 1), (nx, ny - P, nz)), "params": {"preset": "si"}}
+    mats.insert(1, {"type": "lorentz_d", "region": box((4, 30, 1), (9, 34, nz)), "params": {"preset": "si"}})
+    return spec
+
+
 def ref_graphene_small(nsteps=900):
     """A graphene sheet (one cell thick, normal y) with the Raman term over
     a gold strip, y-PML, periodic x/z; the reference's CP4 parameters."""

@@ -252,6 +252,19 @@ int fdtd_add_drude2cp(fdtd_sim* sim, const fdtd_region* region,
 int fdtd_add_lorentz_jp(fdtd_sim* sim, const fdtd_region* region,
                         const fdtd_lorentz_jp* params);
 
+/* Si%initialize_material + update_E, mod3/si_mod.f90:46-82,234-259: Lorentz
+ * poles in D form, a second-order recursion per pole and component:
+ *   E_q = D_q - S_1 - ... - S_npols             (the S of the previous step)
+ *   S_l <- ga_l*S_l + gb_l*S_l(one step older) + gc_l*E_q
+ *   ga = (2 - (dt*w0)^2)/(1 + dt*dnu), gb = (dt*dnu - 1)/(1 + dt*dnu),
+ *   gc = xi*(w0*dt)^2/(1 + dt*dnu)
+ * over the pole table of an fdtd_lorentz_jp (FDTD_LJP_SI holds si_mod's
+ * table, :36-38: the same poles its E-form sibling siLorentzJp_mod uses).
+ * The module has no eps_inf and no nonlinear term: eps_inf must be 1,
+ * nonlinear 0 and jp_mask 7 (all three components).                        */
+int fdtd_add_lorentz_d(fdtd_sim* sim, const fdtd_region* region,
+                       const fdtd_lorentz_jp* params);
+
 /* Graphene%initialize_material + update_E, mod3/graphene_mod.f90:59-283. */
 int fdtd_add_graphene(fdtd_sim* sim, const fdtd_region* region,
                       const fdtd_graphene* params);

@@ -129,6 +129,9 @@ def ref3d_namelist(spec):
             mt = LJP_MTYPE[lp["preset"]]
             if mt >= 8:
                 lines.append(f" gwNL({q})={lp['wNL']!r}, gdnuNL({q})={lp['dnuNL']!r}, gxi3({q})={lp['xi3']!r},")
+        elif m["type"] == "lorentz_d":
+            assert m["params"] == {"preset": "si"}, "mod3/si_mod.f90 has its own table only"
+            mt = 12
         elif m["type"] == "dielectric" and m.get("form") == "E":
             mt = 11  # mod3/simpledielectricLorentzJp_mod.f90
         else:

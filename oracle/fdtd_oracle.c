@@ -72,7 +72,7 @@ static int o_fail(int code, const char* msg) {
 }
 const char* ofdtd_last_error(void) { return o_err; }
 
-enum { M_DIEL = 0, M_DRUDE2CP = 1, M_GRAPHENE = 2, M_LJP = 3 };
+enum { M_DIEL = 0, M_DRUDE2CP = 1, M_GRAPHENE = 2, M_LJP = 3, M_LORD = 4 };
 enum { S_EXY = 0, S_EXZ, S_EYZ, S_EYX, S_EZX, S_EZY, S_HXY, S_HXZ, S_HYZ, S_HYX, S_HZX, S_HZY, S_N };
 
 typedef struct {
@@ -98,6 +98,10 @@ typedef struct {
   fdtd_lorentz_jp lj;
   double lj_ga;
   double *P[4][3], *Jp[4][3], *JpT[3], *Q[3], *G[3];
+  /* Lorentz poles in D form (mod3/si_mod.f90): per pole the recursion
+   * coefficients and, per comp over the box, S and the S one step older */
+  double ld_ga[4], ld_gb[4], ld_gc[4];
+  double *Sl[4][3], *Slm[4][3];
 } omat;
 
 typedef struct {
@@ -1079,6 +1083,31 @@ static void ljp_cell(osim* s, omat* mt, long b, int i, int j, int k, long c) {
   }
 }
 
+/* Si%initialize_material, mod3/si_mod.f90:234-259: the recursion
+ * coefficients of each pole (lj holds the pole table), state zero. */
+int ofdtd_add_lorentz_d(osim* s, const fdtd_region* r, const fdtd_lorentz_jp* p) {
+  if (!s || !r || !p) return o_fail(FDTD_ERR_ARG, "null argument");
+  if (s->committed) return o_fail(FDTD_ERR_STATE, "set-up after commit");
+  if (p->npols < 1 || p->npols > 4 || p->jp_mask != 7 || p->nonlinear || p->ez_uses_jx || p->eps_inf != 1.0)
+    return o_fail(FDTD_ERR_ARG, "bad D-form Lorentz parameters");
+  omat* m = new_mat(s, r, M_LORD);
+  if (!m) return o_fail(FDTD_ERR_ARG, "bad region");
+  m->lj = *p;
+  m->eps_inf = 1.0;
+  double dt = s->dt;
+  for (int l = 0; l < p->npols; ++l) {
+    /* si_mod.f90:249-257 */
+    m->ld_ga[l] = (2.0 - (dt * p->w0[l]) * (dt * p->w0[l])) / (1.0 + dt * p->dnu[l]);
+    m->ld_gb[l] = (dt * p->dnu[l] - 1.0) / (1.0 + dt * p->dnu[l]);
+    m->ld_gc[l] = (p->xi[l] * ((p->w0[l] * dt) * (p->w0[l] * dt))) / (1.0 + dt * p->dnu[l]);
+  }
+  long nb = (long)m->bx * m->by * m->bz;
+  if (nb < 1) nb = 1;
+  for (int c = 0; c < 3; ++c)
+    for (int l = 0; l < p->npols; ++l) { m->Sl[l][c] = zalloc(nb); m->Slm[l][c] = zalloc(nb); }
+  return 0;
+}
+
 /* update_E of every material in call order (sim3/program.f90:411-414) */
 static void update_materials(osim* s) {
   int skip_pml = s->pml_mode == FDTD_PML_ALL_FACES;
@@ -1105,6 +1134,19 @@ static void update_materials(osim* s) {
             for (int q = 0; q < 3; ++q) s->E[q][c] = s->D[q][c] * mt->gax;
           } else if (mt->type == M_LJP) {
             ljp_cell(s, mt, b, i, j, k, c);
+          } else if (mt->type == M_LORD) {
+            /* si_mod.f90:57-63: E from D and the S of the previous step,
+             * then each pole's recursion, then the shift */
+            for (int q = 0; q < 3; ++q) {
+              double e = s->D[q][c];
+              for (int l = 0; l < mt->lj.npols; ++l) e = e - mt->Sl[l][q][b];
+              s->E[q][c] = e;
+              for (int l = 0; l < mt->lj.npols; ++l) {
+                double sn = (mt->ld_ga[l] * mt->Sl[l][q][b] + mt->ld_gb[l] * mt->Slm[l][q][b]) + mt->ld_gc[l] * e;
+                mt->Slm[l][q][b] = mt->Sl[l][q][b];
+                mt->Sl[l][q][b] = sn;
+              }
+            }
           } else if (mt->type == M_DRUDE2CP) {
             /* gold_mod.f90:61-83 */
             for (int q = 0; q < 3; ++q) {

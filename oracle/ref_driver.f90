@@ -20,6 +20,7 @@
 !     so those cells ignore D (test cases keep them apart from the others);
 !     so is the E-form dielectric written for those simulations
 !     (mod3/simpledielectricLorentzJp_mod.f90), mtype 11;
+!   - the D-form Lorentz silicon (mod3/si_mod.f90) is mtype 12;
 !   - source, DFT gating, probes and flux follow sim3/program.f90:396-685.
 ! Usage: ref3d <config.nml> <outdir>
 program ref3d
@@ -38,6 +39,7 @@ program ref3d
     use graphene3LorentzJp1nm_mod
     use nondispersivegrapheneJp_mod
     use simpledielectricLorentzJp_mod
+    use si_mod
     implicit none
     integer :: nx, ny, nz, nsteps
     real*8 :: ds, Ku, dt
@@ -76,6 +78,8 @@ program ref3d
     ! mtype 11: the E-form dielectric of the Lorentz-Jp simulations
     ! (mod3/simpledielectricLorentzJp_mod.f90: E += Ku*ga*curl H)
     type(ConstantDielectricLJp) :: dlj(MAXM)
+    ! mtype 12: the D-form Lorentz silicon (mod3/si_mod.f90)
+    type(Si) :: sid(MAXM)
     type(pmly) :: lowp, highp
     type(Irange) :: absIx, absIy, absIz
     real*8, allocatable :: dre(:,:,:,:,:), dim_(:,:,:,:,:), arg(:), prb(:,:,:), Sy(:,:)
@@ -176,6 +180,10 @@ program ref3d
             dlj(q)%dt = dt; dlj(q)%ds = ds
             dlj(q)%constant_ep = meps(q)
             call dlj(q)%initialize_material(absIx, absIy, absIz, (/ nx, ny, nz /))
+        case (12)
+            call setbox(sid(q)%material, mbox(:, q))
+            sid(q)%dt = dt
+            call sid(q)%initialize_material(absIx, absIy, absIz, (/ nx, ny, nz /))
         end select
     end do
 
@@ -236,6 +244,8 @@ program ref3d
                 call lgrnd(q)%update_EJp(PEx, PEy, PEz, PHx, PHy, PHz)
             case (11)
                 call dlj(q)%update_EJp(PEx, PEy, PEz, PHx, PHy, PHz)
+            case (12)
+                call sid(q)%update_E(PEx, PEy, PEz, PDx, PDy, PDz)
             end select
         end do
         call lowp%updateE(PEx, PEy, PEz, PHx, PHy, PHz)

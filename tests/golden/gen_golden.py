@@ -89,6 +89,7 @@ CASES = {
     "ref_ljp_graphene": (workloads.ref_ljp_graphene, True),
     "ref_ljp_titanium": (lambda: workloads.ref_ljp_metals(nsteps=300, titanium=True), True),
     "ref_ljp_eform": (lambda: workloads.ref_ljp_metals(eform=True), True),
+    "ref_si_small": (workloads.ref_si_small, True),
 }
 
 if __name__ == "__main__":

@@ -47,7 +47,7 @@ def _run_oracle(spec):
 
 
 @pytest.mark.parametrize("case", ["ref_box_small", "ref_graphene_small", "ref_ljp_metals", "ref_ljp_graphene",
-                                  "ref_ljp_titanium"])
+                                  "ref_ljp_titanium", "ref_si_small"])
 def test_staged_fp64_vs_reference_fortran(golden, case):
     spec, z = golden(case)
     sim, ids, pids = _run_gpu(spec)
@@ -188,6 +188,26 @@ def test_fused_lorentz_jp_vs_oracle(precision):
         check(f"lorentz-jp {precision} flux {a}", relerr(gsim.flux(a), osim.flux(b)), stol)
     for a, b in zip(gp, op):
         assert relerr(gsim.probe(a), osim.probe(b)) < (stol if precision == "fp64" else 1e-3)
+
+
+@pytest.mark.parametrize("onepass", ["0", "1"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fused_lorentz_d_vs_oracle(precision, onepass, monkeypatch):
+    """The D-form Lorentz silicon of mod3/si_mod.f90 (ref_si_small's layout)
+    with the y PML as an ALL_FACES PML: the FUSED engine keeps D in the
+    pole state (k_lord), in the two-kernel and the one-pass step."""
+    from fdtd_amd import workloads
+
+    monkeypatch.setenv("FDTD_ONEPASS", onepass)
+    spec = workloads.ref_si_small(nsteps=800)
+    spec["pml"] = {"mode": "all_faces", "npml": [0, 10, 0], "order": 3.0, "R0": 1e-7}
+    gsim, gids, gp = _run_gpu(spec, precision=precision, engine="fused")
+    osim, oids, op = _run_oracle(spec)
+    ftol, stol = (1e-12, 1e-12) if precision == "fp64" else (2e-4, 1e-5)
+    check(f"lorentz-d {precision} onepass {onepass} E", vec_err(gsim.field, osim.field, (0, 1, 2)), ftol)
+    check(f"lorentz-d {precision} onepass {onepass} H", vec_err(gsim.field, osim.field, (3, 4, 5)), ftol)
+    for a, b in zip(gids, oids):
+        check(f"lorentz-d {precision} flux {a}", relerr(gsim.flux(a), osim.flux(b)), stol)
 
 
 def test_staged_fp32_lorentz_jp_graphene_vs_reference_fortran(golden):

@@ -22,12 +22,13 @@ def _vec(get, comps):
 
 
 @pytest.mark.parametrize("case", ["ref_box_small", "ref_graphene_small", "ref_ljp_metals", "ref_ljp_graphene",
-                                  "ref_ljp_titanium"])
+                                  "ref_ljp_titanium", "ref_si_small"])
 def test_oracle_matches_reference_fortran(golden, case):
     """ref_ljp_*: the Lorentz-Jp modules (mod3/goldLorentzJp_mod.f90,
     siLorentzJp_mod.f90, filmGoldLorentzJp_mod.f90, titaniumJp_mod.f90 and
     the three graphene modules) compiled where they lie and driven through
-    their update_EJp; the oracle's presets and E-form update reproduce them."""
+    their update_EJp; the oracle's presets and E-form update reproduce them.
+    ref_si_small: the D-form Lorentz silicon, mod3/si_mod.f90."""
     import oracle
 
     spec, z = golden(case)

@@ -241,6 +241,52 @@ def _impulse_ljp(ds, nsteps, params):
     return dt, sim.probe(pid)[:, 1]
 
 
+def test_lorentz_d_converges_to_the_analytic_lorentz_sum():
+    """The D-form recursion of mod3/si_mod.f90:57-63,249-257 (E = D - sum S,
+    S'' + 2 dnu S' + w0^2 S = xi w0^2 E discretised with a centred damping
+    term and S one step behind E): the discrete permittivity of a 0D cell
+    converges at second order in dt to 1 + sum xi w0^2 / (w0^2 - w^2 - 2i
+    dnu w).  si_mod's own table has an undamped pole, whose impulse response
+    never decays, so the pin uses its poles with every dnu > 0.  Catches a
+    missing factor 2 on dnu, a wrong sign of gb, S taken from the same step."""
+    from fdtd_amd import box
+
+    p = {"preset": "si", "dnu": [0.05 * 2 * PI * C0 / 1e-6, 0.063 * 2 * PI * C0 / 1e-6, 2.5 * 2 * PI * C0 / 1e-6]}
+    from fdtd_amd import lorentz_jp
+
+    t = lorentz_jp(p, _oracle().binding())
+    w = 2 * PI * C0 / np.array([500e-9, 800e-9, 1500e-9])
+    analytic = 1.0 + sum(t.xi[l] * t.w0[l] ** 2 / (t.w0[l] ** 2 - w * w - 2j * t.dnu[l] * w) for l in range(3))
+    errs = []
+    for courant in (0.5, 0.25):
+        sim = _sim(1, 1, 1, 2e-9, courant, periodic=(1, 1, 1))
+        dt = sim.dt
+        sim.add_lorentz_d(box((1, 1, 1), (1, 1, 1)), p)
+        w0 = 3 * PI / (4 * dt)
+        sim.add_plane_source(1, 1, 0, dt, dt / 10, w0, math.sqrt(2.0))
+        sim.add_plane_source(1, 1, 0, 2 * dt, dt / 10, w0, 1.0)
+        pid = sim.add_probe(1, 1, 1)
+        sim.step(int(40.0 / min(t.dnu[l] for l in range(3)) / dt))
+        e = sim.probe(pid)[:, 1]
+        assert abs(e[-1]) < 1e-10 * np.abs(e).max()
+        errs.append(np.abs(_eps_from_impulse(dt, e, w) - analytic) / np.abs(analytic))
+    assert (errs[1] < 1e-3).all(), errs
+    ratio = errs[0] / errs[1]
+    assert ((ratio > 3.5) & (ratio < 4.5)).all(), ratio  # second order in dt
+
+
+def test_lorentz_d_rejects_what_si_mod_cannot_do():
+    """eps_inf other than 1, the nonlinear term or a partial component mask
+    have no counterpart in mod3/si_mod.f90: the oracle refuses them."""
+    from fdtd_amd import FdtdError, box
+
+    for bad in ({"preset": "si", "eps_inf": 2.0}, {"preset": "gold"}, {"preset": "si", "jp_mask": 5},
+                {"preset": "graphene"}):
+        sim = _sim(4, 4, 4, 2e-9, 0.5, periodic=(1, 1, 1))
+        with pytest.raises(FdtdError):
+            sim.add_lorentz_d(box((1, 1, 1), (2, 2, 2)), bad)
+
+
 @pytest.mark.parametrize("preset", ["gold", "film_gold", "graphene3_1nm"])
 def test_lorentz_jp_converges_to_the_analytic_lorentz_sum(preset):
     """update_EJp (mod3/goldLorentzJp_mod.f90:59-86 and siblings): the
