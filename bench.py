@@ -87,22 +87,19 @@ def run_gpu(spec, precision, steps, warmup):
 PROFILES = os.path.join(ROOT, "profiles")
 
 
-def profiled_traffic(kernel, prec):
-    """HBM bytes per launch of `kernel` from a committed rocprofv3 summary
-    (FETCH_SIZE and WRITE_SIZE runs, each scaled by a 1 GiB copy's reading):
-    (bytes, summary file) or (None, None)."""
+def profiled_traffic(kernel, prec, summary):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
+    `summary` under profiles/ (FETCH_SIZE and WRITE_SIZE runs, each scaled
+    by a 1 GiB copy's reading): (bytes, summary file) or (None, None)."""
+    path = os.path.join(PROFILES, summary)
     try:
-        names = sorted(os.listdir(PROFILES))
+        lines = open(path).readlines()
     except OSError:
         return None, None
-    for name in names:
-        if not name.endswith(".md"):
-            continue
-        path = os.path.join(PROFILES, name)
-        for line in open(path):
-            w = line.split()
-            if len(w) == 4 and w[0] == "traffic_bytes_per_launch" and w[1] == kernel and w[2] == prec:
-                return float(w[3]), os.path.relpath(path, ROOT)
+    for line in lines:
+        w = line.split()
+        if len(w) == 4 and w[0] == "traffic_bytes_per_launch" and w[1] == kernel and w[2] == prec:
+            return float(w[3]), os.path.relpath(path, ROOT)
     return None, None
 
 
@@ -250,9 +247,12 @@ def _kernel_breakdown(sim, steps):
     for k in range(abi.K_COUNT):
         ms, n = sim.kernel_stats(k)
         if n:
+            # algorithmic bytes of one step (the ADE figure covers every
+            # dispersive group, one launch each) over the kernel's time per step
             b = sim.algorithmic_bytes(k) if k != abi.K_MON else None
-            kern[abi.KERNEL_NAMES[k]] = {"mean_ms": ms / n, "launches": n, "bytes_per_launch": b,
-                                         "frac": None if not b else b / (ms / n * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            kern[abi.KERNEL_NAMES[k]] = {"mean_ms": ms / n, "launches": n, "bytes_per_step": b,
+                                         "ms_per_step": ms / steps,
+                                         "frac": None if not b else b / (ms / steps * 1e-3) / 1e9 / HBM_PEAK_GBS}
     sim.set_profiling(False)
     return kern
 
@@ -419,7 +419,7 @@ def main():
         print(f"[bench] c3 fp32 {c3['fp32_mcells_per_s']:.0f} fp64 {c3['fp64_mcells_per_s']:.0f} Mcell/s, "
               f"spectra err {max(c3['fp32_spectra_max_rel_err_vs_gpu_fp64']):.2e}", file=sys.stderr, flush=True)
     if args.n == 512 and args.seed == 0:  # the workload the summary profiles
-        tb, src = profiled_traffic(r32["roofline"]["kernel"], "fp32")
+        tb, src = profiled_traffic(r32["roofline"]["kernel"], "fp32", "k_march_plain_gold_dimer_512.md")
         r32["roofline"]["traffic"] = tb
         if tb is not None:
             r32["roofline"]["traffic_source"] = src
