@@ -122,12 +122,22 @@ def _work(spec):
     return g["nx"] * g["ny"] * g["nz"] * spec["nsteps"]
 
 
-def sequential_run(jobs, precision="fp32"):
-    """Each job alone, back to back: create, step all, read the flux."""
+def _prepare(spec, precision, tune):
+    """Handle of a job; with `tune`, fdtd_autotune picks its step form (the
+    two-kernel step or one-pass tiles) from two timed steps of each, which
+    give the same bits."""
+    sim, ids, pids = make_simulation(spec, precision=precision)
+    if tune:
+        sim.autotune(2)
+    return sim, ids, pids
+
+
+def sequential_run(jobs, precision="fp32", tune=True):
+    """Each job alone, back to back: create, tune, step all, read the flux."""
     t0 = time.perf_counter()
     res = []
     for _, spec in jobs:
-        sim, ids, _ = make_simulation(spec, precision=precision)
+        sim, ids, _ = _prepare(spec, precision, tune)
         sim.step(spec["nsteps"])
         res.append(sim.flux(ids[0]))
         sim.close()
@@ -136,9 +146,11 @@ def sequential_run(jobs, precision="fp32"):
     return {"seconds": el, "mcells_per_s": work / el / 1e6, "flux": res}
 
 
-def served_run(jobs, precision="fp32", budget_bytes=200 << 30, max_active=16, chunk_steps=64, realtime=True):
-    """Replay the stream: submit each job at its arrival time, drive the pool,
-    read each result as soon as its job is done."""
+def served_run(jobs, precision="fp32", budget_bytes=200 << 30, max_active=16, chunk_steps=64, realtime=True,
+               tune=True):
+    """Replay the stream: submit each job at its arrival time (created and
+    tuned as sequential_run does), drive the pool, read each result as soon
+    as its job is done."""
     pool = Pool(budget_bytes, max_active, chunk_steps)
     t0 = time.perf_counter()
     live = {}
@@ -149,7 +161,7 @@ def served_run(jobs, precision="fp32", budget_bytes=200 << 30, max_active=16, ch
         now = time.perf_counter() - t0
         while nxt < len(jobs) and (not realtime or jobs[nxt][0] <= now):
             spec = jobs[nxt][1]
-            sim, ids, _ = make_simulation(spec, precision=precision)
+            sim, ids, _ = _prepare(spec, precision, tune)
             arrival = t0 + (jobs[nxt][0] if realtime else now)
             live[pool.submit(sim, spec["nsteps"])] = (nxt, sim, ids, arrival)
             nxt += 1
