@@ -116,6 +116,12 @@ module fdtd_c
             type(fdtd_region), intent(in) :: region
             type(fdtd_lorentz_jp), intent(in) :: p
         end function
+        integer(c_int) function fdtd_add_lorentz_d(sim, region, p) bind(c, name="fdtd_add_lorentz_d")
+            import :: c_int, c_ptr, fdtd_region, fdtd_lorentz_jp
+            type(c_ptr), value :: sim
+            type(fdtd_region), intent(in) :: region
+            type(fdtd_lorentz_jp), intent(in) :: p
+        end function
         integer(c_int) function fdtd_lorentz_jp_preset(kind, p) bind(c, name="fdtd_lorentz_jp_preset")
             import :: c_int, c_int32_t, fdtd_lorentz_jp
             integer(c_int32_t), value :: kind
