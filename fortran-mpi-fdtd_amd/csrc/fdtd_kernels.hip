@@ -320,7 +320,7 @@ __global__ __launch_bounds__(256) void k_fused_e_mixed(FieldArgs<real> a, PmlArg
 
 // Monitor planes sampled by the FUSED H kernel: a cell on such a plane
 // writes its E and new H components to the plane's sample buffer (the same
-// values and layout as k_dft_sample, which would read them again
+// values and layout as k_monitors, which would read them again
 // afterwards; an x-normal plane has one sample per 128-byte line).
 // FP32 only: in FP64 the extra registers cut the H kernel's occupancy
 // (8 -> 5 waves per SIMD) and cost more than the separate sample launch.
@@ -1331,28 +1331,61 @@ __global__ __launch_bounds__(256) void k_ref_pml_h(FieldArgs<real> a, RefPmlArgs
 // every gated step copies the plane's selected components into a ring of
 // kDftBlock samples; a flush adds dt * sum_b cos/sin(w t_b) F_b, t_b =
 // (n_b - 1) dt, to the FP64 accumulators [slot][re|im][m][v][u].
-struct CompList {
-  int32_t c[6];
+// Every monitor of a step in one launch (each kernel boundary costs about
+// 9 us on this GPU): the samples of up to kMonPlanes DFT planes, each laid
+// out as samples[slot][v][u] (u, v the plane's two free axes in x, y, z
+// order), then the probes.  Plane q owns blocks blk[q] .. blk[q+1]-1,
+// (np+255)/256 per sampled component; the blocks from blk[nplanes] on write
+// the probes.
+constexpr int kMonPlanes = 8;
+template <typename real>
+struct MonList {
+  int32_t nplanes, nprobe;
+  int32_t axis[kMonPlanes], index[kMonPlanes], n1[kMonPlanes];
+  int64_t np[kMonPlanes];
+  int32_t comp[kMonPlanes][6];
+  real* dst[kMonPlanes];
+  int32_t blk[kMonPlanes + 1];
+  const int32_t* ppos;
+  double* pout;
+  double tn;
 };
 
 template <typename real>
-__global__ __launch_bounds__(256) void k_dft_sample(FieldArgs<real> a, int axis, int index, int n1, int n2,
-                                                    int nslots, CompList comps, real* samp) {
-  const int64_t np = (int64_t)n1 * n2;
-  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_monitors(FieldArgs<real> a, MonList<real> m) {
+  const int b = blockIdx.x;
+  if (b >= m.blk[m.nplanes]) {  // probes (incident-line probes are written by k_tfsf_h)
+    for (int p = (b - m.blk[m.nplanes]) * 256 + (int)threadIdx.x; p < m.nprobe; p += 256 * (int)gridDim.x) {
+      if (m.ppos[3 * p] < 0) continue;
+      const int64_t c = m.ppos[3 * p] + a.px * m.ppos[3 * p + 1] + a.pxy * m.ppos[3 * p + 2];
+      double* r = m.pout + 7 * p;
+      r[0] = m.tn;
+      for (int q = 0; q < 3; ++q) {
+        r[1 + q] = (double)a.e[q][c];
+        r[4 + q] = (double)a.h[q][c];
+      }
+    }
+    return;
+  }
+  int q = 0;
+  while (q + 1 < m.nplanes && b >= m.blk[q + 1]) ++q;
+  const int64_t np = m.np[q];
+  const int nbx = (int)((np + 255) / 256);
+  const int lb = b - m.blk[q];
+  const int slot = lb / nbx;
+  const int64_t o = (int64_t)(lb - slot * nbx) * 256 + threadIdx.x;
   if (o >= np) return;
-  const int slot = blockIdx.y;
-  const int comp = comps.c[slot];
+  const int n1 = m.n1[q], axis = m.axis[q];
   const int u = (int)(o % n1) + 1, v = (int)(o / n1) + 1;
-  int p[3];
+  int pc[3];
   const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
-  p[axis] = index;
-  p[a1] = u;
-  p[a2] = v;
-  const int64_t c = p[0] + a.px * p[1] + a.pxy * p[2];
+  pc[axis] = m.index[q];
+  pc[a1] = u;
+  pc[a2] = v;
+  const int64_t c = pc[0] + a.px * pc[1] + a.pxy * pc[2];
+  const int comp = m.comp[q][slot];
   const real* F = comp < 3 ? a.e[comp] : a.h[comp - 3];
-  samp[(int64_t)slot * np + o] = F[c];
-  (void)nslots;
+  m.dst[q][(int64_t)slot * np + o] = F[c];
 }
 
 // cos/sin table of one block of samples: tab[(b*nfreq + m)*2 + {0,1}]
@@ -1408,19 +1441,6 @@ __global__ __launch_bounds__(256) void k_dft_accum(const real* samp, int64_t np,
 }
 
 // point probes, sim3/program.f90:445-451
-template <typename real>
-__global__ void k_probe(FieldArgs<real> a, const int32_t* pos, int np, double* out, double tn) {
-  const int p = threadIdx.x;
-  if (p >= np || pos[3 * p] < 0) return;  // incident-line probes are written by k_tfsf_h
-  const int64_t c = pos[3 * p] + a.px * pos[3 * p + 1] + a.pxy * pos[3 * p + 2];
-  double* r = out + 7 * p;
-  r[0] = tn;
-  for (int q = 0; q < 3; ++q) {
-    r[1 + q] = (double)a.e[q][c];
-    r[4 + q] = (double)a.h[q][c];
-  }
-}
-
 // ---------------------------------------------------------------------------
 // TFSF incident line (sim1/program.f90:217-262, mod1/BERpmly_ExHz_mod.f90).
 // One workgroup walks the line; the PML object's own fields pe, ph live on
@@ -2053,46 +2073,82 @@ int step_impl(Sim* s) {
   // overwrites, so each step form is timed with its sampling cost, and
   // neither advance the ring, fold nor record probes.
   (void)t;
+  // the planes this step samples with the monitor kernel (the others were
+  // written by the H kernel), with their ring slots
+  MonList<real> ml{};
+  auto launch_monitors = [&]() -> int {
+    int nb = 0;
+    for (int q = 0; q < ml.nplanes; ++q) {
+      ml.blk[q] = nb;
+      int ns = 0;
+      while (ns < 6 && ml.comp[q][ns] >= 0) ++ns;
+      nb += (int)((ml.np[q] + 255) / 256) * ns;
+    }
+    ml.blk[ml.nplanes] = nb;
+    const int pb = ml.nprobe > 0 ? (ml.nprobe + 255) / 256 : 0;
+    if (nb + pb == 0) return 0;
+    Timer tm(s, FDTD_K_MON);
+    hipLaunchKernelGGL(k_monitors<real>, dim3((unsigned)(nb + pb)), dim3(256), 0, st, a, ml);
+    ml = MonList<real>{};
+    return 0;
+  };
+  auto add_plane = [&](DftPlane& d, real* dst) -> int {
+    if (ml.nplanes == kMonPlanes) {
+      int rc = launch_monitors();
+      if (rc) return rc;
+    }
+    const int q = ml.nplanes++;
+    int ns = 0;
+    for (int c = 0; c < 6; ++c) ml.comp[q][c] = -1;
+    for (int c = 0; c < 6; ++c)
+      if (d.mask & (1 << c)) ml.comp[q][ns++] = c;
+    ml.axis[q] = d.axis;
+    ml.index[q] = d.index;
+    ml.n1[q] = d.n1;
+    ml.np[q] = (int64_t)d.n1 * d.n2;
+    ml.dst[q] = dst;
+    return 0;
+  };
   if (s->dry) {
     for (auto& d : s->dfts) {
       if (!d.fused_now && !(n < d.n_begin || (d.n_end > 0 && n > d.n_end))) {
-        CompList cl{};
         int ns = 0;
-        for (int c = 0; c < 6; ++c)
-          if (d.mask & (1 << c)) cl.c[ns++] = c;
-        real* dst = (real*)d.d_samp + (int64_t)d.nbuf * ns * d.n1 * d.n2;
-        hipLaunchKernelGGL(k_dft_sample<real>, dim3((unsigned)(((int64_t)d.n1 * d.n2 + 255) / 256), (unsigned)ns),
-                           dim3(256), 0, st, a, d.axis, d.index, d.n1, d.n2, ns, cl, dst);
+        for (int c = 0; c < 6; ++c) ns += (d.mask >> c) & 1;
+        int rc = add_plane(d, (real*)d.d_samp + (int64_t)d.nbuf * ns * d.n1 * d.n2);
+        if (rc) return rc;
       }
       d.fused_now = false;
     }
-    return 0;
+    return launch_monitors();
   }
+  std::vector<DftPlane*> active;
   for (auto& d : s->dfts) {
     if (n < d.n_begin) continue;
     if (d.n_end > 0 && n > d.n_end) continue;
-    CompList cl{};
     int ns = 0;
-    for (int c = 0; c < 6; ++c)
-      if (d.mask & (1 << c)) cl.c[ns++] = c;
-    const int64_t np = (int64_t)d.n1 * d.n2;
+    for (int c = 0; c < 6; ++c) ns += (d.mask >> c) & 1;
     if (d.nbuf == 0) d.n_first = n;
-    real* dst = (real*)d.d_samp + (int64_t)d.nbuf * ns * np;
     if (!d.fused_now) {
-      Timer tm(s, FDTD_K_MON);
-      hipLaunchKernelGGL(k_dft_sample<real>, dim3((unsigned)((np + 255) / 256), (unsigned)ns), dim3(256), 0, st, a,
-                         d.axis, d.index, d.n1, d.n2, ns, cl, dst);
+      int rc = add_plane(d, (real*)d.d_samp + (int64_t)d.nbuf * ns * d.n1 * d.n2);
+      if (rc) return rc;
     }
     d.fused_now = false;
-    d.nbuf += 1;
-    if (d.nbuf == kDftBlock) dft_flush_plane<real>(s, d);
+    active.push_back(&d);
   }
   if (!s->probes.empty()) {
-    Timer tm(s, FDTD_K_MON);
-    const int np = (int)s->probes.size();
-    double* out = s->d_probe + (int64_t)7 * np * (s->probe_row0 + (n - 1 - s->call_base));
-    hipLaunchKernelGGL(k_probe<real>, dim3(1), dim3(64), 0, st, a, (const int32_t*)s->d_probe_pos, np, out,
-                       (double)n * s->dt);
+    ml.nprobe = (int)s->probes.size();
+    ml.ppos = (const int32_t*)s->d_probe_pos;
+    ml.pout = s->d_probe + (int64_t)7 * ml.nprobe * (s->probe_row0 + (n - 1 - s->call_base));
+    ml.tn = (double)n * s->dt;
+  }
+  {
+    int rc = launch_monitors();
+    if (rc) return rc;
+  }
+  // fold each full ring after this step's samples are written
+  for (DftPlane* d : active) {
+    d->nbuf += 1;
+    if (d->nbuf == kDftBlock) dft_flush_plane<real>(s, *d);
   }
   return 0;
 }
