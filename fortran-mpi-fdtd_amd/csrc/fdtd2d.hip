@@ -257,6 +257,9 @@ struct Batch {
   int32_t* d_tfl = nullptr;  // simulations with a TFSF
   int ntiles = 0, nade = 0, nmon = 0, nlines = 0, ntf = 0;
   std::vector<int32_t> ctl;  // host mirror
+  // order of the simulations in the packed tile list: largest first, so
+  // the small ones' tiles fill the end of each launch
+  std::vector<int> order;
   std::vector<char> inst;    // host image of the descriptors
   hipGraphExec_t exec = nullptr;  // one captured 32-step block
   std::vector<DevBuf> allocs;
@@ -1674,12 +1677,12 @@ Inst<real> make_inst(Sim* s, int32_t* ctl, int tile0) {
 
 template <typename real>
 int build_inst(Batch* b) {
-  std::vector<Inst<real>> v;
+  std::vector<Inst<real>> v(b->sims.size());
   int tile0 = 0;
-  for (size_t q = 0; q < b->sims.size(); ++q) {
-    Sim* s = b->sims[q];
-    v.push_back(make_inst<real>(s, b->d_ctl + 2 * q, tile0));
-    tile0 += v.back().ntx * v.back().nty;
+  for (int q : b->order) {
+    Sim* s = b->sims[(size_t)q];
+    v[(size_t)q] = make_inst<real>(s, b->d_ctl + 2 * q, tile0);
+    tile0 += v[(size_t)q].ntx * v[(size_t)q].nty;
   }
   const size_t bytes = v.size() * sizeof(Inst<real>);
   b->inst.assign((const char*)v.data(), (const char*)v.data() + bytes);
@@ -1701,10 +1704,18 @@ int batch_build(Batch* b) {
   std::vector<int32_t> tile_inst, tfl;
   std::vector<int4> ade, mon, ljp;
   std::vector<int2> lines;
-  for (int q = 0; q < count; ++q) {
+  b->order.resize((size_t)count);
+  for (int q = 0; q < count; ++q) b->order[(size_t)q] = q;
+  std::stable_sort(b->order.begin(), b->order.end(), [&](int u, int v) {
+    return (int64_t)b->sims[(size_t)u]->nx * b->sims[(size_t)u]->ny > (int64_t)b->sims[(size_t)v]->nx * b->sims[(size_t)v]->ny;
+  });
+  for (int q : b->order) {
     Sim* s = b->sims[q];
     const int nt = ((s->nx + kTX - 1) / kTX) * ((s->ny + kTY - 1) / kTY);
     tile_inst.insert(tile_inst.end(), (size_t)nt, q);
+  }
+  for (int q = 0; q < count; ++q) {
+    Sim* s = b->sims[q];
     for (size_t g = 0; g < s->groups.size(); ++g)
       for (int64_t c0 = 0; c0 < s->groups[g].ncells; c0 += kChunk) ade.push_back(make_int4(q, (int)g, (int)c0, 0));
     for (size_t g = 0; g < s->ljps.size(); ++g)
