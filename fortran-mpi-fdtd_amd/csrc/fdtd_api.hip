@@ -1694,6 +1694,150 @@ int fdtd_sync(fdtd_sim* s) {
   return flush_profile(s);
 }
 
+namespace fdtd {
+// Checkpoint layout: header, then sections, each an int64 byte count and
+// its bytes, in this order: E0..E2, H0..H2 (ghosted arrays), [D0..D2
+// (STAGED)], PML splits, dispersive group states, DFT accumulators, probe
+// records, TFSF lines.
+struct CkHeader {
+  char magic[8];
+  int32_t version, nx, ny, nz, prec, engine, pml_mode, n_done;
+  int32_t ngroups, ndft, nprobe, ntfsf;
+};
+static const char kCkMagic[8] = {'F', 'D', 'T', 'D', 'C', 'K', 'P', '1'};
+
+struct CkSection {
+  void* dev;                 // device bytes, or
+  std::vector<double>* host;  // a host list (probe records)
+  size_t bytes;
+};
+
+static std::vector<CkSection> ck_sections(Sim* s) {
+  std::vector<CkSection> v;
+  const size_t fb = (size_t)s->ncell_alloc * s->wsize;
+  for (int q = 0; q < 3; ++q) v.push_back({s->E[q], nullptr, fb});
+  for (int q = 0; q < 3; ++q) v.push_back({s->H[q], nullptr, fb});
+  if (s->engine == FDTD_ENGINE_STAGED)
+    for (int q = 0; q < 3; ++q) v.push_back({s->D[q], nullptr, fb});
+  if (s->d_split) v.push_back({s->d_split, nullptr, (size_t)S_N * (size_t)std::max<int64_t>(s->npml_cells, 1) * s->wsize});
+  if (s->d_slab) v.push_back({s->d_slab, nullptr, (size_t)2 * S_N * (size_t)s->slab_cells * s->wsize});
+  for (auto& g : s->groups)
+    if (g.ncells) v.push_back({g.d_state, nullptr, (size_t)g.nstate * 3 * (size_t)g.ncells * 8});
+  for (auto& d : s->dfts) v.push_back({d.d_acc, nullptr, sizeof(double) * 2 * (size_t)d.nslots * d.nfreq * (size_t)d.n1 * d.n2});
+  for (auto& p : s->probes) v.push_back({nullptr, &p.rec, p.rec.size() * sizeof(double)});
+  for (auto& t : s->tfsfs) v.push_back({t.d_line, nullptr, (size_t)8 * (t.n + 2) * sizeof(double)});
+  return v;
+}
+
+static CkHeader ck_header(const Sim* s) {
+  CkHeader h{};
+  std::memcpy(h.magic, kCkMagic, 8);
+  h.version = 1;
+  h.nx = s->nx;
+  h.ny = s->ny;
+  h.nz = s->nz;
+  h.prec = s->prec;
+  h.engine = s->engine;
+  h.pml_mode = s->pml_mode;
+  h.n_done = s->n_done;
+  h.ngroups = (int32_t)s->groups.size();
+  h.ndft = (int32_t)s->dfts.size();
+  h.nprobe = (int32_t)s->probes.size();
+  h.ntfsf = (int32_t)s->tfsfs.size();
+  return h;
+}
+}  // namespace fdtd
+
+int fdtd_checkpoint_size(fdtd_sim* s, size_t* bytes) {
+  GUARD(s);
+  if (!bytes) return set_error(FDTD_ERR_ARG, "null argument");
+  if (!s->committed) return set_error(FDTD_ERR_STATE, "checkpoint of a handle that is not committed");
+  FDTD_HIP(hipSetDevice(s->device));
+  FDTD_HIP(hipStreamSynchronize(s->stream));
+  int rc = harvest_probes(s);
+  if (rc) return rc;
+  size_t n = sizeof(CkHeader);
+  for (const auto& c : ck_sections(s)) n += sizeof(int64_t) + c.bytes;
+  *bytes = n;
+  return 0;
+}
+
+int fdtd_checkpoint_save(fdtd_sim* s, void* buf, size_t bytes) {
+  GUARD(s);
+  size_t need = 0;
+  int rc = fdtd_checkpoint_size(s, &need);  // also waits and harvests the probes
+  if (rc) return rc;
+  if (!buf || bytes != need) return set_error(FDTD_ERR_ARG, "checkpoint buffer of the wrong size");
+  for (const auto& d : s->dfts)
+    if (d.nbuf) return set_error(FDTD_ERR_STATE, "DFT samples not folded");  // fdtd_step folds at its end
+  char* o = (char*)buf;
+  const CkHeader h = ck_header(s);
+  std::memcpy(o, &h, sizeof h);
+  o += sizeof h;
+  for (const auto& c : ck_sections(s)) {
+    const int64_t nb = (int64_t)c.bytes;
+    std::memcpy(o, &nb, sizeof nb);
+    o += sizeof nb;
+    if (c.host) {
+      if (c.bytes) std::memcpy(o, c.host->data(), c.bytes);
+    } else if (c.bytes) {
+      FDTD_HIP(hipMemcpyAsync(o, c.dev, c.bytes, hipMemcpyDeviceToHost, s->stream));
+    }
+    o += c.bytes;
+  }
+  FDTD_HIP(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int fdtd_checkpoint_load(fdtd_sim* s, const void* buf, size_t bytes) {
+  GUARD(s);
+  if (!buf || bytes < sizeof(CkHeader)) return set_error(FDTD_ERR_ARG, "bad checkpoint buffer");
+  int rc = ensure_committed(s);
+  if (rc) return rc;
+  if (s->n_done != 0) return set_error(FDTD_ERR_STATE, "checkpoint load after the first step");
+  CkHeader h;
+  std::memcpy(&h, buf, sizeof h);
+  CkHeader mine = ck_header(s);
+  if (std::memcmp(h.magic, kCkMagic, 8) != 0 || h.version != 1) return set_error(FDTD_ERR_ARG, "not a checkpoint");
+  if (h.nx != mine.nx || h.ny != mine.ny || h.nz != mine.nz || h.prec != mine.prec || h.engine != mine.engine ||
+      h.pml_mode != mine.pml_mode || h.ngroups != mine.ngroups || h.ndft != mine.ndft ||
+      h.nprobe != mine.nprobe || h.ntfsf != mine.ntfsf || h.n_done < 0)
+    return set_error(FDTD_ERR_ARG, "checkpoint of a handle set up differently");
+  FDTD_HIP(hipSetDevice(s->device));
+  // probe sections carry the recorded rows: their sizes come from the buffer
+  const char* in = (const char*)buf + sizeof h;
+  const char* end = (const char*)buf + bytes;
+  std::vector<CkSection> secs = ck_sections(s);
+  std::vector<const char*> src(secs.size());
+  std::vector<int64_t> len(secs.size());
+  for (size_t q = 0; q < secs.size(); ++q) {
+    if (end - in < (ptrdiff_t)sizeof(int64_t)) return set_error(FDTD_ERR_ARG, "truncated checkpoint");
+    std::memcpy(&len[q], in, sizeof(int64_t));
+    in += sizeof(int64_t);
+    const bool probe = secs[q].host != nullptr;
+    if (len[q] < 0 || end - in < len[q] || (!probe && (size_t)len[q] != secs[q].bytes) ||
+        (probe && (size_t)len[q] != (size_t)h.n_done * 7 * sizeof(double)))
+      return set_error(FDTD_ERR_ARG, "checkpoint of a handle set up differently");
+    src[q] = in;
+    in += len[q];
+  }
+  if (in != end) return set_error(FDTD_ERR_ARG, "checkpoint of a handle set up differently");
+  FDTD_HIP(hipStreamSynchronize(s->stream));
+  for (size_t q = 0; q < secs.size(); ++q) {
+    if (secs[q].host) {
+      secs[q].host->assign((const double*)src[q], (const double*)(src[q] + len[q]));
+    } else if (len[q]) {
+      FDTD_HIP(hipMemcpyAsync(secs[q].dev, src[q], (size_t)len[q], hipMemcpyHostToDevice, s->stream));
+    }
+  }
+  FDTD_HIP(hipStreamSynchronize(s->stream));
+  for (auto& d : s->dfts) d.nbuf = 0;
+  s->probe_pending = 0;
+  s->n_done = h.n_done;
+  s->fields_set = true;  // the autotuner's dry steps need zero fields
+  return 0;
+}
+
 int fdtd_get_step(fdtd_sim* s, int* n) {
   GUARD(s);
   if (!n) return set_error(FDTD_ERR_ARG, "null argument");

@@ -330,6 +330,21 @@ class Simulation:
         self.b.check(self.b.get_flux(self.h, int(plane), S.ctypes.data_as(abi._PD)))
         return S
 
+    def checkpoint(self):
+        """fdtd_checkpoint_save: the handle's state after the steps done so
+        far, as bytes (waits for queued work)."""
+        n = C.c_size_t()
+        self.b.check(self.b.checkpoint_size(self.h, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        self.b.check(self.b.checkpoint_save(self.h, buf, n.value))
+        return buf.raw
+
+    def restore(self, data):
+        """fdtd_checkpoint_load: continue from a checkpoint of a handle set
+        up the same way (before this handle's first step)."""
+        buf = C.create_string_buffer(bytes(data), len(data))
+        self.b.check(self.b.checkpoint_load(self.h, buf, len(data)))
+
     def flux_noalloc(self, plane):
         """fdtd_get_flux_noalloc: the flux through the library's fixed device
         array, with no device allocation (the handle must be committed)."""

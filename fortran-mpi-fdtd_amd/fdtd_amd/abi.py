@@ -177,6 +177,9 @@ PROTOTYPES = {
 # entry points only the HIP library has
 GPU_PROTOTYPES = {
     "get_flux_noalloc": (_I, [_P, _I, _PD]),
+    "checkpoint_size": (_I, [_P, C.POINTER(C.c_size_t)]),
+    "checkpoint_save": (_I, [_P, C.c_void_p, C.c_size_t]),
+    "checkpoint_load": (_I, [_P, C.c_void_p, C.c_size_t]),
     "memory_bytes": (_I, [_P, C.POINTER(C.c_size_t)]),
     "set_profiling": (_I, [_P, _I]),
     "kernel_stats": (_I, [_P, _I, _PD, C.POINTER(C.c_int64)]),

@@ -178,6 +178,21 @@ module fdtd_c
             import :: c_int, c_ptr
             type(c_ptr), value :: sim
         end function
+        integer(c_int) function fdtd_checkpoint_size(sim, nbytes) bind(c, name="fdtd_checkpoint_size")
+            import :: c_int, c_ptr, c_size_t
+            type(c_ptr), value :: sim
+            integer(c_size_t), intent(out) :: nbytes
+        end function
+        integer(c_int) function fdtd_checkpoint_save(sim, buf, nbytes) bind(c, name="fdtd_checkpoint_save")
+            import :: c_int, c_ptr, c_size_t
+            type(c_ptr), value :: sim, buf
+            integer(c_size_t), value :: nbytes
+        end function
+        integer(c_int) function fdtd_checkpoint_load(sim, buf, nbytes) bind(c, name="fdtd_checkpoint_load")
+            import :: c_int, c_ptr, c_size_t
+            type(c_ptr), value :: sim, buf
+            integer(c_size_t), value :: nbytes
+        end function
         integer(c_int) function fdtd_get_dt(sim, dt) bind(c, name="fdtd_get_dt")
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: sim

@@ -351,6 +351,22 @@ int fdtd_autotune(fdtd_sim* sim, int steps, int* by, int* kc);
  * monitors; the halo exchanges become periodic index wrap on one domain.   */
 int fdtd_step(fdtd_sim* sim, int nsteps);
 int fdtd_sync(fdtd_sim* sim);
+
+/* Checkpoint and restart (no reference counterpart: a reference run
+ * restarts from zero fields).  The state after the steps done so far --
+ * step count, E and H (and D with the STAGED engine), the PML splits, every
+ * dispersive group's state, the DFT accumulators, the probe records and
+ * the TFSF lines -- goes to a host buffer of fdtd_checkpoint_size bytes
+ * (committed handles; waits for the handle's queued work).
+ * fdtd_checkpoint_load restores it into a handle set up with the same
+ * grid, precision, engine, PML, materials, monitors, probes and TFSF waves,
+ * before that handle's first step (committing it if needed); stepping on
+ * gives the same bits as the uninterrupted run, in either FUSED step form.
+ * Mismatched handles or buffers are refused (FDTD_ERR_ARG, FDTD_ERR_STATE
+ * after a step).                                                            */
+int fdtd_checkpoint_size(fdtd_sim* sim, size_t* bytes);
+int fdtd_checkpoint_save(fdtd_sim* sim, void* buf, size_t bytes);
+int fdtd_checkpoint_load(fdtd_sim* sim, const void* buf, size_t bytes);
 int fdtd_get_step(fdtd_sim* sim, int* n_done);
 int fdtd_get_dt(fdtd_sim* sim, double* dt);
 
