@@ -355,7 +355,7 @@ def main():
                     help="batched 2D sweeps: time the batch only, not the separate handles")
     ap.add_argument("--jobs", type=int, default=24)
     ap.add_argument("--rate", type=float, default=20.0, help="served-stream arrival rate [jobs/s]")
-    ap.add_argument("--max-active", type=int, default=16)
+    ap.add_argument("--max-active", type=int, default=4)
     ap.add_argument("--chunk", type=int, default=64)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run a plain device copy of this size (scales rocprof FETCH/WRITE counters)")
