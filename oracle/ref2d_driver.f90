@@ -8,7 +8,11 @@
 ! textually includes the reference's 2D Yee kernels
 ! (FDTDsimulations/2D/air-nanoAntennas_Si02-Sisubstrate/include/myfdtd_2Dfunc.f90),
 ! as sim2/program.f90:535 does.  Everything else restates the driver logic
-! of sim2/program.f90:320-438 for one rank with no PML (PEC y walls):
+! of sim2/program.f90:320-438 for one rank, with no PML (PEC y walls) or
+! with the reference's own y-slab PML module (mod2/BERpmly_HzExEy_mod.f90,
+! one rank: the object carries both slabs, as ycord = 0 = Ymax-1), the main
+! updates then running over rows npmly..N_y-npmly as the reference's local
+! range (sim2/include/mympi_2Dfunc.f90:124-131):
 !   - the halo exchange of sim2/include/mympi_2Dfunc.f90 with PBC_x becomes
 !     the x ghost copies it performs on one rank (Ey after E, Hz after H);
 !   - materials are applied in the order given (last writer wins); the
@@ -27,6 +31,7 @@ program ref2d
     use siLorentzJp_mod
     use filmgoldLorentzJp_mod
     use graphene3LorentzJp1nm_mod
+    use BERpmly_HzExEy_mod
     implicit none
     integer :: nx, ny, nsteps
     real*8 :: ds, Ku, dt
@@ -37,7 +42,11 @@ program ref2d
     integer :: s_index
     real*8 :: s_t0, s_st, s_w0
     integer :: nprobe, p_ij(2, 8)
+    integer :: npmlx, npmly, j0, j1
+    real*8 :: order, R0, eps_low, eps_high, sigmam
+    type(pmly) :: mypmly
     namelist /grid/ nx, ny, ds, Ku, nsteps
+    namelist /pml/ npmlx, npmly, order, R0, eps_low, eps_high
     namelist /mats/ nmat, mtype, mbox, meps, gwNL, gdnuNL, gxi3
     namelist /src/ s_on, s_index, s_t0, s_st, s_w0
     namelist /probes/ nprobe, p_ij
@@ -63,8 +72,10 @@ program ref2d
     nmat = 0; mtype = 0; mbox = 0; meps = 1.d0; gwNL = 0.d0; gdnuNL = 0.d0; gxi3 = 0.d0
     s_on = .false.; s_index = 1; s_t0 = 0.d0; s_st = 1.d0; s_w0 = 0.d0
     nprobe = 0; p_ij = 1
+    npmlx = 0; npmly = 0; order = 3.d0; R0 = dexp(-16.d0); eps_low = 1.d0; eps_high = 1.d0
     open(unit=7, file=trim(cfg), status='old')
     read(7, nml=grid)
+    read(7, nml=pml)
     read(7, nml=mats)
     read(7, nml=src)
     read(7, nml=probes)
@@ -77,6 +88,14 @@ program ref2d
 
     absIx%init = 1; absIx%ends = nx
     absIy%init = 1; absIy%ends = ny
+    j0 = 1; j1 = ny
+    if (npmly > 0) then
+        ! sim2/program.f90:93,237 (one rank: coords 0, Ndim 1)
+        sigmam = (order+1.d0)*c0*log(1.d0/R0)/(2.d0*npmly*ds)
+        call mypmly%initialize(eps_low*ep0, eps_high*ep0, sigmam, (/ npmlx, npmly /), order, dt, ds, &
+                               (/ 0, 0 /), (/ nx, ny /), (/ 1, 1 /))
+        j0 = npmly; j1 = ny-npmly
+    end if
     do q = 1, nmat
         select case (mtype(q))
         case (1)
@@ -115,7 +134,7 @@ program ref2d
     prb = 0.d0
     do n = 1, nsteps
         t = real(n-1, 8)*dt
-        call updateD(1, nx, 1, ny)
+        call updateD(1, nx, j0, j1)
         if (s_on) then
             gauss_t = dexp(-0.5d0*((t-s_t0)/s_st)**2)*dsin(s_w0*t)
             do i = 1, nx
@@ -138,9 +157,11 @@ program ref2d
                 call lgr3(q)%update_EJp(PEx, PEy, PHz)
             end select
         end do
+        if (npmly > 0) call mypmly%updateE(PEx, PHz)
         Ey(nx+1, :) = Ey(1, :)
         Ey(0, :) = Ey(nx, :)
-        call updateH(1, nx, 1, ny)
+        call updateH(1, nx, j0, j1)
+        if (npmly > 0) call mypmly%updateH(PHz)
         Hz(nx+1, :) = Hz(1, :)
         Hz(0, :) = Hz(nx, :)
         do p = 1, nprobe

@@ -2,8 +2,8 @@
 
 Runs oracle/_ref/ref2d (built by ``make -C oracle ref2d`` from the reference
 module sources, in this development container only: modules/src/2D
-material, dielectric, gold, graphene and the Lorentz-Jp modules, with the
-reference's 2D Yee include) on small TEz cases and stores the final fields
+material, dielectric, gold, graphene, the Lorentz-Jp modules and the y-slab
+PML module, with the reference's 2D Yee include) on small TEz cases and stores the final fields
 and point traces next to this script.  The reference never travels; only
 these vectors do.
 
@@ -60,6 +60,19 @@ def cases():
             "sources": [dict(j=15, **pul)],
             "probes": [[12, 30], [12, rows[0]], [12, 90]],
             "nsteps": 700}
+    # the reference's 2D y-slab PML module (x grading inside the slabs over
+    # npmlx cells), gold between air and a substrate, source just above the
+    # low slab as sim2/program.f90 places it
+    P = 10
+    out["ref2d_pml"] = {
+        "grid": {"nx": nx, "ny": ny, "ds": 2e-9, "courant": 0.5, "periodic_x": True},
+        "pml": {"npml": [4, P], "order": 3.0, "R0": 1e-7, "eps_low": 1.0, "eps_high": 2.25},
+        "materials": [{"type": "dielectric", "region": sim2d.box2((1, P), (nx, 80)), "eps": 1.0},
+                      {"type": "drude2cp", "region": sim2d.box2((8, 70), (16, 80))},
+                      {"type": "dielectric", "region": sim2d.box2((1, 81), (nx, ny - P)), "eps": 2.25}],
+        "sources": [dict(j=P + 2, **pul)],
+        "probes": [[12, P + 4], [12, 60], [12, 100], [12, 3], [12, ny - 2]],
+        "nsteps": 900}
     return out
 
 
@@ -67,7 +80,14 @@ def namelist(spec):
     g = spec["grid"]
     mats = spec["materials"]
     lines = ["&grid", f" nx={g['nx']}, ny={g['ny']}, ds={g['ds']!r}, Ku={g['courant']!r}, nsteps={spec['nsteps']}",
-             "/", "&mats", f" nmat={len(mats)},"]
+             "/"]
+    p = spec.get("pml")
+    if p:
+        lines += ["&pml", f" npmlx={p['npml'][0]}, npmly={p['npml'][1]}, order={p['order']!r}, R0={p['R0']!r},",
+                  f" eps_low={p['eps_low']!r}, eps_high={p['eps_high']!r}", "/"]
+    else:
+        lines += ["&pml", " npmlx=0, npmly=0", "/"]
+    lines += ["&mats", f" nmat={len(mats)},"]
     for q, m in enumerate(mats, start=1):
         r = m["region"]
         assert r["kind"] == "box"

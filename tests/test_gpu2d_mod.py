@@ -1,7 +1,8 @@
 """The 2D GPU path against the reference's own 2D modules (golden vectors
 from tests/golden/gen_golden2d_mod.py): graphene CP4 + Raman with gold, and
-the si, film gold and graphene3 1 nm Lorentz-Jp modules.  FP64, fields and
-traces within 1e-12 of their maxima."""
+the si, film gold and graphene3 1 nm Lorentz-Jp modules, and the 2D y-slab
+PML module (mod2/BERpmly_HzExEy_mod.f90) around gold between two
+dielectrics.  FP64, fields and traces within 1e-12 of their maxima."""
 import json
 import os
 
@@ -14,7 +15,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("name", ["ref2d_graphene_gold", "ref2d_ljp_si", "ref2d_ljp_film_gold",
-                                  "ref2d_ljp_graphene3_1nm"])
+                                  "ref2d_ljp_graphene3_1nm", "ref2d_pml"])
 def test_gpu_matches_the_reference_2d_modules(name):
     from fdtd_amd import sim2d
 

@@ -1,9 +1,10 @@
 """The 2D oracle against the reference's own 2D modules (golden vectors of
 tests/golden/gen_golden2d_mod.py: modules/src/2D graphene (CP4 with its
 central-difference Raman term) and gold under a line source, and the 2D
-Lorentz-Jp modules si, film gold and graphene3 1 nm, each run by a
-single-domain driver around the compiled modules and the reference's 2D
-Yee include).  Final fields and point traces."""
+Lorentz-Jp modules si, film gold and graphene3 1 nm, and the 2D y-slab PML
+module mod2/BERpmly_HzExEy_mod.f90 with probes inside both slabs, each run
+by a single-domain driver around the compiled modules and the reference's
+2D Yee include).  Final fields and point traces."""
 import json
 import os
 
@@ -18,7 +19,7 @@ def golden(name):
 
 
 @pytest.mark.parametrize("name", ["ref2d_graphene_gold", "ref2d_ljp_si", "ref2d_ljp_film_gold",
-                                  "ref2d_ljp_graphene3_1nm"])
+                                  "ref2d_ljp_graphene3_1nm", "ref2d_pml"])
 def test_oracle_matches_the_reference_2d_modules(name):
     """Fields and traces bit for bit: the oracle follows the modules'
     operations in order, built without multiply-add contraction as they are,
