@@ -153,3 +153,34 @@ def test_zmarch_fp32_bitwise():
         out.append([sim.field(c) for c in range(6)] + [sim.flux(i) for i in ids])
     for x, y in zip(*out):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_full_size_dimer_forms_agree_and_resume(precision, monkeypatch):
+    """The benchmark's own 512^3 gold dimer (12-cell PML, 64 wavelengths on
+    three planes, 8 probes) at full size, where the oracle cannot follow:
+    the one-pass step with the tile shape the autotuner keeps equals the
+    two-kernel step bit for bit after 96 steps (fields, spectra, probes),
+    and a run saved at step 40 and resumed in the other form lands on the
+    same bits."""
+    import fdtd_amd
+    from fdtd_amd import workloads
+
+    spec = workloads.c2_gold_dimer(nsteps=96)
+    a, ia, pa = _run(spec, precision, {"FDTD_ONEPASS": "0"}, monkeypatch)
+    monkeypatch.setenv("FDTD_ONEPASS", "1")
+    monkeypatch.setenv("FDTD_ONEPASS_BY", "4")
+    monkeypatch.setenv("FDTD_ONEPASS_KC", "16")
+    b, ib, pb = fdtd_amd.make_simulation(spec, precision=precision, engine="fused")
+    b.step(40)
+    ck = b.checkpoint()
+    b.step(56)
+    _same(a, ia, pa, b, ib, pb)
+    b.close()
+    monkeypatch.setenv("FDTD_ONEPASS", "0")
+    c, ic, pc = fdtd_amd.make_simulation(spec, precision=precision, engine="fused")
+    c.restore(ck)
+    c.step(56)
+    _same(a, ia, pa, c, ic, pc)
+    a.close()
+    c.close()
