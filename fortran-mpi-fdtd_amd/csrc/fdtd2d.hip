@@ -365,13 +365,21 @@ __device__ __forceinline__ void e_tmz(const Inst<real>& S, int i, int j, real& c
     const real r = m.gax[1];
     const real* fx = S.tprof;
     const real* fy = S.tprof + 4 * (S.nx + 2);
-    const real ox = S.ezx[c], oy = S.ezy[c];
+    // only a part whose axis's slab holds the cell is kept: a part not
+    // kept is Ez less the other part, or, neither kept, Ezx is the whole of
+    // Ez and Ezy 0 (the other axis's profile there is the interior's)
+    const bool kx = S.Px > 0 && (i <= S.Px || i > S.nx - S.Px);
+    const bool ky = S.P > 0 && (j <= S.P || j > S.ny - S.P);
+    const real sx = kx ? S.ezx[c] : (real)0, sy = ky ? S.ezy[c] : (real)0;
+    const real eo = (kx && ky) ? (real)0 : S.ez[c];
+    const real ox = kx ? sx : (ky ? eo - sy : eo);
+    const real oy = ky ? sy : (kx ? eo - sx : (real)0);
     const real zx = fx[i] * ox + (fx[(S.nx + 2) + i] * r) * (hyc + -hyi);
     real zy = fy[j] * oy + (fy[(S.ny + 2) + j] * r) * (-hxc + hxj);
     if (src) zy = zy + sv * r;
     if (corr) zy = zy + cv * r;
-    S.ezx[c] = zx;
-    S.ezy[c] = zy;
+    if (kx) S.ezx[c] = zx;
+    if (ky) S.ezy[c] = zy;
     ez = zx + zy;
     wrote = true;
   }

@@ -1084,7 +1084,7 @@ int build_march_tiles(Sim* s, int by, int kc) {
       }
   // general boxes: rows of the same x range and z range that follow each
   // other merge; each plane of a box is cut into workgroups of 256 threads,
-  // 64 x 4 (wide boxes) or 16 x 16 cells (the x slabs' narrow boxes)
+  // 64 x 4 (wide boxes), 32 x 8 or 16 x 16 cells (the x edges)
   std::sort(gbox.begin(), gbox.end(), [](const GBox& u, const GBox& v) {
     if (u.k0 != v.k0) return u.k0 < v.k0;
     if (u.x0 != v.x0) return u.x0 < v.x0;

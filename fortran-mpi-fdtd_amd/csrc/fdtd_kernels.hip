@@ -98,6 +98,30 @@ __device__ __forceinline__ bool in_pml(const int32_t* P, int i, int j, int k, in
   return i <= P[0] || i > nx - P[0] || j <= P[1] || j > ny - P[1] || k <= P[2] || k > nz - P[2];
 }
 
+// cell v of an axis of n cells lies in one of the axis's two PML slabs
+__device__ __forceinline__ bool in_slab(int P, int v, int n) { return v <= P || v > n - P; }
+
+// Split parts of an ALL_FACES PML cell before the update.  A component is
+// the sum of two parts, one per transverse axis (Ex = Exy + Exz, Ey = Eyz +
+// Eyx, Ez = Ezx + Ezy; H alike); only a part whose axis's slab holds the
+// cell is stored (sx, sy, sz), since the other axis's profile there is that
+// of the interior (f1 = 1, f2 = Ku).  A part not stored is the old
+// component less its other part, or, with neither part stored, the first
+// part of the sum is the whole old component and the second 0 -- in exact
+// arithmetic the same step, and a face cell moves 13 words a half step
+// instead of 18.  sp: stored parts in the order yx, zx, xy, zy, xz, yz
+// (the x parts, the y parts, the z parts); f: the old components; o: the
+// parts in that order.
+template <typename real>
+__device__ __forceinline__ void pml_parts(bool sx, bool sy, bool sz, const real* sp, const real* f, real* o) {
+  o[0] = sx ? sp[0] : (sz ? f[1] - sp[5] : (real)0);  // yx, second of y = yz + yx
+  o[1] = sx ? sp[1] : (sy ? f[2] - sp[3] : f[2]);     // zx, first of z = zx + zy
+  o[2] = sy ? sp[2] : (sz ? f[0] - sp[4] : f[0]);     // xy, first of x = xy + xz
+  o[3] = sy ? sp[3] : (sx ? f[2] - sp[1] : (real)0);  // zy
+  o[4] = sz ? sp[4] : (sy ? f[0] - sp[2] : (real)0);  // xz
+  o[5] = sz ? sp[5] : (sx ? f[1] - sp[0] : f[1]);     // yz
+}
+
 // Compact index of an ALL_FACES PML cell: z slabs (full xy planes), then y
 // slabs of the middle planes (full x rows), then the low and then the high
 // x slab of the rest, each of those row after row (Px cells a row), so the
@@ -168,14 +192,24 @@ __device__ __forceinline__ void fused_e_cell(const FieldArgs<real>& a, const Pml
   const bool pml = in_pml(p.P, i, j, k, a.nx, a.ny, a.nz);
   const bool any_src = sa[0] >= 0 || sa[1] >= 0 || sa[2] >= 0;
   int s = 0;
-  real sp[6], fx1 = 0, fx2 = 0, e0[3] = {0, 0, 0};
+  bool sx = false, sy = false, sz = false;
+  real sp[6] = {0, 0, 0, 0, 0, 0}, fx1 = 0, fx2 = 0, e0[3] = {0, 0, 0};
   Vec4<real> gx;
   if (pml) {
     s = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
+    sx = in_slab(p.P[0], i, a.nx);
+    sy = in_slab(p.P[1], j, a.ny);
+    sz = in_slab(p.P[2], k, a.nz);
     fx1 = p.f1[0][i];
     fx2 = p.f2[0][i];
-    sp[0] = p.s[S_EYX][s]; sp[1] = p.s[S_EZX][s]; sp[2] = p.s[S_EXY][s];
-    sp[3] = p.s[S_EZY][s]; sp[4] = p.s[S_EXZ][s]; sp[5] = p.s[S_EYZ][s];
+    // only the parts along the axes whose slabs hold the cell are stored
+    // (pml_parts); the old component is read where a part is not
+    if (sx) { sp[0] = p.s[S_EYX][s]; sp[1] = p.s[S_EZX][s]; }
+    if (sy) { sp[2] = p.s[S_EXY][s]; sp[3] = p.s[S_EZY][s]; }
+    if (sz) { sp[4] = p.s[S_EXZ][s]; sp[5] = p.s[S_EYZ][s]; }
+    if (!(sy && sz)) e0[0] = EO0[c];
+    if (!(sz && sx)) e0[1] = EO1[c];
+    if (!(sx && sy)) e0[2] = EO2[c];
   } else if (kHoist) {
     e0[0] = EO0[c]; e0[1] = EO1[c]; e0[2] = EO2[c];
     if (any_src) gx = a.gaxe[id];
@@ -183,12 +217,14 @@ __device__ __forceinline__ void fused_e_cell(const FieldArgs<real>& a, const Pml
   if (pml) {
     // mod3/BERpmly_mod.f90:150-155 with the splits driven by total H
     const real r = cf.v[3];
-    real eyx = fx1 * sp[0] + (fx2 * r) * (-hz + hz_im);
-    real ezx = fx1 * sp[1] + (fx2 * r) * (hy + (-hy_im));
-    real exy = p.f1[1][j] * sp[2] + (p.f2[1][j] * r) * (hz + (-hz_jm));
-    real ezy = p.f1[1][j] * sp[3] + (p.f2[1][j] * r) * (-hx + hx_jm);
-    real exz = p.f1[2][k] * sp[4] + (p.f2[2][k] * r) * (-hy + hy_km);
-    real eyz = p.f1[2][k] * sp[5] + (p.f2[2][k] * r) * (hx + (-hx_km));
+    real o[6];
+    pml_parts(sx, sy, sz, sp, e0, o);
+    real eyx = fx1 * o[0] + (fx2 * r) * (-hz + hz_im);
+    real ezx = fx1 * o[1] + (fx2 * r) * (hy + (-hy_im));
+    real exy = p.f1[1][j] * o[2] + (p.f2[1][j] * r) * (hz + (-hz_jm));
+    real ezy = p.f1[1][j] * o[3] + (p.f2[1][j] * r) * (-hx + hx_jm);
+    real exz = p.f1[2][k] * o[4] + (p.f2[2][k] * r) * (-hy + hy_km);
+    real eyz = p.f1[2][k] * o[5] + (p.f2[2][k] * r) * (hx + (-hx_km));
     // a source plane through the PML feeds the split along its normal
     if (sa[0] == 1) exy = exy + sv[0] * r;
     if (sa[0] == 2) exz = exz + sv[0] * r;
@@ -196,8 +232,9 @@ __device__ __forceinline__ void fused_e_cell(const FieldArgs<real>& a, const Pml
     if (sa[1] == 2) eyz = eyz + sv[1] * r;
     if (sa[2] == 0) ezx = ezx + sv[2] * r;
     if (sa[2] == 1) ezy = ezy + sv[2] * r;
-    pw.s[S_EYX][s] = eyx; pw.s[S_EZX][s] = ezx; pw.s[S_EXY][s] = exy;
-    pw.s[S_EZY][s] = ezy; pw.s[S_EXZ][s] = exz; pw.s[S_EYZ][s] = eyz;
+    if (sx) { pw.s[S_EYX][s] = eyx; pw.s[S_EZX][s] = ezx; }
+    if (sy) { pw.s[S_EXY][s] = exy; pw.s[S_EZY][s] = ezy; }
+    if (sz) { pw.s[S_EXZ][s] = exz; pw.s[S_EYZ][s] = eyz; }
     const real ex = exy + exz, ey = eyz + eyx, ez = ezx + ezy;
     E0[c] = ex;
     E1[c] = ey;
@@ -375,26 +412,37 @@ __device__ __forceinline__ void fused_h_cell(const FieldArgs<real>& a, const Pml
   constexpr bool kHoist = sizeof(real) == 4;
   const bool pml = in_pml(p.P, i, j, k, a.nx, a.ny, a.nz);
   int s = 0;
-  real sp[6], gx1 = 0, gx2 = 0, h0[3] = {0, 0, 0};
+  bool sx = false, sy = false, sz = false;
+  real sp[6] = {0, 0, 0, 0, 0, 0}, gx1 = 0, gx2 = 0, h0[3] = {0, 0, 0};
   if (pml) {
     s = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
+    sx = in_slab(p.P[0], i, a.nx);
+    sy = in_slab(p.P[1], j, a.ny);
+    sz = in_slab(p.P[2], k, a.nz);
     gx1 = p.g1[0][i];
     gx2 = p.g2[0][i];
-    sp[0] = p.s[S_HYX][s]; sp[1] = p.s[S_HZX][s]; sp[2] = p.s[S_HXY][s];
-    sp[3] = p.s[S_HZY][s]; sp[4] = p.s[S_HXZ][s]; sp[5] = p.s[S_HYZ][s];
+    if (sx) { sp[0] = p.s[S_HYX][s]; sp[1] = p.s[S_HZX][s]; }
+    if (sy) { sp[2] = p.s[S_HXY][s]; sp[3] = p.s[S_HZY][s]; }
+    if (sz) { sp[4] = p.s[S_HXZ][s]; sp[5] = p.s[S_HYZ][s]; }
+    if (!(sy && sz)) h0[0] = HO0[c];
+    if (!(sz && sx)) h0[1] = HO1[c];
+    if (!(sx && sy)) h0[2] = HO2[c];
   } else if (kHoist) {
     h0[0] = HO0[c]; h0[1] = HO1[c]; h0[2] = HO2[c];
   }
   if (pml) {
     // mod3/BERpmly_mod.f90:186-191 with the splits driven by total E
-    real hyx = gx1 * sp[0] + gx2 * (-ez + ez_ip);
-    real hzx = gx1 * sp[1] + gx2 * (ey + (-ey_ip));
-    real hxy = p.g1[1][j] * sp[2] + p.g2[1][j] * (ez + (-ez_jp));
-    real hzy = p.g1[1][j] * sp[3] + p.g2[1][j] * (-ex + ex_jp);
-    real hxz = p.g1[2][k] * sp[4] + p.g2[2][k] * (-ey + ey_kp);
-    real hyz = p.g1[2][k] * sp[5] + p.g2[2][k] * (ex + (-ex_kp));
-    pw.s[S_HYX][s] = hyx; pw.s[S_HZX][s] = hzx; pw.s[S_HXY][s] = hxy;
-    pw.s[S_HZY][s] = hzy; pw.s[S_HXZ][s] = hxz; pw.s[S_HYZ][s] = hyz;
+    real o[6];
+    pml_parts(sx, sy, sz, sp, h0, o);
+    real hyx = gx1 * o[0] + gx2 * (-ez + ez_ip);
+    real hzx = gx1 * o[1] + gx2 * (ey + (-ey_ip));
+    real hxy = p.g1[1][j] * o[2] + p.g2[1][j] * (ez + (-ez_jp));
+    real hzy = p.g1[1][j] * o[3] + p.g2[1][j] * (-ex + ex_jp);
+    real hxz = p.g1[2][k] * o[4] + p.g2[2][k] * (-ey + ey_kp);
+    real hyz = p.g1[2][k] * o[5] + p.g2[2][k] * (ex + (-ex_kp));
+    if (sx) { pw.s[S_HYX][s] = hyx; pw.s[S_HZX][s] = hzx; }
+    if (sy) { pw.s[S_HXY][s] = hxy; pw.s[S_HZY][s] = hzy; }
+    if (sz) { pw.s[S_HXZ][s] = hxz; pw.s[S_HYZ][s] = hyz; }
     const real hx = hxy + hxz, hy = hyz + hyx, hz = hzx + hzy;
     H0[c] = hx;
     H1[c] = hy;
@@ -846,9 +894,11 @@ __global__ __launch_bounds__(256) void k_march_plain(FieldArgs<real> a, MarchArg
 // General work: workgroup = 256 threads over cells x0..x1 x j0..j1 of
 // plane k (list entry: x0, x1, j0, j1, k, width), 64 x 4 cells (a
 // wavefront is one row), 32 x 8 or 16 x 16 (narrow boxes: the x edges),
-// one cell per thread, the fused cell updates over the two copies.  k_gen_e reads the
-// old H and E and writes the new E; k_gen_h then reads the new E and the
-// old H and writes the new H.
+// one cell per thread, the fused cell updates over the two copies (a
+// thread marching its column through the box's planes instead is slower:
+// its loads of one plane wait behind the stores of the one before).
+// k_gen_e reads the old H and E and writes the new E; k_gen_h then reads
+// the new E and the old H and writes the new H.
 template <typename real>
 __device__ __forceinline__ bool gen_cell(const FieldArgs<real>& a, const MarchArgs<real>& m, int& i, int& j, int& k,
                                          bool& wide) {
@@ -1401,6 +1451,7 @@ __global__ void k_dft_table(const double* omega, int nfreq, int n_first, int cou
 }
 
 constexpr int kDftMChunk = 32;
+constexpr int kDftMGroup = 8;
 
 template <typename real>
 __global__ __launch_bounds__(256) void k_dft_accum(const real* samp, int64_t np, int nslots, int nfreq, int count,
@@ -1426,16 +1477,37 @@ __global__ __launch_bounds__(256) void k_dft_accum(const real* samp, int64_t np,
     }
     __syncthreads();
     if (!act) continue;
-    for (int mm = 0; mm < mc; ++mm) {
-      double sr = 0.0, si = 0.0;
+    // the accumulators of kDftMGroup frequencies are all loaded before any
+    // is stored: re and im share one allocation, so a load placed after a
+    // store would wait for it, one memory round trip per frequency
+    for (int mb = 0; mb < mc; mb += kDftMGroup) {
+      const int nb = min(kDftMGroup, mc - mb);
+      double ar[kDftMGroup], ai[kDftMGroup];
 #pragma unroll
-      for (int b = 0; b < kDftBlock; ++b) {
-        sr += lt[2 * (b * kDftMChunk + mm)] * f[b];
-        si += lt[2 * (b * kDftMChunk + mm) + 1] * f[b];
+      for (int u = 0; u < kDftMGroup; ++u) {
+        const int64_t qq = (int64_t)(m0 + mb + u) * np + o;
+        ar[u] = u < nb ? re[qq] : 0.0;
+        ai[u] = u < nb ? im[qq] : 0.0;
       }
-      const int64_t qq = (int64_t)(m0 + mm) * np + o;
-      re[qq] += sr * dt;
-      im[qq] += si * dt;
+#pragma unroll
+      for (int u = 0; u < kDftMGroup; ++u) {
+        const int mm = mb + u;
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int b = 0; b < kDftBlock; ++b) {
+          sr += lt[2 * (b * kDftMChunk + mm)] * f[b];
+          si += lt[2 * (b * kDftMChunk + mm) + 1] * f[b];
+        }
+        ar[u] = ar[u] + sr * dt;
+        ai[u] = ai[u] + si * dt;
+      }
+#pragma unroll
+      for (int u = 0; u < kDftMGroup; ++u) {
+        if (u >= nb) break;
+        const int64_t qq = (int64_t)(m0 + mb + u) * np + o;
+        re[qq] = ar[u];
+        im[qq] = ai[u];
+      }
     }
   }
 }
@@ -1555,15 +1627,19 @@ __global__ __launch_bounds__(256) void k_tfsf_h(FieldArgs<real> a, PmlArgs<real>
   const int q = L.hc;
   real hv;
   if (all_faces && in_pml(p.P, i, j, k, a.nx, a.ny, a.nz)) {
-    // split H_q along the axis, then H_q = its two splits summed as
-    // k_fused_h sums them (Hxy+Hxz, Hyz+Hyx, Hzx+Hzy)
+    // the part of H_q along the axis takes the value; with both parts of
+    // H_q stored, H_q = the two summed as k_fused_h sums them (Hxy+Hxz,
+    // Hyz+Hyx, Hzx+Hzy), otherwise H_q takes the value itself (a part not
+    // stored is H_q less the other part)
     const int sa = q == 0 ? (L.axis == 1 ? S_HXY : S_HXZ)
                           : (q == 1 ? (L.axis == 0 ? S_HYX : S_HYZ) : (L.axis == 0 ? S_HZX : S_HZY));
     const int s0 = q == 0 ? S_HXY : (q == 1 ? S_HYZ : S_HZX);
     const int s1 = q == 0 ? S_HXZ : (q == 1 ? S_HYX : S_HZY);
+    const bool st[3] = {in_slab(p.P[0], i, a.nx), in_slab(p.P[1], j, a.ny), in_slab(p.P[2], k, a.nz)};
+    const bool both = st[q == 0 ? 1 : 0] && st[q == 2 ? 1 : 2];
     const int sp = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
-    p.s[sa][sp] = p.s[sa][sp] + (real)val;
-    hv = p.s[s0][sp] + p.s[s1][sp];
+    if (st[L.axis]) p.s[sa][sp] = p.s[sa][sp] + (real)val;
+    hv = both ? p.s[s0][sp] + p.s[s1][sp] : a.h[q][c] + (real)val;
   } else {
     hv = a.h[q][c] + (real)val;
   }

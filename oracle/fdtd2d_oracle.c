@@ -1082,9 +1082,17 @@ static void tmz_step(o2sim* s, int n) {
         const double curlhxy = -hx[c] + hx[c - sx];
         dz[c] = dz[c] + ku * (curlhyx + curlhxy);
       } else {
+        /* a part counts as kept only where its axis's slab holds the cell;
+         * a part not kept is Ez less the other part, or, neither kept, Ezx
+         * is the whole of Ez and Ezy 0 (the 3D oracle's old_parts, at
+         * nz = 1); every part is stored, the kept ones read */
         const double r = s->rz[c];
-        s->ezx[c] = f1x[i] * s->ezx[c] + (f2x[i] * r) * (hy[c] + -hy[c - 1]);
-        s->ezy[c] = f1y[j] * s->ezy[c] + (f2y[j] * r) * (-hx[c] + hx[c - sx]);
+        const int kx = s->tp[0] > 0 && (i <= s->tp[0] || i > nx - s->tp[0]);
+        const int ky = s->tp[1] > 0 && (j <= s->tp[1] || j > ny - s->tp[1]);
+        const double ox = kx ? s->ezx[c] : (ky ? ez[c] - s->ezy[c] : ez[c]);
+        const double oy = ky ? s->ezy[c] : (kx ? ez[c] - s->ezx[c] : 0.0);
+        s->ezx[c] = f1x[i] * ox + (f2x[i] * r) * (hy[c] + -hy[c - 1]);
+        s->ezy[c] = f1y[j] * oy + (f2y[j] * r) * (-hx[c] + hx[c - sx]);
         ez[c] = s->ezx[c] + s->ezy[c];
       }
     }

@@ -178,6 +178,7 @@ typedef struct osim osim;
 
 static double pulse_at(const fdtd_pulse* p, double t);
 static int in_pml_all(const osim* s, int i, int j, int k);
+static int in_slab(const osim* s, int axis, int v, int n);
 #define CIDX(s, i, j, k) ((long)((i)-1) + (long)(s)->nx * ((long)((j)-1) + (long)(s)->ny * (long)((k)-1)))
 
 static double* zalloc(long n) { return (double*)calloc((size_t)n, sizeof(double)); }
@@ -721,9 +722,14 @@ static void plane_add(osim* s, int field, int q, int axis, int index, double v) 
           s->s[si][c] = s->s[si][c] + v * s->pml_gax[c];
           s->E[q][c] = s->s[pairE[q][0]][c] + s->s[pairE[q][1]][c];
         } else {
+          /* the H pass reads a part only where its axis's slab holds the
+           * cell (old_parts): with both parts of H_q kept, H_q is their
+           * sum; otherwise H_q itself takes the value */
           int si = sidxH[q][axis];
-          s->s[si][c] = s->s[si][c] + v;
-          s->H[q][c] = s->s[pairH[q][0]][c] + s->s[pairH[q][1]][c];
+          const int st[3] = {in_slab(s, 0, p[0], s->nx), in_slab(s, 1, p[1], s->ny), in_slab(s, 2, p[2], s->nz)};
+          const int both = st[q == 0 ? 1 : 0] && st[q == 2 ? 1 : 2];
+          if (st[axis]) s->s[si][c] = s->s[si][c] + v;
+          s->H[q][c] = both ? s->s[pairH[q][0]][c] + s->s[pairH[q][1]][c] : s->H[q][c] + v;
         }
       } else if (field == 0) {
         s->D[q][c] = s->D[q][c] + v;
@@ -1285,6 +1291,37 @@ static void ref_pml_H(osim* s) {
 }
 
 /* ---- all-faces split-field PML (extension) --------------------------------- */
+
+/* cell v of an axis of n cells lies in one of the axis's two PML slabs */
+static int in_slab(const osim* s, int axis, int v, int n) {
+  const int P = s->pml.npml[axis];
+  return v <= P || v > n - P;
+}
+
+/* Old split parts of an ALL_FACES PML cell.  A component is the sum of two
+ * parts, one per transverse axis (Ex = Exy + Exz, Ey = Eyz + Eyx, Ez = Ezx +
+ * Ezy; H alike).  A part counts as kept only where its axis's slab holds the
+ * cell (elsewhere that axis's profile is the interior's, f1 = 1, f2 = Ku):
+ * a part not kept is the old component less its other part, or, with
+ * neither part kept, the first part of the sum is the whole old component
+ * and the second 0.  This array keeps every part it computes (the plane
+ * sources after the update read them) but reads only the kept ones.
+ * Parts in the order yx, zx, xy, zy, xz, yz from split index `yx` on (the
+ * enum keeps E and H parts in the same order: xy, xz, yz, yx, zx, zy).
+ * An extension of mod3/BERpmly_mod.f90:117-195's split form to six faces;
+ * in exact arithmetic the step of storing every part. */
+static void old_parts(const osim* s, double** S, int yx, double** F, long c, int i, int j, int k, double* o) {
+  const int sx = in_slab(s, 0, i, s->nx), sy = in_slab(s, 1, j, s->ny), sz = in_slab(s, 2, k, s->nz);
+  const int xy = yx - 3, xz = yx - 2, yz = yx - 1, zx = yx + 1, zy = yx + 2;
+  const double f0 = F[0][c], f1 = F[1][c], f2 = F[2][c];
+  o[0] = sx ? S[yx][c] : (sz ? f1 - S[yz][c] : 0.0);
+  o[1] = sx ? S[zx][c] : (sy ? f2 - S[zy][c] : f2);
+  o[2] = sy ? S[xy][c] : (sz ? f0 - S[xz][c] : f0);
+  o[3] = sy ? S[zy][c] : (sx ? f2 - S[zx][c] : 0.0);
+  o[4] = sz ? S[xz][c] : (sy ? f0 - S[xy][c] : 0.0);
+  o[5] = sz ? S[yz][c] : (sx ? f1 - S[yx][c] : f1);
+}
+
 static void all_pml_E(osim* s) {
   double **S = s->s, *hx = s->H[0], *hy = s->H[1], *hz = s->H[2];
   double *f1x = s->f1E[0], *f2x = s->f2E[0], *f1y = s->f1E[1], *f2y = s->f2E[1];
@@ -1302,12 +1339,14 @@ static void all_pml_E(osim* s) {
         double curlHzx_p = -hz[c], curlHzx_m = hz[IDX(s, i - 1, j, k)];
         double curlHyx_p = hy[c], curlHyx_m = -hy[IDX(s, i - 1, j, k)];
         double curlHxy_p = -hx[c], curlHxy_m = hx[IDX(s, i, j - 1, k)];
-        S[S_EYX][c] = f1x[i] * S[S_EYX][c] + (f2x[i] * r) * (curlHzx_p + curlHzx_m);
-        S[S_EZX][c] = f1x[i] * S[S_EZX][c] + (f2x[i] * r) * (curlHyx_p + curlHyx_m);
-        S[S_EXY][c] = f1y[j] * S[S_EXY][c] + (f2y[j] * r) * (curlHzy_p + curlHzy_m);
-        S[S_EZY][c] = f1y[j] * S[S_EZY][c] + (f2y[j] * r) * (curlHxy_p + curlHxy_m);
-        S[S_EXZ][c] = f1z[k] * S[S_EXZ][c] + (f2z[k] * r) * (curlHyz_p + curlHyz_m);
-        S[S_EYZ][c] = f1z[k] * S[S_EYZ][c] + (f2z[k] * r) * (curlHxz_p + curlHxz_m);
+        double o[6];
+        old_parts(s, S, S_EYX, s->E, c, i, j, k, o);
+        S[S_EYX][c] = f1x[i] * o[0] + (f2x[i] * r) * (curlHzx_p + curlHzx_m);
+        S[S_EZX][c] = f1x[i] * o[1] + (f2x[i] * r) * (curlHyx_p + curlHyx_m);
+        S[S_EXY][c] = f1y[j] * o[2] + (f2y[j] * r) * (curlHzy_p + curlHzy_m);
+        S[S_EZY][c] = f1y[j] * o[3] + (f2y[j] * r) * (curlHxy_p + curlHxy_m);
+        S[S_EXZ][c] = f1z[k] * o[4] + (f2z[k] * r) * (curlHyz_p + curlHyz_m);
+        S[S_EYZ][c] = f1z[k] * o[5] + (f2z[k] * r) * (curlHxz_p + curlHxz_m);
         s->E[0][c] = S[S_EXY][c] + S[S_EXZ][c];
         s->E[1][c] = S[S_EYZ][c] + S[S_EYX][c];
         s->E[2][c] = S[S_EZX][c] + S[S_EZY][c];
@@ -1330,12 +1369,14 @@ static void all_pml_H(osim* s) {
         double curlEzx_p = ez[IDX(s, i + 1, j, k)], curlEzx_m = -ez[c];
         double curlEyx_p = -ey[IDX(s, i + 1, j, k)], curlEyx_m = ey[c];
         double curlExy_p = ex[IDX(s, i, j + 1, k)], curlExy_m = -ex[c];
-        S[S_HYX][c] = g1x[i] * S[S_HYX][c] + g2x[i] * (curlEzx_m + curlEzx_p);
-        S[S_HZX][c] = g1x[i] * S[S_HZX][c] + g2x[i] * (curlEyx_m + curlEyx_p);
-        S[S_HXY][c] = g1y[j] * S[S_HXY][c] + g2y[j] * (curlEzy_m + curlEzy_p);
-        S[S_HZY][c] = g1y[j] * S[S_HZY][c] + g2y[j] * (curlExy_m + curlExy_p);
-        S[S_HXZ][c] = g1z[k] * S[S_HXZ][c] + g2z[k] * (curlEyz_m + curlEyz_p);
-        S[S_HYZ][c] = g1z[k] * S[S_HYZ][c] + g2z[k] * (curlExz_m + curlExz_p);
+        double o[6];
+        old_parts(s, S, S_HYX, s->H, c, i, j, k, o);
+        S[S_HYX][c] = g1x[i] * o[0] + g2x[i] * (curlEzx_m + curlEzx_p);
+        S[S_HZX][c] = g1x[i] * o[1] + g2x[i] * (curlEyx_m + curlEyx_p);
+        S[S_HXY][c] = g1y[j] * o[2] + g2y[j] * (curlEzy_m + curlEzy_p);
+        S[S_HZY][c] = g1y[j] * o[3] + g2y[j] * (curlExy_m + curlExy_p);
+        S[S_HXZ][c] = g1z[k] * o[4] + g2z[k] * (curlEyz_m + curlEyz_p);
+        S[S_HYZ][c] = g1z[k] * o[5] + g2z[k] * (curlExz_m + curlExz_p);
         s->H[0][c] = S[S_HXY][c] + S[S_HXZ][c];
         s->H[1][c] = S[S_HYZ][c] + S[S_HYX][c];
         s->H[2][c] = S[S_HZX][c] + S[S_HZY][c];
