@@ -962,8 +962,9 @@ void* field_ptr(Sim* s, int comp) {
 // are dielectric, and no x- or y-normal source plane crosses
 // them (z-normal ones are added per plane); the z ranges end one plane
 // before the high z slab, so no such cell is a PML cell there.  General
-// work goes to the list as workgroups of one plane: 8 words each, x0, x1,
-// j0, j1, k and the workgroup's width (64, 32 or 16).
+// work goes to the list as workgroups over at most kGenKc planes of a box
+// (FDTD_GEN_KC overrides): 8 words each, x0, x1, j0, j1, k0, the
+// workgroup's width (64, 32 or 16), k1.
 int build_march_tiles(Sim* s, int by, int kc) {
   if (s->tl_by == by && s->tl_kc == kc) return 0;
   const int nx = s->nx, ny = s->ny, nz = s->nz;
@@ -1083,8 +1084,9 @@ int build_march_tiles(Sim* s, int by, int kc) {
         }
       }
   // general boxes: rows of the same x range and z range that follow each
-  // other merge; each plane of a box is cut into workgroups of 256 threads,
-  // 64 x 4 (wide boxes), 32 x 8 or 16 x 16 cells (the x edges)
+  // other merge; a box is cut into workgroups of 256 threads, 64 x 4 (wide
+  // boxes), 32 x 8 or 16 x 16 cells (the x edges), each over a chunk of the
+  // box's planes
   std::sort(gbox.begin(), gbox.end(), [](const GBox& u, const GBox& v) {
     if (u.k0 != v.k0) return u.k0 < v.k0;
     if (u.x0 != v.x0) return u.x0 < v.x0;
@@ -1102,13 +1104,17 @@ int build_march_tiles(Sim* s, int by, int kc) {
     }
     merged.push_back(g);
   }
+  const char* gkc_env = std::getenv("FDTD_GEN_KC");
+  const int gkc = gkc_env && std::atoi(gkc_env) > 0 ? std::atoi(gkc_env) : kGenKc;
+  s->tl_gkc = gkc;
   for (const GBox& g : merged) {
     const int w = g.x1 - g.x0 + 1;
     const int bw = w <= 16 ? 16 : (w <= 32 ? 32 : 64), bh = 256 / bw;
-    for (int k = g.k0; k <= g.k1; ++k)
+    for (int k = g.k0; k <= g.k1; k += gkc)
       for (int j = g.j0; j <= g.j1; j += bh)
         for (int x = g.x0; x <= g.x1; x += bw) {
-          const int32_t e[8] = {x, std::min(x + bw - 1, g.x1), j, std::min(j + bh - 1, g.j1), k, bw, 0, 0};
+          const int32_t e[8] = {x, std::min(x + bw - 1, g.x1), j, std::min(j + bh - 1, g.j1),
+                                k, bw, std::min(k + gkc - 1, g.k1), 0};
           gen.insert(gen.end(), e, e + 8);
         }
   }

@@ -152,79 +152,101 @@ __device__ __forceinline__ int pml_index(const int32_t* P, int i, int j, int k, 
 // FUSED engine: E half step (myfdtd_func.f90:19-41 in E-form, material
 // update_E folded in, ALL_FACES split-field PML, plane sources).
 //
-// Every load is issued before the first store: the field pointers may alias
-// as far as the compiler knows, so a load placed after a store would wait
-// for it and cost one more memory round trip.  UNI: the row has one material
-// id `rm`, whose table entry is read with scalar loads; otherwise the entry
-// is gathered per cell.  The two forms are separate code paths so that no
-// merge of scalar and vector values makes the compiler wait for the loads.
+// Every load is issued before the first store (e_cell_load, then
+// e_cell_store): the field pointers may alias as far as the compiler knows,
+// so a load placed after a store would wait for it and cost one more memory
+// round trip.  The two phases are apart so that a thread marching a column
+// can issue the next plane's loads before this plane's arithmetic.  IDM:
+// how the material id is found -- 0 the row's one id `rm` (wave-uniform,
+// the table entry read with scalar loads), 1 gathered per cell, 2 `rm`
+// where it is >= 0 and gathered otherwise.
 //
 // z march: with use_carry, Hy and Hx of plane k-1 come from cy, cx (the
-// previous plane of the same thread) instead of memory; the cell leaves its
-// own Hy, Hx there for plane k+1.
+// previous plane of the same thread) instead of memory.
 //
 // In place (the two-kernel step) EO* are E* and pw is p.  Over two copies
 // (the general tiles of the one-pass step, pp): the old E comes from EO*,
 // the old splits from p, the new E goes to E* and the new splits to pw, and
 // components no material writes are copied over.
-template <typename real, bool UNI>
-__device__ __forceinline__ void fused_e_cell(const FieldArgs<real>& a, const PmlArgs<real>& p,
-                                             const PmlArgs<real>& pw, bool pp, const real* sv, const int* sa, int rm,
-                                             int i, int j, int k, real& cy, real& cx, bool use_carry,
-                                             const real* H0, const real* H1, const real* H2, const real* EO0,
-                                             const real* EO1, const real* EO2, real* E0, real* E1, real* E2) {
+template <typename real>
+struct ECellIn {
+  int c, s;
+  bool pml, sx, sy, sz, any_src;
+  uint32_t fl, id;
+  Vec4<real> cf;
+  real hx, hy, hz, hz_jm, hx_jm, hy_km, hx_km, hz_im, hy_im;
+  real e0[3], sp[6], fx1, fx2, fy1, fy2, fz1, fz2;
+};
+
+template <typename real, int IDM>
+__device__ __forceinline__ void e_cell_load(const FieldArgs<real>& a, const PmlArgs<real>& p,
+                                            const int* sa, int rm, int i, int j, int k, bool use_carry, real cy,
+                                            real cx, const real* H0, const real* H1, const real* H2,
+                                            const real* EO0, const real* EO1, const real* EO2, ECellIn<real>& in) {
   const int c = i + a.px * j + a.pxy * k;
-  const real hx = H0[c], hy = H1[c], hz = H2[c];
-  const real hz_jm = H2[c - a.px], hx_jm = H0[c - a.px];
-  const real hy_km = use_carry ? cy : H1[c - a.pxy], hx_km = use_carry ? cx : H0[c - a.pxy];
-  cy = hy;
-  cx = hx;
-  const real hz_im = H2[c - 1], hy_im = H1[c - 1];
-  const unsigned id = UNI ? (unsigned)__builtin_amdgcn_readfirstlane(rm) : (unsigned)a.matid[c];
-  const uint32_t fl = a.flags[id];
-  const Vec4<real> cf = a.coef[id];
-  // FP32: the loads of both forms are issued before either computes, so a
-  // wavefront with PML and interior lanes (the x slabs' edges) waits for
-  // memory once rather than once per form (+1.6 % on the E half step).  In
-  // FP64 the extra live registers cost more than that (-3 %): the interior
-  // loads stay after the PML branch.
-  constexpr bool kHoist = sizeof(real) == 4;
-  const bool pml = in_pml(p.P, i, j, k, a.nx, a.ny, a.nz);
-  const bool any_src = sa[0] >= 0 || sa[1] >= 0 || sa[2] >= 0;
-  int s = 0;
-  bool sx = false, sy = false, sz = false;
-  real sp[6] = {0, 0, 0, 0, 0, 0}, fx1 = 0, fx2 = 0, e0[3] = {0, 0, 0};
-  Vec4<real> gx;
-  if (pml) {
-    s = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
-    sx = in_slab(p.P[0], i, a.nx);
-    sy = in_slab(p.P[1], j, a.ny);
-    sz = in_slab(p.P[2], k, a.nz);
-    fx1 = p.f1[0][i];
-    fx2 = p.f2[0][i];
+  in.c = c;
+  in.hx = H0[c]; in.hy = H1[c]; in.hz = H2[c];
+  in.hz_jm = H2[c - a.px]; in.hx_jm = H0[c - a.px];
+  in.hy_km = use_carry ? cy : H1[c - a.pxy];
+  in.hx_km = use_carry ? cx : H0[c - a.pxy];
+  in.hz_im = H2[c - 1]; in.hy_im = H1[c - 1];
+  unsigned id;
+  if constexpr (IDM == 0) id = (unsigned)__builtin_amdgcn_readfirstlane(rm);
+  else if constexpr (IDM == 1) id = (unsigned)a.matid[c];
+  else id = rm >= 0 ? (unsigned)rm : (unsigned)a.matid[c];
+  in.id = id;
+  in.fl = a.flags[id];
+  in.cf = a.coef[id];
+  in.pml = in_pml(p.P, i, j, k, a.nx, a.ny, a.nz);
+  in.any_src = sa[0] >= 0 || sa[1] >= 0 || sa[2] >= 0;
+  in.s = 0;
+  in.sx = in.sy = in.sz = false;
+  for (int q = 0; q < 6; ++q) in.sp[q] = 0;
+  in.e0[0] = in.e0[1] = in.e0[2] = 0;
+  in.fx1 = in.fx2 = in.fy1 = in.fy2 = in.fz1 = in.fz2 = 0;
+  if (in.pml) {
+    const int s = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
+    const bool sx = in_slab(p.P[0], i, a.nx), sy = in_slab(p.P[1], j, a.ny), sz = in_slab(p.P[2], k, a.nz);
+    in.s = s;
+    in.sx = sx;
+    in.sy = sy;
+    in.sz = sz;
+    in.fx1 = p.f1[0][i]; in.fx2 = p.f2[0][i];
+    in.fy1 = p.f1[1][j]; in.fy2 = p.f2[1][j];
+    in.fz1 = p.f1[2][k]; in.fz2 = p.f2[2][k];
     // only the parts along the axes whose slabs hold the cell are stored
     // (pml_parts); the old component is read where a part is not
-    if (sx) { sp[0] = p.s[S_EYX][s]; sp[1] = p.s[S_EZX][s]; }
-    if (sy) { sp[2] = p.s[S_EXY][s]; sp[3] = p.s[S_EZY][s]; }
-    if (sz) { sp[4] = p.s[S_EXZ][s]; sp[5] = p.s[S_EYZ][s]; }
-    if (!(sy && sz)) e0[0] = EO0[c];
-    if (!(sz && sx)) e0[1] = EO1[c];
-    if (!(sx && sy)) e0[2] = EO2[c];
-  } else if (kHoist) {
-    e0[0] = EO0[c]; e0[1] = EO1[c]; e0[2] = EO2[c];
-    if (any_src) gx = a.gaxe[id];
+    if (sx) { in.sp[0] = p.s[S_EYX][s]; in.sp[1] = p.s[S_EZX][s]; }
+    if (sy) { in.sp[2] = p.s[S_EXY][s]; in.sp[3] = p.s[S_EZY][s]; }
+    if (sz) { in.sp[4] = p.s[S_EXZ][s]; in.sp[5] = p.s[S_EYZ][s]; }
+    if (!(sy && sz)) in.e0[0] = EO0[c];
+    if (!(sz && sx)) in.e0[1] = EO1[c];
+    if (!(sx && sy)) in.e0[2] = EO2[c];
+  } else {
+    in.e0[0] = EO0[c]; in.e0[1] = EO1[c]; in.e0[2] = EO2[c];
   }
-  if (pml) {
+}
+
+template <typename real>
+__device__ __forceinline__ void e_cell_store(const FieldArgs<real>& a, const PmlArgs<real>& pw, bool pp,
+                                             const real* sv, const int* sa, int i, int j, int k,
+                                             const ECellIn<real>& in, real* E0, real* E1, real* E2) {
+  const int c = in.c;
+  const real* e0 = in.e0;
+  const real hx = in.hx, hy = in.hy, hz = in.hz, hz_jm = in.hz_jm, hx_jm = in.hx_jm, hy_km = in.hy_km,
+             hx_km = in.hx_km, hz_im = in.hz_im, hy_im = in.hy_im;
+  if (in.pml) {
     // mod3/BERpmly_mod.f90:150-155 with the splits driven by total H
-    const real r = cf.v[3];
+    const real r = in.cf.v[3];
+    const int s = in.s;
     real o[6];
-    pml_parts(sx, sy, sz, sp, e0, o);
-    real eyx = fx1 * o[0] + (fx2 * r) * (-hz + hz_im);
-    real ezx = fx1 * o[1] + (fx2 * r) * (hy + (-hy_im));
-    real exy = p.f1[1][j] * o[2] + (p.f2[1][j] * r) * (hz + (-hz_jm));
-    real ezy = p.f1[1][j] * o[3] + (p.f2[1][j] * r) * (-hx + hx_jm);
-    real exz = p.f1[2][k] * o[4] + (p.f2[2][k] * r) * (-hy + hy_km);
-    real eyz = p.f1[2][k] * o[5] + (p.f2[2][k] * r) * (hx + (-hx_km));
+    pml_parts(in.sx, in.sy, in.sz, in.sp, e0, o);
+    real eyx = in.fx1 * o[0] + (in.fx2 * r) * (-hz + hz_im);
+    real ezx = in.fx1 * o[1] + (in.fx2 * r) * (hy + (-hy_im));
+    real exy = in.fy1 * o[2] + (in.fy2 * r) * (hz + (-hz_jm));
+    real ezy = in.fy1 * o[3] + (in.fy2 * r) * (-hx + hx_jm);
+    real exz = in.fz1 * o[4] + (in.fz2 * r) * (-hy + hy_km);
+    real eyz = in.fz1 * o[5] + (in.fz2 * r) * (hx + (-hx_km));
     // a source plane through the PML feeds the split along its normal
     if (sa[0] == 1) exy = exy + sv[0] * r;
     if (sa[0] == 2) exz = exz + sv[0] * r;
@@ -232,9 +254,9 @@ __device__ __forceinline__ void fused_e_cell(const FieldArgs<real>& a, const Pml
     if (sa[1] == 2) eyz = eyz + sv[1] * r;
     if (sa[2] == 0) ezx = ezx + sv[2] * r;
     if (sa[2] == 1) ezy = ezy + sv[2] * r;
-    if (sx) { pw.s[S_EYX][s] = eyx; pw.s[S_EZX][s] = ezx; }
-    if (sy) { pw.s[S_EXY][s] = exy; pw.s[S_EZY][s] = ezy; }
-    if (sz) { pw.s[S_EXZ][s] = exz; pw.s[S_EYZ][s] = eyz; }
+    if (in.sx) { pw.s[S_EYX][s] = eyx; pw.s[S_EZX][s] = ezx; }
+    if (in.sy) { pw.s[S_EXY][s] = exy; pw.s[S_EZY][s] = ezy; }
+    if (in.sz) { pw.s[S_EXZ][s] = exz; pw.s[S_EYZ][s] = eyz; }
     const real ex = exy + exz, ey = eyz + eyx, ez = ezx + ezy;
     E0[c] = ex;
     E1[c] = ey;
@@ -246,15 +268,15 @@ __device__ __forceinline__ void fused_e_cell(const FieldArgs<real>& a, const Pml
     put_e_ghost(E2, ez, i, j, k, c, a);
     return;
   }
-  if (!kHoist) {
-    e0[0] = EO0[c]; e0[1] = EO1[c]; e0[2] = EO2[c];
-    if (any_src) gx = a.gaxe[id];
-  }
   // E_q += Ku*gax_q*curl_q (+ gax_q * source) for every component a
   // dielectric writes
   const real cur[3] = {(hz - hz_jm) + (-hy + hy_km),   // curlhzy + curlhyz
                        (hx - hx_km) + (-hz + hz_im),   // curlhxz + curlhzx
                        (hy - hy_im) + (-hx + hx_jm)};  // curlhyx + curlhxy
+  const uint32_t fl = in.fl;
+  // the source scale: loaded here, source planes are few
+  Vec4<real> gx;
+  if (in.any_src) gx = a.gaxe[in.id];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     if (!(fl & (1u << q))) {
@@ -265,12 +287,25 @@ __device__ __forceinline__ void fused_e_cell(const FieldArgs<real>& a, const Pml
       }
       continue;
     }
-    real ev = e0[q] + cf.v[q] * cur[q];
+    real ev = e0[q] + in.cf.v[q] * cur[q];
     if (sa[q] >= 0) ev = ev + gx.v[q] * sv[q];
     real* Eq = q == 0 ? E0 : (q == 1 ? E1 : E2);
     Eq[c] = ev;
     put_e_ghost(Eq, ev, i, j, k, c, a);
   }
+}
+
+template <typename real, bool UNI>
+__device__ __forceinline__ void fused_e_cell(const FieldArgs<real>& a, const PmlArgs<real>& p,
+                                             const PmlArgs<real>& pw, bool pp, const real* sv, const int* sa, int rm,
+                                             int i, int j, int k, real& cy, real& cx, bool use_carry,
+                                             const real* H0, const real* H1, const real* H2, const real* EO0,
+                                             const real* EO1, const real* EO2, real* E0, real* E1, real* E2) {
+  ECellIn<real> in;
+  e_cell_load<real, UNI ? 0 : 1>(a, p, sa, rm, i, j, k, use_carry, cy, cx, H0, H1, H2, EO0, EO1, EO2, in);
+  cy = in.hy;
+  cx = in.hx;
+  e_cell_store(a, pw, pp, sv, sa, i, j, k, in, E0, E1, E2);
 }
 
 // plane sources through cell (i,j,k): value and normal axis per E
@@ -390,59 +425,80 @@ __device__ __forceinline__ void dft_fuse_write(const DftFuse<real>& df, int i, i
 }
 
 // FUSED engine: H half step (myfdtd_func.f90:45-64 + split-field H) of one
-// cell.  z march: with use_carry, Ex and Ey of plane k come from cex, cey
-// (loaded as plane k+1 by the previous plane of the same thread); the cell
-// leaves Ex, Ey of plane k+1 there.
-// In place HO* are H* and pw is p; over two copies the old H and splits
-// come from HO* and p, the new ones go to H* and pw.
+// cell, as the E step in a load phase (h_cell_load) and an arithmetic and
+// store phase (h_cell_store).  z march: with use_carry, Ex and Ey of plane
+// k come from cex, cey (loaded as plane k+1 by the previous plane of the
+// same thread).  In place HO* are H* and pw is p; over two copies the old
+// H and splits come from HO* and p, the new ones go to H* and pw.
 template <typename real>
-__device__ __forceinline__ void fused_h_cell(const FieldArgs<real>& a, const PmlArgs<real>& p,
-                                             const PmlArgs<real>& pw, const DftFuse<real>& df, int i, int j, int k,
-                                             real& cex, real& cey, bool use_carry, const real* E0, const real* E1,
-                                             const real* E2, const real* HO0, const real* HO1, const real* HO2,
-                                             real* H0, real* H1, real* H2) {
+struct HCellIn {
+  int c, s;
+  bool pml, sx, sy, sz;
+  real ex, ey, ez, ey_kp, ex_kp, ez_jp, ex_jp, ez_ip, ey_ip;
+  real h0[3], sp[6], gx1, gx2, gy1, gy2, gz1, gz2;
+};
+
+template <typename real>
+__device__ __forceinline__ void h_cell_load(const FieldArgs<real>& a, const PmlArgs<real>& p, int i, int j, int k,
+                                            bool use_carry, real cex, real cey, const real* E0, const real* E1,
+                                            const real* E2, const real* HO0, const real* HO1, const real* HO2,
+                                            HCellIn<real>& in) {
   const int c = i + a.px * j + a.pxy * k;
-  const real ex = use_carry ? cex : E0[c], ey = use_carry ? cey : E1[c], ez = E2[c];
-  const real ey_kp = E1[c + a.pxy], ex_kp = E0[c + a.pxy];
-  const real ez_jp = E2[c + a.px], ex_jp = E0[c + a.px];
-  const real ez_ip = E2[c + 1], ey_ip = E1[c + 1];
-  cex = ex_kp;
-  cey = ey_kp;
-  // FP32: loads of both forms before either computes (as in fused_e_cell)
-  constexpr bool kHoist = sizeof(real) == 4;
-  const bool pml = in_pml(p.P, i, j, k, a.nx, a.ny, a.nz);
-  int s = 0;
-  bool sx = false, sy = false, sz = false;
-  real sp[6] = {0, 0, 0, 0, 0, 0}, gx1 = 0, gx2 = 0, h0[3] = {0, 0, 0};
-  if (pml) {
-    s = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
-    sx = in_slab(p.P[0], i, a.nx);
-    sy = in_slab(p.P[1], j, a.ny);
-    sz = in_slab(p.P[2], k, a.nz);
-    gx1 = p.g1[0][i];
-    gx2 = p.g2[0][i];
-    if (sx) { sp[0] = p.s[S_HYX][s]; sp[1] = p.s[S_HZX][s]; }
-    if (sy) { sp[2] = p.s[S_HXY][s]; sp[3] = p.s[S_HZY][s]; }
-    if (sz) { sp[4] = p.s[S_HXZ][s]; sp[5] = p.s[S_HYZ][s]; }
-    if (!(sy && sz)) h0[0] = HO0[c];
-    if (!(sz && sx)) h0[1] = HO1[c];
-    if (!(sx && sy)) h0[2] = HO2[c];
-  } else if (kHoist) {
-    h0[0] = HO0[c]; h0[1] = HO1[c]; h0[2] = HO2[c];
+  in.c = c;
+  in.ex = use_carry ? cex : E0[c];
+  in.ey = use_carry ? cey : E1[c];
+  in.ez = E2[c];
+  in.ey_kp = E1[c + a.pxy]; in.ex_kp = E0[c + a.pxy];
+  in.ez_jp = E2[c + a.px]; in.ex_jp = E0[c + a.px];
+  in.ez_ip = E2[c + 1]; in.ey_ip = E1[c + 1];
+  in.pml = in_pml(p.P, i, j, k, a.nx, a.ny, a.nz);
+  in.s = 0;
+  in.sx = in.sy = in.sz = false;
+  for (int q = 0; q < 6; ++q) in.sp[q] = 0;
+  in.h0[0] = in.h0[1] = in.h0[2] = 0;
+  in.gx1 = in.gx2 = in.gy1 = in.gy2 = in.gz1 = in.gz2 = 0;
+  if (in.pml) {
+    const int s = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
+    const bool sx = in_slab(p.P[0], i, a.nx), sy = in_slab(p.P[1], j, a.ny), sz = in_slab(p.P[2], k, a.nz);
+    in.s = s;
+    in.sx = sx;
+    in.sy = sy;
+    in.sz = sz;
+    in.gx1 = p.g1[0][i]; in.gx2 = p.g2[0][i];
+    in.gy1 = p.g1[1][j]; in.gy2 = p.g2[1][j];
+    in.gz1 = p.g1[2][k]; in.gz2 = p.g2[2][k];
+    if (sx) { in.sp[0] = p.s[S_HYX][s]; in.sp[1] = p.s[S_HZX][s]; }
+    if (sy) { in.sp[2] = p.s[S_HXY][s]; in.sp[3] = p.s[S_HZY][s]; }
+    if (sz) { in.sp[4] = p.s[S_HXZ][s]; in.sp[5] = p.s[S_HYZ][s]; }
+    if (!(sy && sz)) in.h0[0] = HO0[c];
+    if (!(sz && sx)) in.h0[1] = HO1[c];
+    if (!(sx && sy)) in.h0[2] = HO2[c];
+  } else {
+    in.h0[0] = HO0[c]; in.h0[1] = HO1[c]; in.h0[2] = HO2[c];
   }
-  if (pml) {
+}
+
+template <typename real>
+__device__ __forceinline__ void h_cell_store(const FieldArgs<real>& a, const PmlArgs<real>& pw,
+                                             const DftFuse<real>& df, int i, int j, int k, const HCellIn<real>& in,
+                                             real* H0, real* H1, real* H2) {
+  const int c = in.c;
+  const real ex = in.ex, ey = in.ey, ez = in.ez, ey_kp = in.ey_kp, ex_kp = in.ex_kp, ez_jp = in.ez_jp,
+             ex_jp = in.ex_jp, ez_ip = in.ez_ip, ey_ip = in.ey_ip;
+  if (in.pml) {
     // mod3/BERpmly_mod.f90:186-191 with the splits driven by total E
+    const int s = in.s;
     real o[6];
-    pml_parts(sx, sy, sz, sp, h0, o);
-    real hyx = gx1 * o[0] + gx2 * (-ez + ez_ip);
-    real hzx = gx1 * o[1] + gx2 * (ey + (-ey_ip));
-    real hxy = p.g1[1][j] * o[2] + p.g2[1][j] * (ez + (-ez_jp));
-    real hzy = p.g1[1][j] * o[3] + p.g2[1][j] * (-ex + ex_jp);
-    real hxz = p.g1[2][k] * o[4] + p.g2[2][k] * (-ey + ey_kp);
-    real hyz = p.g1[2][k] * o[5] + p.g2[2][k] * (ex + (-ex_kp));
-    if (sx) { pw.s[S_HYX][s] = hyx; pw.s[S_HZX][s] = hzx; }
-    if (sy) { pw.s[S_HXY][s] = hxy; pw.s[S_HZY][s] = hzy; }
-    if (sz) { pw.s[S_HXZ][s] = hxz; pw.s[S_HYZ][s] = hyz; }
+    pml_parts(in.sx, in.sy, in.sz, in.sp, in.h0, o);
+    real hyx = in.gx1 * o[0] + in.gx2 * (-ez + ez_ip);
+    real hzx = in.gx1 * o[1] + in.gx2 * (ey + (-ey_ip));
+    real hxy = in.gy1 * o[2] + in.gy2 * (ez + (-ez_jp));
+    real hzy = in.gy1 * o[3] + in.gy2 * (-ex + ex_jp);
+    real hxz = in.gz1 * o[4] + in.gz2 * (-ey + ey_kp);
+    real hyz = in.gz1 * o[5] + in.gz2 * (ex + (-ex_kp));
+    if (in.sx) { pw.s[S_HYX][s] = hyx; pw.s[S_HZX][s] = hzx; }
+    if (in.sy) { pw.s[S_HXY][s] = hxy; pw.s[S_HZY][s] = hzy; }
+    if (in.sz) { pw.s[S_HXZ][s] = hxz; pw.s[S_HYZ][s] = hyz; }
     const real hx = hxy + hxz, hy = hyz + hyx, hz = hzx + hzy;
     H0[c] = hx;
     H1[c] = hy;
@@ -456,12 +512,9 @@ __device__ __forceinline__ void fused_h_cell(const FieldArgs<real>& a, const Pml
   const real curleyz = ey_kp - ey, curlezy = -ez_jp + ez;
   const real curlezx = ez_ip - ez, curlexz = -ex_kp + ex;
   const real curlexy = ex_jp - ex, curleyx = -ey_ip + ey;
-  if (!kHoist) {
-    h0[0] = HO0[c]; h0[1] = HO1[c]; h0[2] = HO2[c];
-  }
-  const real hx = h0[0] + a.ku * (curleyz + curlezy);
-  const real hy = h0[1] + a.ku * (curlezx + curlexz);
-  const real hz = h0[2] + a.ku * (curlexy + curleyx);
+  const real hx = in.h0[0] + a.ku * (curleyz + curlezy);
+  const real hy = in.h0[1] + a.ku * (curlezx + curlexz);
+  const real hz = in.h0[2] + a.ku * (curlexy + curleyx);
   H0[c] = hx;
   H1[c] = hy;
   H2[c] = hz;
@@ -469,6 +522,19 @@ __device__ __forceinline__ void fused_h_cell(const FieldArgs<real>& a, const Pml
   put_h_ghost(H1, hy, i, j, k, c, a);
   put_h_ghost(H2, hz, i, j, k, c, a);
   if constexpr (kFuseDftReal<real>) dft_fuse_write(df, i, j, k, ex, ey, ez, hx, hy, hz);
+}
+
+template <typename real>
+__device__ __forceinline__ void fused_h_cell(const FieldArgs<real>& a, const PmlArgs<real>& p,
+                                             const PmlArgs<real>& pw, const DftFuse<real>& df, int i, int j, int k,
+                                             real& cex, real& cey, bool use_carry, const real* E0, const real* E1,
+                                             const real* E2, const real* HO0, const real* HO1, const real* HO2,
+                                             real* H0, real* H1, real* H2) {
+  HCellIn<real> in;
+  h_cell_load(a, p, i, j, k, use_carry, cex, cey, E0, E1, E2, HO0, HO1, HO2, in);
+  cex = in.ex_kp;
+  cey = in.ey_kp;
+  h_cell_store(a, pw, df, i, j, k, in, H0, H1, H2);
 }
 
 // H over tiles of KZ planes, each thread marching its (i,j) column; restrict
@@ -527,7 +593,7 @@ struct MarchArgs {
   const real* eo[3];      // E after the previous step
   const real* ho[3];      // H after the previous step
   // 8 words per entry; plain tiles: i0, j0, k0, k1, xlo, xhi (owned x
-  // range), material id; general workgroups: x0, x1, j0, j1, k, width
+  // range), material id; general workgroups: x0, x1, j0, j1, k0, width, k1
   const int32_t* tiles;
   int32_t ntiles, tper;
 };
@@ -892,16 +958,20 @@ __global__ __launch_bounds__(256) void k_march_plain(FieldArgs<real> a, MarchArg
 }
 
 // General work: workgroup = 256 threads over cells x0..x1 x j0..j1 of
-// plane k (list entry: x0, x1, j0, j1, k, width), 64 x 4 cells (a
-// wavefront is one row), 32 x 8 or 16 x 16 (narrow boxes: the x edges),
-// one cell per thread, the fused cell updates over the two copies (a
-// thread marching its column through the box's planes instead is slower:
-// its loads of one plane wait behind the stores of the one before).
-// k_gen_e reads the old H and E and writes the new E; k_gen_h then reads
-// the new E and the old H and writes the new H.
+// planes k0..k1 (list entry: x0, x1, j0, j1, k0, width, k1), 64 x 4 cells
+// (a wavefront is one row), 32 x 8 or 16 x 16 (narrow boxes: the x edges),
+// the fused cell updates over the two copies.  k_gen_e reads the old H and
+// E and writes the new E, one cell per thread, plane k0 + blockIdx.y;
+// k_gen_h then reads the new E and the old H and writes the new H, each
+// thread marching its (i, j) column up through the planes, pipelined by
+// hand: the loads of plane k+1 are issued before the arithmetic and stores
+// of plane k (no plane reads what the kernel writes, so the order is
+// free), and E of the plane above comes from registers (10 % faster than a
+// cell per thread; the E update marched so holds too many registers and
+// runs slower).
 template <typename real>
-__device__ __forceinline__ bool gen_cell(const FieldArgs<real>& a, const MarchArgs<real>& m, int& i, int& j, int& k,
-                                         bool& wide) {
+__device__ __forceinline__ bool gen_cell(const FieldArgs<real>& a, const MarchArgs<real>& m, int& i, int& j, int& k0,
+                                         int& k1, bool& wide) {
   const int b = blockIdx.x;
   const int lb = a.xcdmap ? (b & 7) * m.tper + (b >> 3) : b;
   if (lb >= m.ntiles) return false;
@@ -911,16 +981,19 @@ __device__ __forceinline__ bool gen_cell(const FieldArgs<real>& a, const MarchAr
   const int t = (int)threadIdx.x;
   i = T[0] + (t & (bw - 1));
   j = T[2] + (wide ? __builtin_amdgcn_readfirstlane(t >> 6) : t / bw);
-  k = T[4];
+  k0 = T[4];
+  k1 = T[6];
   return i <= T[1] && j <= T[3];
 }
 
 template <typename real>
 __global__ __launch_bounds__(256) void k_gen_e(FieldArgs<real> a, PmlArgs<real> p, PmlArgs<real> pw,
                                                MarchArgs<real> m, SrcArgs<real> src, int nsrc) {
-  int i, j, k;
+  int i, j, k0, k1;
   bool wide;
-  if (!gen_cell(a, m, i, j, k, wide)) return;
+  if (!gen_cell(a, m, i, j, k0, k1, wide)) return;
+  const int k = k0 + (int)blockIdx.y;
+  if (k > k1) return;
   real sv[3] = {0, 0, 0};
   int sa[3] = {-1, -1, -1};
   plane_sources(src, nsrc, i, j, k, sv, sa);
@@ -937,13 +1010,19 @@ __global__ __launch_bounds__(256) void k_gen_e(FieldArgs<real> a, PmlArgs<real> 
 template <typename real>
 __global__ __launch_bounds__(256) void k_gen_h(FieldArgs<real> a, PmlArgs<real> p, PmlArgs<real> pw,
                                                MarchArgs<real> m) {
-  int i, j, k;
+  int i, j, k0, k1;
   bool wide;
-  if (!gen_cell(a, m, i, j, k, wide)) return;
+  if (!gen_cell(a, m, i, j, k0, k1, wide)) return;
   const DftFuse<real> df{};
-  real cex = 0, cey = 0;
-  fused_h_cell(a, p, pw, df, i, j, k, cex, cey, false, a.e[0], a.e[1], a.e[2], m.ho[0], m.ho[1], m.ho[2], a.h[0],
-               a.h[1], a.h[2]);
+  HCellIn<real> cur, nxt;
+  h_cell_load(a, p, i, j, k0, false, (real)0, (real)0, a.e[0], a.e[1], a.e[2], m.ho[0], m.ho[1], m.ho[2], cur);
+  for (int k = k0; k <= k1; ++k) {
+    const int kn = min(k + 1, k1);
+    h_cell_load(a, p, i, j, kn, kn > k, cur.ex_kp, cur.ey_kp, a.e[0], a.e[1], a.e[2], m.ho[0], m.ho[1], m.ho[2],
+                nxt);
+    h_cell_store(a, pw, df, i, j, k, cur, a.h[0], a.h[1], a.h[2]);
+    cur = nxt;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2021,14 +2100,15 @@ int step_impl(Sim* s) {
     }
     if (s->n_tl_gen) {
       mg.tiles = s->d_tl_gen;
-      mg.ntiles = s->n_tl_gen;  // one workgroup each
+      mg.ntiles = s->n_tl_gen;  // E: a workgroup per plane of each; H: one each
       mg.tper = (mg.ntiles + 7) / 8;
       const dim3 grid((unsigned)(an.xcdmap ? 8 * mg.tper : mg.ntiles));
       FieldArgs<real> ae = an;  // E: old H in, new E out
       for (int q = 0; q < 3; ++q) ae.h[q] = a.h[q];
       {
         Timer tm(s, FDTD_K_GEN_E);
-        hipLaunchKernelGGL(k_gen_e<real>, grid, dim3(256), 0, st, ae, p, pw, mg, src, nsrc);
+        hipLaunchKernelGGL(k_gen_e<real>, dim3(grid.x, (unsigned)s->tl_gkc), dim3(256), 0, st, ae, p, pw, mg, src,
+                           nsrc);
       }
       {
         Timer tm(s, FDTD_K_GEN_H);
