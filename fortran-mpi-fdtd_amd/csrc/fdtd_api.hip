@@ -1967,18 +1967,29 @@ int fdtd_algorithmic_bytes(fdtd_sim* s, int kernel, double* bytes) {
   const double N = (double)s->nx * s->ny * s->nz;
   double b = 0;
   if (s->engine == FDTD_ENGINE_FUSED) {
-    // per non-PML cell: read H(3) + read/write E(6); per PML cell: read the
-    // other field (3), read/write 6 splits (12), write the main field (3).
+    // per non-PML cell: read H(3) + read/write E(6).  Per PML cell: read the
+    // other field (3) and write the main field (3); per component, read and
+    // write the parts kept (those along the axes whose slabs hold the cell)
+    // and read the old component unless both parts are kept: 13 words in a
+    // cell of one slab (a face), 16 in two (an edge), 18 in three (a corner).
+    double n1 = 0, n2 = 0, n3 = 0;
+    if (s->pml_mode == FDTD_PML_ALL_FACES) {
+      const double X = 2.0 * s->P[0], Y = 2.0 * s->P[1], Z = 2.0 * s->P[2];
+      const double x = s->nx - X, y = s->ny - Y, z = s->nz - Z;
+      n3 = X * Y * Z;
+      n2 = X * Y * z + X * y * Z + x * Y * Z;
+      n1 = X * y * z + x * Y * z + x * y * Z;
+    }
+    const double pml_words = 13.0 * n1 + 16.0 * n2 + 18.0 * n3;
     if (s->onepass) {
       // plain tiles: a whole step of their (non-PML) cells per launch, 18
       // words per cell (the two half steps' 9 each); general tiles: each
       // half step as above.  The tile lists are those of the last step.
       const double Np = (double)s->n_plain_cells;
       if (kernel == FDTD_K_E) b = w * 18.0 * Np;
-      if (kernel == FDTD_K_GEN_E || kernel == FDTD_K_GEN_H)
-        b = w * (9.0 * (N - Np - s->n_pml) + 18.0 * s->n_pml);
+      if (kernel == FDTD_K_GEN_E || kernel == FDTD_K_GEN_H) b = w * (9.0 * (N - Np - s->n_pml) + pml_words);
     } else if (kernel == FDTD_K_E || kernel == FDTD_K_H) {
-      b = w * (9.0 * (N - s->n_pml) + 18.0 * s->n_pml);
+      b = w * (9.0 * (N - s->n_pml) + pml_words);
     }
     if (kernel == FDTD_K_ADE)
       for (auto& g : s->groups) b += (w * 6.0 + 8.0 * 2.0 * 3.0 * g.nstate) * (double)g.ncells;

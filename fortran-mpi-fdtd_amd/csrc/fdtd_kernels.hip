@@ -596,6 +596,7 @@ struct MarchArgs {
   // range), material id; general workgroups: x0, x1, j0, j1, k0, width, k1
   const int32_t* tiles;
   int32_t ntiles, tper;
+  int32_t nsplit;  // general workgroups: workgroups per entry (its planes, k_gen_e) or 1
 };
 
 
@@ -961,7 +962,9 @@ __global__ __launch_bounds__(256) void k_march_plain(FieldArgs<real> a, MarchArg
 // planes k0..k1 (list entry: x0, x1, j0, j1, k0, width, k1), 64 x 4 cells
 // (a wavefront is one row), 32 x 8 or 16 x 16 (narrow boxes: the x edges),
 // the fused cell updates over the two copies.  k_gen_e reads the old H and
-// E and writes the new E, one cell per thread, plane k0 + blockIdx.y;
+// E and writes the new E, one cell per thread, a workgroup per plane (the
+// planes of an entry dealt to neighbouring workgroups, so a plane's
+// neighbour below is still in L2);
 // k_gen_h then reads the new E and the old H and writes the new H, each
 // thread marching its (i, j) column up through the planes, pipelined by
 // hand: the loads of plane k+1 are issued before the arithmetic and stores
@@ -971,10 +974,12 @@ __global__ __launch_bounds__(256) void k_march_plain(FieldArgs<real> a, MarchArg
 // runs slower).
 template <typename real>
 __device__ __forceinline__ bool gen_cell(const FieldArgs<real>& a, const MarchArgs<real>& m, int& i, int& j, int& k0,
-                                         int& k1, bool& wide) {
+                                         int& k1, bool& wide, int& pl) {
   const int b = blockIdx.x;
-  const int lb = a.xcdmap ? (b & 7) * m.tper + (b >> 3) : b;
-  if (lb >= m.ntiles) return false;
+  int lb = a.xcdmap ? (b & 7) * m.tper + (b >> 3) : b;
+  if (lb >= m.ntiles * m.nsplit) return false;
+  pl = lb % m.nsplit;
+  lb /= m.nsplit;
   const int32_t* T = m.tiles + 8 * lb;
   const int bw = T[5];
   wide = bw == 64;
@@ -986,13 +991,16 @@ __device__ __forceinline__ bool gen_cell(const FieldArgs<real>& a, const MarchAr
   return i <= T[1] && j <= T[3];
 }
 
+// FP32: eight wavefronts per SIMD (the compiler keeps some kernel arguments
+// in vector registers to fit), 8 % faster than the seven its register use
+// otherwise allows; the kernel waits on memory for most of its life
 template <typename real>
-__global__ __launch_bounds__(256) void k_gen_e(FieldArgs<real> a, PmlArgs<real> p, PmlArgs<real> pw,
+__global__ __launch_bounds__(256, sizeof(real) == 4 ? 8 : 5) void k_gen_e(FieldArgs<real> a, PmlArgs<real> p, PmlArgs<real> pw,
                                                MarchArgs<real> m, SrcArgs<real> src, int nsrc) {
-  int i, j, k0, k1;
+  int i, j, k0, k1, pl;
   bool wide;
-  if (!gen_cell(a, m, i, j, k0, k1, wide)) return;
-  const int k = k0 + (int)blockIdx.y;
+  if (!gen_cell(a, m, i, j, k0, k1, wide, pl)) return;
+  const int k = k0 + pl;
   if (k > k1) return;
   real sv[3] = {0, 0, 0};
   int sa[3] = {-1, -1, -1};
@@ -1010,9 +1018,9 @@ __global__ __launch_bounds__(256) void k_gen_e(FieldArgs<real> a, PmlArgs<real> 
 template <typename real>
 __global__ __launch_bounds__(256) void k_gen_h(FieldArgs<real> a, PmlArgs<real> p, PmlArgs<real> pw,
                                                MarchArgs<real> m) {
-  int i, j, k0, k1;
+  int i, j, k0, k1, pl;
   bool wide;
-  if (!gen_cell(a, m, i, j, k0, k1, wide)) return;
+  if (!gen_cell(a, m, i, j, k0, k1, wide, pl)) return;
   const DftFuse<real> df{};
   HCellIn<real> cur, nxt;
   h_cell_load(a, p, i, j, k0, false, (real)0, (real)0, a.e[0], a.e[1], a.e[2], m.ho[0], m.ho[1], m.ho[2], cur);
@@ -1532,9 +1540,16 @@ __global__ void k_dft_table(const double* omega, int nfreq, int n_first, int cou
 constexpr int kDftMChunk = 32;
 constexpr int kDftMGroup = 8;
 
+// One thread per plane cell and component slot, the block's samples in
+// registers.  The accumulators of kDftMGroup frequencies are all loaded
+// before any is stored (a load placed after a store to the same allocation
+// would wait for it, one memory round trip per frequency).  A longer block
+// (64 samples, read in two halves) moves fewer accumulator bytes but runs
+// slower a sample: the arithmetic, not the accumulators, then bounds it.
 template <typename real>
-__global__ __launch_bounds__(256) void k_dft_accum(const real* samp, int64_t np, int nslots, int nfreq, int count,
-                                                   const double* tab, double dt, double* acc) {
+__global__ __launch_bounds__(256) void k_dft_accum(const real* __restrict__ samp, int64_t np, int nslots, int nfreq,
+                                                   int count, const double* __restrict__ tab, double dt,
+                                                   double* __restrict__ acc) {
   __shared__ double lt[kDftBlock * kDftMChunk * 2];
   const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int slot = blockIdx.y;
@@ -1556,9 +1571,6 @@ __global__ __launch_bounds__(256) void k_dft_accum(const real* samp, int64_t np,
     }
     __syncthreads();
     if (!act) continue;
-    // the accumulators of kDftMGroup frequencies are all loaded before any
-    // is stored: re and im share one allocation, so a load placed after a
-    // store would wait for it, one memory round trip per frequency
     for (int mb = 0; mb < mc; mb += kDftMGroup) {
       const int nb = min(kDftMGroup, mc - mb);
       double ar[kDftMGroup], ai[kDftMGroup];
@@ -2101,15 +2113,19 @@ int step_impl(Sim* s) {
     if (s->n_tl_gen) {
       mg.tiles = s->d_tl_gen;
       mg.ntiles = s->n_tl_gen;  // E: a workgroup per plane of each; H: one each
-      mg.tper = (mg.ntiles + 7) / 8;
-      const dim3 grid((unsigned)(an.xcdmap ? 8 * mg.tper : mg.ntiles));
       FieldArgs<real> ae = an;  // E: old H in, new E out
       for (int q = 0; q < 3; ++q) ae.h[q] = a.h[q];
       {
+        MarchArgs<real> me = mg;
+        me.nsplit = s->tl_gkc;
+        me.tper = (me.ntiles * me.nsplit + 7) / 8;
+        const dim3 ge((unsigned)(an.xcdmap ? 8 * me.tper : me.ntiles * me.nsplit));
         Timer tm(s, FDTD_K_GEN_E);
-        hipLaunchKernelGGL(k_gen_e<real>, dim3(grid.x, (unsigned)s->tl_gkc), dim3(256), 0, st, ae, p, pw, mg, src,
-                           nsrc);
+        hipLaunchKernelGGL(k_gen_e<real>, ge, dim3(256), 0, st, ae, p, pw, me, src, nsrc);
       }
+      mg.nsplit = 1;
+      mg.tper = (mg.ntiles + 7) / 8;
+      const dim3 grid((unsigned)(an.xcdmap ? 8 * mg.tper : mg.ntiles));
       {
         Timer tm(s, FDTD_K_GEN_H);
         hipLaunchKernelGGL(k_gen_h<real>, grid, dim3(256), 0, st, an, p, pw, mg);

@@ -66,7 +66,7 @@ def test_ragged_grids_vs_oracle(n, P, periodic, precision):
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_stepping_in_pieces_equals_stepping_at_once(precision):
-    """Fields bitwise; the DFT folds its 32-sample blocks at different
+    """Fields bitwise; the DFT folds its blocks of samples at different
     points, so spectra agree to rounding."""
     import fdtd_amd
 
