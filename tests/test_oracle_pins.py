@@ -454,6 +454,40 @@ def test_all_faces_pml_with_vanishing_conductivity_is_plain_yee():
     assert relerr(ta, tb) < 3e-6
 
 
+def test_all_faces_pml_with_vanishing_conductivity_keeps_every_cell_class_plain():
+    """Slabs of unequal thickness on all six faces put cells in one slab (a
+    face: 2 of the 6 split parts of E kept), two (an edge: 4) and three (a
+    corner: all 6); plane sources on the three axes cross every class.  With
+    sigma -> 0 each class, whichever parts it keeps and rebuilds, must sum to
+    the plain Yee update of the same PEC box without PML: the difference is
+    O(alpha) (it grows 4x when alpha does), while a part rebuilt from the
+    wrong one, or a source into the wrong part, leaves an O(1) difference
+    that does not scale."""
+    from fdtd_amd import box
+    from fdtd_amd.workloads import pulse_for_band
+
+    n = (20, 22, 24)
+    dt = 0.5 * 2e-9 / C0
+    pul = pulse_for_band(400e-9, 900e-9, 600e-9)
+
+    def run(P, alpha=0.0):
+        sim = _sim(*n, 2e-9, 0.5)
+        if P:
+            sim.set_pml("all_faces", P, order=3.0, R0=1e-6, sigmam=alpha / dt)
+        sim.add_dielectric(box((1, 1, 1), n), 1.0)
+        for axis, index, comp in ((0, 2, 1), (1, 9, 2), (2, 23, 0)):  # two of them inside a slab
+            sim.add_plane_source(axis, index, comp, pul["t0"], pul["st"], pul["w0"], 1.0)
+        sim.step(200)
+        return sim
+
+    b = run(None)
+    a1, a4 = run((3, 4, 5), 1e-7), run((3, 4, 5), 4e-7)
+    for c in range(6):
+        assert np.abs(b.field(c)).max() > 0
+        e1, e4 = relerr(a1.field(c), b.field(c)), relerr(a4.field(c), b.field(c))
+        assert e1 < 1e-5 and 3.0 < e4 / e1 < 5.0, (c, e1, e4)
+
+
 def test_all_faces_pml_absorbs_a_normally_incident_pulse():
     """Berenger split-field profile (m = 3, R0 = 1e-6, 10 cells) on the z
     faces: the wave reflected back to a probe 15 cells from the PML is below
