@@ -43,7 +43,26 @@ def _cases():
     graphene["sources"][0]["index"] = 8
     graphene["dft"] = [{"axis": 2, "index": 32, "omega": graphene["dft"][0]["omega"], "comps": [0, 1, 3, 4]}]
     return {"gold_sphere": _small_sphere(nsteps=200), "periodic_box": _periodic_box(),
-            "graphene_disks": graphene, "tfsf_pml": _tfsf_through_pml(nsteps=200)}
+            "graphene_disks": graphene, "tfsf_pml": _tfsf_through_pml(nsteps=200), "x_edges": _x_edges()}
+
+
+def _x_edges():
+    """96 cells along x with a 12-cell x PML: one column of plain tiles (x
+    33-64 in FP32) between general x edges 32 cells wide, a gold sphere and
+    an x-normal source plane inside the low edge, an x-normal DFT plane and
+    probes in the high one."""
+    from fdtd_amd import workloads
+    from fdtd_amd import sphere
+
+    spec = workloads.c1_gold_sphere(nsteps=200, n=96)
+    spec["grid"] = dict(spec["grid"], ny=40, nz=36)
+    spec["pml"]["npml"] = [12, 6, 6]
+    spec["materials"][0]["region"] = dict(spec["materials"][0]["region"], hi=[96, 40, 36])
+    spec["materials"][1]["region"] = sphere((22.5, 20.5, 18.5), 5.0)
+    spec["sources"] = [dict(spec["sources"][0], axis=0, index=20, comp=2)]
+    spec["dft"] = [dict(spec["dft"][0], axis=0, index=80, comps=[1, 2, 4, 5])]
+    spec["probes"] = [[22, 20, 18], [80, 20, 18], [5, 20, 18]]
+    return spec
 
 
 def _run(spec, precision, env, monkeypatch):
@@ -76,7 +95,7 @@ def test_onepass_equals_two_kernel_step_fp64(case, by, kc, monkeypatch):
     _same(a, ia, pa, b, ib, pb)
 
 
-@pytest.mark.parametrize("case", ["gold_sphere", "periodic_box", "graphene_disks", "tfsf_pml"])
+@pytest.mark.parametrize("case", ["gold_sphere", "periodic_box", "graphene_disks", "tfsf_pml", "x_edges"])
 @pytest.mark.parametrize("by,kc", [(2, 64), (4, 9)])
 def test_onepass_equals_two_kernel_step_fp32(case, by, kc, monkeypatch):
     spec = _cases()[case]
