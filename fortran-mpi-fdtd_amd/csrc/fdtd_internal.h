@@ -114,7 +114,10 @@ struct HostSource {
 
 // Samples of a DFT plane are buffered for up to kDftBlock gated steps and
 // folded into the FP64 accumulators in one pass (time-blocked running DFT).
-constexpr int kDftBlock = 32;
+// Every fdtd_step call also folds what it buffered, so a job stepped in
+// chunks folds at the same steps as one stepped at once (the same bits)
+// when the chunks are multiples of kDftBlock (the pool's 64, 256).
+constexpr int kDftBlock = 64;
 
 // Planes a general workgroup of the one-pass step marches through (at most).
 constexpr int kGenKc = 4;

@@ -1537,67 +1537,89 @@ __global__ void k_dft_table(const double* omega, int nfreq, int n_first, int cou
   tab[2 * q + 1] = sw;
 }
 
-constexpr int kDftMChunk = 32;
-constexpr int kDftMGroup = 8;
+// The fold of a block as a matrix product on the FP64 matrix cores:
+// D[m][cell] = sum_b T[m][b] F[b][cell] for the cosine and the sine rows,
+// then acc += dt * D.  A workgroup stages the table of up to 64
+// frequencies (all the block's samples, cosine and sine side by side, rows
+// padded against bank conflicts) in LDS once and then walks its share of
+// the plane; each wavefront takes 16 cells of one component slot at a
+// time, reads their samples, then streams them 4 at a time through 8 16x16
+// tiles (4 of cosines, 4
+// of sines; v_mfma_f64_16x16x4f64 on gfx950: A[i][k] comes from lane i +
+// 16k, B[k][j] from lane j + 16k, and D[i][j] lands in lane j + 16 (i % 4),
+// element i / 4).  The samples are not held in registers, so a block can
+// be long: each flush reads and writes every accumulator once, so 64
+// samples a block move half the accumulator bytes of 32.  The block's sum
+// is in another order than a running sum (products fused, four samples at
+// a time), a rounding difference the spectra carry at the 1e-16 level.
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int kDftTabRow = 65;  // double2 entries a sample row of the staged table (64 + 1 pad)
+constexpr int kDftFoldGroups = 512;  // workgroups of a fold: two a CU
 
-// One thread per plane cell and component slot, the block's samples in
-// registers.  The accumulators of kDftMGroup frequencies are all loaded
-// before any is stored (a load placed after a store to the same allocation
-// would wait for it, one memory round trip per frequency).  A longer block
-// (64 samples, read in two halves) moves fewer accumulator bytes but runs
-// slower a sample: the arithmetic, not the accumulators, then bounds it.
 template <typename real>
-__global__ __launch_bounds__(256) void k_dft_accum(const real* __restrict__ samp, int64_t np, int nslots, int nfreq,
+__global__ __launch_bounds__(256, 2) void k_dft_accum(const real* __restrict__ samp, int64_t np, int nslots, int nfreq,
                                                    int count, const double* __restrict__ tab, double dt,
                                                    double* __restrict__ acc) {
-  __shared__ double lt[kDftBlock * kDftMChunk * 2];
-  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int slot = blockIdx.y;
-  const bool act = o < np;
-  double f[kDftBlock];
-#pragma unroll
-  for (int b = 0; b < kDftBlock; ++b)
-    f[b] = (act && b < count) ? (double)samp[((int64_t)b * nslots + slot) * np + o] : 0.0;
-  double* re = acc + (int64_t)slot * 2 * nfreq * np;
-  double* im = re + (int64_t)nfreq * np;
-  for (int m0 = 0; m0 < nfreq; m0 += kDftMChunk) {
-    const int mc = min(kDftMChunk, nfreq - m0);
+  __shared__ double2 lt[kDftBlock * kDftTabRow];
+  const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+  const int jj = lane & 15, kk = lane >> 4;
+  const int64_t ntile = (np + 15) / 16, nwt = ntile * nslots;
+  for (int m0 = 0; m0 < nfreq; m0 += 64) {
     __syncthreads();
-    for (int q = threadIdx.x; q < kDftBlock * mc; q += blockDim.x) {
-      const int b = q / mc, mm = q - b * mc;
-      const bool live = b < count;
-      lt[2 * (b * kDftMChunk + mm)] = live ? tab[2 * (b * nfreq + m0 + mm)] : 0.0;
-      lt[2 * (b * kDftMChunk + mm) + 1] = live ? tab[2 * (b * nfreq + m0 + mm) + 1] : 0.0;
+    for (int q = (int)threadIdx.x; q < kDftBlock * 64; q += 256) {
+      const int b = q >> 6, mm = q & 63;
+      double2 v = {0.0, 0.0};
+      if (b < count && m0 + mm < nfreq) v = *(const double2*)(tab + 2 * ((int64_t)b * nfreq + m0 + mm));
+      lt[b * kDftTabRow + mm] = v;
     }
     __syncthreads();
-    if (!act) continue;
-    for (int mb = 0; mb < mc; mb += kDftMGroup) {
-      const int nb = min(kDftMGroup, mc - mb);
-      double ar[kDftMGroup], ai[kDftMGroup];
+    // wave tiles of 16 cells of one slot, strided over the grid (the loop
+    // count is uniform per wavefront; no barrier inside)
+    for (int64_t wt = (int64_t)blockIdx.x * 4 + w; wt < nwt; wt += (int64_t)gridDim.x * 4) {
+      const int slot = (int)(wt / ntile);
+      const int64_t cell = (wt - (int64_t)slot * ntile) * 16 + jj;
+      const bool cin = cell < np;
+      double f[kDftBlock / 4];  // this lane's samples: b = 4 st + kk
 #pragma unroll
-      for (int u = 0; u < kDftMGroup; ++u) {
-        const int64_t qq = (int64_t)(m0 + mb + u) * np + o;
-        ar[u] = u < nb ? re[qq] : 0.0;
-        ai[u] = u < nb ? im[qq] : 0.0;
+      for (int st = 0; st < kDftBlock / 4; ++st) {
+        const int b = 4 * st + kk;
+        f[st] = (cin && b < count) ? (double)samp[((int64_t)b * nslots + slot) * np + cell] : 0.0;
       }
+      v4d dc[4], ds[4];
 #pragma unroll
-      for (int u = 0; u < kDftMGroup; ++u) {
-        const int mm = mb + u;
-        double sr = 0.0, si = 0.0;
+      for (int t = 0; t < 4; ++t) dc[t] = ds[t] = (v4d){0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+      for (int st = 0; st < kDftBlock / 4; ++st) {  // every lane: the matrix instructions need them all
+        if (4 * st >= count) break;
 #pragma unroll
-        for (int b = 0; b < kDftBlock; ++b) {
-          sr += lt[2 * (b * kDftMChunk + mm)] * f[b];
-          si += lt[2 * (b * kDftMChunk + mm) + 1] * f[b];
+        for (int t = 0; t < 4; ++t) {
+          const double2 cs = lt[(4 * st + kk) * kDftTabRow + 16 * t + jj];
+          dc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(cs.x, f[st], dc[t], 0, 0, 0);
+          ds[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(cs.y, f[st], ds[t], 0, 0, 0);
         }
-        ar[u] = ar[u] + sr * dt;
-        ai[u] = ai[u] + si * dt;
       }
+      if (!cin) continue;  // cells past the plane's end: no accumulator
+      double* re = acc + (int64_t)slot * 2 * nfreq * np;
+      double* im = re + (int64_t)nfreq * np;
+      // accumulators: all loads of a tile pair before its stores
 #pragma unroll
-      for (int u = 0; u < kDftMGroup; ++u) {
-        if (u >= nb) break;
-        const int64_t qq = (int64_t)(m0 + mb + u) * np + o;
-        re[qq] = ar[u];
-        im[qq] = ai[u];
+      for (int t = 0; t < 4; ++t) {
+        double ar[4], ai[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int m = m0 + 16 * t + kk + 4 * v;
+          const int64_t q = (int64_t)m * np + cell;
+          ar[v] = m < nfreq ? re[q] : 0.0;
+          ai[v] = m < nfreq ? im[q] : 0.0;
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int m = m0 + 16 * t + kk + 4 * v;
+          if (m >= nfreq) continue;
+          const int64_t q = (int64_t)m * np + cell;
+          re[q] = ar[v] + dc[t][v] * dt;
+          im[q] = ai[v] + ds[t][v] * dt;
+        }
       }
     }
   }
@@ -1980,7 +2002,10 @@ void dft_flush_plane(Sim* s, DftPlane& d) {
   const int nt = d.nfreq * d.nbuf;
   hipLaunchKernelGGL(k_dft_table, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s->stream, d.d_omega, d.nfreq,
                      d.n_first, d.nbuf, s->dt, d.d_tab);
-  hipLaunchKernelGGL(k_dft_accum<real>, dim3((unsigned)((np + 255) / 256), (unsigned)d.nslots), dim3(256), 0,
+  // two workgroups a CU (the staged table is 65 KB of LDS), each walking
+  // wave tiles of 16 cells of one slot
+  const int64_t wgs = ((np + 15) / 16 * d.nslots + 3) / 4;
+  hipLaunchKernelGGL(k_dft_accum<real>, dim3((unsigned)std::min<int64_t>(wgs, kDftFoldGroups)), dim3(256), 0,
                      s->stream, (const real*)d.d_samp, np, d.nslots, d.nfreq, d.nbuf, d.d_tab, s->dt, d.d_acc);
   d.nbuf = 0;
 }
@@ -2317,10 +2342,14 @@ int step_impl(Sim* s) {
     int rc = launch_monitors();
     if (rc) return rc;
   }
-  // fold each full ring after this step's samples are written
+  // fold after this step's samples are written, when the ring is full or
+  // the step is a multiple of kDftBlock: the blocks then end at the same
+  // steps however the calls that make the steps are cut (a call folds what
+  // it buffered at its end), so runs stepped in chunks of kDftBlock
+  // multiples, or resumed from a checkpoint, equal runs stepped at once
   for (DftPlane* d : active) {
     d->nbuf += 1;
-    if (d->nbuf == kDftBlock) dft_flush_plane<real>(s, *d);
+    if (d->nbuf == kDftBlock || n % kDftBlock == 0) dft_flush_plane<real>(s, *d);
   }
   return 0;
 }

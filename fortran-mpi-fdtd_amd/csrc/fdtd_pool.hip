@@ -148,7 +148,9 @@ int fdtd_pool_create(fdtd_pool** out, size_t hbm_budget, int max_active, int chu
   fdtd_pool* p = new fdtd_pool();
   p->budget = hbm_budget;
   p->max_active = max_active;
-  p->chunk = chunk_steps;
+  // whole DFT blocks: a chunk then ends where a run stepped at once folds
+  // its spectra, so a served job's results equal its run alone bit for bit
+  p->chunk = (chunk_steps + kDftBlock - 1) / kDftBlock * kDftBlock;
   *out = p;
   return 0;
 }

@@ -426,9 +426,10 @@ int fdtd_estimate_bytes(fdtd_sim* sim, size_t* bytes);
  * No counterpart in the reference (it runs one simulation per mpirun).
  * A pool admits set-up handles in arrival order while their device bytes fit
  * the HBM budget and fewer than max_active run; every admitted handle runs
- * on its own stream and advances chunk_steps at a time, one chunk queued per
- * handle per fdtd_pool_progress call, so co-resident jobs overlap on the
- * device.  After every chunk the pool checks the job's E and H for NaN or
+ * on its own stream and advances chunk_steps at a time (rounded up to a
+ * multiple of 64, the DFT fold block, so that a job's spectra equal its run
+ * alone bit for bit), one chunk queued per handle per fdtd_pool_progress
+ * call, so co-resident jobs overlap on the device.  After every chunk the pool checks the job's E and H for NaN or
  * Inf (one read of the fields per chunk, on the job's stream): a job whose
  * fields are no longer finite (an unstable time step, a diverging
  * material) ends FDTD_JOB_FAILED and fdtd_pool_status returns
