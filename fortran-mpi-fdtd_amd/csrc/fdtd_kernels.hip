@@ -156,10 +156,9 @@ __device__ __forceinline__ int pml_index(const int32_t* P, int i, int j, int k, 
 // e_cell_store): the field pointers may alias as far as the compiler knows,
 // so a load placed after a store would wait for it and cost one more memory
 // round trip.  The two phases are apart so that a thread marching a column
-// can issue the next plane's loads before this plane's arithmetic.  IDM:
-// how the material id is found -- 0 the row's one id `rm` (wave-uniform,
-// the table entry read with scalar loads), 1 gathered per cell, 2 `rm`
-// where it is >= 0 and gathered otherwise.
+// can issue the next plane's loads before this plane's arithmetic.  UNI:
+// the row has one material id `rm` (wave-uniform, the table entry read with
+// scalar loads); otherwise the id is gathered per cell.
 //
 // z march: with use_carry, Hy and Hx of plane k-1 come from cy, cx (the
 // previous plane of the same thread) instead of memory.
@@ -178,7 +177,7 @@ struct ECellIn {
   real e0[3], sp[6], fx1, fx2, fy1, fy2, fz1, fz2;
 };
 
-template <typename real, int IDM>
+template <typename real, bool UNI>
 __device__ __forceinline__ void e_cell_load(const FieldArgs<real>& a, const PmlArgs<real>& p,
                                             const int* sa, int rm, int i, int j, int k, bool use_carry, real cy,
                                             real cx, const real* H0, const real* H1, const real* H2,
@@ -190,10 +189,7 @@ __device__ __forceinline__ void e_cell_load(const FieldArgs<real>& a, const PmlA
   in.hy_km = use_carry ? cy : H1[c - a.pxy];
   in.hx_km = use_carry ? cx : H0[c - a.pxy];
   in.hz_im = H2[c - 1]; in.hy_im = H1[c - 1];
-  unsigned id;
-  if constexpr (IDM == 0) id = (unsigned)__builtin_amdgcn_readfirstlane(rm);
-  else if constexpr (IDM == 1) id = (unsigned)a.matid[c];
-  else id = rm >= 0 ? (unsigned)rm : (unsigned)a.matid[c];
+  const unsigned id = UNI ? (unsigned)__builtin_amdgcn_readfirstlane(rm) : (unsigned)a.matid[c];
   in.id = id;
   in.fl = a.flags[id];
   in.cf = a.coef[id];
@@ -302,7 +298,7 @@ __device__ __forceinline__ void fused_e_cell(const FieldArgs<real>& a, const Pml
                                              const real* H0, const real* H1, const real* H2, const real* EO0,
                                              const real* EO1, const real* EO2, real* E0, real* E1, real* E2) {
   ECellIn<real> in;
-  e_cell_load<real, UNI ? 0 : 1>(a, p, sa, rm, i, j, k, use_carry, cy, cx, H0, H1, H2, EO0, EO1, EO2, in);
+  e_cell_load<real, UNI>(a, p, sa, rm, i, j, k, use_carry, cy, cx, H0, H1, H2, EO0, EO1, EO2, in);
   cy = in.hy;
   cx = in.hx;
   e_cell_store(a, pw, pp, sv, sa, i, j, k, in, E0, E1, E2);
