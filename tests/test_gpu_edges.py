@@ -85,6 +85,30 @@ def test_stepping_in_pieces_equals_stepping_at_once(precision):
     b.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_stepping_in_whole_dft_blocks_equals_stepping_at_once_bitwise(precision):
+    """The DFT folds its blocks at steps that are multiples of the block
+    (64) and where a call ends: calls that end on such steps (64, 128, then
+    the rest) fold at the same steps as one call, so the spectra are equal
+    bit for bit as well as the fields and the probes."""
+    import fdtd_amd
+
+    spec = _ragged((37, 29, 23), (4, 3, 5), (0, 0, 0), nsteps=200)
+    a, aids, ap = fdtd_amd.make_simulation(spec, precision=precision)
+    b, bids, bp = fdtd_amd.make_simulation(spec, precision=precision)
+    a.step(200)
+    for k in (64, 64, 72):
+        b.step(k)
+    for q in range(6):
+        assert np.array_equal(a.field(q), b.field(q)), q
+    assert np.array_equal(a.flux(aids[0]), b.flux(bids[0]))
+    assert np.abs(a.flux(aids[0])).max() > 0
+    for x, y in zip(ap, bp):
+        assert np.array_equal(a.probe(x), b.probe(y))
+    a.close()
+    b.close()
+
+
 def test_step_zero_changes_nothing():
     import fdtd_amd
 
