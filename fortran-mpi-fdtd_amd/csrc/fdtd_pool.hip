@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <chrono>
 #include <deque>
 #include <vector>
@@ -150,7 +151,8 @@ int fdtd_pool_create(fdtd_pool** out, size_t hbm_budget, int max_active, int chu
   p->max_active = max_active;
   // whole DFT blocks: a chunk then ends where a run stepped at once folds
   // its spectra, so a served job's results equal its run alone bit for bit
-  p->chunk = (chunk_steps + kDftBlock - 1) / kDftBlock * kDftBlock;
+  const int64_t whole = ((int64_t)chunk_steps + kDftBlock - 1) / kDftBlock * kDftBlock;
+  p->chunk = (int)std::min<int64_t>(whole, (int64_t)INT_MAX / kDftBlock * kDftBlock);
   *out = p;
   return 0;
 }
