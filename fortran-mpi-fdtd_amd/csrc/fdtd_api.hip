@@ -1020,6 +1020,42 @@ int build_march_tiles(Sim* s, int by, int kc) {
   std::vector<int32_t> plain, gen;
   struct GBox {
     int x0, x1, j0, j1, k0, k1;
+    int face;  // 1, 2: every cell a face cell of the y or z slabs (see face_of)
+  };
+  // A general box whose cells are all face cells of the y slabs (1) or of
+  // the z slabs (2) -- x inside the x interior, the other axis's slabs not
+  // touched -- in rows of one plain dielectric each, with no source or TFSF
+  // plane through it: the general kernels update it with the slab axis
+  // fixed at compile time (the same arithmetic, fewer tests and loads).
+  auto face_of = [&](int x0, int x1, int j0, int j1, int k0, int k1) {
+    // 64-wide workgroups only (more than 32 cells): the E update reads the
+    // row's id with scalar loads, which needs one row per wavefront
+    if (!af || x0 <= Px || x1 > nx - Px || x1 - x0 + 1 <= 32) return 0;
+    bool allz = true, noz = true, ally = true, noy = true;
+    for (int k = k0; k <= k1; ++k) {
+      allz = allz && pml_z(k);
+      noz = noz && !pml_z(k);
+    }
+    for (int j = j0; j <= j1; ++j) {
+      ally = ally && pml_y(j);
+      noy = noy && !pml_y(j);
+    }
+    const int face = (allz && noy) ? 2 : ((ally && noz) ? 1 : 0);
+    if (!face) return 0;
+    for (int k = k0; k <= k1; ++k)
+      for (int j = j0; j <= j1; ++j) {
+        const int id = seg_id(j, k, 1, nx);  // the row's one id (the kernel reads it from rowmat)
+        if (id < 0 || !plain_id(id)) return 0;
+      }
+    auto crosses = [&](int axis, int index) {
+      const int lo = axis == 0 ? x0 : (axis == 1 ? j0 : k0), hi = axis == 0 ? x1 : (axis == 1 ? j1 : k1);
+      return index >= lo && index <= hi;
+    };
+    for (const auto& src : s->srcs)
+      if (crosses(src.axis, src.index)) return 0;
+    for (const auto& t : s->tfsfs)
+      if (crosses(t.p.axis, t.p.lo) || (t.p.hi > 0 && crosses(t.p.axis, t.p.hi + 1))) return 0;
+    return face;
   };
   std::vector<GBox> gbox;
   int64_t nplain = 0;
@@ -1031,7 +1067,7 @@ int build_march_tiles(Sim* s, int by, int kc) {
         const int jend = std::min(j0 + by - 1, ny);
         if (tx < 0 || tx == ntx) {  // the x edges: general
           const int a = tx < 0 ? 1 : std::max(xe + 1, xs), b = tx < 0 ? std::min(xs - 1, nx) : nx;
-          if (a <= b) gbox.push_back({a, b, j0, jend, k0, k1});
+          if (a <= b) gbox.push_back({a, b, j0, jend, k0, k1, 0});
           continue;
         }
         const int i0 = xs + 64 * tx;
@@ -1080,7 +1116,7 @@ int build_march_tiles(Sim* s, int by, int kc) {
           plain.insert(plain.end(), e, e + 8);
           nplain += (int64_t)(xhi - xlo + 1) * (jend - j0 + 1) * (k1 - k0 + 1);
         } else {
-          gbox.push_back({xlo, xhi, j0, jend, k0, k1});
+          gbox.push_back({xlo, xhi, j0, jend, k0, k1, face_of(xlo, xhi, j0, jend, k0, k1)});
         }
       }
   // general boxes: rows of the same x range and z range that follow each
@@ -1091,13 +1127,14 @@ int build_march_tiles(Sim* s, int by, int kc) {
     if (u.k0 != v.k0) return u.k0 < v.k0;
     if (u.x0 != v.x0) return u.x0 < v.x0;
     if (u.x1 != v.x1) return u.x1 < v.x1;
+    if (u.face != v.face) return u.face < v.face;
     return u.j0 < v.j0;
   });
   std::vector<GBox> merged;
   for (const GBox& g : gbox) {
     if (!merged.empty()) {
       GBox& m = merged.back();
-      if (m.k0 == g.k0 && m.k1 == g.k1 && m.x0 == g.x0 && m.x1 == g.x1 && m.j1 + 1 == g.j0) {
+      if (m.k0 == g.k0 && m.k1 == g.k1 && m.x0 == g.x0 && m.x1 == g.x1 && m.face == g.face && m.j1 + 1 == g.j0) {
         m.j1 = g.j1;
         continue;
       }
@@ -1114,7 +1151,7 @@ int build_march_tiles(Sim* s, int by, int kc) {
       for (int j = g.j0; j <= g.j1; j += bh)
         for (int x = g.x0; x <= g.x1; x += bw) {
           const int32_t e[8] = {x, std::min(x + bw - 1, g.x1), j, std::min(j + bh - 1, g.j1),
-                                k, bw, std::min(k + gkc - 1, g.k1), 0};
+                                k, bw, std::min(k + gkc - 1, g.k1), g.face};
           gen.insert(gen.end(), e, e + 8);
         }
   }

@@ -66,6 +66,7 @@ struct PmlArgs {
   const real* f2[3];
   const real* g1[3];
   const real* g2[3];
+  real kup;  // f2 and g2 outside an axis's slabs (f1 and g1 are 1 there)
 };
 
 template <typename real>
@@ -177,7 +178,7 @@ struct ECellIn {
   real e0[3], sp[6], fx1, fx2, fy1, fy2, fz1, fz2;
 };
 
-template <typename real, bool UNI>
+template <typename real, bool UNI, int FACE = 0>
 __device__ __forceinline__ void e_cell_load(const FieldArgs<real>& a, const PmlArgs<real>& p,
                                             const int* sa, int rm, int i, int j, int k, bool use_carry, real cy,
                                             real cx, const real* H0, const real* H1, const real* H2,
@@ -193,7 +194,7 @@ __device__ __forceinline__ void e_cell_load(const FieldArgs<real>& a, const PmlA
   in.id = id;
   in.fl = a.flags[id];
   in.cf = a.coef[id];
-  in.pml = in_pml(p.P, i, j, k, a.nx, a.ny, a.nz);
+  in.pml = FACE ? true : in_pml(p.P, i, j, k, a.nx, a.ny, a.nz);
   in.any_src = sa[0] >= 0 || sa[1] >= 0 || sa[2] >= 0;
   in.s = 0;
   in.sx = in.sy = in.sz = false;
@@ -202,14 +203,26 @@ __device__ __forceinline__ void e_cell_load(const FieldArgs<real>& a, const PmlA
   in.fx1 = in.fx2 = in.fy1 = in.fy2 = in.fz1 = in.fz2 = 0;
   if (in.pml) {
     const int s = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
-    const bool sx = in_slab(p.P[0], i, a.nx), sy = in_slab(p.P[1], j, a.ny), sz = in_slab(p.P[2], k, a.nz);
+    bool sx, sy, sz;
+    if constexpr (FACE == 0) {
+      sx = in_slab(p.P[0], i, a.nx);
+      sy = in_slab(p.P[1], j, a.ny);
+      sz = in_slab(p.P[2], k, a.nz);
+      in.fx1 = p.f1[0][i]; in.fx2 = p.f2[0][i];
+      in.fy1 = p.f1[1][j]; in.fy2 = p.f2[1][j];
+      in.fz1 = p.f1[2][k]; in.fz2 = p.f2[2][k];
+    } else {  // a face cell of the y (FACE 1) or z (FACE 2) slabs: the other axes' profiles are 1, Ku
+      sx = false;
+      sy = FACE == 1;
+      sz = FACE == 2;
+      in.fx1 = (real)1; in.fx2 = p.kup;
+      in.fy1 = sy ? p.f1[1][j] : (real)1; in.fy2 = sy ? p.f2[1][j] : p.kup;
+      in.fz1 = sz ? p.f1[2][k] : (real)1; in.fz2 = sz ? p.f2[2][k] : p.kup;
+    }
     in.s = s;
     in.sx = sx;
     in.sy = sy;
     in.sz = sz;
-    in.fx1 = p.f1[0][i]; in.fx2 = p.f2[0][i];
-    in.fy1 = p.f1[1][j]; in.fy2 = p.f2[1][j];
-    in.fz1 = p.f1[2][k]; in.fz2 = p.f2[2][k];
     // only the parts along the axes whose slabs hold the cell are stored
     // (pml_parts); the old component is read where a part is not
     if (sx) { in.sp[0] = p.s[S_EYX][s]; in.sp[1] = p.s[S_EZX][s]; }
@@ -434,7 +447,7 @@ struct HCellIn {
   real h0[3], sp[6], gx1, gx2, gy1, gy2, gz1, gz2;
 };
 
-template <typename real>
+template <typename real, int FACE = 0>
 __device__ __forceinline__ void h_cell_load(const FieldArgs<real>& a, const PmlArgs<real>& p, int i, int j, int k,
                                             bool use_carry, real cex, real cey, const real* E0, const real* E1,
                                             const real* E2, const real* HO0, const real* HO1, const real* HO2,
@@ -447,7 +460,7 @@ __device__ __forceinline__ void h_cell_load(const FieldArgs<real>& a, const PmlA
   in.ey_kp = E1[c + a.pxy]; in.ex_kp = E0[c + a.pxy];
   in.ez_jp = E2[c + a.px]; in.ex_jp = E0[c + a.px];
   in.ez_ip = E2[c + 1]; in.ey_ip = E1[c + 1];
-  in.pml = in_pml(p.P, i, j, k, a.nx, a.ny, a.nz);
+  in.pml = FACE ? true : in_pml(p.P, i, j, k, a.nx, a.ny, a.nz);
   in.s = 0;
   in.sx = in.sy = in.sz = false;
   for (int q = 0; q < 6; ++q) in.sp[q] = 0;
@@ -455,14 +468,26 @@ __device__ __forceinline__ void h_cell_load(const FieldArgs<real>& a, const PmlA
   in.gx1 = in.gx2 = in.gy1 = in.gy2 = in.gz1 = in.gz2 = 0;
   if (in.pml) {
     const int s = pml_index(p.P, i, j, k, a.nx, a.ny, a.nz);
-    const bool sx = in_slab(p.P[0], i, a.nx), sy = in_slab(p.P[1], j, a.ny), sz = in_slab(p.P[2], k, a.nz);
+    bool sx, sy, sz;
+    if constexpr (FACE == 0) {
+      sx = in_slab(p.P[0], i, a.nx);
+      sy = in_slab(p.P[1], j, a.ny);
+      sz = in_slab(p.P[2], k, a.nz);
+      in.gx1 = p.g1[0][i]; in.gx2 = p.g2[0][i];
+      in.gy1 = p.g1[1][j]; in.gy2 = p.g2[1][j];
+      in.gz1 = p.g1[2][k]; in.gz2 = p.g2[2][k];
+    } else {
+      sx = false;
+      sy = FACE == 1;
+      sz = FACE == 2;
+      in.gx1 = (real)1; in.gx2 = p.kup;
+      in.gy1 = sy ? p.g1[1][j] : (real)1; in.gy2 = sy ? p.g2[1][j] : p.kup;
+      in.gz1 = sz ? p.g1[2][k] : (real)1; in.gz2 = sz ? p.g2[2][k] : p.kup;
+    }
     in.s = s;
     in.sx = sx;
     in.sy = sy;
     in.sz = sz;
-    in.gx1 = p.g1[0][i]; in.gx2 = p.g2[0][i];
-    in.gy1 = p.g1[1][j]; in.gy2 = p.g2[1][j];
-    in.gz1 = p.g1[2][k]; in.gz2 = p.g2[2][k];
     if (sx) { in.sp[0] = p.s[S_HYX][s]; in.sp[1] = p.s[S_HZX][s]; }
     if (sy) { in.sp[2] = p.s[S_HXY][s]; in.sp[3] = p.s[S_HZY][s]; }
     if (sz) { in.sp[4] = p.s[S_HXZ][s]; in.sp[5] = p.s[S_HYZ][s]; }
@@ -970,7 +995,7 @@ __global__ __launch_bounds__(256) void k_march_plain(FieldArgs<real> a, MarchArg
 // runs slower).
 template <typename real>
 __device__ __forceinline__ bool gen_cell(const FieldArgs<real>& a, const MarchArgs<real>& m, int& i, int& j, int& k0,
-                                         int& k1, bool& wide, int& pl) {
+                                         int& k1, bool& wide, int& pl, int& face) {
   const int b = blockIdx.x;
   int lb = a.xcdmap ? (b & 7) * m.tper + (b >> 3) : b;
   if (lb >= m.ntiles * m.nsplit) return false;
@@ -984,6 +1009,7 @@ __device__ __forceinline__ bool gen_cell(const FieldArgs<real>& a, const MarchAr
   j = T[2] + (wide ? __builtin_amdgcn_readfirstlane(t >> 6) : t / bw);
   k0 = T[4];
   k1 = T[6];
+  face = T[7];
   return i <= T[1] && j <= T[3];
 }
 
@@ -993,16 +1019,30 @@ __device__ __forceinline__ bool gen_cell(const FieldArgs<real>& a, const MarchAr
 template <typename real>
 __global__ __launch_bounds__(256, sizeof(real) == 4 ? 8 : 5) void k_gen_e(FieldArgs<real> a, PmlArgs<real> p, PmlArgs<real> pw,
                                                MarchArgs<real> m, SrcArgs<real> src, int nsrc) {
-  int i, j, k0, k1, pl;
+  int i, j, k0, k1, pl, face;
   bool wide;
-  if (!gen_cell(a, m, i, j, k0, k1, wide, pl)) return;
+  if (!gen_cell(a, m, i, j, k0, k1, wide, pl, face)) return;
   const int k = k0 + pl;
   if (k > k1) return;
   real sv[3] = {0, 0, 0};
   int sa[3] = {-1, -1, -1};
-  plane_sources(src, nsrc, i, j, k, sv, sa);
   real cy = 0, cx = 0;
   const int rm = wide ? a.rowmat[__builtin_amdgcn_readfirstlane(j + (a.ny + 2) * k)] : -1;
+  // face cells of one slab in rows of one dielectric, no source plane
+  // (build_march_tiles); the row's id is read with scalar loads, so only
+  // 64-wide rows with an id take this path
+  if (face && wide && rm >= 0) {
+    ECellIn<real> in;
+    if (face == 2)
+      e_cell_load<real, true, 2>(a, p, sa, rm, i, j, k, false, cy, cx, a.h[0], a.h[1], a.h[2], m.eo[0], m.eo[1],
+                                 m.eo[2], in);
+    else
+      e_cell_load<real, true, 1>(a, p, sa, rm, i, j, k, false, cy, cx, a.h[0], a.h[1], a.h[2], m.eo[0], m.eo[1],
+                                 m.eo[2], in);
+    e_cell_store(a, pw, true, sv, sa, i, j, k, in, a.e[0], a.e[1], a.e[2]);
+    return;
+  }
+  plane_sources(src, nsrc, i, j, k, sv, sa);
   if (rm >= 0)
     fused_e_cell<real, true>(a, p, pw, true, sv, sa, rm, i, j, k, cy, cx, false, a.h[0], a.h[1], a.h[2], m.eo[0],
                              m.eo[1], m.eo[2], a.e[0], a.e[1], a.e[2]);
@@ -1011,22 +1051,35 @@ __global__ __launch_bounds__(256, sizeof(real) == 4 ? 8 : 5) void k_gen_e(FieldA
                               m.eo[1], m.eo[2], a.e[0], a.e[1], a.e[2]);
 }
 
-template <typename real>
-__global__ __launch_bounds__(256) void k_gen_h(FieldArgs<real> a, PmlArgs<real> p, PmlArgs<real> pw,
-                                               MarchArgs<real> m) {
-  int i, j, k0, k1, pl;
-  bool wide;
-  if (!gen_cell(a, m, i, j, k0, k1, wide, pl)) return;
+// one (i, j) column of planes k0..k1, pipelined by hand (see above)
+template <typename real, int FACE>
+__device__ __forceinline__ void gen_h_column(const FieldArgs<real>& a, const PmlArgs<real>& p, const PmlArgs<real>& pw,
+                                             const MarchArgs<real>& m, int i, int j, int k0, int k1) {
   const DftFuse<real> df{};
   HCellIn<real> cur, nxt;
-  h_cell_load(a, p, i, j, k0, false, (real)0, (real)0, a.e[0], a.e[1], a.e[2], m.ho[0], m.ho[1], m.ho[2], cur);
+  h_cell_load<real, FACE>(a, p, i, j, k0, false, (real)0, (real)0, a.e[0], a.e[1], a.e[2], m.ho[0], m.ho[1],
+                          m.ho[2], cur);
   for (int k = k0; k <= k1; ++k) {
     const int kn = min(k + 1, k1);
-    h_cell_load(a, p, i, j, kn, kn > k, cur.ex_kp, cur.ey_kp, a.e[0], a.e[1], a.e[2], m.ho[0], m.ho[1], m.ho[2],
-                nxt);
+    h_cell_load<real, FACE>(a, p, i, j, kn, kn > k, cur.ex_kp, cur.ey_kp, a.e[0], a.e[1], a.e[2], m.ho[0],
+                            m.ho[1], m.ho[2], nxt);
     h_cell_store(a, pw, df, i, j, k, cur, a.h[0], a.h[1], a.h[2]);
     cur = nxt;
   }
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_gen_h(FieldArgs<real> a, PmlArgs<real> p, PmlArgs<real> pw,
+                                               MarchArgs<real> m) {
+  int i, j, k0, k1, pl, face;
+  bool wide;
+  if (!gen_cell(a, m, i, j, k0, k1, wide, pl, face)) return;
+  if (face == 2)
+    gen_h_column<real, 2>(a, p, pw, m, i, j, k0, k1);
+  else if (face == 1)
+    gen_h_column<real, 1>(a, p, pw, m, i, j, k0, k1);
+  else
+    gen_h_column<real, 0>(a, p, pw, m, i, j, k0, k1);
 }
 
 // ---------------------------------------------------------------------------
@@ -1880,6 +1933,7 @@ PmlArgs<real> pml_args(Sim* s) {
   PmlArgs<real> p{};
   for (int q = 0; q < 3; ++q) p.P[q] = s->P[q];
   for (int q = 0; q < S_N; ++q) p.s[q] = s->d_split ? (real*)s->d_split + (int64_t)q * s->npml_cells : nullptr;
+  p.kup = (real)s->ku_pml;
   if (s->d_prof) {
     const real* base = (const real*)s->d_prof;
     int64_t off = 0;
