@@ -964,7 +964,7 @@ void* field_ptr(Sim* s, int comp) {
 // before the high z slab, so no such cell is a PML cell there.  General
 // work goes to the list as workgroups over at most kGenKc planes of a box
 // (FDTD_GEN_KC overrides): 8 words each, x0, x1, j0, j1, k0, the
-// workgroup's width (64, 32 or 16), k1.
+// workgroup's width (64, 32 or 16), k1, and the face kind (face_of).
 int build_march_tiles(Sim* s, int by, int kc) {
   if (s->tl_by == by && s->tl_kc == kc) return 0;
   const int nx = s->nx, ny = s->ny, nz = s->nz;

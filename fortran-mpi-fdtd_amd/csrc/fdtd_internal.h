@@ -235,7 +235,7 @@ struct Sim {
   int tl_by = -1, tl_kc = -1;        // tile lists below built for these
   int tl_gkc = 1;                    // planes per general workgroup in them
   int32_t* d_tl_plain = nullptr;     // i0, j0, k0, k1, xlo, xhi, material id, 0
-  int32_t* d_tl_gen = nullptr;       // workgroups: x0, x1, j0, j1, k0, width (64, 32 or 16), k1, 0
+  int32_t* d_tl_gen = nullptr;       // workgroups: x0, x1, j0, j1, k0, width (64, 32 or 16), k1, face (0, 1: y, 2: z)
   int n_tl_plain = 0, n_tl_gen = 0;
   int64_t n_plain_cells = 0;         // cells owned by plain tiles
   // material runs of every (j,k) row r = j + (ny+2)*k: runs h_runoff[r] ..

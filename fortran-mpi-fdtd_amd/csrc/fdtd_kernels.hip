@@ -614,7 +614,7 @@ struct MarchArgs {
   const real* eo[3];      // E after the previous step
   const real* ho[3];      // H after the previous step
   // 8 words per entry; plain tiles: i0, j0, k0, k1, xlo, xhi (owned x
-  // range), material id; general workgroups: x0, x1, j0, j1, k0, width, k1
+  // range), material id; general workgroups: x0, x1, j0, j1, k0, width, k1, face
   const int32_t* tiles;
   int32_t ntiles, tper;
   int32_t nsplit;  // general workgroups: workgroups per entry (its planes, k_gen_e) or 1
@@ -980,7 +980,7 @@ __global__ __launch_bounds__(256) void k_march_plain(FieldArgs<real> a, MarchArg
 }
 
 // General work: workgroup = 256 threads over cells x0..x1 x j0..j1 of
-// planes k0..k1 (list entry: x0, x1, j0, j1, k0, width, k1), 64 x 4 cells
+// planes k0..k1 (list entry: x0, x1, j0, j1, k0, width, k1, face), 64 x 4 cells
 // (a wavefront is one row), 32 x 8 or 16 x 16 (narrow boxes: the x edges),
 // the fused cell updates over the two copies.  k_gen_e reads the old H and
 // E and writes the new E, one cell per thread, a workgroup per plane (the
