@@ -1092,11 +1092,9 @@ __global__ __launch_bounds__(256) void k_gen_h(FieldArgs<real> a, PmlArgs<real> 
 // (an FP32 state gave 6e-5 relative spectra error on the 16-disk case);
 // E is stored in field precision.
 template <typename real, bool FUSED>
-__global__ __launch_bounds__(256) void k_ade(FieldArgs<real> a, AdeCoef<double> co, const int32_t* cells,
-                                             const uint8_t* masks, double* state, int64_t n, SrcArgs<real> src,
-                                             int nsrc) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
+__device__ __forceinline__ void ade_cell(const FieldArgs<real>& a, const AdeCoef<double>& co, const int32_t* cells,
+                                         const uint8_t* masks, double* state, int64_t n, const SrcArgs<real>& src,
+                                         int nsrc, int64_t e) {
   const int64_t c = cells[e];
   const int mask = masks[e];
   const int k = (int)(c / a.pxy);
@@ -1172,6 +1170,39 @@ __global__ __launch_bounds__(256) void k_ade(FieldArgs<real> a, AdeCoef<double> 
           (co.gbr[pp] * gi[pp] + co.gbi[pp] * gr[pp]) + co.gci[pp] * ev;
     }
   }
+}
+
+template <typename real, bool FUSED>
+__global__ __launch_bounds__(256) void k_ade(FieldArgs<real> a, AdeCoef<double> co, const int32_t* cells,
+                                             const uint8_t* masks, double* state, int64_t n, SrcArgs<real> src,
+                                             int nsrc) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  ade_cell<real, FUSED>(a, co, cells, masks, state, n, src, nsrc, e);
+}
+
+// Several groups in one launch (the one-pass step): group q takes blocks
+// blk[q] .. blk[q+1]-1.  A small group alone is a short launch that leaves
+// most of the GPU idle, and every launch boundary costs a drain.
+constexpr int kAdeMulti = 4;
+struct AdeMulti {
+  int32_t ng;
+  int32_t blk[kAdeMulti + 1];
+  AdeCoef<double> co[kAdeMulti];
+  const int32_t* cells[kAdeMulti];
+  const uint8_t* masks[kAdeMulti];
+  double* state[kAdeMulti];
+  int64_t n[kAdeMulti];
+};
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_ade_multi(FieldArgs<real> a, AdeMulti g, SrcArgs<real> src, int nsrc) {
+  const int b = (int)blockIdx.x;
+  int q = 0;
+  while (q + 1 < g.ng && b >= g.blk[q + 1]) ++q;
+  const int64_t e = (int64_t)(b - g.blk[q]) * blockDim.x + threadIdx.x;
+  if (e >= g.n[q]) return;
+  ade_cell<real, true>(a, g.co[q], g.cells[q], g.masks[q], g.state[q], g.n[q], src, nsrc, e);
 }
 
 // ---------------------------------------------------------------------------
@@ -2141,17 +2172,35 @@ int step_impl(Sim* s) {
       an.e[q] = (real*)s->E2[q];
       an.h[q] = (real*)s->H2[q];
     }
-    for (auto& g : s->groups) {
-      if (!g.ncells) continue;
-      Timer tm(s, FDTD_K_ADE);
+    {
       FieldArgs<real> ad = a;  // D form from the old H into the new E
       for (int q = 0; q < 3; ++q) ad.e[q] = an.e[q];
-      if (g.kind == 2)
-        hipLaunchKernelGGL((k_lord<real, true>), dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, ad,
-                           lord_coef(g), g.d_cells, g.d_mask, (double*)g.d_state, g.ncells, src, nsrc);
-      else
-        hipLaunchKernelGGL((k_ade<real, true>), dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, ad,
-                           ade_coef(g), g.d_cells, g.d_mask, (double*)g.d_state, g.ncells, src, nsrc);
+      AdeMulti gm{};
+      auto launch_multi = [&]() {
+        if (!gm.ng) return;
+        Timer tm(s, FDTD_K_ADE);
+        hipLaunchKernelGGL(k_ade_multi<real>, dim3((unsigned)gm.blk[gm.ng]), dim3(256), 0, st, ad, gm, src, nsrc);
+        gm = AdeMulti{};
+      };
+      for (auto& g : s->groups) {
+        if (!g.ncells) continue;
+        if (g.kind == 2) {
+          Timer tm(s, FDTD_K_ADE);
+          hipLaunchKernelGGL((k_lord<real, true>), dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, ad,
+                             lord_coef(g), g.d_cells, g.d_mask, (double*)g.d_state, g.ncells, src, nsrc);
+          continue;
+        }
+        const int q = gm.ng;
+        gm.co[q] = ade_coef(g);
+        gm.cells[q] = g.d_cells;
+        gm.masks[q] = g.d_mask;
+        gm.state[q] = (double*)g.d_state;
+        gm.n[q] = g.ncells;
+        gm.blk[q + 1] = gm.blk[q] + (int32_t)((g.ncells + 255) / 256);
+        gm.ng = q + 1;
+        if (gm.ng == kAdeMulti) launch_multi();
+      }
+      launch_multi();
     }
     {
       int rc = build_march_tiles(s, s->op_by, s->op_kc);
