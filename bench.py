@@ -48,7 +48,7 @@ def run_gpu(spec, precision, steps, warmup):
     nx, ny, nz = sim.shape
     cells = nx * ny * nz
     kern = {}
-    for k in (abi.K_E, abi.K_H, abi.K_ADE, abi.K_MON, abi.K_GEN_E, abi.K_GEN_H):
+    for k in (abi.K_E, abi.K_H, abi.K_ADE, abi.K_MON, abi.K_GEN_E, abi.K_GEN_H, abi.K_XF):
         ms, n = sim.kernel_stats(k)
         if n:
             kern[abi.KERNEL_NAMES[k]] = {"mean_ms": ms / n, "launches": n,

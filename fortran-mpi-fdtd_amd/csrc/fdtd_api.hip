@@ -944,7 +944,7 @@ void* field_ptr(Sim* s, int comp) {
 // Tile lists of the one-pass step for tiles of 64 x by cells (x from 1 in
 // steps of 64, y from 1 in steps of by) and z ranges of at most kc planes;
 // 8 words per tile: i0, j0, k0, k1, xlo, xhi, then (plain tiles) the
-// material id and 0.
+// material id and 1 for an x-face tile (0 otherwise).
 //
 // Plain tiles cover x from the first 128-byte line after the low x PML
 // slab to the last line before the cell next to the high slab, in steps of
@@ -961,7 +961,14 @@ void* field_ptr(Sim* s, int comp) {
 // periodic axis -- is a non-PML cell of one material whose three components
 // are dielectric, and no x- or y-normal source plane crosses
 // them (z-normal ones are added per plane); the z ranges end one plane
-// before the high z slab, so no such cell is a PML cell there.  General
+// before the high z slab, so no such cell is a PML cell there.  FP32 with
+// ALL_FACES PML on a non-periodic x axis and no TFSF line (FDTD_XFACE=0
+// turns it off): plain tiles cover every x from 1, and those whose lanes
+// reach into an x slab (x-face tiles, bx rows each, listed after the
+// others) update the x-slab cells with the split-field rule of an x face
+// cell -- their rows and planes, the row and plane after included, lie
+// between the y and z slabs, the cell after xhi is no x-slab cell, and no
+// z-normal source plane crosses them.  General
 // work goes to the list as workgroups over at most kGenKc planes of a box
 // (FDTD_GEN_KC overrides): 8 words each, x0, x1, j0, j1, k0, the
 // workgroup's width (64, 32 or 16), k1, and the face kind (face_of).
@@ -1012,12 +1019,18 @@ int build_march_tiles(Sim* s, int by, int kc) {
   auto wrap = [](int v, int n, int per) { return v <= n ? v : (v == n + 1 && per ? 1 : 0); };
   // plain tiles start at the first 128-byte line after the low x slab and
   // end with the last line before the cell next to the high slab (i = 1
-  // starts a line), so the general x edges are whole lines
+  // starts a line), so the general x edges are whole lines -- or, in FP32
+  // (x-face tiles), they cover every x from 1, and the lanes of a tile that
+  // lie in an x slab take the split update of an x face cell
   const int le = (int)(128 / s->wsize);
-  const int xs = Px > 0 ? 1 + le * ((Px + le - 1) / le) : 1;
-  const int xe = Px > 0 ? le * ((nx - Px - 1) / le) : nx;
-  const int ntx = xe >= xs ? (xe - xs + 64) / 64 : 0, nty = (ny + by - 1) / by;
-  std::vector<int32_t> plain, gen;
+  const char* xf_env = std::getenv("FDTD_XFACE");
+  const bool xface = af && Px > 0 && !pers[0] && s->wsize == 4 && s->tfsfs.empty() &&
+                     !(xf_env && std::atoi(xf_env) == 0);
+  const int xs = xface ? 1 : (Px > 0 ? 1 + le * ((Px + le - 1) / le) : 1);
+  const int xe = xface ? nx : (Px > 0 ? le * ((nx - Px - 1) / le) : nx);
+  auto pml_x = [&](int i) { return i <= Px || i > nx - Px; };
+  const int ntx = xe >= xs ? (xe - xs + 64) / 64 : 0;
+  std::vector<int32_t> plain, gen, xfl;
   struct GBox {
     int x0, x1, j0, j1, k0, k1;
     int face;  // 1, 2: every cell a face cell of the y or z slabs (see face_of)
@@ -1058,13 +1071,31 @@ int build_march_tiles(Sim* s, int by, int kc) {
     return face;
   };
   std::vector<GBox> gbox;
-  int64_t nplain = 0;
+  int64_t nplain = 0, nxfi = 0, nxf = 0;
+  int ntxf = 0;
   std::vector<int> ys, zs;
+  // pass 0: the plain tiles of by rows; pass 1 (x-face tiles only): the
+  // tiles with lanes in an x slab, bx rows each (FDTD_XFACE_BY, 2 or 4),
+  // listed after the others and marched by their own launch
+  const char* xb_env = std::getenv("FDTD_XFACE_BY");
+  const int bx = xb_env && std::atoi(xb_env) == 4 ? 4 : 2;
+  for (int pass = 0; pass < (xface ? 2 : 1); ++pass) {
+  const int bt = pass ? bx : by;
+  const int ntyp = (ny + bt - 1) / bt;
   for (const auto& z : zr)
-    for (int ty = 0; ty < nty; ++ty)
+    for (int ty = 0; ty < ntyp; ++ty)
       for (int tx = -1; tx <= ntx; ++tx) {
-        const int j0 = 1 + by * ty, k0 = z.first, k1 = z.second;
-        const int jend = std::min(j0 + by - 1, ny);
+        const int j0 = 1 + bt * ty, k0 = z.first, k1 = z.second;
+        const int jend = std::min(j0 + bt - 1, ny);
+        if (pass == 0 && xface && tx >= 0 && tx < ntx) {  // x-face columns: pass 1
+          const int a = xs + 64 * tx, b = std::min(a + 63, xe);
+          if (pml_x(a) || pml_x(b)) continue;
+        }
+        if (pass == 1) {
+          if (tx < 0 || tx == ntx) continue;
+          const int a = xs + 64 * tx, b = std::min(a + 63, xe);
+          if (!(pml_x(a) || pml_x(b))) continue;
+        }
         if (tx < 0 || tx == ntx) {  // the x edges: general
           const int a = tx < 0 ? 1 : std::max(xe + 1, xs), b = tx < 0 ? std::min(xs - 1, nx) : nx;
           if (a <= b) gbox.push_back({a, b, j0, jend, k0, k1, 0});
@@ -1078,9 +1109,9 @@ int build_march_tiles(Sim* s, int by, int kc) {
         // after (wrapped on a periodic axis); none may be a PML row or plane
         ys.clear();
         zs.clear();
-        for (int j = j0; j <= std::min(j0 + by - 1, ny); ++j) ys.push_back(j);
-        if (j0 + by - 1 <= ny) {
-          if (const int jh = wrap(j0 + by, ny, pers[1])) ys.push_back(jh);
+        for (int j = j0; j <= std::min(j0 + bt - 1, ny); ++j) ys.push_back(j);
+        if (j0 + bt - 1 <= ny) {
+          if (const int jh = wrap(j0 + bt, ny, pers[1])) ys.push_back(jh);
         } else if (pers[1]) {
           ys.push_back(1);  // the wrapped row after ny inside the tile
         }
@@ -1088,6 +1119,15 @@ int build_march_tiles(Sim* s, int by, int kc) {
         if (const int kh = wrap(k1 + 1, nz, pers[2])) zs.push_back(kh);
         for (int j : ys) ok = ok && !pml_y(j);
         for (int k : zs) ok = ok && !pml_z(k);
+        // the cell after xhi (whose new E the tile computes) is no x-slab
+        // cell; a tile with lanes in an x slab (x face) takes no z-normal
+        // source plane either (its split update has no source term)
+        const bool xft = xface && (pml_x(xlo) || pml_x(xhi));
+        if (xhi + 1 <= nx && pml_x(xhi + 1)) ok = false;
+        if (xft)
+          for (const auto& src : s->srcs)
+            if (src.axis == 2)
+              for (int k : zs) ok = ok && k != src.index;
         // one plain material over the cells computed: [xlo, xhi] and the
         // cell after xhi (cell 1 past nx on a periodic x axis) of every row
         int mid = -1;
@@ -1112,13 +1152,22 @@ int build_march_tiles(Sim* s, int by, int kc) {
           }
         }
         if (ok) {
-          const int32_t e[8] = {i0, j0, k0, k1, xlo, xhi, mid, 0};
-          plain.insert(plain.end(), e, e + 8);
-          nplain += (int64_t)(xhi - xlo + 1) * (jend - j0 + 1) * (k1 - k0 + 1);
+          const int32_t e[8] = {i0, j0, k0, k1, xlo, xhi, mid, xft ? 1 : 0};
+          std::vector<int32_t>& to = xft ? xfl : plain;
+          to.insert(to.end(), e, e + 8);
+          const int64_t rk = (int64_t)(jend - j0 + 1) * (k1 - k0 + 1);
+          // x-slab cells of the owned range: [xlo, Px] and [nx-Px+1, xhi]
+          const int64_t nsl = xft ? (int64_t)std::max(0, std::min(xhi, Px) - xlo + 1) +
+                                        std::max(0, xhi - std::max(xlo, nx - Px + 1) + 1)
+                                  : 0;
+          (xft ? nxfi : nplain) += (xhi - xlo + 1 - nsl) * rk;
+          nxf += nsl * rk;
+          ntxf += xft ? 1 : 0;
         } else {
           gbox.push_back({xlo, xhi, j0, jend, k0, k1, face_of(xlo, xhi, j0, jend, k0, k1)});
         }
       }
+  }
   // general boxes: rows of the same x range and z range that follow each
   // other merge; a box is cut into workgroups of 256 threads, 64 x 4 (wide
   // boxes), 32 x 8 or 16 x 16 cells (the x edges), each over a chunk of the
@@ -1158,13 +1207,18 @@ int build_march_tiles(Sim* s, int by, int kc) {
   if (s->d_tl_plain) dev_free(s, s->d_tl_plain);
   if (s->d_tl_gen) dev_free(s, s->d_tl_gen);
   s->d_tl_plain = s->d_tl_gen = nullptr;
-  s->n_tl_plain = (int)(plain.size() / 8);
+  s->n_tl_plain = (int)(plain.size() / 8);  // the x-face tiles (n_tl_xf) follow in the same list
+  plain.insert(plain.end(), xfl.begin(), xfl.end());
   s->n_tl_gen = (int)(gen.size() / 8);
-  if (s->n_tl_plain) s->d_tl_plain = (int32_t*)upload_bytes(s, plain.data(), plain.size() * sizeof(int32_t));
+  if (!plain.empty()) s->d_tl_plain = (int32_t*)upload_bytes(s, plain.data(), plain.size() * sizeof(int32_t));
   if (s->n_tl_gen) s->d_tl_gen = (int32_t*)upload_bytes(s, gen.data(), gen.size() * sizeof(int32_t));
-  if ((s->n_tl_plain && !s->d_tl_plain) || (s->n_tl_gen && !s->d_tl_gen))
+  if ((!plain.empty() && !s->d_tl_plain) || (s->n_tl_gen && !s->d_tl_gen))
     return set_error(FDTD_ERR_NOMEM, "one-pass tile lists");
   s->n_plain_cells = nplain;
+  s->n_xfi_cells = nxfi;
+  s->n_xf_cells = nxf;
+  s->n_tl_xf = ntxf;
+  s->tl_bx = bx;
   s->tl_by = by;
   s->tl_kc = kc;
   return 0;
@@ -2022,9 +2076,14 @@ int fdtd_algorithmic_bytes(fdtd_sim* s, int kernel, double* bytes) {
       // plain tiles: a whole step of their (non-PML) cells per launch, 18
       // words per cell (the two half steps' 9 each); general tiles: each
       // half step as above.  The tile lists are those of the last step.
-      const double Np = (double)s->n_plain_cells;
+      // x-face tiles: 18 words a step for their non-PML cells, 26 (13 a
+      // half step) for their x-slab face cells, which leave the general
+      // kernels' share
+      const double Np = (double)s->n_plain_cells, Ni = (double)s->n_xfi_cells, Nx = (double)s->n_xf_cells;
       if (kernel == FDTD_K_E) b = w * 18.0 * Np;
-      if (kernel == FDTD_K_GEN_E || kernel == FDTD_K_GEN_H) b = w * (9.0 * (N - Np - s->n_pml) + pml_words);
+      if (kernel == FDTD_K_XF) b = w * (18.0 * Ni + 26.0 * Nx);
+      if (kernel == FDTD_K_GEN_E || kernel == FDTD_K_GEN_H)
+        b = w * (9.0 * (N - Np - Ni - s->n_pml) + pml_words - 13.0 * Nx);
     } else if (kernel == FDTD_K_E || kernel == FDTD_K_H) {
       b = w * (9.0 * (N - s->n_pml) + pml_words);
     }

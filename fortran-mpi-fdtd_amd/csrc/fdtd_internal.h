@@ -234,10 +234,14 @@ struct Sim {
   int op_by = 4, op_kc = 64;
   int tl_by = -1, tl_kc = -1;        // tile lists below built for these
   int tl_gkc = 1;                    // planes per general workgroup in them
-  int32_t* d_tl_plain = nullptr;     // i0, j0, k0, k1, xlo, xhi, material id, 0
+  int32_t* d_tl_plain = nullptr;     // i0, j0, k0, k1, xlo, xhi, material id, x face (0, 1)
   int32_t* d_tl_gen = nullptr;       // workgroups: x0, x1, j0, j1, k0, width (64, 32 or 16), k1, face (0, 1: y, 2: z)
   int n_tl_plain = 0, n_tl_gen = 0;
-  int64_t n_plain_cells = 0;         // cells owned by plain tiles
+  int n_tl_xf = 0;                   // plain tiles that reach into an x slab (x-face tiles), after the others
+  int tl_bx = 2;                     // rows of an x-face tile
+  int64_t n_plain_cells = 0;         // cells owned by plain tiles (x-face tiles not counted)
+  int64_t n_xfi_cells = 0;           // non-PML cells owned by x-face tiles
+  int64_t n_xf_cells = 0;            // x-slab face cells owned by x-face tiles
   // material runs of every (j,k) row r = j + (ny+2)*k: runs h_runoff[r] ..
   // h_runoff[r+1]-1, run q starting at cell h_runx[q] with id h_runid[q]
   std::vector<int32_t> h_runoff, h_runx;

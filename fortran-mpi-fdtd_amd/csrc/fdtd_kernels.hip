@@ -123,6 +123,36 @@ __device__ __forceinline__ void pml_parts(bool sx, bool sy, bool sz, const real*
   o[5] = sz ? sp[5] : (sx ? f[1] - sp[0] : f[1]);     // yz
 }
 
+// The six split updates of an ALL_FACES PML cell (mod3/BERpmly_mod.f90:150-155
+// for E, :186-191 for H) from the parts o (pml_parts order) and the
+// neighbouring fields: f1/g1 and f2/g2 are the cell's profiles along each
+// axis (1 and Ku outside that axis's slabs), r the medium's 1/eps.  Every
+// update of a PML cell, whichever kernel makes it, goes through these, so
+// all of them round alike.
+template <typename real>
+__device__ __forceinline__ void e_pml_split(real fx1, real fx2, real fy1, real fy2, real fz1, real fz2, real r,
+                                            const real* o, real hx, real hy, real hz, real hz_jm, real hx_jm,
+                                            real hy_km, real hx_km, real hz_im, real hy_im, real* n) {
+  n[0] = fx1 * o[0] + (fx2 * r) * (-hz + hz_im);    // yx
+  n[1] = fx1 * o[1] + (fx2 * r) * (hy + (-hy_im));  // zx
+  n[2] = fy1 * o[2] + (fy2 * r) * (hz + (-hz_jm));  // xy
+  n[3] = fy1 * o[3] + (fy2 * r) * (-hx + hx_jm);    // zy
+  n[4] = fz1 * o[4] + (fz2 * r) * (-hy + hy_km);    // xz
+  n[5] = fz1 * o[5] + (fz2 * r) * (hx + (-hx_km));  // yz
+}
+
+template <typename real>
+__device__ __forceinline__ void h_pml_split(real gx1, real gx2, real gy1, real gy2, real gz1, real gz2, const real* o,
+                                            real ex, real ey, real ez, real ey_kp, real ex_kp, real ez_jp, real ex_jp,
+                                            real ez_ip, real ey_ip, real* n) {
+  n[0] = gx1 * o[0] + gx2 * (-ez + ez_ip);    // yx
+  n[1] = gx1 * o[1] + gx2 * (ey + (-ey_ip));  // zx
+  n[2] = gy1 * o[2] + gy2 * (ez + (-ez_jp));  // xy
+  n[3] = gy1 * o[3] + gy2 * (-ex + ex_jp);    // zy
+  n[4] = gz1 * o[4] + gz2 * (-ey + ey_kp);    // xz
+  n[5] = gz1 * o[5] + gz2 * (ex + (-ex_kp));  // yz
+}
+
 // Compact index of an ALL_FACES PML cell: z slabs (full xy planes), then y
 // slabs of the middle planes (full x rows), then the low and then the high
 // x slab of the rest, each of those row after row (Px cells a row), so the
@@ -250,12 +280,10 @@ __device__ __forceinline__ void e_cell_store(const FieldArgs<real>& a, const Pml
     const int s = in.s;
     real o[6];
     pml_parts(in.sx, in.sy, in.sz, in.sp, e0, o);
-    real eyx = in.fx1 * o[0] + (in.fx2 * r) * (-hz + hz_im);
-    real ezx = in.fx1 * o[1] + (in.fx2 * r) * (hy + (-hy_im));
-    real exy = in.fy1 * o[2] + (in.fy2 * r) * (hz + (-hz_jm));
-    real ezy = in.fy1 * o[3] + (in.fy2 * r) * (-hx + hx_jm);
-    real exz = in.fz1 * o[4] + (in.fz2 * r) * (-hy + hy_km);
-    real eyz = in.fz1 * o[5] + (in.fz2 * r) * (hx + (-hx_km));
+    real n[6];
+    e_pml_split(in.fx1, in.fx2, in.fy1, in.fy2, in.fz1, in.fz2, r, o, hx, hy, hz, hz_jm, hx_jm, hy_km, hx_km, hz_im,
+                hy_im, n);
+    real eyx = n[0], ezx = n[1], exy = n[2], ezy = n[3], exz = n[4], eyz = n[5];
     // a source plane through the PML feeds the split along its normal
     if (sa[0] == 1) exy = exy + sv[0] * r;
     if (sa[0] == 2) exz = exz + sv[0] * r;
@@ -511,12 +539,10 @@ __device__ __forceinline__ void h_cell_store(const FieldArgs<real>& a, const Pml
     const int s = in.s;
     real o[6];
     pml_parts(in.sx, in.sy, in.sz, in.sp, in.h0, o);
-    real hyx = in.gx1 * o[0] + in.gx2 * (-ez + ez_ip);
-    real hzx = in.gx1 * o[1] + in.gx2 * (ey + (-ey_ip));
-    real hxy = in.gy1 * o[2] + in.gy2 * (ez + (-ez_jp));
-    real hzy = in.gy1 * o[3] + in.gy2 * (-ex + ex_jp);
-    real hxz = in.gz1 * o[4] + in.gz2 * (-ey + ey_kp);
-    real hyz = in.gz1 * o[5] + in.gz2 * (ex + (-ex_kp));
+    real n[6];
+    h_pml_split(in.gx1, in.gx2, in.gy1, in.gy2, in.gz1, in.gz2, o, ex, ey, ez, ey_kp, ex_kp, ez_jp, ex_jp, ez_ip, ey_ip,
+                n);
+    const real hyx = n[0], hzx = n[1], hxy = n[2], hzy = n[3], hxz = n[4], hyz = n[5];
     if (in.sx) { pw.s[S_HYX][s] = hyx; pw.s[S_HZX][s] = hzx; }
     if (in.sy) { pw.s[S_HXY][s] = hxy; pw.s[S_HZY][s] = hzy; }
     if (in.sz) { pw.s[S_HXZ][s] = hxz; pw.s[S_HYZ][s] = hyz; }
@@ -602,6 +628,13 @@ __global__ __launch_bounds__(512) void k_fused_h(FieldArgs<real> a, PmlArgs<real
 //  it computes (its own, the row, plane and column after them) is a
 //  non-PML cell of its material.
 //
+//  x-face tiles (FP32): plain tiles whose lanes reach into an x PML slab;
+//  those lanes keep the x slab's parts (yx, zx) of E and H and take the
+//  split-field update of a face cell (e_pml_split / h_pml_split, as the
+//  general kernels do), the other lanes the plain one.  They run as a
+//  launch of their own (k_march_plain<real, R, true>), so the plain tiles'
+//  kernel keeps its registers.
+//
 //  general tiles (PML, dispersive or mixed materials, other sources, and
 //  the x-slab parts of plain tiles) go to k_gen_e and then k_gen_h, the
 //  fused cell updates of k_fused_e / k_fused_h over the two copies.
@@ -614,7 +647,8 @@ struct MarchArgs {
   const real* eo[3];      // E after the previous step
   const real* ho[3];      // H after the previous step
   // 8 words per entry; plain tiles: i0, j0, k0, k1, xlo, xhi (owned x
-  // range), material id; general workgroups: x0, x1, j0, j1, k0, width, k1, face
+  // range), material id, x face; general workgroups: x0, x1, j0, j1, k0,
+  // width, k1, face
   const int32_t* tiles;
   int32_t ntiles, tper;
   int32_t nsplit;  // general workgroups: workgroups per entry (its planes, k_gen_e) or 1
@@ -683,16 +717,55 @@ struct PlainPlane {
   real yh_ex, yh_ez, yh_hx, yh_hy, yh_hz;  // row j0+R
   real xh_ey, xh_ez, xh_hx, xh_hy, xh_hz;  // column i0+64, lane q: row j0-1+q
   real xl_hz, xl_hy;                       // column i0-1, lane q: tile row q (q = R: row j0+R)
+  // x-slab lanes of x-face tiles: the kept parts (yx, zx) of the old E and H
+  real pe_yx[R], pe_zx[R], ph_yx[R], ph_zx[R];
+  real yh_pe_zx;  // row j0+R: old Ezx
 };
+
+// x-slab face cells in the march (tiles at the ends of a row of tiles whose
+// lanes reach into an x slab, list word 7 set): the x slabs' kept parts yx,
+// zx over the two copies, the x profiles, and the compact index of the x
+// slabs (pml_index: base, then Px cells a row over the rows and planes
+// between the y and z slabs, the high slab hioff after the low one).
+template <typename real>
+struct XfArgs {
+  const real* eo[2];  // old Eyx, Ezx
+  const real* ho[2];  // old Hyx, Hzx
+  real* en[2];        // new Eyx, Ezx
+  real* hn[2];        // new Hyx, Hzx
+  const real *f1, *f2, *g1, *g2;
+  real kup;
+  int32_t Px, Py, Pz, nym, nzm, base, hioff;
+};
+
+// a lane of an x-face tile: in an x slab (on), where its row of parts
+// starts, its x profiles
+template <typename real>
+struct XfLane {
+  bool on;
+  int sb;
+  real f1, f2, g1, g2;
+};
+
+// offset of row j, plane k in an x slab's parts (clamped to the slab's rows
+// and planes: rows and planes a tile only loads for, never stores, stay in
+// bounds)
+template <typename real>
+__device__ __forceinline__ int xf_row(const XfArgs<real>& x, int j, int k) {
+  const int km = min(max(k - x.Pz - 1, 0), x.nzm - 1);
+  const int jm = min(max(j - x.Py - 1, 0), x.nym - 1);
+  return (km * x.nym + jm) * x.Px;
+}
 
 // Every address below lies inside the ghosted box (rows, columns and planes
 // are mapped to 0..n+1 before use), so the loads are unconditional.  A value
 // loaded for a dead lane, row or plane is read only by that position's own
 // new E, which is zeroed; nothing consumes the loaded registers early, so
 // the loads stay in flight until the plane that uses them.
-template <typename real, int R>
-__device__ __forceinline__ void plain_load(const FieldArgs<real>& a, const MarchArgs<real>& m, const MarchTile<R>& w,
-                                           int k, PlainPlane<real, R>& P) {
+template <typename real, int R, bool XF>
+__device__ __forceinline__ void plain_load(const FieldArgs<real>& a, const MarchArgs<real>& m, const XfArgs<real>& x,
+                                           const XfLane<real>& xl, const MarchTile<R>& w, int k,
+                                           PlainPlane<real, R>& P) {
   const int px = a.px;
   const int kb = a.pxy * k;
 #pragma unroll
@@ -731,6 +804,23 @@ __device__ __forceinline__ void plain_load(const FieldArgs<real>& a, const March
     P.xl_hz = m.ho[2][c];
     P.xl_hy = m.ho[1][c];
   }
+  if constexpr (XF) {
+    if (xl.on) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int sc = xl.sb + xf_row(x, w.jrow[r], k);
+        P.pe_yx[r] = x.eo[0][sc];
+        P.pe_zx[r] = x.eo[1][sc];
+        P.ph_yx[r] = x.ho[0][sc];
+        P.ph_zx[r] = x.ho[1][sc];
+      }
+      P.yh_pe_zx = w.yh_need ? x.eo[1][xl.sb + xf_row(x, w.jyh, k)] : (real)0;
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) P.pe_yx[r] = P.pe_zx[r] = P.ph_yx[r] = P.ph_zx[r] = 0;
+      P.yh_pe_zx = 0;
+    }
+  }
 }
 
 // registers carried from plane k-1 to plane k
@@ -740,6 +830,7 @@ struct PlainCarry {
   real ex[R], ey[R], ez[R];  // new E of plane k-1
   real yh_hy, yh_ex, yh_ez;  // row j0+R: old Hy, new Ex, Ez of plane k-1
   real xh_hx, xh_ey, xh_ez;  // column i0+64 (lane q: row j0-1+q): old Hx, new Ey, Ez of plane k-1
+  real ph_yx[R], ph_zx[R];   // x-slab lanes: old H parts of plane k-1
 };
 
 // material coefficients of a plain tile (wave-uniform) and this plane's
@@ -748,6 +839,7 @@ template <typename real>
 struct PlainCoef {
   real c[3];   // Ku*gax
   real g[3];   // gax (source scaling)
+  real r;      // 1/eps (the PML update's medium)
   real sv[3];  // sources of this plane per component
   bool sa[3];
 };
@@ -759,15 +851,16 @@ __device__ __forceinline__ real plain_e(const PlainCoef<real>& q, int c, real e0
   return ev;
 }
 
-template <typename real, int R>
+template <typename real, int R, bool XF>
 __device__ __forceinline__ void plain_plane(const FieldArgs<real>& a, const MarchArgs<real>& m,
+                                            const XfArgs<real>& x, const XfLane<real>& xl,
                                             const SrcArgs<real>& src, int nsrc, PlainCoef<real> q,
                                             const MarchTile<R>& w, int lane, int k,
                                             const PlainPlane<real, R>& cur, PlainPlane<real, R>& nxt,
                                             PlainCarry<real, R>& c) {
   if (k + 1 <= w.k1 + 1) {
     bool lv;
-    plain_load(a, m, w, march_map(k + 1, a.nz, a.per[2], lv), nxt);
+    plain_load<real, R, XF>(a, m, x, xl, w, march_map(k + 1, a.nz, a.per[2], lv), nxt);
   }
   bool live_k;
   const int kk = march_map(k, a.nz, a.per[2], live_k);
@@ -785,6 +878,7 @@ __device__ __forceinline__ void plain_plane(const FieldArgs<real>& a, const Marc
     }
   // new E of plane k on the tile's rows
   real exn[R], eyn[R], ezn[R];
+  real pyx[R], pzx[R];  // x-slab lanes: the new E parts yx, zx
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const real hz_jm = r ? cur.hz[r - 1] : cur.yl_hz;
@@ -798,9 +892,25 @@ __device__ __forceinline__ void plain_plane(const FieldArgs<real>& a, const Marc
     }
     const bool live = live_k && w.lane_live && w.row_live[r];
     const real hx = cur.hx[r], hy = cur.hy[r], hz = cur.hz[r];
-    const real ex = plain_e(q, 0, cur.ex[r], (hz - hz_jm) + (-hy + c.hy[r]));  // curlhzy + curlhyz
-    const real ey = plain_e(q, 1, cur.ey[r], (hx - c.hx[r]) + (-hz + hz_im));  // curlhxz + curlhzx
-    const real ez = plain_e(q, 2, cur.ez[r], (hy - hy_im) + (-hx + hx_jm));    // curlhyx + curlhxy
+    real ex = plain_e(q, 0, cur.ex[r], (hz - hz_jm) + (-hy + c.hy[r]));  // curlhzy + curlhyz
+    real ey = plain_e(q, 1, cur.ey[r], (hx - c.hx[r]) + (-hz + hz_im));  // curlhxz + curlhzx
+    real ez = plain_e(q, 2, cur.ez[r], (hy - hy_im) + (-hx + hx_jm));    // curlhyx + curlhxy
+    pyx[r] = pzx[r] = 0;
+    if constexpr (XF) {
+      if (xl.on) {  // a face cell of an x slab: the split update, its x parts kept
+        const real f[3] = {cur.ex[r], cur.ey[r], cur.ez[r]};
+        const real sp[6] = {cur.pe_yx[r], cur.pe_zx[r], 0, 0, 0, 0};
+        real o[6], n[6];
+        pml_parts(true, false, false, sp, f, o);
+        e_pml_split(xl.f1, xl.f2, (real)1, x.kup, (real)1, x.kup, q.r, o, hx, hy, hz, hz_jm, hx_jm, c.hy[r], c.hx[r],
+                    hz_im, hy_im, n);
+        ex = n[2] + n[4];
+        ey = n[5] + n[0];
+        ez = n[1] + n[3];
+        pyx[r] = n[0];
+        pzx[r] = n[1];
+      }
+    }
     exn[r] = live ? ex : (real)0;
     eyn[r] = live ? ey : (real)0;
     ezn[r] = live ? ez : (real)0;
@@ -812,8 +922,20 @@ __device__ __forceinline__ void plain_plane(const FieldArgs<real>& a, const Marc
     const real ly = lane_at(cur.xl_hy, R);
     if (lane == 0) hy_im = ly;
     const bool live = live_k && w.lane_live && w.yh_live;
-    const real ex = plain_e(q, 0, cur.yh_ex, (cur.yh_hz - cur.hz[R - 1]) + (-cur.yh_hy + c.yh_hy));
-    const real ez = plain_e(q, 2, cur.yh_ez, (cur.yh_hy - hy_im) + (-cur.yh_hx + cur.hx[R - 1]));
+    real ex = plain_e(q, 0, cur.yh_ex, (cur.yh_hz - cur.hz[R - 1]) + (-cur.yh_hy + c.yh_hy));
+    real ez = plain_e(q, 2, cur.yh_ez, (cur.yh_hy - hy_im) + (-cur.yh_hx + cur.hx[R - 1]));
+    if constexpr (XF) {
+      if (xl.on) {  // Ex = Exy + Exz, Ez = Ezx + Ezy (the Ey inputs are not needed)
+        const real f[3] = {cur.yh_ex, (real)0, cur.yh_ez};
+        const real sp[6] = {(real)0, cur.yh_pe_zx, 0, 0, 0, 0};
+        real o[6], n[6];
+        pml_parts(true, false, false, sp, f, o);
+        e_pml_split(xl.f1, xl.f2, (real)1, x.kup, (real)1, x.kup, q.r, o, cur.yh_hx, cur.yh_hy, cur.yh_hz,
+                    cur.hz[R - 1], cur.hx[R - 1], c.yh_hy, (real)0, (real)0, hy_im, n);
+        ex = n[2] + n[4];
+        ez = n[1] + n[3];
+      }
+    }
     yex = live ? ex : (real)0;
     yez = live ? ez : (real)0;
   }
@@ -848,6 +970,13 @@ __device__ __forceinline__ void plain_plane(const FieldArgs<real>& a, const Marc
       put_e_ghost(a.e[0], exn[r], w.ip, w.jrow[r], kk, cc, a);
       put_e_ghost(a.e[1], eyn[r], w.ip, w.jrow[r], kk, cc, a);
       put_e_ghost(a.e[2], ezn[r], w.ip, w.jrow[r], kk, cc, a);
+      if constexpr (XF) {
+        if (xl.on) {
+          const int sc = xl.sb + xf_row(x, w.jrow[r], kk);
+          x.en[0][sc] = pyx[r];
+          x.en[1][sc] = pzx[r];
+        }
+      }
     }
   }
   // new H of plane k-1 (myfdtd_func.f90:45-64)
@@ -864,10 +993,26 @@ __device__ __forceinline__ void plain_plane(const FieldArgs<real>& a, const Marc
         ey_ip = xy;
       }
       if (!(w.lane_own && w.row_own[r])) continue;
-      const real hx = c.hx[r] + a.ku * ((eyn[r] - c.ey[r]) + (-ez_jp + c.ez[r]));
-      const real hy = c.hy[r] + a.ku * ((ez_ip - c.ez[r]) + (-exn[r] + c.ex[r]));
-      const real hz = c.hz[r] + a.ku * ((ex_jp - c.ex[r]) + (-ey_ip + c.ey[r]));
+      real hx = c.hx[r] + a.ku * ((eyn[r] - c.ey[r]) + (-ez_jp + c.ez[r]));
+      real hy = c.hy[r] + a.ku * ((ez_ip - c.ez[r]) + (-exn[r] + c.ex[r]));
+      real hz = c.hz[r] + a.ku * ((ex_jp - c.ex[r]) + (-ey_ip + c.ey[r]));
       const int kh = k - 1;
+      if constexpr (XF) {
+        if (xl.on) {
+          const real f[3] = {c.hx[r], c.hy[r], c.hz[r]};
+          const real sp[6] = {c.ph_yx[r], c.ph_zx[r], 0, 0, 0, 0};
+          real o[6], n[6];
+          pml_parts(true, false, false, sp, f, o);
+          h_pml_split(xl.g1, xl.g2, (real)1, x.kup, (real)1, x.kup, o, c.ex[r], c.ey[r], c.ez[r], eyn[r], exn[r],
+                      ez_jp, ex_jp, ez_ip, ey_ip, n);
+          hx = n[2] + n[4];
+          hy = n[5] + n[0];
+          hz = n[1] + n[3];
+          const int sc = xl.sb + xf_row(x, w.jrow[r], kh);
+          x.hn[0][sc] = n[0];
+          x.hn[1][sc] = n[1];
+        }
+      }
       const int cc = w.ip + a.px * w.jrow[r] + a.pxy * kh;
       a.h[0][cc] = hx;
       a.h[1][cc] = hy;
@@ -885,6 +1030,10 @@ __device__ __forceinline__ void plain_plane(const FieldArgs<real>& a, const Marc
     c.ex[r] = exn[r];
     c.ey[r] = eyn[r];
     c.ez[r] = ezn[r];
+    if constexpr (XF) {
+      c.ph_yx[r] = cur.ph_yx[r];
+      c.ph_zx[r] = cur.ph_zx[r];
+    }
   }
   c.yh_hy = cur.yh_hy;
   c.yh_ex = yex;
@@ -904,9 +1053,9 @@ __device__ __forceinline__ void plain_plane(const FieldArgs<real>& a, const Marc
 // j0+R (its new Ex, Ez), column i0-1 (old Hz, Hy: lanes 0..R gather one
 // row each), column i0+64 (its new Ey, Ez: lanes 1..R), plane k1+1 (its new
 // E).  No LDS and no barrier.
-template <typename real, int R>
+template <typename real, int R, bool XF>
 __global__ __launch_bounds__(256) void k_march_plain(FieldArgs<real> a, MarchArgs<real> m, SrcArgs<real> src,
-                                                     int nsrc) {
+                                                     int nsrc, XfArgs<real> x) {
   const int lane = threadIdx.x & 63;
   const int b = blockIdx.x;
   const int lb = a.xcdmap ? (b & 7) * m.tper + (b >> 3) : b;
@@ -926,6 +1075,7 @@ __global__ __launch_bounds__(256) void k_march_plain(FieldArgs<real> a, MarchArg
       q.c[d] = cf.v[d];
       q.g[d] = gx.v[d];
     }
+    q.r = cf.v[3];
   }
   // lanes xlo..xhi are owned; lane xhi+1 supplies the new E after the
   // range (the next cell, or the wrapped cell 1)
@@ -970,12 +1120,29 @@ __global__ __launch_bounds__(256) void k_march_plain(FieldArgs<real> a, MarchArg
     c.yh_ex = c.yh_ez = 0;
     c.xh_hx = m.ho[0][kb + w.ixh + a.px * w.jxg];
     c.xh_ey = c.xh_ez = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) c.ph_yx[r] = c.ph_zx[r] = 0;
+  }
+  // x-face tiles (word 7): lanes in an x slab take the split update; their
+  // column of the x slab's parts and their x profiles are fixed per lane
+  XfLane<real> xl{};
+  if constexpr (XF) {
+    const int i = w.ip;
+    xl.on = T[7] != 0 && (i <= x.Px || i > a.nx - x.Px);
+    if (xl.on) {
+      const bool hi = i > a.nx - x.Px;
+      xl.sb = x.base + (hi ? x.hioff + (i - (a.nx - x.Px + 1)) : i - 1);
+      xl.f1 = x.f1[i];
+      xl.f2 = x.f2[i];
+      xl.g1 = x.g1[i];
+      xl.g2 = x.g2[i];
+    }
   }
   PlainPlane<real, R> A, B;
-  plain_load(a, m, w, w.k0, A);
+  plain_load<real, R, XF>(a, m, x, xl, w, w.k0, A);
   for (int k = w.k0; k <= w.k1 + 1; k += 2) {
-    plain_plane(a, m, src, nsrc, q, w, lane, k, A, B, c);
-    if (k + 1 <= w.k1 + 1) plain_plane(a, m, src, nsrc, q, w, lane, k + 1, B, A, c);
+    plain_plane<real, R, XF>(a, m, x, xl, src, nsrc, q, w, lane, k, A, B, c);
+    if (k + 1 <= w.k1 + 1) plain_plane<real, R, XF>(a, m, x, xl, src, nsrc, q, w, lane, k + 1, B, A, c);
   }
 }
 
@@ -2218,20 +2385,65 @@ int step_impl(Sim* s) {
     const int R = s->op_by;
     // monitor samples: the separate sample kernel (written from the march
     // instead, they cost the plain tiles more than the launches they save)
-    if (s->n_tl_plain) {
-      mg.tiles = s->d_tl_plain;
-      mg.ntiles = s->n_tl_plain;
-      const int blocks = (mg.ntiles + 3) / 4;  // four wavefronts per workgroup
-      mg.tper = (blocks + 7) / 8;
-      const dim3 grid((unsigned)(an.xcdmap ? 8 * mg.tper : blocks));
-      Timer tm(s, FDTD_K_E);
-      if (R == 2) {
-        hipLaunchKernelGGL((k_march_plain<real, 2>), grid, dim3(256), 0, st, an, mg, src, nsrc);
-      } else if (R == 4) {
-        hipLaunchKernelGGL((k_march_plain<real, 4>), grid, dim3(256), 0, st, an, mg, src, nsrc);
-      } else {
-        if constexpr (sizeof(real) == 4)  // FP64 tiles stop at 4 rows (8 would spill)
-          hipLaunchKernelGGL((k_march_plain<real, 8>), grid, dim3(256), 0, st, an, mg, src, nsrc);
+    if (s->n_tl_plain || s->n_tl_xf) {
+      // x-face tiles (FP32 only, build_march_tiles): the split parts of
+      // the x slabs, old (p) and new (pw), and the x profiles
+      XfArgs<real> xa{};
+      if (s->n_tl_xf) {
+        const int Px = p.P[0], Py = p.P[1], Pz = p.P[2];
+        xa.eo[0] = p.s[S_EYX];
+        xa.eo[1] = p.s[S_EZX];
+        xa.ho[0] = p.s[S_HYX];
+        xa.ho[1] = p.s[S_HZX];
+        xa.en[0] = pw.s[S_EYX];
+        xa.en[1] = pw.s[S_EZX];
+        xa.hn[0] = pw.s[S_HYX];
+        xa.hn[1] = pw.s[S_HZX];
+        xa.f1 = p.f1[0];
+        xa.f2 = p.f2[0];
+        xa.g1 = p.g1[0];
+        xa.g2 = p.g2[0];
+        xa.kup = p.kup;
+        xa.Px = Px;
+        xa.Py = Py;
+        xa.Pz = Pz;
+        xa.nym = s->ny - 2 * Py;
+        xa.nzm = s->nz - 2 * Pz;
+        xa.base = 2 * Pz * s->nx * s->ny + (s->nz - 2 * Pz) * 2 * Py * s->nx;
+        xa.hioff = (s->nz - 2 * Pz) * xa.nym * Px;
+      }
+      // the plain tiles, then the x-face tiles (their own launch, tiles of
+      // tl_bx rows, the kernel with the split update)
+      if (s->n_tl_plain) {
+        Timer tm(s, FDTD_K_E);
+        mg.tiles = s->d_tl_plain;
+        mg.ntiles = s->n_tl_plain;
+        const int blocks = (mg.ntiles + 3) / 4;  // four wavefronts per workgroup
+        mg.tper = (blocks + 7) / 8;
+        const dim3 grid((unsigned)(an.xcdmap ? 8 * mg.tper : blocks));
+        if (R == 2) {
+          hipLaunchKernelGGL((k_march_plain<real, 2, false>), grid, dim3(256), 0, st, an, mg, src, nsrc, xa);
+        } else if (R == 4) {
+          hipLaunchKernelGGL((k_march_plain<real, 4, false>), grid, dim3(256), 0, st, an, mg, src, nsrc, xa);
+        } else {
+          if constexpr (sizeof(real) == 4)  // FP64 tiles stop at 4 rows (8 would spill)
+            hipLaunchKernelGGL((k_march_plain<real, 8, false>), grid, dim3(256), 0, st, an, mg, src, nsrc, xa);
+        }
+      }
+      if constexpr (sizeof(real) == 4) {
+        if (s->n_tl_xf) {
+          Timer tm(s, FDTD_K_XF);
+          MarchArgs<real> mx = mg;
+          mx.tiles = s->d_tl_plain + 8 * (int64_t)s->n_tl_plain;
+          mx.ntiles = s->n_tl_xf;
+          const int blocks = (mx.ntiles + 3) / 4;
+          mx.tper = (blocks + 7) / 8;
+          const dim3 grid((unsigned)(an.xcdmap ? 8 * mx.tper : blocks));
+          if (s->tl_bx == 4)
+            hipLaunchKernelGGL((k_march_plain<real, 4, true>), grid, dim3(256), 0, st, an, mx, src, nsrc, xa);
+          else
+            hipLaunchKernelGGL((k_march_plain<real, 2, true>), grid, dim3(256), 0, st, an, mx, src, nsrc, xa);
+        }
       }
     }
     if (s->n_tl_gen) {

@@ -21,8 +21,8 @@ COMP_NAMES = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz", "Dx", "Dy", "Dz")
 REGION_BOX, REGION_SPHERE, REGION_CYLINDER = 0, 1, 2
 GRAPHENE_CP4, GRAPHENE_KUBO = 0, 1
 ENGINE_AUTO, ENGINE_STAGED, ENGINE_FUSED = 0, 1, 2
-K_E, K_H, K_ADE, K_PML_E, K_PML_H, K_MAT, K_MON, K_GEN_E, K_GEN_H, K_COUNT = range(10)
-KERNEL_NAMES = ("e_update", "h_update", "ade", "pml_e", "pml_h", "material", "monitor", "gen_e", "gen_h")
+K_E, K_H, K_ADE, K_PML_E, K_PML_H, K_MAT, K_MON, K_GEN_E, K_GEN_H, K_XF, K_COUNT = range(11)
+KERNEL_NAMES = ("e_update", "h_update", "ade", "pml_e", "pml_h", "material", "monitor", "gen_e", "gen_h", "x_face")
 
 
 class Grid(C.Structure):

@@ -99,7 +99,9 @@ enum {
   FDTD_K_MON = 6,     /* DFT monitors and probes                    */
   FDTD_K_GEN_E = 7,   /* one-pass step: E of its general tiles      */
   FDTD_K_GEN_H = 8,   /* one-pass step: H of its general tiles      */
-  FDTD_K_COUNT = 9
+  FDTD_K_XF = 9,      /* one-pass step (FP32): its plain tiles that reach
+                         into an x PML slab (x-face tiles)           */
+  FDTD_K_COUNT = 10
 };
 
 typedef struct fdtd_sim fdtd_sim;

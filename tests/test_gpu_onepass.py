@@ -105,6 +105,36 @@ def test_onepass_equals_two_kernel_step_fp32(case, by, kc, monkeypatch):
     _same(a, ia, pa, b, ib, pb)
 
 
+@pytest.mark.parametrize("case", ["gold_sphere", "x_edges", "graphene_disks"])
+@pytest.mark.parametrize("bx", ["2", "4"])
+def test_x_face_tiles_equal_general_x_edges_fp32(case, bx, monkeypatch):
+    """FP32 one-pass tiles whose lanes reach into an x PML slab (x-face
+    tiles, their own launch) update those lanes with the split-field rule
+    of the general kernels: the run equals the one with the x edges left to
+    the general kernels (FDTD_XFACE=0) bit for bit, and the x-face launch
+    did run (its timer counted launches, and its algorithmic bytes include
+    x-slab cells)."""
+    from fdtd_amd import abi
+
+    spec = _cases()[case]
+    env = {"FDTD_ONEPASS": "1", "FDTD_ONEPASS_BY": "4", "FDTD_ONEPASS_KC": "16", "FDTD_XFACE_BY": bx}
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    import fdtd_amd
+
+    b, ib, pb = fdtd_amd.make_simulation(spec, precision="fp32", engine="fused")
+    b.set_profiling(True)
+    b.step(spec["nsteps"])
+    assert b.kernel_stats(abi.K_XF)[1] > 0
+    assert b.algorithmic_bytes(abi.K_XF) > 0
+    monkeypatch.setenv("FDTD_XFACE", "0")
+    a, ia, pa = fdtd_amd.make_simulation(spec, precision="fp32", engine="fused")
+    a.set_profiling(True)
+    a.step(spec["nsteps"])
+    assert a.kernel_stats(abi.K_XF)[1] == 0
+    _same(a, ia, pa, b, ib, pb)
+
+
 def test_autotune_leaves_the_state_untouched(monkeypatch):
     """fdtd_autotune times the step forms with dry steps on the zero fields:
     the run that follows equals an untuned run bit for bit (every step form
