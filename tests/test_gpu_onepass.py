@@ -105,7 +105,22 @@ def test_onepass_equals_two_kernel_step_fp32(case, by, kc, monkeypatch):
     _same(a, ia, pa, b, ib, pb)
 
 
-@pytest.mark.parametrize("case", ["gold_sphere", "x_edges", "graphene_disks"])
+def _x_face_cases():
+    """Ragged grids for the x-face tiles: a tile holding both x slabs (37
+    cells), the second tile's lanes reaching into the high slab (100), the
+    first tile left general because the cell after it lies in the high slab
+    (70, periodic z without z slabs)."""
+    from test_gpu_edges import _ragged
+
+    r37 = _ragged((37, 29, 23), (4, 3, 5), (0, 0, 0), nsteps=150)
+    del r37["materials"][1]  # no dielectric block across the whole tile
+    return {"ragged_37": r37,
+            "ragged_100": _ragged((100, 24, 30), (12, 4, 4), (0, 0, 0), nsteps=150),
+            "ragged_70_zper": _ragged((70, 20, 16), (12, 4, 0), (0, 0, 1), nsteps=150)}
+
+
+@pytest.mark.parametrize("case", ["gold_sphere", "x_edges", "graphene_disks", "ragged_37", "ragged_100",
+                                  "ragged_70_zper"])
 @pytest.mark.parametrize("bx", ["2", "4"])
 def test_x_face_tiles_equal_general_x_edges_fp32(case, bx, monkeypatch):
     """FP32 one-pass tiles whose lanes reach into an x PML slab (x-face
@@ -116,8 +131,12 @@ def test_x_face_tiles_equal_general_x_edges_fp32(case, bx, monkeypatch):
     x-slab cells)."""
     from fdtd_amd import abi
 
-    spec = _cases()[case]
-    env = {"FDTD_ONEPASS": "1", "FDTD_ONEPASS_BY": "4", "FDTD_ONEPASS_KC": "16", "FDTD_XFACE_BY": bx}
+    own = case in ("gold_sphere", "x_edges", "graphene_disks")
+    spec = _cases()[case] if own else _x_face_cases()[case]
+    # 4-plane chunks on the small ragged grids, so that some chunks miss the
+    # source plane, the sphere and the dielectric block
+    env = {"FDTD_ONEPASS": "1", "FDTD_ONEPASS_BY": "4", "FDTD_ONEPASS_KC": "16" if own else "4",
+           "FDTD_XFACE_BY": bx}
     for k, v in env.items():
         monkeypatch.setenv(k, v)
     import fdtd_amd
