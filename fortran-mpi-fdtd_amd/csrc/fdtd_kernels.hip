@@ -1130,7 +1130,7 @@ __global__ __launch_bounds__(256) void k_march_plain(FieldArgs<real> a, MarchArg
     const int i = w.ip;
     xl.on = T[7] != 0 && (i <= x.Px || i > a.nx - x.Px);
     if (xl.on) {
-      const bool hi = i > a.nx - x.Px;
+      const bool hi = i > x.Px;  // as pml_index (the slabs never overlap: 2 Px <= nx)
       xl.sb = x.base + (hi ? x.hioff + (i - (a.nx - x.Px + 1)) : i - 1);
       xl.f1 = x.f1[i];
       xl.f2 = x.f2[i];
