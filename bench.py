@@ -418,11 +418,22 @@ def main():
         c3 = c3_mixed_precision(args)
         print(f"[bench] c3 fp32 {c3['fp32_mcells_per_s']:.0f} fp64 {c3['fp64_mcells_per_s']:.0f} Mcell/s, "
               f"spectra err {max(c3['fp32_spectra_max_rel_err_vs_gpu_fp64']):.2e}", file=sys.stderr, flush=True)
+    # the FP32 x-face tiles (plain tiles reaching into an x PML slab, a
+    # launch of their own beside the dominant kernel): the same figures
+    xf = r32["kernels"].get("x_face")
+    xface = None
+    if xf:
+        ach = xf["bytes_per_launch"] / (xf["mean_ms"] * 1e-3) / 1e9
+        xface = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                 "traffic": None, "kernel": "k_march_plain_xf", "bytes_per_launch": xf["bytes_per_launch"],
+                 "mean_launch_ms": xf["mean_ms"]}
     if args.n == 512 and args.seed == 0:  # the workload the summary profiles
         tb, src = profiled_traffic(r32["roofline"]["kernel"], "fp32", "k_march_plain_gold_dimer_512.md")
         r32["roofline"]["traffic"] = tb
         if tb is not None:
             r32["roofline"]["traffic_source"] = src
+        if xface:
+            xface["traffic"], _ = profiled_traffic("k_march_plain_xf", "fp32", "k_march_plain_gold_dimer_512.md")
     line = {
         "metric": METRIC,
         "value": r32["value"],
@@ -440,6 +451,7 @@ def main():
                    "gap_nm": spec["gap_nm"], "seed": args.seed, "dft_planes": 3, "nfreq": 64, "probes": 8,
                    "engine": "fused", "dft_accumulation": "f64"},
         "roofline": r32["roofline"],
+        "roofline_x_face_tiles": xface,
         "fp64": None if r64 is None else {"value": r64["value"], "ms_per_step": r64["ms_per_step"],
                                           "dtype": "f64", "roofline": r64["roofline"]},
         "fp32_over_fp64": None if r64 is None else r32["value"] / r64["value"],
