@@ -99,3 +99,15 @@ def test_struct_layouts_match_the_header():
     ours = [C.sizeof(t) for t in (abi.Grid, abi.Pml, abi.Region, abi.Drude2CP, abi.Graphene, abi.Pulse,
                                   abi2d.Grid2D, abi2d.Pml2D, abi2d.Region2D, abi.LorentzJp, abi.Tfsf, abi2d.Tfsf2D)]
     assert sizes == ours
+
+
+def test_kernel_ids_mirror_the_header():
+    """The ctypes mirror's kernel ids and names (fdtd_kernel_stats,
+    fdtd_algorithmic_bytes) follow the header's FDTD_K_* enum one for one."""
+    from fdtd_amd import abi
+
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    ids = {m.group(1): int(m.group(2)) for m in re.finditer(r"\bFDTD_K_([A-Z0-9_]+)\s*=\s*(\d+)", text)}
+    assert ids["COUNT"] == abi.K_COUNT == len(abi.KERNEL_NAMES)
+    for name, value in ids.items():
+        assert getattr(abi, "K_" + name) == value, name
