@@ -75,6 +75,19 @@ def test_fused_dimer_resumes_bit_for_bit(precision, forms, monkeypatch):
     assert np.abs(whole["flux"][0]).max() > 0
 
 
+@pytest.mark.parametrize("xface", [("1", "0"), ("0", "1")])
+def test_fp32_x_face_tiles_resume_across_settings(xface, monkeypatch):
+    """FP32 one-pass step with the x slabs marched in x-face tiles or left
+    to the general kernels (FDTD_XFACE): a state saved under one setting
+    resumes under the other to the same bits (the split parts live in the
+    same arrays either way)."""
+    env = {"FDTD_ONEPASS": "1", "FDTD_ONEPASS_BY": "4", "FDTD_ONEPASS_KC": "16"}
+    whole, resumed, _ = _run(_dimer(400), "fp32", 150, dict(env, FDTD_XFACE=xface[0]),
+                             dict(env, FDTD_XFACE=xface[1]), monkeypatch)
+    _same(whole, resumed)
+    assert np.abs(whole["flux"][0]).max() > 0
+
+
 @pytest.mark.parametrize("maker", [_ref_box, _tfsf])
 def test_staged_and_tfsf_resume_bit_for_bit(maker, monkeypatch):
     """The reference-order engine with its y-slab PML and gated planes, and
